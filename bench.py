@@ -1,0 +1,168 @@
+"""SpTC benchmark on one MI355X.
+
+A step is one full SpTC -- linearize, sort, index match, partition and ESC
+accumulation -- of the NIPS-shaped workload (order-4 2482x2862x14036x17,
+3.1e6 nnz, Zipf(1.2) modes 0-2, lognormal values) contracted with itself over
+modes {2,3}, inputs resident in HBM when the timed region starts, output left
+resident in HBM (freed after the step).
+
+Prints one JSON line: output nnz/s plus the roofline of the dominant kernel
+and the reference CPU SpTC (Sparta ttt) timed on a bounded sample.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--workload", default="nips", choices=["nips", "plain_small", "nell2"])
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-sample-rows", type=int, default=0,
+                   help="X free rows in the CPU-baseline sample (0: auto)")
+    return p.parse_args()
+
+
+def to_device(t):
+    import torch
+
+    class Dev:
+        pass
+    d = Dev()
+    d.dims = t.dims
+    d.order = t.order
+    d.idx = [torch.from_numpy(a.view(np.int32)).cuda() for a in t.idx]
+    d.val = torch.from_numpy(t.val).cuda()
+    d.nnz = t.nnz
+    return d
+
+
+def cpu_baseline(spec, X, n_rows):
+    """Reference CPU SpTC (Sparta ttt, EXPERIMENT_MODES=3) on a bounded
+    sample: the nonzeros of the first `n_rows` X free rows (in free-row key
+    order) contracted against all of X."""
+    import oracle as orc
+    from gsparc_amd.coo import COOTensor, write_tns_fast
+    if not os.path.exists(orc.REF_TTT):
+        return None
+    free = [m for m in range(X.order) if m not in spec.cx]
+    fk = np.zeros(X.nnz, np.int64)
+    for m in free:
+        fk = fk * X.dims[m] + X.idx[m].astype(np.int64)
+    rows = np.unique(fk)
+    pick = rows[:n_rows]
+    sel = np.isin(fk, pick)
+    Xs = COOTensor(X.dims, [a[sel] for a in X.idx], X.val[sel])
+    # the sample's products (for the rate): sum over its nonzeros of the
+    # matching contract-key count in X
+    ck = np.zeros(X.nnz, np.int64)
+    for m in spec.cx:
+        ck = ck * X.dims[m] + X.idx[m].astype(np.int64)
+    u, cnt = np.unique(ck, return_counts=True)
+    look = dict(zip(u.tolist(), cnt.tolist()))
+    prods = int(sum(look.get(k, 0) for k in ck[sel].tolist()))
+    threads = os.cpu_count() or 1
+    with tempfile.TemporaryDirectory() as d:
+        xs_path = os.path.join(d, "xs.tns")
+        x_path = os.path.join(d, "x.tns")
+        write_tns_fast(xs_path, Xs)
+        write_tns_fast(x_path, X)
+        cmd = [orc.REF_TTT, "-X", xs_path, "-Y", x_path, "-m", str(len(spec.cx)), "-x",
+               *[str(c) for c in spec.cx], "-y", *[str(c) for c in spec.cy], "-t", str(threads)]
+        env = dict(os.environ, EXPERIMENT_MODES="3")
+        p = subprocess.run(cmd, env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
+                           text=True, timeout=600)
+    tot = None
+    for line in p.stderr.splitlines():
+        if line.startswith("[Total time]"):
+            tot = float(line.split(":")[1].split()[0])
+    if tot is None:
+        return None
+    return {"total_s": tot, "threads": threads, "sample_rows": int(len(pick)),
+            "sample_nnz": int(sel.sum()), "sample_products": prods}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
+              f"(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from gsparc_amd import sptc
+    from gsparc_amd.workloads import CONFIGS, NELL2_CAP_BYTES
+
+    spec = CONFIGS[args.workload]
+    X = spec.make()
+    dX = to_device(X)
+    torch.cuda.synchronize()
+    kw = {}
+    if args.workload == "nell2":
+        kw = dict(count_only=True, mem_budget_bytes=NELL2_CAP_BYTES)
+
+    def step():
+        r = sptc.SpTC(dX, dX, list(spec.cx), list(spec.cy), on_device=True, **kw)
+        return r
+
+    for _ in range(args.warmup):
+        r = step()
+        del r
+    torch.cuda.synchronize()
+    times, phases = [], []
+    out_nnz = ir = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        r = step()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - ts)
+        phases.append(r.phase_ms)
+        out_nnz, ir = r.nnz, r.ir_nnz
+        del r
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = elapsed / args.steps * 1e3
+    ph = np.median(np.array(phases), axis=0).tolist()
+    res = {
+        "metric": "SpTC output nnz/s",
+        "value": out_nnz / (elapsed / args.steps),
+        "unit": "nnz/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": spec.name, "contract_x": list(spec.cx),
+                   "contract_y": list(spec.cy), "nnz": X.nnz, "ir_nnz": ir,
+                   "out_nnz": out_nnz},
+        "phase_ms_median": {"linearize_sort": ph[1], "match": ph[2], "partition": ph[3],
+                            "accumulate": ph[4], "total": ph[5]},
+    }
+    if not args.no_cpu_baseline and args.workload != "nell2":
+        rows = args.cpu_sample_rows or (2000 if args.workload == "nips" else 4000)
+        cb = cpu_baseline(spec, X, rows)
+        if cb:
+            rate = cb["sample_products"] / cb["total_s"]
+            res["cpu_baseline"] = {
+                "value": rate, "unit": "products/s", "cores": cb["threads"],
+                "kind": "reference",
+                "sample": (f"Sparta ttt (EXPERIMENT_MODES=3) on the first {cb['sample_rows']} "
+                           f"X free rows ({cb['sample_nnz']} nnz, {cb['sample_products']} "
+                           f"products) against all of X; {cb['total_s']:.3f} s"),
+                "gpu_products_per_s": ir / (elapsed / args.steps),
+            }
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,86 @@
+"""Build recipe for the native libraries (run: python -m gsparc_amd.build).
+
+  lib/libsptc_gen.so  host-only workload generator (g++)
+  lib/libsptc_hip.so  gfx950 kernels + C ABI (hipcc, --offload-arch=gfx950)
+
+Every call recompiles everything (no incremental state), in parallel.
+"""
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+from ._lib import LIB_DIR, PKG_DIR, REPO_DIR
+
+CSRC = os.path.join(PKG_DIR, "csrc")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+HIP_SOURCES = ["sort.hip", "kernels.hip", "device.cpp", "engine.cpp", "api.cpp"]
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    # no packed-FP32 instructions anywhere in the device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
+    "-I" + os.path.join(REPO_DIR, "include"),
+]
+
+
+def _run(cmd):
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in p.stdout.splitlines()
+                    if "packed-fp32-ops" not in l and "not a recognized feature" not in l)
+    if p.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{out}")
+    return out
+
+
+def build_gen():
+    os.makedirs(LIB_DIR, exist_ok=True)
+    out = os.path.join(LIB_DIR, "libsptc_gen.so")
+    _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-o", out,
+          os.path.join(CSRC, "gen.cpp")])
+    return out
+
+
+def build_hip(jobs=8):
+    os.makedirs(LIB_DIR, exist_ok=True)
+    objdir = os.path.join(PKG_DIR, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    cmds = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(objdir, src.replace(".", "_") + ".o")
+        lang = ["-x", "hip"] if src.endswith(".hip") else []
+        cmds.append([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        for out in ex.map(_run, cmds):
+            if out.strip():
+                print(out)
+    lib = os.path.join(LIB_DIR, "libsptc_hip.so")
+    tmp = lib + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
+    os.replace(tmp, lib)
+    return lib
+
+
+def build_oracle():
+    """The CPU checker (oracle/liboracle.so) and, when the reference sources
+    are present, the reference CPU SpTC binary (oracle/_ref/ttt)."""
+    odir = os.path.join(REPO_DIR, "oracle")
+    _run(["make", "-s", "-B", "-C", odir, "all"])
+    if os.path.isdir("/root/reference/baselines/Sparta/src"):
+        _run(["make", "-s", "-C", odir, "ref"])
+
+
+def build_all():
+    build_gen()
+    build_hip()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    for w in what:
+        {"all": build_all, "gen": build_gen, "hip": build_hip, "oracle": build_oracle}[w]()
+    print("built:", ", ".join(what))

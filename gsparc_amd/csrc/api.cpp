@@ -1,0 +1,473 @@
+// extern "C" boundary of libsptc_hip.so (declared in include/sptc.h).
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "engine.hpp"
+
+using namespace sptc;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+sptc_status fail(sptc_status s, const std::string& m) {
+  g_last_error = m;
+  return s;
+}
+
+sptc_status from_hip(const HipError& e) {
+  char buf[256];
+  std::snprintf(buf, sizeof(buf), "%s failed: %s (line %d)", e.what, hipGetErrorString(e.err),
+                e.line);
+  return fail(SPTC_ERR_HIP, buf);
+}
+
+sptc_status check_coo(const sptc_coo* t, const char* name) {
+  if (!t) return fail(SPTC_ERR_BAD_ARGS, std::string(name) + " is NULL");
+  if (t->order < 1 || t->order > SPTC_MAX_ORDER)
+    return fail(SPTC_ERR_BAD_ARGS, std::string(name) + ": order out of range");
+  if (!t->dims) return fail(SPTC_ERR_BAD_ARGS, std::string(name) + ": dims is NULL");
+  if (t->idx_bytes != 4 && t->idx_bytes != 8)
+    return fail(SPTC_ERR_BAD_ARGS, std::string(name) + ": idx_bytes must be 4 or 8");
+  if (t->nnz) {
+    if (!t->idx || !t->val) return fail(SPTC_ERR_BAD_ARGS, std::string(name) + ": NULL arrays");
+    for (int m = 0; m < t->order; ++m)
+      if (!t->idx[m]) return fail(SPTC_ERR_BAD_ARGS, std::string(name) + ": NULL index array");
+  }
+  if (t->nnz >= (1ull << 32))
+    return fail(SPTC_ERR_BAD_ARGS, std::string(name) + ": nnz must be below 2^32");
+  return SPTC_OK;
+}
+
+bool device_ok() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sptc_status_string(sptc_status s) {
+  switch (s) {
+    case SPTC_OK: return "ok";
+    case SPTC_ERR_BAD_ARGS: return "bad arguments";
+    case SPTC_ERR_OOM_BUDGET: return "out of memory budget";
+    case SPTC_ERR_KEY_OVERFLOW: return "key needs more than 64 bits";
+    case SPTC_ERR_HIP: return "HIP runtime error";
+    case SPTC_ERR_NO_DEVICE: return "no usable GPU";
+    case SPTC_ERR_INTERNAL: return "internal consistency check failed";
+  }
+  return "unknown status";
+}
+
+const char* sptc_last_error(void) { return g_last_error.c_str(); }
+
+sptc_status sptc_layout(int32_t order, const uint64_t* dims, int32_t nc, const int32_t* cmodes,
+                        const int32_t* cwidth, int32_t role, int32_t* shift, int32_t* width,
+                        int32_t* nfbits, int32_t* ncbits) {
+  if (!dims || (nc && !cmodes) || (role != 1 && role != 2))
+    return fail(SPTC_ERR_BAD_ARGS, "sptc_layout: bad arguments");
+  Layout L;
+  sptc_status st = make_layout(order, dims, nc, cmodes, cwidth, role, &L);
+  if (st == SPTC_ERR_BAD_ARGS) return fail(st, "sptc_layout: bad modes or dims");
+  for (int m = 0; m < order; ++m) {
+    if (shift) shift[m] = L.shift[m];
+    if (width) width[m] = L.width[m];
+  }
+  if (nfbits) *nfbits = L.nfbits;
+  if (ncbits) *ncbits = L.ncbits;
+  return st;
+}
+
+sptc_status sptc_contract(const sptc_coo* X, const sptc_coo* Y, int32_t nc, const int32_t* cx,
+                          const int32_t* cy, const sptc_opts* opts, void* stream,
+                          sptc_result** out) {
+  if (!out) return fail(SPTC_ERR_BAD_ARGS, "out is NULL");
+  *out = nullptr;
+  sptc_status st;
+  if ((st = check_coo(X, "X")) != SPTC_OK) return st;
+  if ((st = check_coo(Y, "Y")) != SPTC_OK) return st;
+  if (nc < 0 || nc > X->order || nc > Y->order || (nc && (!cx || !cy)))
+    return fail(SPTC_ERR_BAD_ARGS, "bad contracted-mode count");
+  if (!device_ok()) return fail(SPTC_ERR_NO_DEVICE, "no GPU");
+  sptc_opts o;
+  std::memset(&o, 0, sizeof(o));
+  if (opts) o = *opts;
+  // One contract-field layout for X and Y: the wider of the two dims per
+  // contracted pair (equal to the reference's layout whenever the dims agree).
+  int32_t cw[SPTC_MAX_ORDER];
+  for (int i = 0; i < nc; ++i) {
+    if (cx[i] < 0 || cx[i] >= X->order || cy[i] < 0 || cy[i] >= Y->order)
+      return fail(SPTC_ERR_BAD_ARGS, "contracted mode out of range");
+    cw[i] = std::max(mode_bits(X->dims[cx[i]]), mode_bits(Y->dims[cy[i]]));
+  }
+  Layout xl, yl;
+  st = make_layout(X->order, X->dims, nc, cx, cw, 1, &xl);
+  if (st != SPTC_OK)
+    return fail(st, st == SPTC_ERR_KEY_OVERFLOW ? "X key needs more than 64 bits"
+                                                : "bad X modes");
+  st = make_layout(Y->order, Y->dims, nc, cy, cw, 2, &yl);
+  if (st != SPTC_OK)
+    return fail(st, st == SPTC_ERR_KEY_OVERFLOW ? "Y key needs more than 64 bits"
+                                                : "bad Y modes");
+  sptc_result_s* res = new (std::nothrow) sptc_result_s();
+  if (!res) return fail(SPTC_ERR_BAD_ARGS, "host allocation failed");
+  std::memset(&res->info, 0, sizeof(res->info));
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  try {
+    Engine e(s, o);
+    StagedCOO sx, sy;
+    Operand ox, oy;
+    e.record(0);
+    e.stage(*X, &sx);
+    e.stage(*Y, &sy);
+    e.record(1);
+    e.prepare(sx, xl, &ox);
+    e.prepare(sy, yl, &oy);
+    e.record(2);
+    st = e.contract(ox, oy, res);
+    if (st != SPTC_OK) {
+      delete res;
+      return fail(st, e.msg);
+    }
+    std::memcpy(res->info.phase_ms, e.times.ms, sizeof(res->info.phase_ms));
+    res->info.accumulator = 0;
+    if (o.verbose) {
+      std::fprintf(stderr,
+                   "sptc: ir_nnz %llu out_nnz %llu chunks %d | linearize+sort %.3f ms, "
+                   "match %.3f ms, partition %.3f ms, accumulate %.3f ms, total %.3f ms\n",
+                   (unsigned long long)res->info.ir_nnz, (unsigned long long)res->info.nnz,
+                   res->info.nchunks, e.times.ms[1], e.times.ms[2], e.times.ms[3],
+                   e.times.ms[4], e.times.ms[5]);
+    }
+  } catch (const HipError& he) {
+    delete res;
+    return from_hip(he);
+  } catch (const std::bad_alloc&) {
+    delete res;
+    return fail(SPTC_ERR_OOM_BUDGET, "host allocation failed");
+  }
+  *out = res;
+  return SPTC_OK;
+}
+
+sptc_status sptc_result_get_info(const sptc_result* r, sptc_result_info* info) {
+  if (!r || !info) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");
+  *info = r->info;
+  return SPTC_OK;
+}
+
+sptc_status sptc_result_get_chunk(const sptc_result* r, int32_t i, uint64_t* nnz,
+                                  const uint32_t** row, const uint64_t** col,
+                                  const float** val) {
+  if (!r || i < 0 || (size_t)i >= r->chunks.size())
+    return fail(SPTC_ERR_BAD_ARGS, "bad chunk index");
+  const OutChunk& c = r->chunks[i];
+  if (nnz) *nnz = c.nnz;
+  if (row) *row = c.row.get();
+  if (col) *col = c.col.get();
+  if (val) *val = c.val.get();
+  return SPTC_OK;
+}
+
+sptc_status sptc_result_copy_to_host(const sptc_result* r, uint64_t* keys, uint64_t* xfree,
+                                     uint64_t* yfree, float* vals) {
+  if (!r) return fail(SPTC_ERR_BAD_ARGS, "NULL result");
+  if (keys && r->info.out_bits > 64)
+    return fail(SPTC_ERR_KEY_OVERFLOW, "packed output key needs more than 64 bits");
+  try {
+    std::vector<uint64_t> rk(r->info.nrows);
+    if (r->info.nrows)
+      SPTC_HIP(hipMemcpy(rk.data(), r->row_key.get(), r->info.nrows * 8, hipMemcpyDeviceToHost));
+    uint64_t off = 0;
+    std::vector<uint32_t> row;
+    std::vector<uint64_t> col;
+    for (const OutChunk& c : r->chunks) {
+      if (!c.nnz) continue;
+      row.resize(c.nnz);
+      col.resize(c.nnz);
+      SPTC_HIP(hipMemcpy(row.data(), c.row.get(), c.nnz * 4, hipMemcpyDeviceToHost));
+      SPTC_HIP(hipMemcpy(col.data(), c.col.get(), c.nnz * 8, hipMemcpyDeviceToHost));
+      if (vals)
+        SPTC_HIP(hipMemcpy(vals + off, c.val.get(), c.nnz * 4, hipMemcpyDeviceToHost));
+      const int ynf = r->info.y_nfbits;
+      for (uint64_t i = 0; i < c.nnz; ++i) {
+        const uint64_t xf = rk[row[i]];
+        if (xfree) xfree[off + i] = xf;
+        if (yfree) yfree[off + i] = col[i];
+        if (keys) keys[off + i] = shl64(xf, ynf) | col[i];
+      }
+      off += c.nnz;
+    }
+  } catch (const HipError& he) {
+    return from_hip(he);
+  }
+  return SPTC_OK;
+}
+
+sptc_status sptc_result_row_keys(const sptc_result* r, uint64_t* row_keys) {
+  if (!r || (!row_keys && r->info.nrows)) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");
+  try {
+    if (r->info.nrows)
+      SPTC_HIP(hipMemcpy(row_keys, r->row_key.get(), r->info.nrows * 8, hipMemcpyDeviceToHost));
+  } catch (const HipError& he) {
+    return from_hip(he);
+  }
+  return SPTC_OK;
+}
+
+sptc_status sptc_result_row_range(const sptc_result* r, uint64_t row, uint64_t* begin,
+                                  uint64_t* end) {
+  if (!r || !begin || !end) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");
+  try {
+    uint64_t off = 0;
+    for (const OutChunk& c : r->chunks) {
+      if (row >= c.r0 && row < c.r1 && c.nnz) {
+        // lower/upper bound of `row` in the chunk's sorted row array
+        auto at = [&](uint64_t i) {
+          uint32_t v;
+          SPTC_HIP(hipMemcpy(&v, c.row.get() + i, 4, hipMemcpyDeviceToHost));
+          return (uint64_t)v;
+        };
+        uint64_t lo = 0, hi = c.nnz;
+        while (lo < hi) {
+          uint64_t mid = (lo + hi) / 2;
+          if (at(mid) < row) lo = mid + 1; else hi = mid;
+        }
+        uint64_t b = lo;
+        hi = c.nnz;
+        while (lo < hi) {
+          uint64_t mid = (lo + hi) / 2;
+          if (at(mid) <= row) lo = mid + 1; else hi = mid;
+        }
+        *begin = off + b;
+        *end = off + lo;
+        return SPTC_OK;
+      }
+      off += c.nnz;
+    }
+    *begin = *end = off;
+  } catch (const HipError& he) {
+    return from_hip(he);
+  }
+  return SPTC_OK;
+}
+
+sptc_status sptc_result_copy_range(const sptc_result* r, uint64_t begin, uint64_t end,
+                                   uint64_t* xfree, uint64_t* yfree, float* vals) {
+  if (!r || begin > end || end > r->info.nnz) return fail(SPTC_ERR_BAD_ARGS, "bad range");
+  try {
+    uint64_t off = 0, o = 0;
+    std::vector<uint32_t> row;
+    for (const OutChunk& c : r->chunks) {
+      const uint64_t cb = std::max(begin, off), ce = std::min(end, off + c.nnz);
+      if (cb < ce) {
+        const uint64_t n = ce - cb, lo = cb - off;
+        if (xfree) {
+          row.resize(n);
+          SPTC_HIP(hipMemcpy(row.data(), c.row.get() + lo, n * 4, hipMemcpyDeviceToHost));
+          // rows are ascending within the range: fetch that slice of row_key once
+          const uint64_t rlo = row.front(), rhi = row.back();
+          std::vector<uint64_t> rk(rhi - rlo + 1);
+          SPTC_HIP(hipMemcpy(rk.data(), r->row_key.get() + rlo, rk.size() * 8,
+                             hipMemcpyDeviceToHost));
+          for (uint64_t i = 0; i < n; ++i) xfree[o + i] = rk[row[i] - rlo];
+        }
+        if (yfree)
+          SPTC_HIP(hipMemcpy(yfree + o, c.col.get() + lo, n * 8, hipMemcpyDeviceToHost));
+        if (vals) SPTC_HIP(hipMemcpy(vals + o, c.val.get() + lo, n * 4, hipMemcpyDeviceToHost));
+        o += n;
+      }
+      off += c.nnz;
+    }
+  } catch (const HipError& he) {
+    return from_hip(he);
+  }
+  return SPTC_OK;
+}
+
+sptc_status sptc_result_decode(const sptc_result* r, uint32_t* const* mode_idx, float* vals) {
+  if (!r || (!mode_idx && r->info.out_order)) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");
+  const uint64_t n = r->info.nnz;
+  std::vector<uint64_t> xf(n), yf(n);
+  sptc_status st = sptc_result_copy_to_host(r, nullptr, xf.data(), yf.data(), vals);
+  if (st != SPTC_OK) return st;
+  const Layout& xl = r->xl;
+  const Layout& yl = r->yl;
+  int o = 0;
+  for (int i = 0; i < xl.nf; ++i, ++o) {
+    const int m = xl.fmodes[i];
+    const int sh = xl.shift[m] - xl.ncbits;
+    const uint64_t mk = low_mask(xl.width[m]);
+    for (uint64_t k = 0; k < n; ++k) mode_idx[o][k] = (uint32_t)(shr64(xf[k], sh) & mk);
+  }
+  for (int i = 0; i < yl.nf; ++i, ++o) {
+    const int m = yl.fmodes[i];
+    const uint64_t mk = low_mask(yl.width[m]);
+    for (uint64_t k = 0; k < n; ++k) mode_idx[o][k] = (uint32_t)(shr64(yf[k], yl.shift[m]) & mk);
+  }
+  return SPTC_OK;
+}
+
+void sptc_result_free(sptc_result* r) { delete r; }
+
+// ---- per-phase entry points ---------------------------------------------
+
+sptc_status sptc_linearize(int32_t order, uint64_t nnz, const void* const* idx, int32_t idx_bytes,
+                           const int32_t* shift, const int32_t* width, uint64_t* keys,
+                           void* stream) {
+  if (order < 1 || order > SPTC_MAX_ORDER || !shift || (nnz && (!idx || !keys)) ||
+      (idx_bytes != 4 && idx_bytes != 8))
+    return fail(SPTC_ERR_BAD_ARGS, "sptc_linearize: bad arguments");
+  int total = 0;
+  for (int m = 0; m < order; ++m) {
+    if (shift[m] < 0 || shift[m] > 63) return fail(SPTC_ERR_BAD_ARGS, "shift out of range");
+    if (width && (width[m] < 0 || width[m] > 64))
+      return fail(SPTC_ERR_BAD_ARGS, "width out of range");
+    if (width) total = std::max(total, shift[m] + width[m]);
+  }
+  if (total > 64) return fail(SPTC_ERR_KEY_OVERFLOW, "fields exceed 64 bits");
+  if (!device_ok()) return fail(SPTC_ERR_NO_DEVICE, "no GPU");
+  try {
+    LinArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.order = order;
+    for (int m = 0; m < order; ++m) {
+      a.idx[m] = idx[m];
+      a.shift[m] = shift[m];
+      a.width[m] = width ? width[m] : 64;
+    }
+    launch_linearize(a, idx_bytes, nnz, keys, nullptr, 0, 8,
+                     reinterpret_cast<hipStream_t>(stream));
+    SPTC_HIP(hipGetLastError());
+  } catch (const HipError& he) {
+    return from_hip(he);
+  }
+  return SPTC_OK;
+}
+
+sptc_status sptc_sort_pairs(uint64_t* keys, float* vals, uint64_t n, int32_t nbits,
+                            int32_t radix_bits, void* stream) {
+  if (nbits < 0 || nbits > 64 || (n && (!keys || !vals)))
+    return fail(SPTC_ERR_BAD_ARGS, "sptc_sort_pairs: bad arguments");
+  if (radix_bits != 0 && radix_bits != 8 && radix_bits != 11)
+    return fail(SPTC_ERR_BAD_ARGS, "radix_bits must be 0, 8 or 11");
+  if (!device_ok()) return fail(SPTC_ERR_NO_DEVICE, "no GPU");
+  const int rb = radix_bits ? radix_bits : 8;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  try {
+    SortWorkspace ws;
+    ws.ensure(n, rb, s);
+    const int passes = radix_passes(nbits, rb);
+    ws.prepare(passes, s);
+    histogram(ws, keys, n, passes, s);
+    if (sort_after_histogram(ws, keys, vals, n, nbits, s)) {
+      SPTC_HIP(hipMemcpyAsync(keys, ws.alt_keys.get(), n * 8, hipMemcpyDeviceToDevice, s));
+      SPTC_HIP(hipMemcpyAsync(vals, ws.alt_vals.get(), n * 4, hipMemcpyDeviceToDevice, s));
+    }
+    uint32_t err = 0;
+    SPTC_HIP(hipMemcpyAsync(&err, ws.err(), 4, hipMemcpyDeviceToHost, s));
+    SPTC_HIP(hipStreamSynchronize(s));
+    if (err) return fail(SPTC_ERR_INTERNAL, "look-back spin limit exceeded");
+  } catch (const HipError& he) {
+    return from_hip(he);
+  }
+  return SPTC_OK;
+}
+
+sptc_status sptc_index_match(const uint64_t* xkeys, uint64_t nx, const uint64_t* ykeys,
+                             uint64_t ny, int32_t ncbits, int32_t y_nfbits, uint64_t* mpos,
+                             uint64_t* mcnt, uint64_t* prefix, uint64_t* ir_nnz, void* stream) {
+  if (ncbits < 0 || ncbits > 64 || y_nfbits < 0 || y_nfbits > 64 || !prefix ||
+      (nx && (!xkeys || !mpos || !mcnt)) || (ny && !ykeys))
+    return fail(SPTC_ERR_BAD_ARGS, "sptc_index_match: bad arguments");
+  if (!device_ok()) return fail(SPTC_ERR_NO_DEVICE, "no GPU");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  try {
+    DevBuf<uint64_t> tlo, thi, ukey, ustart;
+    YTable tab;
+    std::memset(&tab, 0, sizeof(tab));
+    if (ncbits <= kDirectBits) {
+      const uint64_t K = 1ull << ncbits;
+      tlo.reset(K);
+      thi.reset(K);
+      fill_u64(tlo.get(), 0, K, s);
+      fill_u64(thi.get(), 0, K, s);
+      launch_ytable_direct(ykeys, ny, y_nfbits, K, tlo.get(), thi.get(), s);
+      tab.direct = 1;
+      tab.lo = tlo.get();
+      tab.hi = thi.get();
+    } else {
+      ukey.reset(ny + 1);
+      ustart.reset(ny + 1);
+      DevBuf<U2> tmp(scan_tmp_count(ny) + 1);
+      run_yunique(ykeys, ny, y_nfbits, ukey.get(), ustart.get(), tmp.get(), s);
+      U2 t;
+      SPTC_HIP(hipMemcpyAsync(&t, tmp.get() + scan_tmp_count(ny), sizeof(U2),
+                              hipMemcpyDeviceToHost, s));
+      SPTC_HIP(hipStreamSynchronize(s));
+      tab.direct = 0;
+      tab.ukey = ukey.get();
+      tab.ustart = ustart.get();
+      tab.U = t.a;
+    }
+    MatchOut mo;
+    std::memset(&mo, 0, sizeof(mo));
+    mo.mpos = mpos;
+    mo.mcnt = mcnt;
+    mo.prefix = prefix;
+    DevBuf<U2> tmp(scan_tmp_count(nx) + 1);
+    run_match(xkeys, nullptr, nx, ncbits, tab, mo, 1, tmp.get(), s);
+    U2 t;
+    SPTC_HIP(hipMemcpyAsync(&t, tmp.get() + scan_tmp_count(nx), sizeof(U2),
+                            hipMemcpyDeviceToHost, s));
+    if (nx == 0) fill_u64(prefix, 0, 1, s);
+    SPTC_HIP(hipStreamSynchronize(s));
+    if (ir_nnz) *ir_nnz = t.a;
+  } catch (const HipError& he) {
+    return from_hip(he);
+  }
+  return SPTC_OK;
+}
+
+sptc_status sptc_exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n,
+                                    void* stream) {
+  if (!out || (n && !in)) return fail(SPTC_ERR_BAD_ARGS, "sptc_exclusive_scan_u64: NULL");
+  if (!device_ok()) return fail(SPTC_ERR_NO_DEVICE, "no GPU");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  try {
+    DevBuf<U2> tmp(scan_tmp_count(n) + 1);
+    if (n == 0) fill_u64(out, 0, 1, s);
+    else run_exclusive_scan(in, out, n, tmp.get(), s);
+    SPTC_HIP(hipStreamSynchronize(s));
+  } catch (const HipError& he) {
+    return from_hip(he);
+  }
+  return SPTC_OK;
+}
+
+sptc_status sptc_device_info(int32_t* num_cus, uint64_t* free_bytes, uint64_t* total_bytes,
+                             double* peak_hbm_gbs) {
+  if (!device_ok()) return fail(SPTC_ERR_NO_DEVICE, "no GPU");
+  try {
+    const DeviceInfo& d = device_info();
+    if (num_cus) *num_cus = d.num_cus;
+    if (total_bytes) *total_bytes = d.total_bytes;
+    if (peak_hbm_gbs) *peak_hbm_gbs = d.peak_hbm_gbs;
+    if (free_bytes) *free_bytes = device_free_bytes();
+  } catch (const HipError& he) {
+    return from_hip(he);
+  }
+  return SPTC_OK;
+}
+
+void sptc_trim_cache(void) { DevicePool::get().trim(); }
+
+}  // extern "C"

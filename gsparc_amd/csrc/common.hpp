@@ -1,0 +1,116 @@
+// Shared definitions for the gfx950 SpTC kernels and their host launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+
+#include "../../include/sptc.h"
+
+namespace sptc {
+
+constexpr int kBlock = 256;  // 4 wave64s per workgroup
+constexpr int kWave = 64;
+
+// ---- host-side error plumbing -------------------------------------------
+struct HipError {
+  hipError_t err;
+  const char* what;
+  int line;
+};
+
+#define SPTC_HIP(call)                                                  \
+  do {                                                                  \
+    hipError_t e_ = (call);                                             \
+    if (e_ != hipSuccess) throw ::sptc::HipError{e_, #call, __LINE__}; \
+  } while (0)
+
+__host__ __device__ inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+// Shifts that are defined for every count in [0, 64].
+__host__ __device__ __forceinline__ uint64_t shr64(uint64_t k, int s) { return s >= 64 ? 0 : k >> s; }
+__host__ __device__ __forceinline__ uint64_t shl64(uint64_t k, int s) { return s >= 64 ? 0 : k << s; }
+__host__ __device__ __forceinline__ uint64_t low_mask(int bits) {
+  return bits >= 64 ? ~0ull : ((1ull << bits) - 1);
+}
+
+// ---- device helpers ------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+// Number of set bits of m in lanes strictly below this lane.
+__device__ __forceinline__ uint32_t popc_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// Agent-scope (whole device, across XCD L2s) relaxed atomics for look-back.
+__device__ __forceinline__ uint64_t ld_agent(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wave-level inclusive scan of uint64 via LDS-free shuffles.
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    uint64_t o = __shfl_up(v, d, kWave);
+    if (lane >= (uint32_t)d) v += o;
+  }
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_incl_scan32(uint32_t v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    uint32_t o = __shfl_up(v, d, kWave);
+    if (lane >= (uint32_t)d) v += o;
+  }
+  return v;
+}
+
+// Block-wide exclusive scan (blockDim == kBlock), returns exclusive prefix
+// and the block total through *total.  `scratch` holds kBlock/kWave values.
+template <typename T>
+__device__ __forceinline__ T block_excl_scan(T v, T* scratch, T* total) {
+  const uint32_t lane = lane_id();
+  const uint32_t wave = threadIdx.x / kWave;
+  T inc;
+  if constexpr (sizeof(T) == 8) inc = wave_incl_scan(v);
+  else inc = wave_incl_scan32(v);
+  if (lane == kWave - 1) scratch[wave] = inc;
+  __syncthreads();
+  T wave_off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / kWave; ++w) {
+    T s = scratch[w];
+    if ((uint32_t)w < wave) wave_off += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return wave_off + inc - v;
+}
+
+// Neumaier-compensated FP32 accumulation (error ~2u independent of count).
+struct KahanF32 {
+  float s = 0.f, c = 0.f;
+  __device__ __forceinline__ void add(float x) {
+    float t = s + x;
+    if (fabsf(s) >= fabsf(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+  }
+  __device__ __forceinline__ float value() const { return s + c; }
+};
+
+// ---- launchers (defined in the .hip files) --------------------------------
+struct SortWorkspace;
+
+// fill / copy helpers (graph-safe: kernels, not hipMemsetAsync)
+void fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t s);
+void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+
+}  // namespace sptc

@@ -1,0 +1,76 @@
+// Device context: caching allocator with budget accounting, events, props.
+#pragma once
+#include <map>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "common.hpp"
+
+namespace sptc {
+
+// Caching device allocator.  Freed blocks are kept for reuse (keyed by
+// size); `live_bytes` counts what callers currently hold and is what memory
+// budgets are checked against -- the analogue of the reference's
+// CudaMemoryPool::is_available (cuda_memory.cpp:37-51), which returns a
+// status here instead of exit(1) on exhaustion.
+class DevicePool {
+ public:
+  static DevicePool& get();
+  void* alloc(size_t bytes);  // throws HipError on failure
+  void free(void* p);
+  void trim();                // return cached blocks to the runtime
+  size_t live_bytes() const { return live_; }
+  size_t cached_bytes() const { return cached_; }
+
+ private:
+  std::mutex mu_;
+  std::multimap<size_t, void*> cache_;
+  std::unordered_map<void*, size_t> sizes_;
+  size_t live_ = 0, cached_ = 0;
+};
+
+// RAII device buffer from the pool.
+template <typename T>
+class DevBuf {
+ public:
+  DevBuf() = default;
+  explicit DevBuf(uint64_t n) { reset(n); }
+  ~DevBuf() { release(); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+    return *this;
+  }
+  void reset(uint64_t n) {
+    release();
+    n_ = n;
+    p_ = n ? static_cast<T*>(DevicePool::get().alloc(n * sizeof(T))) : nullptr;
+  }
+  void release() {
+    if (p_) DevicePool::get().free(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  T* get() const { return p_; }
+  uint64_t size() const { return n_; }
+  uint64_t bytes() const { return n_ * sizeof(T); }
+
+ private:
+  T* p_ = nullptr;
+  uint64_t n_ = 0;
+};
+
+struct DeviceInfo {
+  int device = 0;
+  int num_cus = 0;
+  size_t total_bytes = 0;
+  double peak_hbm_gbs = 0;
+  char arch[64] = {0};
+};
+const DeviceInfo& device_info();
+size_t device_free_bytes();
+
+}  // namespace sptc

@@ -1,0 +1,356 @@
+// SpTC engine host orchestration (built with hipcc; launches live in .hip).
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace sptc {
+
+int mode_bits(uint64_t d) {
+  // 64 - clz(d-1) (tensor_manager.tpp:109); a dim of 1 needs no bits.
+  return d <= 1 ? 0 : 64 - __builtin_clzll(d - 1);
+}
+
+// Free modes ascending with the last one at bit 0; contracted modes in the
+// caller's order with the last one at bit 0 (tensor_manager.tpp:153-174).
+// X (role 1): key = f << ncbits | c; Y (role 2): key = c << nfbits | f.
+sptc_status make_layout(int order, const uint64_t* dims, int nc, const int32_t* cmodes,
+                        const int32_t* cwidth, int role, Layout* L) {
+  if (order < 1 || order > SPTC_MAX_ORDER || nc < 0 || nc > order) return SPTC_ERR_BAD_ARGS;
+  *L = Layout();
+  L->order = order;
+  L->role = role;
+  bool is_c[SPTC_MAX_ORDER] = {false};
+  for (int i = 0; i < nc; ++i) {
+    if (cmodes[i] < 0 || cmodes[i] >= order || is_c[cmodes[i]]) return SPTC_ERR_BAD_ARGS;
+    is_c[cmodes[i]] = true;
+    L->cmodes[i] = cmodes[i];
+  }
+  L->nc = nc;
+  for (int m = 0; m < order; ++m) {
+    if (dims[m] == 0) return SPTC_ERR_BAD_ARGS;
+    L->dims[m] = dims[m];
+    L->width[m] = mode_bits(dims[m]);
+    if (!is_c[m]) L->fmodes[L->nf++] = m;
+  }
+  for (int i = 0; i < nc; ++i)
+    if (cwidth && cwidth[i] >= 0) L->width[cmodes[i]] = cwidth[i];
+  int s = 0;
+  for (int i = L->nf - 1; i >= 0; --i) {
+    L->shift[L->fmodes[i]] = s;
+    s += L->width[L->fmodes[i]];
+  }
+  L->nfbits = s;
+  s = 0;
+  for (int i = nc - 1; i >= 0; --i) {
+    L->shift[cmodes[i]] = s;
+    s += L->width[cmodes[i]];
+  }
+  L->ncbits = s;
+  L->nbits = L->nfbits + L->ncbits;
+  for (int m = 0; m < order; ++m) {
+    if (is_c[m] && role == 2) L->shift[m] += L->nfbits;
+    if (!is_c[m] && role == 1) L->shift[m] += L->ncbits;
+  }
+  return L->nbits > 64 ? SPTC_ERR_KEY_OVERFLOW : SPTC_OK;
+}
+
+Engine::Engine(hipStream_t s, const sptc_opts& o) : s_(s), o_(o) {
+  for (auto& e : ev_) SPTC_HIP(hipEventCreate(&e));
+  SPTC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_total_), sizeof(U2) * 2, 0));
+  SPTC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_u64_), sizeof(uint64_t) * 8, 0));
+  const uint64_t live = DevicePool::get().live_bytes();
+  if (o.mem_budget_bytes) {
+    budget_ = o.mem_budget_bytes;
+  } else {
+    const uint64_t fr = device_free_bytes();
+    const uint64_t margin = 1ull << 30;
+    budget_ = live + (fr > margin ? fr - margin : 0);
+  }
+  rb_ = (o.radix_bits == 8 || o.radix_bits == 11) ? o.radix_bits : 8;
+}
+
+Engine::~Engine() {
+  for (auto& e : ev_) (void)hipEventDestroy(e);
+  (void)hipHostFree(host_total_);
+  (void)hipHostFree(host_u64_);
+}
+
+void Engine::record(int i) {
+  SPTC_HIP(hipEventRecord(ev_[i], s_));
+  ev_used_[i] = true;
+}
+
+U2 Engine::read_total(const U2* dev) {
+  SPTC_HIP(hipMemcpyAsync(host_total_, dev, sizeof(U2), hipMemcpyDeviceToHost, s_));
+  SPTC_HIP(hipStreamSynchronize(s_));
+  return host_total_[0];
+}
+
+void Engine::stage(const sptc_coo& t, StagedCOO* out) {
+  const uint64_t n = t.nnz;
+  out->nnz = n;
+  out->order = t.order;
+  out->idx_bytes = t.idx_bytes;
+  out->own.clear();
+  if (t.on_device || n == 0) {
+    for (int m = 0; m < t.order; ++m) out->idx[m] = t.idx[m];
+    out->val = t.val;
+    return;
+  }
+  for (int m = 0; m < t.order; ++m) {
+    out->own.emplace_back(n * t.idx_bytes);
+    SPTC_HIP(hipMemcpyAsync(out->own.back().get(), t.idx[m], n * t.idx_bytes,
+                            hipMemcpyHostToDevice, s_));
+    out->idx[m] = out->own.back().get();
+  }
+  out->own.emplace_back(n * sizeof(float));
+  SPTC_HIP(hipMemcpyAsync(out->own.back().get(), t.val, n * sizeof(float),
+                          hipMemcpyHostToDevice, s_));
+  out->val = reinterpret_cast<const float*>(out->own.back().get());
+}
+
+void Engine::prepare(const StagedCOO& t, const Layout& lay, Operand* out) {
+  const uint64_t n = t.nnz;
+  out->lay = lay;
+  out->nnz = n;
+  out->keys.reset(n ? n : 1);
+  out->vals.reset(n ? n : 1);
+  if (!n) return;
+  LinArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.order = lay.order;
+  for (int m = 0; m < lay.order; ++m) {
+    a.shift[m] = lay.shift[m];
+    a.width[m] = lay.width[m];
+    a.idx[m] = t.idx[m];
+  }
+  SPTC_HIP(hipMemcpyAsync(out->vals.get(), t.val, n * sizeof(float),
+                          hipMemcpyDeviceToDevice, s_));
+  const int passes = radix_passes(lay.nbits, rb_);
+  ws_.ensure(n, rb_, s_);
+  ws_.prepare(passes, s_);
+  launch_linearize(a, t.idx_bytes, n, out->keys.get(), ws_.hist.get(),
+                   passes <= kMaxFusedPasses ? passes : 0, rb_, s_);
+  if (passes > kMaxFusedPasses) histogram(ws_, out->keys.get(), n, passes, s_);
+  if (sort_after_histogram(ws_, out->keys.get(), out->vals.get(), n, lay.nbits, s_)) {
+    std::swap(out->keys, ws_.alt_keys);
+    std::swap(out->vals, ws_.alt_vals);
+  }
+}
+
+sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* res) {
+  const Layout& xl = X.lay;
+  const Layout& yl = Y.lay;
+  const uint64_t nx = X.nnz, ny = Y.nnz;
+  const int ncb = xl.ncbits, ynf = yl.nfbits;
+  sptc_result_info& info = res->info;
+  info.x_nfbits = xl.nfbits;
+  info.y_nfbits = ynf;
+  info.ncbits = ncb;
+  info.out_bits = xl.nfbits + ynf;
+  info.out_order = xl.nf + yl.nf;
+  res->xl = xl;
+  res->yl = yl;
+
+  // ---- (c) index match ---------------------------------------------------
+  DevBuf<uint64_t> tlo, thi, ukey, ustart;
+  YTable tab;
+  std::memset(&tab, 0, sizeof(tab));
+  if (ncb <= kDirectBits) {
+    const uint64_t K = 1ull << ncb;
+    tlo.reset(K);
+    thi.reset(K);
+    fill_u64(tlo.get(), 0, K, s_);
+    fill_u64(thi.get(), 0, K, s_);
+    launch_ytable_direct(Y.keys.get(), ny, ynf, K, tlo.get(), thi.get(), s_);
+    tab.direct = 1;
+    tab.lo = tlo.get();
+    tab.hi = thi.get();
+  } else {
+    ukey.reset(ny + 1);
+    ustart.reset(ny + 1);
+    DevBuf<U2> tmp(scan_tmp_count(ny) + 1);
+    run_yunique(Y.keys.get(), ny, ynf, ukey.get(), ustart.get(), tmp.get(), s_);
+    tab.direct = 0;
+    tab.ukey = ukey.get();
+    tab.ustart = ustart.get();
+    tab.U = read_total(tmp.get() + scan_tmp_count(ny)).a;
+  }
+  DevBuf<uint64_t> act_prefix(nx + 1), act_ypos(nx + 1), row_key(nx + 1), row_prod(nx + 1),
+      row_act(nx + 1);
+  DevBuf<float> act_xval(nx + 1);
+  DevBuf<uint32_t> act_row(nx + 1);
+  MatchOut mo;
+  std::memset(&mo, 0, sizeof(mo));
+  mo.act_prefix = act_prefix.get();
+  mo.act_ypos = act_ypos.get();
+  mo.act_xval = act_xval.get();
+  mo.act_row = act_row.get();
+  mo.row_key = row_key.get();
+  mo.row_prod = row_prod.get();
+  mo.row_act = row_act.get();
+  U2 tot;
+  {
+    DevBuf<U2> tmp(scan_tmp_count(nx) + 1);
+    run_match(X.keys.get(), X.vals.get(), nx, ncb, tab, mo, 0, tmp.get(), s_);
+    tot = read_total(tmp.get() + scan_tmp_count(nx));
+  }
+  const uint64_t ir = tot.a;
+  const uint64_t A = tot.b >> 32;
+  const uint64_t R = tot.b & 0xFFFFFFFFull;
+  host_u64_[0] = ir;
+  host_u64_[1] = A;
+  SPTC_HIP(hipMemcpyAsync(row_prod.get() + R, &host_u64_[0], 8, hipMemcpyHostToDevice, s_));
+  SPTC_HIP(hipMemcpyAsync(row_act.get() + R, &host_u64_[1], 8, hipMemcpyHostToDevice, s_));
+  tlo.release();
+  thi.release();
+  ukey.release();
+  ustart.release();
+  record(3);
+  info.ir_nnz = ir;
+  info.nrows = R;
+  res->row_key = std::move(row_key);
+
+  // ---- (e) partition: chunk the X rows so ESC buffers fit the budget -------
+  if (o_.dense_accumulator == 1) {
+    msg = "dense accumulator is not available for this contraction";
+    return SPTC_ERR_BAD_ARGS;
+  }
+  std::vector<uint64_t> bounds_r{0}, bounds_p{0}, bounds_a{0};
+  DevBuf<double> vsum(1);
+  fill_u64(reinterpret_cast<uint64_t*>(vsum.get()), 0, 1, s_);
+  if (ir) {
+    DevBuf<uint64_t> scratch(4);
+    launch_max_row(row_prod.get(), R, scratch.get(), s_);
+    SPTC_HIP(hipMemcpyAsync(host_u64_, scratch.get(), 8, hipMemcpyDeviceToHost, s_));
+    SPTC_HIP(hipStreamSynchronize(s_));
+    const uint64_t max_row = host_u64_[0];
+    const uint64_t live = DevicePool::get().live_bytes();
+    const uint64_t free_b = budget_ > live ? budget_ - live : 0;
+    const uint64_t per_prod = 12 + SortWorkspace::bytes_per_key(rb_) + 1;
+    const uint64_t out_per = o_.count_only ? 0 : 16;
+    const uint64_t slack = 64ull << 20;
+    uint64_t cap;
+    if (free_b > slack + out_per * ir + per_prod * std::min<uint64_t>(ir, 1ull << 20))
+      cap = (free_b - slack - out_per * ir) / per_prod;
+    else
+      cap = free_b > slack ? (free_b - slack) / (per_prod + out_per) : 0;
+    cap = std::min<uint64_t>(cap, std::min<uint64_t>(ir, 4000000000ull));
+    if (cap < max_row) {
+      msg = "one X row's products exceed the memory budget";
+      return SPTC_ERR_OOM_BUDGET;
+    }
+    const uint64_t rmax = ynf >= 32 ? (ynf >= 64 ? 1 : (1ull << (64 - ynf))) : (1ull << 32);
+    uint64_t r0 = 0;
+    while (r0 < R) {
+      launch_chunk_end(row_prod.get(), row_act.get(), R, r0, rmax, cap, scratch.get(), s_);
+      SPTC_HIP(hipMemcpyAsync(host_u64_, scratch.get(), 24, hipMemcpyDeviceToHost, s_));
+      SPTC_HIP(hipStreamSynchronize(s_));
+      uint64_t r1 = host_u64_[0];
+      if (r1 <= r0) {
+        msg = "partitioning made no progress";
+        return SPTC_ERR_INTERNAL;
+      }
+      bounds_r.push_back(r1);
+      bounds_p.push_back(host_u64_[1]);
+      bounds_a.push_back(host_u64_[2]);
+      r0 = r1;
+    }
+    record(4);
+
+    // ---- (d) ESC accumulate per chunk --------------------------------------
+    uint64_t chunk_cap = 0;
+    for (size_t c = 1; c < bounds_p.size(); ++c)
+      chunk_cap = std::max(chunk_cap, bounds_p[c] - bounds_p[c - 1]);
+    DevBuf<uint64_t> kA(chunk_cap);
+    DevBuf<float> vA(chunk_cap);
+    DevBuf<U2> tmp(scan_tmp_count(chunk_cap) + 1);
+    ws_.ensure(chunk_cap, rb_, s_);
+    const uint64_t fmask = low_mask(ynf);
+    for (size_t c = 1; c < bounds_r.size(); ++c) {
+      const uint64_t r0c = bounds_r[c - 1], r1c = bounds_r[c];
+      const uint64_t P0 = bounds_p[c - 1], P1 = bounds_p[c];
+      const uint64_t n = P1 - P0;
+      if (!n) continue;
+      const int bits = mode_bits(r1c - r0c) + ynf;
+      const int passes = radix_passes(bits, rb_);
+      ws_.prepare(passes, s_);
+      ExpandArgs ea;
+      ea.act_prefix = act_prefix.get();
+      ea.act_ypos = act_ypos.get();
+      ea.act_xval = act_xval.get();
+      ea.act_row = act_row.get();
+      ea.A0 = bounds_a[c - 1];
+      ea.A1 = bounds_a[c];
+      ea.P0 = P0;
+      ea.P1 = P1;
+      ea.r0 = (uint32_t)r0c;
+      ea.ykeys = Y.keys.get();
+      ea.yvals = Y.vals.get();
+      ea.ynf = ynf;
+      ea.fmask = fmask;
+      ea.out_keys = kA.get();
+      ea.out_vals = vA.get();
+      ea.hist = ws_.hist.get();
+      ea.passes = passes <= kMaxFusedPasses ? passes : 0;
+      ea.vsum = vsum.get();
+      launch_expand(ea, rb_, s_);
+      if (passes > kMaxFusedPasses) histogram(ws_, kA.get(), n, passes, s_);
+      const bool alt = sort_after_histogram(ws_, kA.get(), vA.get(), n, bits, s_);
+      const uint64_t* sk = alt ? ws_.alt_keys.get() : kA.get();
+      const float* sv = alt ? ws_.alt_vals.get() : vA.get();
+      run_reduce_count(sk, n, tmp.get(), s_);
+      const uint64_t out_n = read_total(tmp.get() + scan_tmp_count(n)).a;
+      info.nnz += out_n;
+      info.nchunks++;
+      if (o_.count_only) continue;
+      if (DevicePool::get().live_bytes() + out_n * 16 > budget_) {
+        msg = "output does not fit the memory budget";
+        return SPTC_ERR_OOM_BUDGET;
+      }
+      OutChunk oc;
+      oc.nnz = out_n;
+      oc.r0 = r0c;
+      oc.r1 = r1c;
+      oc.row.reset(out_n);
+      oc.col.reset(out_n);
+      oc.val.reset(out_n);
+      run_reduce_emit(sk, sv, n, oc.row.get(), oc.col.get(), oc.val.get(), r0c, ynf, fmask,
+                      tmp.get(), s_);
+      res->chunks.push_back(std::move(oc));
+    }
+  } else {
+    record(4);
+  }
+  SPTC_HIP(hipMemcpyAsync(&info.value_sum, vsum.get(), sizeof(double), hipMemcpyDeviceToHost,
+                          s_));
+  SPTC_HIP(hipMemcpyAsync(host_u64_, ws_.err(), 4, hipMemcpyDeviceToHost, s_));
+  record(5);
+  SPTC_HIP(hipStreamSynchronize(s_));
+  if (*reinterpret_cast<uint32_t*>(host_u64_) != 0) {
+    msg = "radix sort look-back exceeded its spin limit";
+    return SPTC_ERR_INTERNAL;
+  }
+  float t;
+  auto el = [&](int a, int b) -> double {
+    if (!ev_used_[a] || !ev_used_[b]) return 0.0;
+    SPTC_HIP(hipEventElapsedTime(&t, ev_[a], ev_[b]));
+    return t;
+  };
+  times.ms[0] = el(0, 1);
+  times.ms[1] = el(1, 2);
+  times.ms[2] = el(2, 3);
+  times.ms[3] = el(3, 4);
+  times.ms[4] = el(4, 5);
+  times.ms[5] = el(0, 5);
+  if (res->chunks.size() == 1) {
+    info.row = res->chunks[0].row.get();
+    info.col = res->chunks[0].col.get();
+    info.val = res->chunks[0].val.get();
+  }
+  info.row_key = res->row_key.get();
+  return SPTC_OK;
+}
+
+}  // namespace sptc

@@ -1,0 +1,101 @@
+// SpTC engine: operand preparation (linearize + sort), index match,
+// memory-aware chunking and ESC accumulation.  Host orchestration only; the
+// work runs in the kernels of kernels.hip and sort.hip.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace sptc {
+
+// Bit layout of one operand's linearized key (the reference's SLITOM layout,
+// tensor_manager.tpp:63-186).
+struct Layout {
+  int order = 0;
+  uint64_t dims[SPTC_MAX_ORDER] = {0};
+  int shift[SPTC_MAX_ORDER] = {0};
+  int width[SPTC_MAX_ORDER] = {0};
+  int nc = 0, cmodes[SPTC_MAX_ORDER] = {0};
+  int nf = 0, fmodes[SPTC_MAX_ORDER] = {0};
+  int nfbits = 0, ncbits = 0, nbits = 0;
+  int role = 1;
+};
+
+int mode_bits(uint64_t d);
+sptc_status make_layout(int order, const uint64_t* dims, int nc, const int32_t* cmodes,
+                        const int32_t* cwidth, int role, Layout* out);
+
+// A linearized, sorted operand resident in device memory.
+struct Operand {
+  Layout lay;
+  uint64_t nnz = 0;
+  DevBuf<uint64_t> keys;
+  DevBuf<float> vals;
+};
+
+// One accumulation chunk of the output (a contiguous X-row range).
+struct OutChunk {
+  uint64_t nnz = 0;
+  uint64_t r0 = 0, r1 = 0;  // X-row range [r0, r1) of the chunk
+  DevBuf<uint32_t> row;
+  DevBuf<uint64_t> col;
+  DevBuf<float> val;
+};
+
+}  // namespace sptc
+
+struct sptc_result_s {
+  sptc_result_info info;
+  sptc::Layout xl, yl;
+  sptc::DevBuf<uint64_t> row_key;
+  std::vector<sptc::OutChunk> chunks;
+};
+
+namespace sptc {
+
+struct PhaseTimes {
+  double ms[6] = {0, 0, 0, 0, 0, 0};
+};
+
+// A COO operand's arrays in device memory (caller's, or staged copies).
+struct StagedCOO {
+  uint64_t nnz = 0;
+  int order = 0;
+  int idx_bytes = 4;
+  const void* idx[SPTC_MAX_ORDER] = {nullptr};
+  const float* val = nullptr;
+  std::vector<DevBuf<uint8_t>> own;
+};
+
+class Engine {
+ public:
+  Engine(hipStream_t s, const sptc_opts& o);
+  ~Engine();
+  // Copies host-resident COO arrays to the device (no-op for device ones).
+  void stage(const sptc_coo& t, StagedCOO* out);
+  // Linearize and sort one operand.
+  void prepare(const StagedCOO& t, const Layout& lay, Operand* out);
+  // Match + partition + accumulate.  X and Y are prepared operands.
+  sptc_status contract(const Operand& X, const Operand& Y, sptc_result_s* res);
+  uint64_t budget() const { return budget_; }
+  hipStream_t stream() const { return s_; }
+  void record(int i);
+  PhaseTimes times;
+  std::string msg;  // last error detail
+
+ private:
+  U2 read_total(const U2* dev);
+  hipStream_t s_;
+  sptc_opts o_;
+  uint64_t budget_ = 0;
+  int rb_ = 8;
+  SortWorkspace ws_;
+  hipEvent_t ev_[8];
+  bool ev_used_[8] = {false};
+  U2* host_total_ = nullptr;
+  uint64_t* host_u64_ = nullptr;
+};
+
+}  // namespace sptc

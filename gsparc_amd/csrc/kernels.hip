@@ -1,0 +1,445 @@
+// SpTC phase kernels for gfx950: linearize, index match, ESC expand and
+// reduce.  Host launchers are declared in kernels.hpp.
+#include "kernels.hpp"
+
+namespace sptc {
+
+// --------------------------------------------------------------------------
+// Tile-sum scan (one workgroup, loops over the tile sums with a carry).
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(U2* sums, uint64_t nt, U2* total) {
+  __shared__ U2 scratch[kBlock / kWave];
+  U2 carry = mk2(0, 0);
+  for (uint64_t base = 0; base < nt; base += kScanTile) {
+    const uint64_t first = base + (uint64_t)threadIdx.x * kScanItems;
+    U2 v[kScanItems];
+    U2 s = mk2(0, 0);
+#pragma unroll
+    for (int e = 0; e < kScanItems; ++e) {
+      v[e] = first + e < nt ? sums[first + e] : mk2(0, 0);
+      s = s + v[e];
+    }
+    U2 tot;
+    U2 ex = block_excl_scan_u2(s, scratch, &tot) + carry;
+#pragma unroll
+    for (int e = 0; e < kScanItems; ++e) {
+      if (first + e < nt) sums[first + e] = ex;
+      ex = ex + v[e];
+    }
+    carry = carry + tot;
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+void scan_tiles(U2* sums, uint64_t nt, U2* total, hipStream_t s) {
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, s, sums, nt, total);
+}
+
+// --------------------------------------------------------------------------
+// (a) Linearize: key = OR_m (idx[m] & (2^w[m]-1)) << shift[m].  The fields
+// are contiguous, so the reference's pdep with mask (2^w-1) << s is a masked
+// shift (pdep likewise keeps only the low w bits of an index).  Fused: the
+// digit histograms of every radix pass of the following sort.
+template <typename IdxT, int RB>
+__global__ __launch_bounds__(kBlock) void k_linearize(LinArgs a, uint64_t n,
+                                                      uint64_t* __restrict__ keys,
+                                                      uint64_t* __restrict__ hist,
+                                                      int passes) {
+  constexpr int R = 1 << RB;
+  __shared__ uint32_t sh[kMaxFusedPasses * R];
+  for (int i = threadIdx.x; i < passes * R; i += kBlock) sh[i] = 0;
+  __syncthreads();
+  uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (; i < n; i += stride) {
+    uint64_t k = 0;
+#pragma unroll
+    for (int m = 0; m < SPTC_MAX_ORDER; ++m)
+      if (m < a.order)
+        k |= shl64((uint64_t)static_cast<const IdxT*>(a.idx[m])[i] & low_mask(a.width[m]),
+                   a.shift[m]);
+    keys[i] = k;
+    if (passes) hist_add<RB>(sh, k, passes);
+  }
+  if (passes) {
+    __syncthreads();
+    hist_flush<RB>(sh, hist, passes);
+  }
+}
+
+void launch_linearize(const LinArgs& a, int idx_bytes, uint64_t n, uint64_t* keys,
+                      uint64_t* hist, int passes, int rb, hipStream_t s) {
+  if (!n) return;
+  uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(n, kBlock * 4), 2048);
+  if (passes > kMaxFusedPasses) passes = 0;  // caller falls back to histogram()
+  if (idx_bytes == 4) {
+    if (rb == 8)
+      hipLaunchKernelGGL((k_linearize<uint32_t, 8>), dim3(grid), dim3(kBlock), 0, s, a, n, keys, hist, passes);
+    else
+      hipLaunchKernelGGL((k_linearize<uint32_t, 11>), dim3(grid), dim3(kBlock), 0, s, a, n, keys, hist, passes);
+  } else {
+    if (rb == 8)
+      hipLaunchKernelGGL((k_linearize<uint64_t, 8>), dim3(grid), dim3(kBlock), 0, s, a, n, keys, hist, passes);
+    else
+      hipLaunchKernelGGL((k_linearize<uint64_t, 11>), dim3(grid), dim3(kBlock), 0, s, a, n, keys, hist, passes);
+  }
+}
+
+// --------------------------------------------------------------------------
+// (c) Index match.
+//
+// Y is sorted by key = c << y_nfbits | f, so every contract value c owns one
+// contiguous segment.  With a small contract field (ncbits <= kDirectBits)
+// the segment table is direct-addressed: lo[c], hi[c]; otherwise it is the
+// compacted list of distinct c with their start offsets, searched per X
+// nonzero.
+
+__global__ __launch_bounds__(kBlock) void k_ytable_direct(const uint64_t* __restrict__ ykeys,
+                                                          uint64_t ny, int ynf, uint64_t K,
+                                                          uint64_t* __restrict__ lo,
+                                                          uint64_t* __restrict__ hi) {
+  uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (; i < ny; i += stride) {
+    uint64_t c = shr64(ykeys[i], ynf);
+    if (c >= K) continue;  // key outside the contract field: matches nothing
+    if (i == 0 || shr64(ykeys[i - 1], ynf) != c) lo[c] = i;
+    if (i == ny - 1 || shr64(ykeys[i + 1], ynf) != c) hi[c] = i + 1;
+  }
+}
+
+void launch_ytable_direct(const uint64_t* ykeys, uint64_t ny, int ynf, uint64_t K,
+                          uint64_t* lo, uint64_t* hi, hipStream_t s) {
+  if (!ny) return;
+  uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(ny, kBlock), 4096);
+  hipLaunchKernelGGL(k_ytable_direct, dim3(grid), dim3(kBlock), 0, s, ykeys, ny, ynf, K, lo,
+                     hi);
+}
+
+struct YUniqueOp {
+  const uint64_t* ykeys;
+  uint64_t ny;
+  int ynf;
+  uint64_t* ukey;
+  uint64_t* ustart;
+  __device__ U2 value(uint64_t i) const {
+    return mk2(i == 0 || shr64(ykeys[i - 1], ynf) != shr64(ykeys[i], ynf), 0);
+  }
+  __device__ void emit(uint64_t i, U2 excl, U2 v) const {
+    if (v.a) {
+      ukey[excl.a] = shr64(ykeys[i], ynf);
+      ustart[excl.a] = i;
+    }
+    if (i == ny - 1) ustart[excl.a + v.a] = ny;
+  }
+};
+
+void run_yunique(const uint64_t* ykeys, uint64_t ny, int ynf, uint64_t* ukey,
+                 uint64_t* ustart, U2* tmp, hipStream_t s) {
+  YUniqueOp op{ykeys, ny, ynf, ukey, ustart};
+  run_scan(op, ny, tmp, s);
+}
+
+__device__ __forceinline__ void ytable_lookup(const YTable& t, uint64_t c,
+                                              uint64_t& pos, uint64_t& cnt) {
+  if (t.direct) {
+    uint64_t l = t.lo[c], h = t.hi[c];
+    pos = l;
+    cnt = h - l;
+  } else {
+    uint64_t lo = 0, hi = t.U;
+    while (lo < hi) {
+      uint64_t mid = (lo + hi) >> 1;
+      if (t.ukey[mid] < c) lo = mid + 1; else hi = mid;
+    }
+    if (lo < t.U && t.ukey[lo] == c) {
+      pos = t.ustart[lo];
+      cnt = t.ustart[lo + 1] - pos;
+    } else {
+      pos = 0;
+      cnt = 0;
+    }
+  }
+}
+
+// value: a = match count; b = (active << 32) | row_head
+struct MatchOp {
+  const uint64_t* xkeys;
+  const float* xvals;
+  uint64_t nx;
+  uint64_t cmask;
+  int ncbits;
+  YTable t;
+  MatchOut o;
+  int api;  // 1: write mpos/mcnt/prefix (per-phase entry point)
+
+  __device__ U2 value(uint64_t i) const {
+    const uint64_t k = xkeys[i];
+    uint64_t pos, cnt;
+    ytable_lookup(t, k & cmask, pos, cnt);
+    const uint64_t head = (i == 0) || (shr64(xkeys[i - 1], ncbits) != shr64(k, ncbits));
+    return mk2(cnt, ((uint64_t)(cnt != 0) << 32) | head);
+  }
+  __device__ void emit(uint64_t i, U2 excl, U2 v) const {
+    const uint64_t k = xkeys[i];
+    uint64_t pos, cnt;
+    ytable_lookup(t, k & cmask, pos, cnt);
+    if (api) {
+      o.mpos[i] = cnt ? pos : 0;
+      o.mcnt[i] = cnt;
+      o.prefix[i] = excl.a;
+      if (i == nx - 1) o.prefix[nx] = excl.a + v.a;
+      return;
+    }
+    const uint64_t head = v.b & 1;
+    const uint64_t row = (excl.b & 0xFFFFFFFFull) + head - 1;
+    if (head) {
+      o.row_key[row] = shr64(k, ncbits);
+      o.row_prod[row] = excl.a;
+      o.row_act[row] = excl.b >> 32;
+    }
+    if (cnt) {
+      const uint64_t j = excl.b >> 32;
+      o.act_prefix[j] = excl.a;
+      o.act_ypos[j] = pos;
+      o.act_xval[j] = xvals[i];
+      o.act_row[j] = (uint32_t)row;
+    }
+  }
+};
+
+void run_match(const uint64_t* xkeys, const float* xvals, uint64_t nx, int ncbits,
+               const YTable& t, const MatchOut& o, int api, U2* tmp, hipStream_t s) {
+  MatchOp op{xkeys, xvals, nx, low_mask(ncbits), ncbits, t, o, api};
+  run_scan(op, nx, tmp, s);
+}
+
+struct ExclScanOp {
+  const uint64_t* in;
+  uint64_t* out;
+  uint64_t n;
+  __device__ U2 value(uint64_t i) const { return mk2(in[i], 0); }
+  __device__ void emit(uint64_t i, U2 excl, U2 v) const {
+    out[i] = excl.a;
+    if (i == n - 1) out[n] = excl.a + v.a;
+  }
+};
+
+void run_exclusive_scan(const uint64_t* in, uint64_t* out, uint64_t n, U2* tmp,
+                        hipStream_t s) {
+  ExclScanOp op{in, out, n};
+  run_scan(op, n, tmp, s);
+}
+
+// --------------------------------------------------------------------------
+// Partition: end row of the next chunk.  Largest r1 in (r0, min(R, r0+rmax)]
+// with row_prod[r1] - row_prod[r0] <= cap; r1 = r0 when even one row does
+// not fit.  out = {r1, row_prod[r1], row_act[r1]}.
+__global__ void k_chunk_end(const uint64_t* row_prod, const uint64_t* row_act,
+                            uint64_t R, uint64_t r0, uint64_t rmax, uint64_t cap,
+                            uint64_t* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint64_t p0 = row_prod[r0];
+  uint64_t hi_r = r0 + rmax < R ? r0 + rmax : R;
+  uint64_t lo = r0, hi = hi_r;  // answer in [r0, hi_r]
+  while (lo < hi) {
+    uint64_t mid = lo + (hi - lo + 1) / 2;
+    if (row_prod[mid] - p0 <= cap) lo = mid; else hi = mid - 1;
+  }
+  out[0] = lo;
+  out[1] = row_prod[lo];
+  out[2] = row_act[lo];
+}
+
+void launch_chunk_end(const uint64_t* row_prod, const uint64_t* row_act, uint64_t R,
+                      uint64_t r0, uint64_t rmax, uint64_t cap, uint64_t* out,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_chunk_end, dim3(1), dim3(64), 0, s, row_prod, row_act, R, r0,
+                     rmax, cap, out);
+}
+
+// Largest product count of any row: max over r of row_prod[r+1]-row_prod[r].
+__global__ __launch_bounds__(kBlock) void k_max_row(const uint64_t* row_prod, uint64_t R,
+                                                    unsigned long long* out) {
+  uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  uint64_t m = 0;
+  for (; r < R; r += stride) {
+    uint64_t d = row_prod[r + 1] - row_prod[r];
+    m = d > m ? d : m;
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    uint64_t o = __shfl_xor(m, off, kWave);
+    m = o > m ? o : m;
+  }
+  if (lane_id() == 0 && m) atomicMax(out, (unsigned long long)m);
+}
+
+void launch_max_row(const uint64_t* row_prod, uint64_t R, uint64_t* out, hipStream_t s) {
+  fill_u64(out, 0, 1, s);
+  if (!R) return;
+  uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(R, kBlock), 1024);
+  hipLaunchKernelGGL(k_max_row, dim3(grid), dim3(kBlock), 0, s, row_prod, R,
+                     (unsigned long long*)out);
+}
+
+// --------------------------------------------------------------------------
+// (d) ESC expand with load-balanced search.  The chunk's products
+// [P0, P1) are cut into tiles of kExpandTile; each active X nonzero owns at
+// least one product, so a tile spans at most kExpandTile+1 of them.  One
+// global binary search per tile finds the first; their product offsets are
+// staged in LDS and each product finds its owner with an LDS search.
+// Product key = (row - r0) << ynf | (ykey & fmask); value = vx * vy (FP32,
+// contraction.cuh:132-136).  Fused: digit histograms for the product sort
+// and an FP64 running sum of products.
+constexpr int kExpandItems = 8;
+constexpr uint32_t kExpandTile = kBlock * kExpandItems;
+
+template <int RB>
+__global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs a) {
+  constexpr int R = 1 << RB;
+  __shared__ uint64_t s_pref[kExpandTile + 1];
+  __shared__ uint32_t sh[kMaxFusedPasses * R];
+  __shared__ uint64_t s_ja;
+  __shared__ double s_sum[kBlock / kWave];
+  for (int i = threadIdx.x; i < a.passes * R; i += kBlock) sh[i] = 0;
+  double vsum = 0.0;
+  const uint64_t nprod = a.P1 - a.P0;
+  const uint64_t ntiles = ceil_div(nprod, kExpandTile);
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint64_t pb = a.P0 + tile * kExpandTile;
+    const uint64_t pe = pb + kExpandTile < a.P1 ? pb + kExpandTile : a.P1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      // last j in [A0, A1) with act_prefix[j] <= pb
+      uint64_t lo = a.A0, hi = a.A1;
+      while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (a.act_prefix[mid] <= pb) lo = mid + 1; else hi = mid;
+      }
+      s_ja = lo - 1;
+    }
+    __syncthreads();
+    const uint64_t ja = s_ja;
+    const uint64_t avail = a.A1 - ja;
+    const uint32_t m = (uint32_t)(avail < kExpandTile + 1 ? avail : kExpandTile + 1);
+    for (uint32_t k = threadIdx.x; k < m; k += kBlock) s_pref[k] = a.act_prefix[ja + k];
+    __syncthreads();
+#pragma unroll 2
+    for (int e = 0; e < kExpandItems; ++e) {
+      const uint64_t p = pb + (uint64_t)e * kBlock + threadIdx.x;
+      if (p >= pe) break;
+      uint32_t lo = 0, hi = m;  // last k with s_pref[k] <= p
+      while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (s_pref[mid] <= p) lo = mid + 1; else hi = mid;
+      }
+      const uint32_t k = lo - 1;
+      const uint64_t j = ja + k;
+      const uint64_t y = a.act_ypos[j] + (p - s_pref[k]);
+      const uint64_t yk = a.ykeys[y];
+      const float v = a.act_xval[j] * a.yvals[y];
+      const uint64_t key = shl64((uint64_t)(a.act_row[j] - a.r0), a.ynf) | (yk & a.fmask);
+      a.out_keys[p - a.P0] = key;
+      a.out_vals[p - a.P0] = v;
+      vsum += (double)v;
+      if (a.passes) hist_add<RB>(sh, key, a.passes);
+    }
+  }
+  __syncthreads();
+  if (a.passes) hist_flush<RB>(sh, a.hist, a.passes);
+  for (int off = kWave / 2; off > 0; off >>= 1) vsum += __shfl_xor(vsum, off, kWave);
+  if (lane_id() == 0) s_sum[threadIdx.x / kWave] = vsum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) t += s_sum[w];
+    atomicAdd(a.vsum, t);
+  }
+}
+
+void launch_expand(const ExpandArgs& a, int rb, hipStream_t s) {
+  const uint64_t nprod = a.P1 - a.P0;
+  if (!nprod) return;
+  uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(nprod, kExpandTile), 2048);
+  if (rb == 8)
+    hipLaunchKernelGGL(k_expand<8>, dim3(grid), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_expand<11>, dim3(grid), dim3(kBlock), 0, s, a);
+}
+
+// --------------------------------------------------------------------------
+// (d) Reduce by key over sorted products: one output per run of equal keys;
+// the run's values are summed in sorted (stable) order with compensated FP32
+// arithmetic (cub::DeviceReduce::ReduceByKey(Sum), contraction.cuh:512-537).
+struct ReduceOp {
+  const uint64_t* keys;
+  const float* vals;
+  uint64_t n;
+  uint32_t* out_row;
+  uint64_t* out_col;
+  float* out_val;
+  uint64_t r0;
+  int ynf;
+  uint64_t fmask;
+  __device__ U2 value(uint64_t i) const {
+    return mk2(i == 0 || keys[i] != keys[i - 1], 0);
+  }
+  __device__ void emit(uint64_t i, U2 excl, U2 v) const {
+    if (!v.a) return;
+    const uint64_t k = keys[i];
+    KahanF32 acc;
+    acc.add(vals[i]);
+    for (uint64_t j = i + 1; j < n && keys[j] == k; ++j) acc.add(vals[j]);
+    const uint64_t o = excl.a;
+    out_row[o] = (uint32_t)(r0 + shr64(k, ynf));
+    out_col[o] = k & fmask;
+    out_val[o] = acc.value();
+  }
+};
+
+void run_reduce_count(const uint64_t* keys, uint64_t n, U2* tmp, hipStream_t s) {
+  ReduceOp op{keys, nullptr, n, nullptr, nullptr, nullptr, 0, 0, 0};
+  run_scan(op, n, tmp, s, /*emit=*/false);
+}
+
+void run_reduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32_t* out_row,
+                     uint64_t* out_col, float* out_val, uint64_t r0, int ynf,
+                     uint64_t fmask, const U2* tmp, hipStream_t s) {
+  if (!n) return;
+  ReduceOp op{keys, vals, n, out_row, out_col, out_val, r0, ynf, fmask};
+  hipLaunchKernelGGL(k_scan_emit<ReduceOp>, dim3((uint32_t)scan_tmp_count(n)), dim3(kBlock),
+                     0, s, op, n, tmp);
+}
+
+// --------------------------------------------------------------------------
+// Result decode helpers.
+__global__ __launch_bounds__(kBlock) void k_pack_keys(const uint32_t* row, const uint64_t* col,
+                                                      const uint64_t* row_key, uint64_t n,
+                                                      int ynf, uint64_t* out) {
+  uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (; i < n; i += stride) out[i] = shl64(row_key[row[i]], ynf) | col[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_rowkey(const uint32_t* row,
+                                                          const uint64_t* row_key,
+                                                          uint64_t n, uint64_t* out) {
+  uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (; i < n; i += stride) out[i] = row_key[row[i]];
+}
+
+void launch_pack_keys(const uint32_t* row, const uint64_t* col, const uint64_t* row_key,
+                      uint64_t n, int ynf, uint64_t* out, hipStream_t s) {
+  if (!n) return;
+  uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(n, kBlock), 4096);
+  hipLaunchKernelGGL(k_pack_keys, dim3(grid), dim3(kBlock), 0, s, row, col, row_key, n, ynf, out);
+}
+void launch_gather_rowkey(const uint32_t* row, const uint64_t* row_key, uint64_t n,
+                          uint64_t* out, hipStream_t s) {
+  if (!n) return;
+  uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(n, kBlock), 4096);
+  hipLaunchKernelGGL(k_gather_rowkey, dim3(grid), dim3(kBlock), 0, s, row, row_key, n, out);
+}
+
+}  // namespace sptc

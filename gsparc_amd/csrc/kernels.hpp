@@ -1,0 +1,85 @@
+// Host launchers and argument structs of the SpTC phase kernels.
+#pragma once
+#include "common.hpp"
+#include "scan.hpp"
+#include "sort.hpp"
+
+namespace sptc {
+
+constexpr int kMaxFusedPasses = 8;  // producers fuse histograms up to this many passes
+constexpr int kDirectBits = 22;     // direct-addressed Y segment table up to 2^22 keys
+
+struct LinArgs {
+  const void* idx[SPTC_MAX_ORDER];
+  int shift[SPTC_MAX_ORDER];
+  int width[SPTC_MAX_ORDER];
+  int order;
+};
+
+struct YTable {
+  int direct;
+  const uint64_t* lo;      // direct: [2^ncbits]
+  const uint64_t* hi;
+  const uint64_t* ukey;    // search: [U] distinct contract values
+  const uint64_t* ustart;  // [U+1]
+  uint64_t U;
+};
+
+struct MatchOut {
+  // engine form (compacted active X nonzeros and X rows)
+  uint64_t* act_prefix;
+  uint64_t* act_ypos;
+  float* act_xval;
+  uint32_t* act_row;
+  uint64_t* row_key;
+  uint64_t* row_prod;
+  uint64_t* row_act;
+  // per-phase API form
+  uint64_t* mpos;
+  uint64_t* mcnt;
+  uint64_t* prefix;
+};
+
+struct ExpandArgs {
+  const uint64_t* act_prefix;
+  const uint64_t* act_ypos;
+  const float* act_xval;
+  const uint32_t* act_row;
+  uint64_t A0, A1, P0, P1;
+  uint32_t r0;
+  const uint64_t* ykeys;
+  const float* yvals;
+  int ynf;
+  uint64_t fmask;
+  uint64_t* out_keys;
+  float* out_vals;
+  uint64_t* hist;
+  int passes;
+  double* vsum;
+};
+
+void launch_linearize(const LinArgs& a, int idx_bytes, uint64_t n, uint64_t* keys,
+                      uint64_t* hist, int passes, int rb, hipStream_t s);
+void launch_ytable_direct(const uint64_t* ykeys, uint64_t ny, int ynf, uint64_t K,
+                          uint64_t* lo, uint64_t* hi, hipStream_t s);
+void run_yunique(const uint64_t* ykeys, uint64_t ny, int ynf, uint64_t* ukey,
+                 uint64_t* ustart, U2* tmp, hipStream_t s);
+void run_match(const uint64_t* xkeys, const float* xvals, uint64_t nx, int ncbits,
+               const YTable& t, const MatchOut& o, int api, U2* tmp, hipStream_t s);
+void run_exclusive_scan(const uint64_t* in, uint64_t* out, uint64_t n, U2* tmp,
+                        hipStream_t s);
+void launch_chunk_end(const uint64_t* row_prod, const uint64_t* row_act, uint64_t R,
+                      uint64_t r0, uint64_t rmax, uint64_t cap, uint64_t* out,
+                      hipStream_t s);
+void launch_max_row(const uint64_t* row_prod, uint64_t R, uint64_t* out, hipStream_t s);
+void launch_expand(const ExpandArgs& a, int rb, hipStream_t s);
+void run_reduce_count(const uint64_t* keys, uint64_t n, U2* tmp, hipStream_t s);
+void run_reduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32_t* out_row,
+                     uint64_t* out_col, float* out_val, uint64_t r0, int ynf,
+                     uint64_t fmask, const U2* tmp, hipStream_t s);
+void launch_pack_keys(const uint32_t* row, const uint64_t* col, const uint64_t* row_key,
+                      uint64_t n, int ynf, uint64_t* out, hipStream_t s);
+void launch_gather_rowkey(const uint32_t* row, const uint64_t* row_key, uint64_t n,
+                          uint64_t* out, hipStream_t s);
+
+}  // namespace sptc

@@ -1,0 +1,122 @@
+// Generic device-wide scan with a fused producer and consumer.
+//
+// An Op supplies, for item i of [0, n):
+//   __device__ V value(uint64_t i) const;            // the item's contribution
+//   __device__ void emit(uint64_t i, V excl, V val);   // consumer (optional)
+// where V is U2 (two uint64 lanes, added lane-wise).  Three kernels:
+//   k_scan_reduce : per tile of kBlock*kScanItems items, the tile sum
+//   k_scan_tiles  : exclusive scan of tile sums (single workgroup), total
+//   k_scan_emit   : recompute values, block scan + tile offset, emit()
+// Items are handled in a blocked arrangement: thread t of a tile owns items
+// [t*kScanItems, t*kScanItems + kScanItems).
+#pragma once
+#include "common.hpp"
+
+namespace sptc {
+
+constexpr int kScanItems = 8;
+constexpr uint64_t kScanTile = (uint64_t)kBlock * kScanItems;
+
+struct U2 {
+  uint64_t a, b;
+};
+__host__ __device__ __forceinline__ U2 mk2(uint64_t a, uint64_t b) {
+  U2 r;
+  r.a = a;
+  r.b = b;
+  return r;
+}
+__host__ __device__ __forceinline__ U2 operator+(const U2& x, const U2& y) {
+  return mk2(x.a + y.a, x.b + y.b);
+}
+__host__ __device__ __forceinline__ U2 operator-(const U2& x, const U2& y) {
+  return mk2(x.a - y.a, x.b - y.b);
+}
+
+__device__ __forceinline__ U2 block_excl_scan_u2(U2 v, U2* scratch, U2* total) {
+  const uint32_t lane = lane_id();
+  const uint32_t wave = threadIdx.x / kWave;
+  U2 inc = v;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    uint64_t oa = __shfl_up(inc.a, d, kWave);
+    uint64_t ob = __shfl_up(inc.b, d, kWave);
+    if (lane >= (uint32_t)d) { inc.a += oa; inc.b += ob; }
+  }
+  if (lane == kWave - 1) scratch[wave] = inc;
+  __syncthreads();
+  U2 off = mk2(0, 0), tot = mk2(0, 0);
+#pragma unroll
+  for (int w = 0; w < kBlock / kWave; ++w) {
+    U2 s = scratch[w];
+    if ((uint32_t)w < wave) off = off + s;
+    tot = tot + s;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - v;
+}
+
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(Op op, uint64_t n, U2* tile_sums) {
+  __shared__ U2 scratch[kBlock / kWave];
+  const uint64_t tile = blockIdx.x;
+  const uint64_t first = tile * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+  U2 s = mk2(0, 0);
+#pragma unroll
+  for (int e = 0; e < kScanItems; ++e) {
+    uint64_t i = first + e;
+    if (i < n) s = s + op.value(i);
+  }
+  U2 tot;
+  block_excl_scan_u2(s, scratch, &tot);
+  if (threadIdx.x == 0) tile_sums[tile] = tot;
+}
+
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_scan_emit(Op op, uint64_t n, const U2* tile_offs) {
+  __shared__ U2 scratch[kBlock / kWave];
+  const uint64_t tile = blockIdx.x;
+  const uint64_t first = tile * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+  U2 vals[kScanItems];
+  U2 s = mk2(0, 0);
+#pragma unroll
+  for (int e = 0; e < kScanItems; ++e) {
+    uint64_t i = first + e;
+    vals[e] = i < n ? op.value(i) : mk2(0, 0);
+    s = s + vals[e];
+  }
+  U2 tot;
+  U2 excl = block_excl_scan_u2(s, scratch, &tot) + tile_offs[tile];
+#pragma unroll
+  for (int e = 0; e < kScanItems; ++e) {
+    uint64_t i = first + e;
+    if (i < n) op.emit(i, excl, vals[e]);
+    excl = excl + vals[e];
+  }
+}
+
+// In-place exclusive scan of nt tile sums by one workgroup; total -> *total
+// (defined in kernels.hip).
+void scan_tiles(U2* sums, uint64_t nt, U2* total, hipStream_t s);
+
+// Runs reduce -> tile scan -> emit.  `tmp` must hold scan_tmp_count(n)+1 U2
+// entries; the grand total lands in tmp[scan_tmp_count(n)].
+inline uint64_t scan_tmp_count(uint64_t n) { return ceil_div(n ? n : 1, kScanTile); }
+
+template <class Op>
+void run_scan(const Op& op, uint64_t n, U2* tmp, hipStream_t s, bool emit = true) {
+  const uint64_t nt = scan_tmp_count(n);
+  U2* total = tmp + nt;
+  if (n == 0) {
+    scan_tiles(tmp, 0, total, s);
+    return;
+  }
+  hipLaunchKernelGGL(k_scan_reduce<Op>, dim3((uint32_t)nt), dim3(kBlock), 0, s, op, n, tmp);
+  scan_tiles(tmp, nt, total, s);
+  if (emit)
+    hipLaunchKernelGGL(k_scan_emit<Op>, dim3((uint32_t)nt), dim3(kBlock), 0, s, op, n,
+                       (const U2*)tmp);
+}
+
+}  // namespace sptc

@@ -1,0 +1,350 @@
+// Stable LSD radix sort of (uint64 key, float value) pairs for gfx950.
+//
+// Replaces cub::DeviceRadixSort::SortPairs in the reference (sort.cuh:90-108,
+// contraction.cuh:506-530, tensor_manager.tpp:779-786).  Design:
+//   * Only the significant key bits are sorted: ceil(nbits / RB) passes.
+//   * One up-front pass computes every digit histogram at once; producers
+//     (linearize, ESC expand) fuse it, so the sort never re-reads keys for
+//     counting.  digit_scan turns them into per-pass digit bases.
+//   * Each pass is a single kernel ("onesweep"): a workgroup takes the next
+//     tile id from an atomic counter, ranks its 256*IPT keys with wave64
+//     ballot matching (one ballot per digit bit, stable within the wave),
+//     publishes per-digit tile counts, resolves its global offsets by
+//     decoupled look-back over predecessor tiles, stages the tile in LDS in
+//     digit order and writes it out in coalesced digit runs.
+//   * Look-back status words carry a pass epoch so the status array is never
+//     cleared between passes: [63:62] flag (1 aggregate, 2 inclusive prefix),
+//     [61:48] epoch, [47:0] count.
+#include "common.hpp"
+#include "sort.hpp"
+
+namespace sptc {
+
+namespace {
+
+constexpr uint64_t kFlagAgg = 1ull << 62;
+constexpr uint64_t kFlagPre = 2ull << 62;
+constexpr uint64_t kValMask = (1ull << 48) - 1;
+constexpr uint32_t kSpinLimit = 1u << 28;
+
+__device__ __forceinline__ uint64_t pack_status(uint64_t flag, uint32_t epoch,
+                                                uint64_t v) {
+  return flag | ((uint64_t)epoch << 48) | (v & kValMask);
+}
+
+__global__ void k_fill_u64(uint64_t* p, uint64_t v, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+// Standalone histogram of all passes (used when no producer fused it).
+template <int RB>
+__global__ __launch_bounds__(kBlock) void k_histogram(const uint64_t* __restrict__ keys,
+                                                      uint64_t n, int passes,
+                                                      uint64_t* __restrict__ hist) {
+  constexpr int R = 1 << RB;
+  __shared__ uint32_t sh[kMaxPasses * R];
+  for (int i = threadIdx.x; i < passes * R; i += kBlock) sh[i] = 0;
+  __syncthreads();
+  uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (; i < n; i += stride) {
+    uint64_t k = keys[i];
+    for (int p = 0; p < passes; ++p)
+      atomicAdd(&sh[p * R + ((k >> (p * RB)) & (R - 1))], 1u);
+  }
+  __syncthreads();
+  for (int i2 = threadIdx.x; i2 < passes * R; i2 += kBlock)
+    if (sh[i2]) atomicAdd((unsigned long long*)&hist[i2], (unsigned long long)sh[i2]);
+}
+
+// hist [passes][R] counts -> base [passes][R] exclusive starts.  One block per
+// pass.
+template <int RB>
+__global__ __launch_bounds__(kBlock) void k_digit_scan(const uint64_t* __restrict__ hist,
+                                                       uint64_t* __restrict__ base) {
+  constexpr int R = 1 << RB;
+  constexpr int PER = (R + kBlock - 1) / kBlock;
+  __shared__ uint64_t scratch[kBlock / kWave];
+  const int p = blockIdx.x;
+  uint64_t local[PER];
+  uint64_t sum = 0;
+  // thread t owns digits [t*PER, t*PER+PER)
+  for (int j = 0; j < PER; ++j) {
+    int d = threadIdx.x * PER + j;
+    local[j] = d < R ? hist[p * R + d] : 0;
+    sum += local[j];
+  }
+  uint64_t tot;
+  uint64_t ex = block_excl_scan<uint64_t>(sum, scratch, &tot);
+  for (int j = 0; j < PER; ++j) {
+    int d = threadIdx.x * PER + j;
+    if (d < R) base[p * R + d] = ex;
+    ex += local[j];
+  }
+}
+
+template <int RB, int IPT>
+__global__ __launch_bounds__(kBlock) void k_onesweep(
+    const uint64_t* __restrict__ kin, const float* __restrict__ vin,
+    uint64_t* __restrict__ kout, float* __restrict__ vout, uint64_t n, int shift,
+    const uint64_t* __restrict__ digit_base, uint64_t* status,
+    uint32_t* tile_counter, uint32_t epoch, uint32_t* err) {
+  constexpr int R = 1 << RB;
+  constexpr int TILE = kBlock * IPT;
+  constexpr int NW = kBlock / kWave;
+  __shared__ uint32_t s_cnt[NW][R];     // per-wave running counts, then prefixes
+  __shared__ uint64_t s_gbase[R];       // global start of this tile's digit run
+  __shared__ uint32_t s_dstart[R];      // tile-local start of each digit run
+  __shared__ uint64_t s_keys[TILE];
+  __shared__ float s_vals[TILE];
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_scan[NW];
+
+  const uint32_t tid = threadIdx.x;
+  const uint32_t wave = tid / kWave;
+  const uint32_t lane = lane_id();
+  if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+  for (int i = tid; i < NW * R; i += kBlock) (&s_cnt[0][0])[i] = 0;
+  __syncthreads();
+  const uint64_t tile = s_tile;
+  const uint64_t tile_begin = tile * TILE;
+  const uint64_t wbase = tile_begin + (uint64_t)wave * kWave * IPT;
+
+  uint64_t k[IPT];
+  float v[IPT];
+  uint32_t rank[IPT];
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+    bool valid = idx < n;
+    k[i] = valid ? kin[idx] : 0;
+    v[i] = valid ? vin[idx] : 0.f;
+  }
+  // Rank: rounds in order, lanes in order => stable within the wave's run.
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+    bool valid = idx < n;
+    uint32_t d = (uint32_t)(k[i] >> shift) & (R - 1);
+    uint64_t peers = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+      bool bit = (d >> b) & 1u;
+      uint64_t bb = ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    uint32_t before = 0;
+    if (valid) before = s_cnt[wave][d];
+    __builtin_amdgcn_wave_barrier();
+    if (valid) {
+      rank[i] = before + popc_below(peers);
+      // highest lane of the peer group advances the counter
+      bool last = (peers >> lane) == 1ull;
+      if (last) s_cnt[wave][d] = before + (uint32_t)__popcll(peers);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+
+  // Per digit: wave prefixes, tile total, publish + look-back.
+  for (int d = tid; d < R; d += kBlock) {
+    uint32_t acc = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      uint32_t c = s_cnt[w][d];
+      s_cnt[w][d] = acc;
+      acc += c;
+    }
+    const uint64_t total = acc;
+    uint64_t excl = 0;
+    uint64_t* my = status + tile * R + d;
+    if (tile == 0) {
+      st_agent(my, pack_status(kFlagPre, epoch, total));
+    } else {
+      st_agent(my, pack_status(kFlagAgg, epoch, total));
+      int64_t j = (int64_t)tile - 1;
+      uint32_t spins = 0;
+      while (j >= 0) {
+        uint64_t s = ld_agent(status + (uint64_t)j * R + d);
+        uint32_t se = (uint32_t)((s >> 48) & 0x3FFF);
+        uint64_t fl = s & (3ull << 62);
+        if (se != epoch || fl == 0) {
+          if (++spins > kSpinLimit) { atomicOr(err, 1u); break; }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        excl += s & kValMask;
+        if (fl == kFlagPre) break;
+        --j;
+      }
+      st_agent(my, pack_status(kFlagPre, epoch, excl + total));
+    }
+    s_gbase[d] = digit_base[d] + excl;
+    s_dstart[d] = acc;  // temporarily the tile total
+  }
+  __syncthreads();
+  // Exclusive scan of tile totals over digits -> s_dstart.
+  {
+    constexpr int PER = (R + kBlock - 1) / kBlock;
+    uint32_t loc[PER];
+    uint32_t sum = 0;
+    for (int j = 0; j < PER; ++j) {
+      int d = tid * PER + j;
+      loc[j] = d < R ? s_dstart[d] : 0;
+      sum += loc[j];
+    }
+    uint32_t tot;
+    uint32_t ex = block_excl_scan<uint32_t>(sum, s_scan, &tot);
+    for (int j = 0; j < PER; ++j) {
+      int d = tid * PER + j;
+      if (d < R) s_dstart[d] = ex;
+      ex += loc[j];
+    }
+  }
+  __syncthreads();
+  // Stage in LDS in digit order.
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+    if (idx < n) {
+      uint32_t d = (uint32_t)(k[i] >> shift) & (R - 1);
+      uint32_t pos = s_dstart[d] + s_cnt[wave][d] + rank[i];
+      s_keys[pos] = k[i];
+      s_vals[pos] = v[i];
+    }
+  }
+  __syncthreads();
+  const uint32_t cnt = (uint32_t)((n - tile_begin) < (uint64_t)TILE ? (n - tile_begin) : TILE);
+  for (uint32_t j = tid; j < cnt; j += kBlock) {
+    uint64_t key = s_keys[j];
+    uint32_t d = (uint32_t)(key >> shift) & (R - 1);
+    uint64_t dst = s_gbase[d] + (j - s_dstart[d]);
+    kout[dst] = key;
+    vout[dst] = s_vals[j];
+  }
+}
+
+template <int RB>
+constexpr int ipt_for() { return RB <= 8 ? 16 : 12; }
+
+}  // namespace
+
+void fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(n, kBlock), 4096);
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid), dim3(kBlock), 0, s, p, v, n);
+}
+void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(n, kBlock), 4096);
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(kBlock), 0, s, p, v, n);
+}
+
+int radix_passes(int nbits, int rb) { return nbits <= 0 ? 0 : (nbits + rb - 1) / rb; }
+
+uint64_t sort_tile_items(int rb) {
+  switch (rb) {
+    case 8: return (uint64_t)kBlock * ipt_for<8>();
+    case 11: return (uint64_t)kBlock * ipt_for<11>();
+    default: return (uint64_t)kBlock * ipt_for<8>();
+  }
+}
+
+uint64_t SortWorkspace::bytes_per_key(int rb) {
+  return sizeof(uint64_t) + sizeof(float) + (8ull << rb) / sort_tile_items(rb) + 1;
+}
+
+void SortWorkspace::ensure(uint64_t n, int rb_, hipStream_t s) {
+  const uint64_t R = 1ull << rb_;
+  if (counters.size() == 0) {
+    counters.reset(64);
+    fill_u32(counters.get(), 0, 64, s);
+  }
+  if (hist.size() < kMaxPasses * R) {
+    hist.reset(kMaxPasses * R);
+    base.reset(kMaxPasses * R);
+  }
+  if (alt_keys.size() < n || alt_vals.size() < n) {
+    alt_keys.reset(n ? n : 1);
+    alt_vals.reset(n ? n : 1);
+  }
+  const uint64_t tiles = ceil_div(n ? n : 1, sort_tile_items(rb_));
+  if (status.size() < tiles * R) {
+    status.reset(tiles * R);
+    fill_u64(status.get(), 0, tiles * R, s);
+    epoch = 0;
+  }
+  rb = rb_;
+  cap = alt_keys.size();
+  max_tiles = status.size() / R;
+}
+
+void SortWorkspace::prepare(int passes, hipStream_t s) {
+  uint64_t R = 1ull << rb;
+  fill_u64(hist.get(), 0, (uint64_t)passes * R, s);
+  fill_u32(counters.get(), 0, kMaxPasses, s);
+}
+
+void histogram(const SortWorkspace& ws, const uint64_t* keys, uint64_t n,
+               int passes, hipStream_t s) {
+  if (!n || !passes) return;
+  uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(n, kBlock * 8), 2048);
+  if (ws.rb == 8)
+    hipLaunchKernelGGL(k_histogram<8>, dim3(grid), dim3(kBlock), 0, s, keys, n,
+                       passes, ws.hist.get());
+  else
+    hipLaunchKernelGGL(k_histogram<11>, dim3(grid), dim3(kBlock), 0, s, keys, n,
+                       passes, ws.hist.get());
+}
+
+// Sorts keys/vals (n <= ws.cap) on bits [0, nbits) assuming ws.hist already
+// holds the digit histograms of all passes.  Result ends in (keys, vals) when
+// the pass count is even, else in (ws.alt_keys, ws.alt_vals); the returned
+// flag says which (true = alt).
+bool sort_after_histogram(SortWorkspace& ws, uint64_t* keys, float* vals,
+                          uint64_t n, int nbits, hipStream_t s) {
+  const int rb = ws.rb;
+  const int passes = radix_passes(nbits, rb);
+  if (!n || !passes) return false;
+  const uint64_t R = 1ull << rb;
+  if (rb == 8)
+    hipLaunchKernelGGL(k_digit_scan<8>, dim3(passes), dim3(kBlock), 0, s, ws.hist.get(),
+                       ws.base.get());
+  else
+    hipLaunchKernelGGL(k_digit_scan<11>, dim3(passes), dim3(kBlock), 0, s, ws.hist.get(),
+                       ws.base.get());
+  const uint64_t tiles = ceil_div(n, sort_tile_items(rb));
+  uint64_t* ka = keys;
+  float* va = vals;
+  uint64_t* kb = ws.alt_keys.get();
+  float* vb = ws.alt_vals.get();
+  for (int p = 0; p < passes; ++p) {
+    ws.epoch = (ws.epoch + 1) & 0x3FFF;
+    if (ws.epoch == 0) {
+      fill_u64(ws.status.get(), 0, ws.max_tiles * R, s);
+      ws.epoch = 1;
+    }
+    if (rb == 8)
+      hipLaunchKernelGGL((k_onesweep<8, ipt_for<8>()>), dim3((uint32_t)tiles),
+                         dim3(kBlock), 0, s, ka, va, kb, vb, n, p * rb,
+                         ws.base.get() + p * R, ws.status.get(), ws.counters.get() + p,
+                         ws.epoch, ws.err());
+    else
+      hipLaunchKernelGGL((k_onesweep<11, ipt_for<11>()>), dim3((uint32_t)tiles),
+                         dim3(kBlock), 0, s, ka, va, kb, vb, n, p * rb,
+                         ws.base.get() + p * R, ws.status.get(), ws.counters.get() + p,
+                         ws.epoch, ws.err());
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  return (passes & 1) != 0;
+}
+
+}  // namespace sptc

@@ -1,0 +1,260 @@
+"""Python mirror of the reference SpTC entry point over libsptc_hip.so.
+
+``SpTC(X, Y, cmodes_x, cmodes_y, dense_accumulator=0)`` mirrors
+``gsparc::SpTC`` (include/gsparc/sptc.cuh:16-26) together with the
+TensorManager preprocessing that precedes it in main.cu:293-321: it takes COO
+tensors and contracted-mode lists, and returns the contraction's COO output
+(which the reference computes and then discards; only its nnz is printed,
+main.cu:428).  Errors raise :class:`SpTCError` carrying the library status
+instead of exiting the process (the reference's exit(1) paths,
+cuda_memory.cpp:75-79, sptc.cuh:206-211).
+
+Everything runs on the GPU through the C ABI of include/sptc.h; there is no
+CPU fallback.
+"""
+import ctypes
+from ctypes import (POINTER, Structure, byref, c_char_p, c_double, c_int32, c_uint32,
+                    c_uint64, c_void_p)
+
+import numpy as np
+
+from ._lib import hip_lib
+
+OK, ERR_BAD_ARGS, ERR_OOM_BUDGET, ERR_KEY_OVERFLOW, ERR_HIP, ERR_NO_DEVICE, ERR_INTERNAL = range(7)
+MAX_ORDER = 8
+
+
+class SpTCError(RuntimeError):
+    def __init__(self, status, detail=""):
+        self.status = status
+        super().__init__(f"sptc status {status}: {detail}")
+
+
+class _COO(Structure):
+    _fields_ = [("order", c_int32), ("nnz", c_uint64), ("dims", POINTER(c_uint64)),
+                ("idx", POINTER(c_void_p)), ("idx_bytes", c_int32),
+                ("val", c_void_p), ("on_device", c_int32)]
+
+
+class _Opts(Structure):
+    _fields_ = [("dense_accumulator", c_int32), ("count_only", c_int32),
+                ("mem_budget_bytes", c_uint64), ("radix_bits", c_int32),
+                ("verbose", c_int32)]
+
+
+class _Info(Structure):
+    _fields_ = [("nnz", c_uint64), ("ir_nnz", c_uint64), ("nrows", c_uint64),
+                ("x_nfbits", c_int32), ("y_nfbits", c_int32), ("ncbits", c_int32),
+                ("out_bits", c_int32), ("out_order", c_int32), ("nchunks", c_int32),
+                ("accumulator", c_int32), ("value_sum", c_double),
+                ("row", c_void_p), ("col", c_void_p), ("val", c_void_p),
+                ("row_key", c_void_p), ("phase_ms", c_double * 6)]
+
+
+_bound = False
+
+
+def lib():
+    global _bound
+    L = hip_lib()
+    if not _bound:
+        L.sptc_contract.argtypes = [POINTER(_COO), POINTER(_COO), c_int32, POINTER(c_int32),
+                                    POINTER(c_int32), POINTER(_Opts), c_void_p,
+                                    POINTER(c_void_p)]
+        L.sptc_result_get_info.argtypes = [c_void_p, POINTER(_Info)]
+        L.sptc_result_get_chunk.argtypes = [c_void_p, c_int32, POINTER(c_uint64),
+                                            POINTER(c_void_p), POINTER(c_void_p),
+                                            POINTER(c_void_p)]
+        L.sptc_result_copy_to_host.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+        L.sptc_result_decode.argtypes = [c_void_p, POINTER(c_void_p), c_void_p]
+        L.sptc_result_row_keys.argtypes = [c_void_p, c_void_p]
+        L.sptc_result_row_range.argtypes = [c_void_p, c_uint64, POINTER(c_uint64),
+                                            POINTER(c_uint64)]
+        L.sptc_result_copy_range.argtypes = [c_void_p, c_uint64, c_uint64, c_void_p, c_void_p,
+                                             c_void_p]
+        L.sptc_result_free.argtypes = [c_void_p]
+        L.sptc_result_free.restype = None
+        L.sptc_layout.argtypes = [c_int32, POINTER(c_uint64), c_int32, POINTER(c_int32),
+                                  POINTER(c_int32), c_int32, POINTER(c_int32),
+                                  POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]
+        L.sptc_linearize.argtypes = [c_int32, c_uint64, POINTER(c_void_p), c_int32,
+                                     POINTER(c_int32), POINTER(c_int32), c_void_p, c_void_p]
+        L.sptc_sort_pairs.argtypes = [c_void_p, c_void_p, c_uint64, c_int32, c_int32, c_void_p]
+        L.sptc_index_match.argtypes = [c_void_p, c_uint64, c_void_p, c_uint64, c_int32,
+                                       c_int32, c_void_p, c_void_p, c_void_p,
+                                       POINTER(c_uint64), c_void_p]
+        L.sptc_exclusive_scan_u64.argtypes = [c_void_p, c_void_p, c_uint64, c_void_p]
+        L.sptc_device_info.argtypes = [POINTER(c_int32), POINTER(c_uint64), POINTER(c_uint64),
+                                       POINTER(c_double)]
+        L.sptc_status_string.argtypes = [c_int32]
+        L.sptc_status_string.restype = c_char_p
+        L.sptc_last_error.argtypes = []
+        L.sptc_last_error.restype = c_char_p
+        L.sptc_trim_cache.argtypes = []
+        L.sptc_trim_cache.restype = None
+        for name in ("sptc_contract", "sptc_result_get_info", "sptc_result_get_chunk",
+                     "sptc_result_copy_to_host", "sptc_result_decode", "sptc_layout",
+                     "sptc_result_row_keys", "sptc_result_row_range", "sptc_result_copy_range",
+                     "sptc_linearize", "sptc_sort_pairs", "sptc_index_match",
+                     "sptc_exclusive_scan_u64", "sptc_device_info"):
+            getattr(L, name).restype = c_int32
+        _bound = True
+    return L
+
+
+def check(status):
+    if status != OK:
+        L = lib()
+        raise SpTCError(status, (L.sptc_last_error() or b"").decode())
+
+
+def layout(dims, cmodes, role, cwidth=None):
+    """Key layout of one operand: (shift[], width[], nfbits, ncbits)."""
+    L = lib()
+    order = len(dims)
+    d = (c_uint64 * order)(*dims)
+    cm = (c_int32 * max(1, len(cmodes)))(*cmodes)
+    cw = (c_int32 * max(1, len(cmodes)))(*(cwidth or [-1] * len(cmodes)))
+    sh = (c_int32 * order)()
+    wd = (c_int32 * order)()
+    nf, nc = c_int32(), c_int32()
+    st = L.sptc_layout(order, d, len(cmodes), cm, cw if cwidth else None, role, sh, wd,
+                       byref(nf), byref(nc))
+    check(st)
+    return list(sh), list(wd), nf.value, nc.value
+
+
+class _HostCOO:
+    """Keeps the ctypes views of a COOTensor alive for one call."""
+
+    def __init__(self, t, on_device=False):
+        self.order = t.order
+        self.dims = (c_uint64 * t.order)(*t.dims)
+        if on_device:
+            self.arrays = t.idx
+            ptrs = [a.data_ptr() for a in t.idx]
+            val_ptr = t.val.data_ptr()
+            idx_bytes = t.idx[0].element_size() if t.order else 4
+        else:
+            self.arrays = [np.ascontiguousarray(a) for a in t.idx]
+            ptrs = [a.ctypes.data for a in self.arrays]
+            self.val = np.ascontiguousarray(t.val, dtype=np.float32)
+            val_ptr = self.val.ctypes.data
+            idx_bytes = self.arrays[0].dtype.itemsize if t.order else 4
+        self.ptrs = (c_void_p * t.order)(*ptrs)
+        self.s = _COO(t.order, t.nnz if not on_device else int(t.val.numel()), self.dims,
+                      self.ptrs, idx_bytes, val_ptr, 1 if on_device else 0)
+
+
+class Result:
+    """Device-resident contraction output (freed with the object)."""
+
+    def __init__(self, handle):
+        self._h = c_void_p(handle)
+        self.info = _Info()
+        check(lib().sptc_result_get_info(self._h, byref(self.info)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().sptc_result_free(h)
+            self._h = c_void_p(0)
+
+    @property
+    def nnz(self):
+        return int(self.info.nnz)
+
+    @property
+    def ir_nnz(self):
+        return int(self.info.ir_nnz)
+
+    @property
+    def phase_ms(self):
+        return list(self.info.phase_ms)
+
+    def chunks(self):
+        out = []
+        for i in range(self.info.nchunks):
+            n, r, c, v = c_uint64(), c_void_p(), c_void_p(), c_void_p()
+            check(lib().sptc_result_get_chunk(self._h, i, byref(n), byref(r), byref(c), byref(v)))
+            out.append((n.value, r.value, c.value, v.value))
+        return out
+
+    def to_host(self):
+        """(xfree u64, yfree u64, vals f32) sorted by (xfree, yfree)."""
+        n = self.nnz
+        xf = np.empty(n, np.uint64)
+        yf = np.empty(n, np.uint64)
+        v = np.empty(n, np.float32)
+        check(lib().sptc_result_copy_to_host(self._h, None, xf.ctypes.data, yf.ctypes.data,
+                                             v.ctypes.data))
+        return xf, yf, v
+
+    def packed_keys(self):
+        """Packed 64-bit keys xfree << y_nfbits | yfree (the reference's
+        output key, contraction.cuh:135) and values."""
+        n = self.nnz
+        k = np.empty(n, np.uint64)
+        v = np.empty(n, np.float32)
+        check(lib().sptc_result_copy_to_host(self._h, k.ctypes.data, None, None, v.ctypes.data))
+        return k, v
+
+    def row_keys(self):
+        rk = np.empty(int(self.info.nrows), np.uint64)
+        check(lib().sptc_result_row_keys(self._h, rk.ctypes.data))
+        return rk
+
+    def row_range(self, row):
+        b, e = c_uint64(), c_uint64()
+        check(lib().sptc_result_row_range(self._h, int(row), byref(b), byref(e)))
+        return b.value, e.value
+
+    def copy_range(self, begin, end):
+        n = end - begin
+        xf = np.empty(n, np.uint64)
+        yf = np.empty(n, np.uint64)
+        v = np.empty(n, np.float32)
+        check(lib().sptc_result_copy_range(self._h, begin, end, xf.ctypes.data, yf.ctypes.data,
+                                           v.ctypes.data))
+        return xf, yf, v
+
+    def decode(self):
+        """Per-mode 0-based indices (X free modes ascending, then Y free
+        modes ascending) and values."""
+        n = self.nnz
+        order = self.info.out_order
+        idx = [np.empty(n, np.uint32) for _ in range(order)]
+        v = np.empty(n, np.float32)
+        ptrs = (c_void_p * max(1, order))(*[a.ctypes.data for a in idx])
+        check(lib().sptc_result_decode(self._h, ptrs, v.ctypes.data))
+        return idx, v
+
+
+def SpTC(X, Y, cmodes_x, cmodes_y, dense_accumulator=0, count_only=False,
+         mem_budget_bytes=0, radix_bits=0, verbose=False, stream=None, on_device=False):
+    """Z = X x_{cmodes_x, cmodes_y} Y on the GPU; returns a :class:`Result`.
+
+    X and Y are COOTensor (host numpy arrays) or, with on_device=True,
+    objects with .dims, .idx (torch device tensors) and .val."""
+    L = lib()
+    hx = _HostCOO(X, on_device)
+    hy = X is Y and hx or _HostCOO(Y, on_device)
+    nc = len(cmodes_x)
+    if len(cmodes_y) != nc:
+        raise SpTCError(ERR_BAD_ARGS, "contracted mode lists differ in length")
+    cx = (c_int32 * max(1, nc))(*cmodes_x)
+    cy = (c_int32 * max(1, nc))(*cmodes_y)
+    o = _Opts(int(dense_accumulator), int(bool(count_only)), int(mem_budget_bytes),
+              int(radix_bits), int(bool(verbose)))
+    out = c_void_p()
+    st = L.sptc_contract(byref(hx.s), byref(hy.s), nc, cx, cy, byref(o),
+                         c_void_p(stream or 0), byref(out))
+    check(st)
+    return Result(out.value)
+
+
+def device_info():
+    cus, fr, tot, peak = c_int32(), c_uint64(), c_uint64(), c_double()
+    check(lib().sptc_device_info(byref(cus), byref(fr), byref(tot), byref(peak)))
+    return {"num_cus": cus.value, "free_bytes": fr.value, "total_bytes": tot.value,
+            "peak_hbm_gbs": peak.value}

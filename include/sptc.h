@@ -1,0 +1,198 @@
+/*
+ * sptc.h -- C ABI of libsptc_hip.so, the MI355X (gfx950) sparse tensor
+ * contraction engine.
+ *
+ * Drop-in boundary for GSPARC's SpTC path.  The reference entry point is the
+ * C++ template
+ *     gsparc::SpTC<TensorType,SLITOMType>(SX, SY, SZ, pools, gpu_count,
+ *         copy_timers, idxmatch_timers, dprtn_timers, contraction_timers,
+ *         dense_accumulator)                          (include/gsparc/sptc.cuh:16-26)
+ * preceded by TensorManager::{FindPartitionNum, ConvertTensor, SortSlitomXY,
+ * Partition} (main.cu:293-321) on SparseTensor<uint64_t,float> inputs
+ * (sparse_tensor.hpp:12-44).  Here the whole chain -- linearize, sort,
+ * partition, index match, accumulate -- sits behind sptc_contract(), which
+ * takes the same COO tensors and contracted-mode lists and returns the
+ * contraction's COO output.  The per-phase entry points below replace the
+ * individual reference routines named in their comments.
+ *
+ * Conventions: plain pointers and sizes; every call returns sptc_status (the
+ * reference exit()s instead, cuda_helper.cpp:83-92, cuda_memory.cpp:75-79).
+ * `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ * Device pointers are HIP device addresses.  No call allocates from, or
+ * frees, memory that the caller owns.
+ */
+#ifndef SPTC_H_
+#define SPTC_H_
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  SPTC_OK = 0,
+  SPTC_ERR_BAD_ARGS = 1,     /* malformed tensor, modes or options        */
+  SPTC_ERR_OOM_BUDGET = 2,   /* work does not fit the memory budget       */
+  SPTC_ERR_KEY_OVERFLOW = 3, /* a linearized key needs more than 64 bits  */
+  SPTC_ERR_HIP = 4,          /* a HIP runtime call failed                 */
+  SPTC_ERR_NO_DEVICE = 5,    /* no usable gfx950 device                   */
+  SPTC_ERR_INTERNAL = 6      /* a device-side consistency check failed    */
+} sptc_status;
+
+#define SPTC_MAX_ORDER 8
+
+/* COO tensor in SoA form (the reference's SparseTensor<uint64_t,float>,
+ * sparse_tensor.hpp:12-44): 0-based per-mode index arrays and FP32 values.
+ * dims[m] sets the bit width of mode m (64 - clz(dims[m]-1),
+ * tensor_manager.tpp:109; the .tns reader sets dims = max index + 1). */
+typedef struct {
+  int32_t order;
+  uint64_t nnz;
+  const uint64_t* dims;     /* host array [order]                        */
+  const void* const* idx;   /* [order] index arrays (host array of ptrs) */
+  int32_t idx_bytes;        /* 4 (uint32) or 8 (uint64)                  */
+  const float* val;         /* [nnz]                                     */
+  int32_t on_device;        /* 1: idx[m], val are device pointers        */
+} sptc_coo;
+
+/* Options.  dense_accumulator follows the reference's -d flag
+ * (sptc.cuh:201-218): 1 forces the dense accumulator (error if it does not
+ * apply), -1 forces ESC, 0 lets the planner choose. */
+typedef struct {
+  int32_t dense_accumulator;
+  int32_t count_only;        /* 1: return nnz and checksums, no entries   */
+  uint64_t mem_budget_bytes; /* 0: free device memory minus 1 GiB         */
+  int32_t radix_bits;        /* 0: autotune; else 4..11                   */
+  int32_t verbose;           /* 1: phase log on stderr                    */
+} sptc_opts;
+
+/* Result of a contraction (device-resident).  Entry i has output free
+ * coordinates (X free modes ascending, then Y free modes ascending):
+ *   X part = row_key[row[i]]  (X free field of the reference's X key,
+ *            key >> ncbits, tensor_manager.tpp:266)
+ *   Y part = col[i]           (Y free field, key & (2^Ynf - 1))
+ * Entries are sorted by (row, col): for out_bits <= 64 that is ascending
+ * packed key (row_key << y_nfbits | col), the order of the reference's ESC
+ * output (contraction.cuh:506-537). */
+typedef struct {
+  uint64_t nnz;          /* output nonzeros                                */
+  uint64_t ir_nnz;       /* intermediate products (index-match total)      */
+  uint64_t nrows;        /* entries in row_key (X free rows of X)          */
+  int32_t x_nfbits, y_nfbits, ncbits, out_bits;
+  int32_t out_order;
+  int32_t nchunks;       /* accumulation chunks (dynamic partitions)       */
+  int32_t accumulator;   /* 0 = ESC, 1 = dense                             */
+  double value_sum;      /* sum of all output values (FP64 on device)      */
+  const uint32_t* row;   /* device [nnz]                                   */
+  const uint64_t* col;   /* device [nnz]                                   */
+  const float* val;      /* device [nnz]                                   */
+  const uint64_t* row_key; /* device [nrows]                               */
+  /* phase times in ms (HIP events): 0 upload, 1 linearize+sort, 2 match,
+   * 3 partition, 4 accumulate (expand+sort+reduce), 5 total */
+  double phase_ms[6];
+} sptc_result_info;
+
+typedef struct sptc_result_s sptc_result;
+
+/* Full SpTC: Z = X x_{cx,cy} Y.  Replaces TensorManager::ConvertTensor +
+ * SortSlitomXY + Partition + gsparc::SpTC (main.cu:293-376).  On success
+ * *out holds the result (free with sptc_result_free). */
+sptc_status sptc_contract(const sptc_coo* X, const sptc_coo* Y, int32_t nc,
+                          const int32_t* cx, const int32_t* cy,
+                          const sptc_opts* opts, void* stream,
+                          sptc_result** out);
+
+sptc_status sptc_result_get_info(const sptc_result* r, sptc_result_info* info);
+
+/* Chunk i (0 <= i < info.nchunks) of the result: outputs of one contiguous
+ * X-row range, in ascending order; chunks are in ascending order too.  The
+ * reference keeps one ESC partition's output at a time (contraction.cuh:
+ * 440-562); here every chunk stays resident. */
+sptc_status sptc_result_get_chunk(const sptc_result* r, int32_t i, uint64_t* nnz,
+                                  const uint32_t** row, const uint64_t** col,
+                                  const float** val);
+
+/* Copies entries to host arrays of r->nnz elements (any may be NULL):
+ * packed keys (row_key << y_nfbits | col; only when out_bits <= 64, else
+ * SPTC_ERR_KEY_OVERFLOW), the row/col parts, values. */
+sptc_status sptc_result_copy_to_host(const sptc_result* r, uint64_t* keys,
+                                     uint64_t* xfree, uint64_t* yfree,
+                                     float* vals);
+
+/* Row table (X free keys, ascending) to a host array of info.nrows. */
+sptc_status sptc_result_row_keys(const sptc_result* r, uint64_t* row_keys);
+
+/* Global entry range [*begin, *end) of row `row` (an index into the row
+ * table); empty when the row has no outputs. */
+sptc_status sptc_result_row_range(const sptc_result* r, uint64_t row,
+                                  uint64_t* begin, uint64_t* end);
+
+/* Copies entries [begin, end) (global entry order) to host arrays of
+ * end-begin elements; any output pointer may be NULL. */
+sptc_status sptc_result_copy_range(const sptc_result* r, uint64_t begin,
+                                   uint64_t end, uint64_t* xfree,
+                                   uint64_t* yfree, float* vals);
+
+/* Decodes entries to 0-based per-mode uint32 indices (mode_idx[out_order]
+ * host arrays of nnz each), X free modes ascending then Y free modes. */
+sptc_status sptc_result_decode(const sptc_result* r, uint32_t* const* mode_idx,
+                               float* vals);
+
+void sptc_result_free(sptc_result* r);
+
+/* ---- per-phase entry points (device pointers, caller-owned buffers) ---- */
+
+/* Key layout of one operand (get_fmode + compute_nbtis + setup_slitom,
+ * tensor_manager.tpp:63-186).  role 1 = X: key = f << ncbits | c; role 2 =
+ * Y: key = c << nfbits | f.  cwidth[i] >= 0 overrides the width of
+ * contracted mode cmodes[i] (NULL: widths from dims).  shift/width are
+ * [order] outputs. */
+sptc_status sptc_layout(int32_t order, const uint64_t* dims, int32_t nc,
+                        const int32_t* cmodes, const int32_t* cwidth,
+                        int32_t role, int32_t* shift, int32_t* width,
+                        int32_t* nfbits, int32_t* ncbits);
+
+/* Linearize (ConvertTensor, tensor_manager.tpp:212-298, on the GPU):
+ * keys[i] = OR_m idx[m][i] << shift[m].  idx are device pointers. */
+sptc_status sptc_linearize(int32_t order, uint64_t nnz, const void* const* idx,
+                           int32_t idx_bytes, const int32_t* shift,
+                           const int32_t* width, uint64_t* keys, void* stream);
+
+/* Stable LSD radix sort of (key, value) pairs on key bits [0, nbits)
+ * (sort_64's cub::DeviceRadixSort::SortPairs, sort.cuh:90-108).  Sorts
+ * in place; radix_bits 0 = autotune. */
+sptc_status sptc_sort_pairs(uint64_t* keys, float* vals, uint64_t n,
+                            int32_t nbits, int32_t radix_bits, void* stream);
+
+/* Index match (IndexMatch, indexMatch.cuh:231-287): for each X nonzero,
+ * mpos = first Y index whose key >> y_nfbits equals the X contract field,
+ * mcnt = run length, prefix = exclusive scan of mcnt ([nx] entries + 1,
+ * prefix[nx] = ir).  *ir_nnz receives the total. */
+sptc_status sptc_index_match(const uint64_t* xkeys, uint64_t nx,
+                             const uint64_t* ykeys, uint64_t ny,
+                             int32_t ncbits, int32_t y_nfbits, uint64_t* mpos,
+                             uint64_t* mcnt, uint64_t* prefix,
+                             uint64_t* ir_nnz, void* stream);
+
+/* Exclusive scan of n uint64 (cub::DeviceScan::InclusiveSum shifted,
+ * indexMatch.cuh:275-279); out has n+1 entries, out[n] = total. */
+sptc_status sptc_exclusive_scan_u64(const uint64_t* in, uint64_t* out,
+                                    uint64_t n, void* stream);
+
+/* Device properties used by the planner and the benchmark. */
+sptc_status sptc_device_info(int32_t* num_cus, uint64_t* free_bytes,
+                             uint64_t* total_bytes, double* peak_hbm_gbs);
+
+const char* sptc_status_string(sptc_status s);
+
+/* Detail of the last failure on the calling thread (empty when none). */
+const char* sptc_last_error(void);
+
+/* Returns cached device blocks to the HIP runtime. */
+void sptc_trim_cache(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,256 @@
+"""GPU parity: the HIP path (libsptc_hip.so, through its C ABI) against the CPU
+oracle and the golden fixtures.  Index work must be bit-exact; FP32 values
+within REL_TOL (1e-5) relative error of the FP64 oracle."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from gsparc_amd import sptc
+from gsparc_amd.coo import COOTensor
+from gsparc_amd.workloads import PLAIN_SMALL
+from helpers import (REL_TOL, assert_same_coo, decode_x, decode_y, load_fixtures, max_rel_err,
+                     random_coo)
+
+pytestmark = pytest.mark.gpu
+FIXTURES = load_fixtures()
+
+
+def _dev(torch, a):
+    if a.dtype == np.uint64:
+        return torch.from_numpy(a.view(np.int64)).cuda()
+    if a.dtype == np.uint32:
+        return torch.from_numpy(a.view(np.int32)).cuda()
+    return torch.from_numpy(a).cuda()
+
+
+def _host_u64(t):
+    return t.cpu().numpy().view(np.uint64)
+
+
+def _stream(torch):
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+# ---- (a) linearize ---------------------------------------------------------
+
+@pytest.mark.parametrize("seed", range(4))
+def test_linearize_matches_oracle(gpu, seed):
+    import torch
+    rng = np.random.default_rng(seed)
+    order = int(rng.integers(2, 6))
+    dims = [int(x) for x in rng.integers(2, 5000, size=order)]
+    nc = int(rng.integers(1, order))
+    cm = [int(x) for x in rng.permutation(order)[:nc]]
+    t = random_coo(rng, dims, 100_000)
+    for role in (1, 2):
+        sh, wd, fp, nf, ncb = orc.layout(dims, cm, role)
+        sh2, wd2, nf2, ncb2 = sptc.layout(dims, cm, role)
+        assert (sh2, wd2, nf2, ncb2) == (sh, wd, nf, ncb)
+        idx = [_dev(torch, a) for a in t.idx]
+        keys = torch.empty(t.nnz, dtype=torch.int64, device="cuda")
+        ptrs = (ctypes.c_void_p * order)(*[a.data_ptr() for a in idx])
+        sptc.check(sptc.lib().sptc_linearize(order, t.nnz, ptrs, 4, (ctypes.c_int32 * order)(*sh),
+                                             (ctypes.c_int32 * order)(*wd),
+                                             ctypes.c_void_p(keys.data_ptr()), _stream(torch)))
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(_host_u64(keys), orc.linearize(t.idx, sh, wd))
+
+
+# ---- (b) radix sort --------------------------------------------------------
+
+@pytest.mark.parametrize("n,bits,rb", [(0, 8, 8), (1, 3, 8), (4095, 20, 8), (4096, 20, 8),
+                                       (4097, 33, 8), (100_000, 43, 8), (1_000_003, 48, 8),
+                                       (2_000_000, 64, 8), (300_000, 31, 11),
+                                       (1_500_000, 44, 11)])
+def test_sort_pairs_stable_bit_exact(gpu, n, bits, rb):
+    import torch
+    rng = np.random.default_rng(n + bits)
+    if bits == 64:
+        keys = rng.integers(0, 2 ** 63, size=n, dtype=np.uint64) * np.uint64(2) + \
+            rng.integers(0, 2, size=n, dtype=np.uint64)
+    else:
+        # Zipf-like skew: many repeated keys exercise stability and hot digits
+        base = rng.integers(0, 1 << bits, size=max(1, n // 7), dtype=np.uint64)
+        keys = base[rng.zipf(1.3, size=n) % len(base)] if n else np.zeros(0, np.uint64)
+    vals = np.arange(n, dtype=np.float32)  # input positions (exact below 2^24)
+    dk = _dev(torch, keys.astype(np.uint64))
+    dv = _dev(torch, vals)
+    sptc.check(sptc.lib().sptc_sort_pairs(ctypes.c_void_p(dk.data_ptr() if n else 0),
+                                          ctypes.c_void_p(dv.data_ptr() if n else 0), n, bits, rb,
+                                          _stream(torch)))
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(_host_u64(dk), keys[order])
+    np.testing.assert_array_equal(dv.cpu().numpy(), vals[order])
+
+
+# ---- scan ------------------------------------------------------------------
+
+@pytest.mark.parametrize("n", [0, 1, 2047, 2048, 2049, 1_000_000, 5_000_001])
+def test_exclusive_scan(gpu, n):
+    import torch
+    rng = np.random.default_rng(n)
+    a = rng.integers(0, 1000, size=n, dtype=np.uint64)
+    d_in = _dev(torch, a)
+    d_out = torch.empty(n + 1, dtype=torch.int64, device="cuda")
+    sptc.check(sptc.lib().sptc_exclusive_scan_u64(ctypes.c_void_p(d_in.data_ptr() if n else 0),
+                                                  ctypes.c_void_p(d_out.data_ptr()), n,
+                                                  _stream(torch)))
+    want = np.concatenate([[0], np.cumsum(a)]).astype(np.uint64)
+    np.testing.assert_array_equal(_host_u64(d_out), want)
+
+
+# ---- (c) index match -------------------------------------------------------
+
+@pytest.mark.parametrize("dims,cx,cy", [((200, 150, 120), [2], [0]),           # direct table
+                                        ((300, 2000, 4000), [1, 2], [1, 2]),   # search (23 bits)
+                                        ((50, 40, 30), [0, 2], [0, 2])])
+def test_index_match_matches_oracle(gpu, dims, cx, cy):
+    import torch
+    rng = np.random.default_rng(sum(dims))
+    X = random_coo(rng, dims, 200_000)
+    Y = random_coo(rng, dims if cx == cy else (dims[2], dims[1], dims[0]), 150_000)
+    cw = orc.common_cwidth(X.dims, Y.dims, cx, cy)
+    xl = orc.layout(X.dims, cx, 1, cw)
+    yl = orc.layout(Y.dims, cy, 2, cw)
+    xk, _ = orc.sort_pairs(orc.linearize(X.idx, xl[0], xl[1]), X.val)
+    yk, _ = orc.sort_pairs(orc.linearize(Y.idx, yl[0], yl[1]), Y.val)
+    mpos, mcnt, prefix, ir = orc.index_match(xk, yk, xl[4], yl[3])
+    dx, dy = _dev(torch, xk), _dev(torch, yk)
+    nx, ny = len(xk), len(yk)
+    gp = torch.empty(nx, dtype=torch.int64, device="cuda")
+    gc = torch.empty(nx, dtype=torch.int64, device="cuda")
+    gpre = torch.empty(nx + 1, dtype=torch.int64, device="cuda")
+    gir = ctypes.c_uint64()
+    sptc.check(sptc.lib().sptc_index_match(
+        ctypes.c_void_p(dx.data_ptr()), nx, ctypes.c_void_p(dy.data_ptr()), ny, xl[4], yl[3],
+        ctypes.c_void_p(gp.data_ptr()), ctypes.c_void_p(gc.data_ptr()),
+        ctypes.c_void_p(gpre.data_ptr()), ctypes.byref(gir), _stream(torch)))
+    assert gir.value == ir
+    np.testing.assert_array_equal(_host_u64(gc), mcnt)
+    np.testing.assert_array_equal(_host_u64(gpre), prefix)
+    np.testing.assert_array_equal(_host_u64(gp), mpos)
+
+
+# ---- full SpTC -------------------------------------------------------------
+
+def _gpu_coords(res):
+    idx, v = res.decode()
+    return [a.astype(np.int64) for a in idx], v
+
+
+@pytest.mark.parametrize("case", FIXTURES, ids=[c["name"] for c in FIXTURES])
+def test_contract_matches_golden(gpu, case):
+    res = sptc.SpTC(case["X"], case["Y"], case["cx"], case["cy"])
+    coords, v = _gpu_coords(res)
+    assert_same_coo(coords, v, case["zcoords"], case["zval"], rtol=REL_TOL)
+
+
+def _compare_with_oracle(X, Y, cx, cy, res, x_rows=None):
+    r = orc.pipeline(X, Y, cx, cy, x_rows=x_rows)
+    if x_rows is None:
+        xf, yf, v = res.to_host()
+    else:
+        rk = res.row_keys()
+        parts = []
+        for key in np.sort(np.asarray(x_rows, np.uint64)):
+            j = int(np.searchsorted(rk, key))
+            if j < len(rk) and rk[j] == key:
+                b, e = res.row_range(j)
+                parts.append(res.copy_range(b, e))
+        xf = np.concatenate([p[0] for p in parts]) if parts else np.zeros(0, np.uint64)
+        yf = np.concatenate([p[1] for p in parts]) if parts else np.zeros(0, np.uint64)
+        v = np.concatenate([p[2] for p in parts]) if parts else np.zeros(0, np.float32)
+    np.testing.assert_array_equal(xf, r["xf"])
+    np.testing.assert_array_equal(yf, r["yf"])
+    err = max_rel_err(v, r["v64"])
+    assert err <= REL_TOL, err
+    return r, err
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_contract_random_matches_oracle(gpu, seed):
+    rng = np.random.default_rng(1000 + seed)
+    xo = int(rng.integers(2, 5))
+    nc = int(rng.integers(1, xo))
+    yo = int(rng.integers(max(nc, 2), 5))
+    cdims = [int(x) for x in rng.integers(2, 60, size=nc)]
+    xdims = [int(x) for x in rng.integers(2, 300, size=xo)]
+    ydims = [int(x) for x in rng.integers(2, 300, size=yo)]
+    cx = [int(x) for x in rng.permutation(xo)[:nc]]
+    cy = [int(x) for x in rng.permutation(yo)[:nc]]
+    for a, b, d in zip(cx, cy, cdims):
+        xdims[a] = d
+        ydims[b] = d
+    X = random_coo(rng, xdims, 20_000, positive=bool(seed % 2))
+    Y = random_coo(rng, ydims, 15_000, positive=bool(seed % 2))
+    res = sptc.SpTC(X, Y, cx, cy)
+    r, _ = _compare_with_oracle(X, Y, cx, cy, res)
+    assert res.ir_nnz == r["ir"]
+
+
+def test_chunked_equals_unchunked(gpu):
+    rng = np.random.default_rng(7)
+    X = random_coo(rng, (300, 200, 40), 60_000)
+    full = sptc.SpTC(X, X, [2], [2])
+    assert full.info.nchunks == 1
+    # a budget that forces many accumulation chunks
+    small = sptc.SpTC(X, X, [2], [2], mem_budget_bytes=full.ir_nnz * 8 + (96 << 20))
+    assert small.info.nchunks > 1
+    a = full.to_host()
+    b = small.to_host()
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    cnt = sptc.SpTC(X, X, [2], [2], count_only=True)
+    assert cnt.nnz == full.nnz and cnt.ir_nnz == full.ir_nnz
+
+
+def test_plain_small_config(gpu):
+    """The 2000x2000x500, 2e5-nnz self-contraction over mode 2: global
+    invariants on the whole output plus oracle parity on sampled rows."""
+    X = PLAIN_SMALL.make()
+    res = sptc.SpTC(X, X, [2], [2])
+    # ir = sum over contract index of count^2
+    cnt = np.bincount(X.idx[2].astype(np.int64), minlength=X.dims[2]).astype(np.float64)
+    assert res.ir_nnz == int((cnt ** 2).sum())
+    # value checksum: sum of all outputs = sum_c (sum of X values with c)^2
+    s = np.bincount(X.idx[2].astype(np.int64), weights=X.val.astype(np.float64),
+                    minlength=X.dims[2])
+    assert abs(res.info.value_sum - (s ** 2).sum()) <= 1e-9 * (s ** 2).sum()
+    xf, yf, v = res.to_host()
+    key = (xf.astype(np.uint64) << np.uint64(res.info.y_nfbits)) | yf
+    assert np.all(np.diff(key.astype(np.uint64)) > 0)  # strictly ascending: unique, sorted
+    assert abs(v.astype(np.float64).sum() - (s ** 2).sum()) <= 1e-5 * (s ** 2).sum()
+    rk = res.row_keys()
+    pick = np.unique(np.concatenate([rk[:40], rk[-10:], rk[:: max(1, len(rk) // 50)]]))
+    _compare_with_oracle(X, X, [2], [2], res, x_rows=pick)
+
+
+def test_errors_are_status_codes(gpu):
+    rng = np.random.default_rng(3)
+    X = random_coo(rng, (10, 10, 10), 50)
+    with pytest.raises(sptc.SpTCError) as e:
+        sptc.SpTC(X, X, [5], [2])
+    assert e.value.status == sptc.ERR_BAD_ARGS
+    big = COOTensor((2 ** 31, 2 ** 31, 2 ** 31), [a for a in X.idx], X.val)
+    with pytest.raises(sptc.SpTCError) as e:
+        sptc.SpTC(big, big, [0], [0])
+    assert e.value.status == sptc.ERR_KEY_OVERFLOW
+    Xb = random_coo(rng, (100, 100, 5), 20_000)
+    with pytest.raises(sptc.SpTCError) as e:
+        sptc.SpTC(Xb, Xb, [2], [2], mem_budget_bytes=1 << 20)
+    assert e.value.status == sptc.ERR_OOM_BUDGET
+
+
+def test_empty_and_disjoint(gpu):
+    X = COOTensor((5, 6), [np.array([0, 1], np.uint32), np.array([0, 1], np.uint32)],
+                  np.array([1.0, 2.0], np.float32))
+    Y = COOTensor((6, 4), [np.array([2, 3], np.uint32), np.array([0, 1], np.uint32)],
+                  np.array([1.0, 2.0], np.float32))
+    res = sptc.SpTC(X, Y, [1], [0])
+    assert res.nnz == 0 and res.ir_nnz == 0
+    E = COOTensor((5, 6), [np.zeros(0, np.uint32), np.zeros(0, np.uint32)],
+                  np.zeros(0, np.float32))
+    res = sptc.SpTC(E, Y, [1], [0])
+    assert res.nnz == 0
