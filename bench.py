@@ -12,6 +12,7 @@ and the reference CPU SpTC (Sparta ttt) timed on a bounded sample.
 import argparse
 import json
 import os
+import re
 import subprocess
 import sys
 import tempfile
@@ -32,6 +33,9 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-rows", type=int, default=0,
                    help="X free rows in the CPU-baseline sample (0: auto)")
+    p.add_argument("--radix-bits", type=int, default=0, help="0: autotune, 8 or 11")
+    p.add_argument("--copy-probe-bytes", type=int, default=0,
+                   help="run one device-to-device copy of this size first (counter calibration)")
     return p.parse_args()
 
 
@@ -82,16 +86,18 @@ def cpu_baseline(spec, X, n_rows):
         cmd = [orc.REF_TTT, "-X", xs_path, "-Y", x_path, "-m", str(len(spec.cx)), "-x",
                *[str(c) for c in spec.cx], "-y", *[str(c) for c in spec.cy], "-t", str(threads)]
         env = dict(os.environ, EXPERIMENT_MODES="3")
-        p = subprocess.run(cmd, env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
+        p = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
                            text=True, timeout=600)
     tot = None
     for line in p.stderr.splitlines():
         if line.startswith("[Total time]"):
             tot = float(line.split(":")[1].split()[0])
-    if tot is None:
+    nnz_z = re.findall(r"NNZ=(\d+)", p.stdout)  # the output tensor's status is last
+    if tot is None or not nnz_z:
         return None
     return {"total_s": tot, "threads": threads, "sample_rows": int(len(pick)),
-            "sample_nnz": int(sel.sum()), "sample_products": prods}
+            "sample_nnz": int(sel.sum()), "sample_products": prods,
+            "sample_out_nnz": int(nnz_z[-1])}
 
 
 def main():
@@ -112,23 +118,35 @@ def main():
     if args.workload == "nell2":
         kw = dict(count_only=True, mem_budget_bytes=NELL2_CAP_BYTES)
 
-    def step():
-        r = sptc.SpTC(dX, dX, list(spec.cx), list(spec.cy), on_device=True, **kw)
-        return r
+    if args.copy_probe_bytes:
+        a = torch.empty(args.copy_probe_bytes // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
+    def step(timing):
+        return sptc.SpTC(dX, dX, list(spec.cx), list(spec.cy), on_device=True,
+                         radix_bits=args.radix_bits, kernel_timing=timing, **kw)
 
     for _ in range(args.warmup):
-        r = step()
+        r = step(False)
         del r
     torch.cuda.synchronize()
     times, phases = [], []
+    kstats = {}
     out_nnz = ir = 0
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ts = time.perf_counter()
-        r = step()
+        r = step(True)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - ts)
         phases.append(r.phase_ms)
+        for k, (n, kms, kb) in r.kernel_stats().items():
+            a0 = kstats.get(k, (0, 0.0, 0.0))
+            kstats[k] = (a0[0] + n, a0[1] + kms, a0[2] + kb)
         out_nnz, ir = r.nnz, r.ir_nnz
         del r
     torch.cuda.synchronize()
@@ -148,17 +166,33 @@ def main():
         "phase_ms_median": {"linearize_sort": ph[1], "match": ph[2], "partition": ph[3],
                             "accumulate": ph[4], "total": ph[5]},
     }
+    peak = sptc.device_info()["peak_hbm_gbs"]
+    kern = {}
+    for k, (n, kms, kb) in kstats.items():
+        if n:
+            kern[k] = {"launches": n, "avg_ms": kms / n, "alg_bytes_per_launch": kb / n,
+                       "achieved_gbs": kb / (kms * 1e-3) / 1e9 if kms else None}
+    res["kernels"] = kern
+    if kern:
+        dom = max(kern, key=lambda k: kstats[k][1])
+        d = kern[dom]
+        res["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": d["achieved_gbs"],
+                           "peak": peak, "unit": "GB/s", "frac": d["achieved_gbs"] / peak,
+                           "traffic": None, "avg_launch_ms": d["avg_ms"],
+                           "alg_bytes_per_launch": d["alg_bytes_per_launch"]}
+    res["pool"] = sptc.pool_stats()
     if not args.no_cpu_baseline and args.workload != "nell2":
         rows = args.cpu_sample_rows or (2000 if args.workload == "nips" else 4000)
         cb = cpu_baseline(spec, X, rows)
         if cb:
-            rate = cb["sample_products"] / cb["total_s"]
             res["cpu_baseline"] = {
-                "value": rate, "unit": "products/s", "cores": cb["threads"],
-                "kind": "reference",
-                "sample": (f"Sparta ttt (EXPERIMENT_MODES=3) on the first {cb['sample_rows']} "
-                           f"X free rows ({cb['sample_nnz']} nnz, {cb['sample_products']} "
-                           f"products) against all of X; {cb['total_s']:.3f} s"),
+                "value": cb["sample_out_nnz"] / cb["total_s"], "unit": "nnz/s",
+                "cores": cb["threads"], "kind": "reference",
+                "sample": (f"Sparta ttt (EXPERIMENT_MODES=3, [Total time]) on the first "
+                           f"{cb['sample_rows']} X free rows ({cb['sample_nnz']} nnz, "
+                           f"{cb['sample_products']} products, {cb['sample_out_nnz']} output "
+                           f"nnz) against all of X: {cb['total_s']:.3f} s"),
+                "products_per_s": cb["sample_products"] / cb["total_s"],
                 "gpu_products_per_s": ir / (elapsed / args.steps),
             }
     print(json.dumps(res))

@@ -138,6 +138,8 @@ sptc_status sptc_contract(const sptc_coo* X, const sptc_coo* Y, int32_t nc, cons
       return fail(st, e.msg);
     }
     std::memcpy(res->info.phase_ms, e.times.ms, sizeof(res->info.phase_ms));
+    // operand preparation ran before contract() collected: add its launches
+    e.collect_kernel_stats(res->kstats);
     res->info.accumulator = 0;
     if (o.verbose) {
       std::fprintf(stderr,
@@ -317,6 +319,22 @@ sptc_status sptc_result_decode(const sptc_result* r, uint32_t* const* mode_idx, 
 }
 
 void sptc_result_free(sptc_result* r) { delete r; }
+
+sptc_status sptc_result_kernel_stats(const sptc_result* r, int32_t cls, uint64_t* launches,
+                                     double* total_ms, double* alg_bytes) {
+  if (!r || cls < 0 || cls >= SPTC_K_COUNT) return fail(SPTC_ERR_BAD_ARGS, "bad kernel class");
+  if (launches) *launches = r->kstats[cls].launches;
+  if (total_ms) *total_ms = r->kstats[cls].ms;
+  if (alg_bytes) *alg_bytes = r->kstats[cls].bytes;
+  return SPTC_OK;
+}
+
+void sptc_pool_stats(uint64_t* live_bytes, uint64_t* reserved_bytes, uint64_t* hip_mallocs) {
+  DevicePool& p = DevicePool::get();
+  if (live_bytes) *live_bytes = p.live_bytes();
+  if (reserved_bytes) *reserved_bytes = p.reserved_bytes();
+  if (hip_mallocs) *hip_mallocs = p.hip_mallocs();
+}
 
 // ---- per-phase entry points ---------------------------------------------
 

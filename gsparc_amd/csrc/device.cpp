@@ -6,15 +6,9 @@
 namespace sptc {
 
 namespace {
-constexpr size_t kGranule = 2u << 20;  // round large blocks to 2 MiB
-size_t round_size(size_t b) {
-  if (b <= (1u << 20)) {
-    size_t r = 256;
-    while (r < b) r <<= 1;
-    return r;
-  }
-  return (b + kGranule - 1) / kGranule * kGranule;
-}
+constexpr size_t kAlign = 256;
+constexpr size_t kSlabMin = 8ull << 30;  // new slabs are at least 8 GiB
+size_t align_up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
 }  // namespace
 
 DevicePool& DevicePool::get() {
@@ -22,34 +16,90 @@ DevicePool& DevicePool::get() {
   return *pool;
 }
 
-void* DevicePool::alloc(size_t bytes) {
-  const size_t want = round_size(bytes ? bytes : 1);
-  std::lock_guard<std::mutex> g(mu_);
-  auto it = cache_.lower_bound(want);
-  if (it != cache_.end() && it->first <= want + want / 4 + kGranule) {
-    void* p = it->second;
-    size_t sz = it->first;
-    cache_.erase(it);
-    cached_ -= sz;
-    live_ += sz;
-    sizes_[p] = sz;
-    return p;
+// Best fit over every slab's free ranges; ranges split on allocation and
+// coalesce with their neighbours on release.
+bool DevicePool::take_locked(size_t want, void** out) {
+  auto it = by_size_.lower_bound(want);
+  if (it == by_size_.end()) return false;
+  const size_t sz = it->first;
+  char* base = it->second;
+  by_size_.erase(it);
+  by_addr_.erase(base);
+  if (sz > want) {
+    by_addr_[base + want] = sz - want;
+    by_size_.emplace(sz - want, base + want);
   }
+  sizes_[base] = want;
+  live_ += want;
+  *out = base;
+  return true;
+}
+
+void DevicePool::insert_free_locked(char* p, size_t sz) {
+  // merge with the following free range
+  auto next = by_addr_.find(p + sz);
+  if (next != by_addr_.end() && slab_of(p) == slab_of(next->first)) {
+    erase_size_locked(next->second, next->first);
+    sz += next->second;
+    by_addr_.erase(next);
+  }
+  // merge with the preceding free range
+  auto prev = by_addr_.lower_bound(p);
+  if (prev != by_addr_.begin()) {
+    --prev;
+    if (prev->first + prev->second == p && slab_of(prev->first) == slab_of(p)) {
+      erase_size_locked(prev->second, prev->first);
+      p = prev->first;
+      sz += prev->second;
+      by_addr_.erase(prev);
+    }
+  }
+  by_addr_[p] = sz;
+  by_size_.emplace(sz, p);
+}
+
+void DevicePool::erase_size_locked(size_t sz, char* p) {
+  auto r = by_size_.equal_range(sz);
+  for (auto i = r.first; i != r.second; ++i)
+    if (i->second == p) {
+      by_size_.erase(i);
+      return;
+    }
+}
+
+int DevicePool::slab_of(const char* p) const {
+  for (size_t i = 0; i < slabs_.size(); ++i)
+    if (p >= slabs_[i].first && p < slabs_[i].first + slabs_[i].second) return (int)i;
+  return -1;
+}
+
+void* DevicePool::alloc(size_t bytes) {
+  const size_t want = align_up(bytes ? bytes : 1);
+  std::lock_guard<std::mutex> g(mu_);
   void* p = nullptr;
-  hipError_t e = hipMalloc(&p, want);
+  if (take_locked(want, &p)) return p;
+  // New slab: at least kSlabMin, else exactly what is needed.
+  size_t fr = 0, tot = 0;
+  (void)hipMemGetInfo(&fr, &tot);
+  size_t slab = std::max(want, kSlabMin);
+  if (slab > fr) slab = want;
+  char* base = nullptr;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&base), slab);
   if (e != hipSuccess) {
     (void)hipGetLastError();
-    for (auto& kv : cache_) (void)hipFree(kv.second);
-    cache_.clear();
-    cached_ = 0;
-    e = hipMalloc(&p, want);
+    release_empty_slabs_locked();
+    e = hipMalloc(reinterpret_cast<void**>(&base), want);
+    slab = want;
     if (e != hipSuccess) {
       (void)hipGetLastError();
       throw HipError{e, "hipMalloc", __LINE__};
     }
   }
-  sizes_[p] = want;
-  live_ += want;
+  ++hip_mallocs_;
+  slabs_.emplace_back(base, slab);
+  reserved_ += slab;
+  insert_free_locked(base, slab);
+  if (!take_locked(want, &p)) throw HipError{hipErrorOutOfMemory, "pool", __LINE__};
   return p;
 }
 
@@ -58,19 +108,35 @@ void DevicePool::free(void* p) {
   std::lock_guard<std::mutex> g(mu_);
   auto it = sizes_.find(p);
   if (it == sizes_.end()) return;
-  size_t sz = it->second;
+  const size_t sz = it->second;
   sizes_.erase(it);
   live_ -= sz;
-  cache_.emplace(sz, p);
-  cached_ += sz;
+  insert_free_locked(static_cast<char*>(p), sz);
+}
+
+void DevicePool::release_empty_slabs_locked() {
+  for (size_t i = 0; i < slabs_.size();) {
+    char* b = slabs_[i].first;
+    const size_t s = slabs_[i].second;
+    auto f = by_addr_.find(b);
+    if (f != by_addr_.end() && f->second == s) {
+      erase_size_locked(s, b);
+      by_addr_.erase(f);
+      (void)hipFree(b);
+      reserved_ -= s;
+      slabs_.erase(slabs_.begin() + i);
+    } else {
+      ++i;
+    }
+  }
 }
 
 void DevicePool::trim() {
   std::lock_guard<std::mutex> g(mu_);
-  for (auto& kv : cache_) (void)hipFree(kv.second);
-  cache_.clear();
-  cached_ = 0;
+  release_empty_slabs_locked();
 }
+
+size_t DevicePool::cached_bytes() const { return reserved_ - live_; }
 
 const DeviceInfo& device_info() {
   static DeviceInfo info = [] {

@@ -1,5 +1,6 @@
 // Device context: caching allocator with budget accounting, events, props.
 #pragma once
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <unordered_map>
@@ -9,8 +10,9 @@
 
 namespace sptc {
 
-// Caching device allocator.  Freed blocks are kept for reuse (keyed by
-// size); `live_bytes` counts what callers currently hold and is what memory
+// Device memory arena: large hipMalloc slabs, best-fit sub-allocation with
+// coalescing on release, so repeated calls do not return to the runtime
+// allocator.  `live_bytes` counts what callers hold and is what memory
 // budgets are checked against -- the analogue of the reference's
 // CudaMemoryPool::is_available (cuda_memory.cpp:37-51), which returns a
 // status here instead of exit(1) on exhaustion.
@@ -19,15 +21,25 @@ class DevicePool {
   static DevicePool& get();
   void* alloc(size_t bytes);  // throws HipError on failure
   void free(void* p);
-  void trim();                // return cached blocks to the runtime
+  void trim();                // return fully free slabs to the runtime
   size_t live_bytes() const { return live_; }
-  size_t cached_bytes() const { return cached_; }
+  size_t cached_bytes() const;
+  size_t reserved_bytes() const { return reserved_; }
+  uint64_t hip_mallocs() const { return hip_mallocs_; }
 
  private:
+  bool take_locked(size_t want, void** out);
+  void insert_free_locked(char* p, size_t sz);
+  void erase_size_locked(size_t sz, char* p);
+  void release_empty_slabs_locked();
+  int slab_of(const char* p) const;
   std::mutex mu_;
-  std::multimap<size_t, void*> cache_;
+  std::vector<std::pair<char*, size_t>> slabs_;
+  std::map<char*, size_t> by_addr_;        // free ranges by address
+  std::multimap<size_t, char*> by_size_;   // free ranges by size
   std::unordered_map<void*, size_t> sizes_;
-  size_t live_ = 0, cached_ = 0;
+  size_t live_ = 0, reserved_ = 0;
+  uint64_t hip_mallocs_ = 0;
 };
 
 // RAII device buffer from the pool.

@@ -59,15 +59,18 @@ Engine::Engine(hipStream_t s, const sptc_opts& o) : s_(s), o_(o) {
   for (auto& e : ev_) SPTC_HIP(hipEventCreate(&e));
   SPTC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_total_), sizeof(U2) * 2, 0));
   SPTC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_u64_), sizeof(uint64_t) * 8, 0));
+  // The budget caps what this call holds at once, on top of what was
+  // already live when it started.
   const uint64_t live = DevicePool::get().live_bytes();
   if (o.mem_budget_bytes) {
-    budget_ = o.mem_budget_bytes;
+    budget_ = live + o.mem_budget_bytes;
   } else {
     const uint64_t fr = device_free_bytes();
     const uint64_t margin = 1ull << 30;
     budget_ = live + (fr > margin ? fr - margin : 0);
   }
   rb_ = (o.radix_bits == 8 || o.radix_bits == 11) ? o.radix_bits : 8;
+  kt_.on = o.kernel_timing != 0;
 }
 
 Engine::~Engine() {
@@ -130,10 +133,13 @@ void Engine::prepare(const StagedCOO& t, const Layout& lay, Operand* out) {
   const int passes = radix_passes(lay.nbits, rb_);
   ws_.ensure(n, rb_, s_);
   ws_.prepare(passes, s_);
+  int tok = kt_.begin(s_);
   launch_linearize(a, t.idx_bytes, n, out->keys.get(), ws_.hist.get(),
                    passes <= kMaxFusedPasses ? passes : 0, rb_, s_);
+  kt_.end(tok, SPTC_K_LINEARIZE, (double)n * (lay.order * t.idx_bytes + 8), s_);
   if (passes > kMaxFusedPasses) histogram(ws_, out->keys.get(), n, passes, s_);
-  if (sort_after_histogram(ws_, out->keys.get(), out->vals.get(), n, lay.nbits, s_)) {
+  if (sort_after_histogram(ws_, out->keys.get(), out->vals.get(), n, lay.nbits, s_, &kt_,
+                           SPTC_K_SORT_INPUT)) {
     std::swap(out->keys, ws_.alt_keys);
     std::swap(out->vals, ws_.alt_vals);
   }
@@ -193,7 +199,10 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
   U2 tot;
   {
     DevBuf<U2> tmp(scan_tmp_count(nx) + 1);
+    int tok = kt_.begin(s_);
     run_match(X.keys.get(), X.vals.get(), nx, ncb, tab, mo, 0, tmp.get(), s_);
+    // keys read twice, plus the compacted active and row tables written
+    kt_.end(tok, SPTC_K_MATCH, (double)nx * (16 + 24 + 24), s_);
     tot = read_total(tmp.get() + scan_tmp_count(nx));
   }
   const uint64_t ir = tot.a;
@@ -295,12 +304,18 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       ea.hist = ws_.hist.get();
       ea.passes = passes <= kMaxFusedPasses ? passes : 0;
       ea.vsum = vsum.get();
+      int tok = kt_.begin(s_);
       launch_expand(ea, rb_, s_);
+      // product (key, value) written, matched Y (key, value) read
+      kt_.end(tok, SPTC_K_EXPAND, 24.0 * (double)n, s_);
       if (passes > kMaxFusedPasses) histogram(ws_, kA.get(), n, passes, s_);
-      const bool alt = sort_after_histogram(ws_, kA.get(), vA.get(), n, bits, s_);
+      const bool alt = sort_after_histogram(ws_, kA.get(), vA.get(), n, bits, s_, &kt_,
+                                            SPTC_K_SORT_PROD);
       const uint64_t* sk = alt ? ws_.alt_keys.get() : kA.get();
       const float* sv = alt ? ws_.alt_vals.get() : vA.get();
+      tok = kt_.begin(s_);
       run_reduce_count(sk, n, tmp.get(), s_);
+      kt_.end(tok, SPTC_K_REDUCE, 8.0 * (double)n, s_);
       const uint64_t out_n = read_total(tmp.get() + scan_tmp_count(n)).a;
       info.nnz += out_n;
       info.nchunks++;
@@ -316,8 +331,10 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       oc.row.reset(out_n);
       oc.col.reset(out_n);
       oc.val.reset(out_n);
+      tok = kt_.begin(s_);
       run_reduce_emit(sk, sv, n, oc.row.get(), oc.col.get(), oc.val.get(), r0c, ynf, fmask,
                       tmp.get(), s_);
+      kt_.end(tok, SPTC_K_REDUCE, 12.0 * (double)n + 16.0 * (double)out_n, s_);
       res->chunks.push_back(std::move(oc));
     }
   } else {
@@ -328,6 +345,7 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
   SPTC_HIP(hipMemcpyAsync(host_u64_, ws_.err(), 4, hipMemcpyDeviceToHost, s_));
   record(5);
   SPTC_HIP(hipStreamSynchronize(s_));
+  kt_.collect(res->kstats, SPTC_K_COUNT);
   if (*reinterpret_cast<uint32_t*>(host_u64_) != 0) {
     msg = "radix sort look-back exceeded its spin limit";
     return SPTC_ERR_INTERNAL;

@@ -7,6 +7,7 @@
 
 #include "device.hpp"
 #include "kernels.hpp"
+#include "ktimer.hpp"
 
 namespace sptc {
 
@@ -48,6 +49,7 @@ struct OutChunk {
 
 struct sptc_result_s {
   sptc_result_info info;
+  sptc::KStat kstats[SPTC_K_COUNT];
   sptc::Layout xl, yl;
   sptc::DevBuf<uint64_t> row_key;
   std::vector<sptc::OutChunk> chunks;
@@ -82,6 +84,9 @@ class Engine {
   uint64_t budget() const { return budget_; }
   hipStream_t stream() const { return s_; }
   void record(int i);
+  // Collects per-kernel timings (after the stream is synchronized).
+  void collect_kernel_stats(sptc::KStat* out) { kt_.collect(out, SPTC_K_COUNT); }
+  KTimer& ktimer() { return kt_; }
   PhaseTimes times;
   std::string msg;  // last error detail
 
@@ -92,6 +97,7 @@ class Engine {
   uint64_t budget_ = 0;
   int rb_ = 8;
   SortWorkspace ws_;
+  KTimer kt_;
   hipEvent_t ev_[8];
   bool ev_used_[8] = {false};
   U2* host_total_ = nullptr;

@@ -298,6 +298,8 @@ template <int RB>
 __global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs a) {
   constexpr int R = 1 << RB;
   __shared__ uint64_t s_pref[kExpandTile + 1];
+  __shared__ uint64_t s_key[kExpandTile];
+  __shared__ float s_val[kExpandTile];
   __shared__ uint32_t sh[kMaxFusedPasses * R];
   __shared__ uint64_t s_ja;
   __shared__ double s_sum[kBlock / kWave];
@@ -308,6 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs a) {
   for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const uint64_t pb = a.P0 + tile * kExpandTile;
     const uint64_t pe = pb + kExpandTile < a.P1 ? pb + kExpandTile : a.P1;
+    const uint32_t cnt = (uint32_t)(pe - pb);
     __syncthreads();
     if (threadIdx.x == 0) {
       // last j in [A0, A1) with act_prefix[j] <= pb
@@ -324,25 +327,53 @@ __global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs a) {
     const uint32_t m = (uint32_t)(avail < kExpandTile + 1 ? avail : kExpandTile + 1);
     for (uint32_t k = threadIdx.x; k < m; k += kBlock) s_pref[k] = a.act_prefix[ja + k];
     __syncthreads();
-#pragma unroll 2
-    for (int e = 0; e < kExpandItems; ++e) {
-      const uint64_t p = pb + (uint64_t)e * kBlock + threadIdx.x;
-      if (p >= pe) break;
-      uint32_t lo = 0, hi = m;  // last k with s_pref[k] <= p
+    // Thread t expands products [t*E, t*E+E) of the tile: one LDS search for
+    // the first, then a forward walk over the owning active nonzeros.
+    const uint32_t q0 = threadIdx.x * kExpandItems;
+    if (q0 < cnt) {
+      const uint64_t p0 = pb + q0;
+      uint32_t lo = 0, hi = m;  // last k with s_pref[k] <= p0
       while (lo < hi) {
         uint32_t mid = (lo + hi) >> 1;
-        if (s_pref[mid] <= p) lo = mid + 1; else hi = mid;
+        if (s_pref[mid] <= p0) lo = mid + 1; else hi = mid;
       }
-      const uint32_t k = lo - 1;
-      const uint64_t j = ja + k;
-      const uint64_t y = a.act_ypos[j] + (p - s_pref[k]);
-      const uint64_t yk = a.ykeys[y];
-      const float v = a.act_xval[j] * a.yvals[y];
-      const uint64_t key = shl64((uint64_t)(a.act_row[j] - a.r0), a.ynf) | (yk & a.fmask);
-      a.out_keys[p - a.P0] = key;
-      a.out_vals[p - a.P0] = v;
-      vsum += (double)v;
-      if (a.passes) hist_add<RB>(sh, key, a.passes);
+      uint32_t k = lo - 1;
+      uint64_t j = ja + k;
+      uint64_t ybase = a.act_ypos[j] - s_pref[k];
+      float xv = a.act_xval[j];
+      uint64_t rbits = shl64((uint64_t)(a.act_row[j] - a.r0), a.ynf);
+      uint64_t next = k + 1 < m ? s_pref[k + 1] : ~0ull;
+#pragma unroll
+      for (int e = 0; e < kExpandItems; ++e) {
+        const uint64_t p = p0 + e;
+        if (q0 + e < cnt) {
+          while (p >= next) {
+            ++k;
+            j = ja + k;
+            ybase = a.act_ypos[j] - s_pref[k];
+            xv = a.act_xval[j];
+            rbits = shl64((uint64_t)(a.act_row[j] - a.r0), a.ynf);
+            next = k + 1 < m ? s_pref[k + 1] : ~0ull;
+          }
+          const uint64_t y = ybase + p;
+          const float v = xv * a.yvals[y];
+          s_key[q0 + e] = rbits | (a.ykeys[y] & a.fmask);
+          s_val[q0 + e] = v;
+          vsum += (double)v;
+        }
+      }
+    }
+    __syncthreads();
+    // Coalesced stores of the staged tile, and the digit histograms.
+    const uint64_t obase = pb - a.P0;
+    for (uint32_t i = threadIdx.x; i < kExpandTile; i += kBlock) {
+      const bool valid = i < cnt;
+      const uint64_t key = valid ? s_key[i] : 0;
+      if (valid) {
+        a.out_keys[obase + i] = key;
+        a.out_vals[obase + i] = s_val[i];
+      }
+      if (a.passes) hist_add_wave<RB>(sh, key, valid, a.passes);
     }
   }
   __syncthreads();
@@ -402,13 +433,75 @@ void run_reduce_count(const uint64_t* keys, uint64_t n, U2* tmp, hipStream_t s) 
   run_scan(op, n, tmp, s, /*emit=*/false);
 }
 
+// Emit for a tile of kScanTile sorted products (same tiling as the count
+// pass, whose scanned tile sums give each tile's first output slot): the
+// tile is staged in LDS with coalesced loads, thread t owns items
+// [t*8, t*8+8), each head sums its run (continuing past the tile end in
+// global memory when the run crosses it), and outputs are staged in LDS and
+// stored coalesced.
+__global__ __launch_bounds__(kBlock) void k_reduce_emit(ReduceOp op, const U2* tile_offs) {
+  __shared__ uint64_t s_k[kScanTile];
+  __shared__ float s_v[kScanTile];
+  __shared__ uint64_t s_ocol[kScanTile];
+  __shared__ float s_oval[kScanTile];
+  __shared__ uint32_t s_orow[kScanTile];
+  __shared__ U2 scratch[kBlock / kWave];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  const uint32_t cnt = (uint32_t)(op.n - base < kScanTile ? op.n - base : kScanTile);
+  for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) {
+    s_k[i] = op.keys[base + i];
+    s_v[i] = op.vals[base + i];
+  }
+  const uint64_t prev_key = base ? op.keys[base - 1] : 0;
+  __syncthreads();
+  const uint32_t q0 = threadIdx.x * kScanItems;
+  uint32_t heads = 0;
+#pragma unroll
+  for (int e = 0; e < kScanItems; ++e) {
+    const uint32_t i = q0 + e;
+    if (i < cnt) {
+      const uint64_t pk = i ? s_k[i - 1] : prev_key;
+      heads += (base + i == 0 || s_k[i] != pk) ? 1u : 0u;
+    }
+  }
+  U2 tot;
+  uint32_t o = (uint32_t)block_excl_scan_u2(mk2(heads, 0), scratch, &tot).a;
+  const uint32_t nout = (uint32_t)tot.a;
+#pragma unroll 1
+  for (int e = 0; e < kScanItems; ++e) {
+    const uint32_t i = q0 + e;
+    if (i >= cnt) break;
+    const uint64_t k = s_k[i];
+    const uint64_t pk = i ? s_k[i - 1] : prev_key;
+    if (!(base + i == 0 || k != pk)) continue;
+    KahanF32 acc;
+    acc.add(s_v[i]);
+    uint32_t jj = i + 1;
+    while (jj < cnt && s_k[jj] == k) acc.add(s_v[jj++]);
+    if (jj == cnt) {
+      for (uint64_t g = base + cnt; g < op.n && op.keys[g] == k; ++g) acc.add(op.vals[g]);
+    }
+    s_ocol[o] = k & op.fmask;
+    s_orow[o] = (uint32_t)(op.r0 + shr64(k, op.ynf));
+    s_oval[o] = acc.value();
+    ++o;
+  }
+  __syncthreads();
+  const uint64_t obase = tile_offs[blockIdx.x].a;
+  for (uint32_t i = threadIdx.x; i < nout; i += kBlock) {
+    op.out_row[obase + i] = s_orow[i];
+    op.out_col[obase + i] = s_ocol[i];
+    op.out_val[obase + i] = s_oval[i];
+  }
+}
+
 void run_reduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32_t* out_row,
                      uint64_t* out_col, float* out_val, uint64_t r0, int ynf,
                      uint64_t fmask, const U2* tmp, hipStream_t s) {
   if (!n) return;
   ReduceOp op{keys, vals, n, out_row, out_col, out_val, r0, ynf, fmask};
-  hipLaunchKernelGGL(k_scan_emit<ReduceOp>, dim3((uint32_t)scan_tmp_count(n)), dim3(kBlock),
-                     0, s, op, n, tmp);
+  hipLaunchKernelGGL(k_reduce_emit, dim3((uint32_t)scan_tmp_count(n)), dim3(kBlock), 0, s, op,
+                     tmp);
 }
 
 // --------------------------------------------------------------------------

@@ -1,0 +1,66 @@
+// Per-launch HIP-event timing of hot kernels, grouped by kernel class.
+#pragma once
+#include <vector>
+
+#include "common.hpp"
+
+namespace sptc {
+
+struct KStat {
+  uint64_t launches = 0;
+  double ms = 0;
+  double bytes = 0;  // algorithmic HBM bytes
+};
+
+class KTimer {
+ public:
+  ~KTimer() {
+    for (auto e : ev_) (void)hipEventDestroy(e);
+  }
+  bool on = false;
+  // Brackets one launch on stream s; returns a token for end().
+  int begin(hipStream_t s) {
+    if (!on) return -1;
+    const int slot = (int)used_;
+    if (ev_.size() < 2 * (used_ + 1)) {
+      hipEvent_t a, b;
+      SPTC_HIP(hipEventCreate(&a));
+      SPTC_HIP(hipEventCreate(&b));
+      ev_.push_back(a);
+      ev_.push_back(b);
+    }
+    SPTC_HIP(hipEventRecord(ev_[2 * slot], s));
+    ++used_;
+    cls_.push_back(-1);
+    bytes_.push_back(0);
+    return slot;
+  }
+  void end(int tok, int cls, double bytes, hipStream_t s) {
+    if (tok < 0) return;
+    SPTC_HIP(hipEventRecord(ev_[2 * tok + 1], s));
+    cls_[tok] = cls;
+    bytes_[tok] = bytes;
+  }
+  // Call after the stream has been synchronized.
+  void collect(KStat* out, int nclasses) {
+    for (size_t i = 0; i < used_; ++i) {
+      if (cls_[i] < 0 || cls_[i] >= nclasses) continue;
+      float ms = 0;
+      SPTC_HIP(hipEventElapsedTime(&ms, ev_[2 * i], ev_[2 * i + 1]));
+      out[cls_[i]].launches += 1;
+      out[cls_[i]].ms += ms;
+      out[cls_[i]].bytes += bytes_[i];
+    }
+    used_ = 0;
+    cls_.clear();
+    bytes_.clear();
+  }
+
+ private:
+  std::vector<hipEvent_t> ev_;
+  std::vector<int> cls_;
+  std::vector<double> bytes_;
+  size_t used_ = 0;
+};
+
+}  // namespace sptc

@@ -309,7 +309,7 @@ void histogram(const SortWorkspace& ws, const uint64_t* keys, uint64_t n,
 // the pass count is even, else in (ws.alt_keys, ws.alt_vals); the returned
 // flag says which (true = alt).
 bool sort_after_histogram(SortWorkspace& ws, uint64_t* keys, float* vals,
-                          uint64_t n, int nbits, hipStream_t s) {
+                          uint64_t n, int nbits, hipStream_t s, KTimer* kt, int cls) {
   const int rb = ws.rb;
   const int passes = radix_passes(nbits, rb);
   if (!n || !passes) return false;
@@ -331,6 +331,7 @@ bool sort_after_histogram(SortWorkspace& ws, uint64_t* keys, float* vals,
       fill_u64(ws.status.get(), 0, ws.max_tiles * R, s);
       ws.epoch = 1;
     }
+    const int tok = kt ? kt->begin(s) : -1;
     if (rb == 8)
       hipLaunchKernelGGL((k_onesweep<8, ipt_for<8>()>), dim3((uint32_t)tiles),
                          dim3(kBlock), 0, s, ka, va, kb, vb, n, p * rb,
@@ -341,6 +342,7 @@ bool sort_after_histogram(SortWorkspace& ws, uint64_t* keys, float* vals,
                          dim3(kBlock), 0, s, ka, va, kb, vb, n, p * rb,
                          ws.base.get() + p * R, ws.status.get(), ws.counters.get() + p,
                          ws.epoch, ws.err());
+    if (kt) kt->end(tok, cls, 24.0 * (double)n, s);
     std::swap(ka, kb);
     std::swap(va, vb);
   }

@@ -2,6 +2,7 @@
 #pragma once
 #include "common.hpp"
 #include "device.hpp"
+#include "ktimer.hpp"
 
 namespace sptc {
 
@@ -34,8 +35,10 @@ void histogram(const SortWorkspace& ws, const uint64_t* keys, uint64_t n,
 // Sorts (keys, vals) on bits [0, nbits), ws.hist holding the histograms of
 // all passes.  Returns true when the sorted data ended in (ws.alt_keys,
 // ws.alt_vals) (odd pass count).
+// kt (optional) times every pass under kernel class `cls`.
 bool sort_after_histogram(SortWorkspace& ws, uint64_t* keys, float* vals,
-                          uint64_t n, int nbits, hipStream_t s);
+                          uint64_t n, int nbits, hipStream_t s, KTimer* kt = nullptr,
+                          int cls = 0);
 
 // Device-side histogram accumulation for producers: call once per key with
 // an LDS array of passes*R uint32 cleared by the block, then flush.
@@ -45,6 +48,29 @@ __device__ __forceinline__ void hist_add(uint32_t* sh, uint64_t key, int passes)
   for (int p = 0; p < passes; ++p)
     atomicAdd(&sh[p * R + ((uint32_t)(key >> (p * RB)) & (R - 1))], 1u);
 }
+// Wave-aggregated variant: when every valid lane of the wave has the same
+// digit (typical for the high, row-index digits of product keys) one lane
+// adds the whole count; otherwise each lane adds one.  All lanes of the wave
+// must call it together.
+template <int RB>
+__device__ __forceinline__ void hist_add_wave(uint32_t* sh, uint64_t key, bool valid,
+                                             int passes) {
+  constexpr uint32_t R = 1u << RB;
+  const uint64_t vmask = __ballot(valid);
+  if (!vmask) return;
+  const int first = __builtin_ctzll(vmask);
+  for (int p = 0; p < passes; ++p) {
+    const uint32_t d = (uint32_t)(key >> (p * RB)) & (R - 1);
+    const uint32_t d0 = __shfl(d, first, 64);
+    const uint64_t same = __ballot(valid && d == d0);
+    if (same == vmask) {
+      if ((int)__lane_id() == first) atomicAdd(&sh[p * R + d0], (uint32_t)__popcll(vmask));
+    } else if (valid) {
+      atomicAdd(&sh[p * R + d], 1u);
+    }
+  }
+}
+
 template <int RB>
 __device__ __forceinline__ void hist_flush(const uint32_t* sh, uint64_t* hist,
                                            int passes) {

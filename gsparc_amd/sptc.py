@@ -39,7 +39,11 @@ class _COO(Structure):
 class _Opts(Structure):
     _fields_ = [("dense_accumulator", c_int32), ("count_only", c_int32),
                 ("mem_budget_bytes", c_uint64), ("radix_bits", c_int32),
-                ("verbose", c_int32)]
+                ("verbose", c_int32), ("kernel_timing", c_int32)]
+
+
+KERNEL_CLASSES = ["linearize", "sort_input", "match", "expand", "sort_products", "reduce",
+                  "dense"]
 
 
 class _Info(Structure):
@@ -92,6 +96,11 @@ def lib():
         L.sptc_last_error.restype = c_char_p
         L.sptc_trim_cache.argtypes = []
         L.sptc_trim_cache.restype = None
+        L.sptc_result_kernel_stats.argtypes = [c_void_p, c_int32, POINTER(c_uint64),
+                                               POINTER(c_double), POINTER(c_double)]
+        L.sptc_result_kernel_stats.restype = c_int32
+        L.sptc_pool_stats.argtypes = [POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64)]
+        L.sptc_pool_stats.restype = None
         for name in ("sptc_contract", "sptc_result_get_info", "sptc_result_get_chunk",
                      "sptc_result_copy_to_host", "sptc_result_decode", "sptc_layout",
                      "sptc_result_row_keys", "sptc_result_row_range", "sptc_result_copy_range",
@@ -199,6 +208,16 @@ class Result:
         check(lib().sptc_result_copy_to_host(self._h, k.ctypes.data, None, None, v.ctypes.data))
         return k, v
 
+    def kernel_stats(self):
+        """{class: (launches, total_ms, algorithmic_bytes)} (needs
+        kernel_timing=True)."""
+        out = {}
+        for i, name in enumerate(KERNEL_CLASSES):
+            n, ms, b = c_uint64(), c_double(), c_double()
+            check(lib().sptc_result_kernel_stats(self._h, i, byref(n), byref(ms), byref(b)))
+            out[name] = (n.value, ms.value, b.value)
+        return out
+
     def row_keys(self):
         rk = np.empty(int(self.info.nrows), np.uint64)
         check(lib().sptc_result_row_keys(self._h, rk.ctypes.data))
@@ -231,7 +250,8 @@ class Result:
 
 
 def SpTC(X, Y, cmodes_x, cmodes_y, dense_accumulator=0, count_only=False,
-         mem_budget_bytes=0, radix_bits=0, verbose=False, stream=None, on_device=False):
+         mem_budget_bytes=0, radix_bits=0, verbose=False, stream=None, on_device=False,
+         kernel_timing=False):
     """Z = X x_{cmodes_x, cmodes_y} Y on the GPU; returns a :class:`Result`.
 
     X and Y are COOTensor (host numpy arrays) or, with on_device=True,
@@ -245,12 +265,18 @@ def SpTC(X, Y, cmodes_x, cmodes_y, dense_accumulator=0, count_only=False,
     cx = (c_int32 * max(1, nc))(*cmodes_x)
     cy = (c_int32 * max(1, nc))(*cmodes_y)
     o = _Opts(int(dense_accumulator), int(bool(count_only)), int(mem_budget_bytes),
-              int(radix_bits), int(bool(verbose)))
+              int(radix_bits), int(bool(verbose)), int(bool(kernel_timing)))
     out = c_void_p()
     st = L.sptc_contract(byref(hx.s), byref(hy.s), nc, cx, cy, byref(o),
                          c_void_p(stream or 0), byref(out))
     check(st)
     return Result(out.value)
+
+
+def pool_stats():
+    live, res, n = c_uint64(), c_uint64(), c_uint64()
+    lib().sptc_pool_stats(byref(live), byref(res), byref(n))
+    return {"live_bytes": live.value, "reserved_bytes": res.value, "hip_mallocs": n.value}
 
 
 def device_info():

@@ -62,9 +62,13 @@ typedef struct {
 typedef struct {
   int32_t dense_accumulator;
   int32_t count_only;        /* 1: return nnz and checksums, no entries   */
-  uint64_t mem_budget_bytes; /* 0: free device memory minus 1 GiB         */
-  int32_t radix_bits;        /* 0: autotune; else 4..11                   */
+  uint64_t mem_budget_bytes; /* device bytes this call may hold at once
+                                (operands, workspaces, output); 0: free
+                                device memory minus 1 GiB               */
+  int32_t radix_bits;        /* 0: autotune; else 8 or 11                 */
   int32_t verbose;           /* 1: phase log on stderr                    */
+  int32_t kernel_timing;     /* 1: HIP-event time every hot-kernel launch
+                                (see sptc_result_kernel_stats)          */
 } sptc_opts;
 
 /* Result of a contraction (device-resident).  Entry i has output free
@@ -138,6 +142,29 @@ sptc_status sptc_result_copy_range(const sptc_result* r, uint64_t begin,
  * host arrays of nnz each), X free modes ascending then Y free modes. */
 sptc_status sptc_result_decode(const sptc_result* r, uint32_t* const* mode_idx,
                                float* vals);
+
+/* Kernel classes timed when opts.kernel_timing is set. */
+typedef enum {
+  SPTC_K_LINEARIZE = 0,   /* linearize + fused digit histograms           */
+  SPTC_K_SORT_INPUT = 1,  /* radix pass over operand keys                 */
+  SPTC_K_MATCH = 2,       /* index match (reduce + emit)                  */
+  SPTC_K_EXPAND = 3,      /* ESC product expansion                        */
+  SPTC_K_SORT_PROD = 4,   /* radix pass over product keys                 */
+  SPTC_K_REDUCE = 5,      /* reduce-by-key (count + emit)                 */
+  SPTC_K_DENSE = 6,       /* dense per-row accumulation                   */
+  SPTC_K_COUNT = 7
+} sptc_kernel_class;
+
+/* Launch count, summed HIP-event time (ms) and algorithmic HBM bytes of one
+ * kernel class over this contraction (zeros unless opts.kernel_timing). */
+sptc_status sptc_result_kernel_stats(const sptc_result* r, int32_t cls,
+                                     uint64_t* launches, double* total_ms,
+                                     double* alg_bytes);
+
+/* Device arena statistics: bytes held by live buffers, bytes reserved from
+ * the HIP runtime, and hipMalloc calls so far. */
+void sptc_pool_stats(uint64_t* live_bytes, uint64_t* reserved_bytes,
+                     uint64_t* hip_mallocs);
 
 void sptc_result_free(sptc_result* r);
 
