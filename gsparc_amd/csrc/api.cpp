@@ -375,10 +375,10 @@ sptc_status sptc_sort_pairs(uint64_t* keys, float* vals, uint64_t n, int32_t nbi
                             int32_t radix_bits, void* stream) {
   if (nbits < 0 || nbits > 64 || (n && (!keys || !vals)))
     return fail(SPTC_ERR_BAD_ARGS, "sptc_sort_pairs: bad arguments");
-  if (radix_bits != 0 && radix_bits != 8 && radix_bits != 11)
-    return fail(SPTC_ERR_BAD_ARGS, "radix_bits must be 0, 8 or 11");
+  if (radix_bits != 0 && !radix_bits_supported(radix_bits))
+    return fail(SPTC_ERR_BAD_ARGS, "radix_bits must be 0 or 8..11");
   if (!device_ok()) return fail(SPTC_ERR_NO_DEVICE, "no GPU");
-  const int rb = radix_bits ? radix_bits : 8;
+  const int rb = radix_bits ? radix_bits : choose_radix_bits(nbits);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   try {
     SortWorkspace ws;

@@ -9,6 +9,16 @@ namespace {
 constexpr size_t kAlign = 256;
 constexpr size_t kSlabMin = 8ull << 30;  // new slabs are at least 8 GiB
 size_t align_up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
+// Size classes for large blocks (>= 64 MiB): round up to 1/8 of the leading
+// power of two, so the slightly different sizes that successive calls ask
+// for land in the same class and reuse each other's ranges.
+size_t size_class(size_t b) {
+  b = align_up(b ? b : 1);
+  if (b < (64ull << 20)) return b;
+  int lg = 63 - __builtin_clzll(b);
+  size_t step = 1ull << (lg - 3);
+  return (b + step - 1) / step * step;
+}
 }  // namespace
 
 DevicePool& DevicePool::get() {
@@ -74,7 +84,7 @@ int DevicePool::slab_of(const char* p) const {
 }
 
 void* DevicePool::alloc(size_t bytes) {
-  const size_t want = align_up(bytes ? bytes : 1);
+  const size_t want = size_class(bytes);
   std::lock_guard<std::mutex> g(mu_);
   void* p = nullptr;
   if (take_locked(want, &p)) return p;

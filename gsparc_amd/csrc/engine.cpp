@@ -69,7 +69,7 @@ Engine::Engine(hipStream_t s, const sptc_opts& o) : s_(s), o_(o) {
     const uint64_t margin = 1ull << 30;
     budget_ = live + (fr > margin ? fr - margin : 0);
   }
-  rb_ = (o.radix_bits == 8 || o.radix_bits == 11) ? o.radix_bits : 8;
+  rb_ = radix_bits_supported(o.radix_bits) ? o.radix_bits : 0;  // 0: per sort
   kt_.on = o.kernel_timing != 0;
 }
 
@@ -130,12 +130,13 @@ void Engine::prepare(const StagedCOO& t, const Layout& lay, Operand* out) {
   }
   SPTC_HIP(hipMemcpyAsync(out->vals.get(), t.val, n * sizeof(float),
                           hipMemcpyDeviceToDevice, s_));
-  const int passes = radix_passes(lay.nbits, rb_);
-  ws_.ensure(n, rb_, s_);
+  const int rb = rb_ ? rb_ : choose_radix_bits(lay.nbits);
+  const int passes = radix_passes(lay.nbits, rb);
+  ws_.ensure(n, rb, s_);
   ws_.prepare(passes, s_);
   int tok = kt_.begin(s_);
   launch_linearize(a, t.idx_bytes, n, out->keys.get(), ws_.hist.get(),
-                   passes <= kMaxFusedPasses ? passes : 0, rb_, s_);
+                   passes <= kMaxFusedPasses ? passes : 0, rb, s_);
   kt_.end(tok, SPTC_K_LINEARIZE, (double)n * (lay.order * t.idx_bytes + 8), s_);
   if (passes > kMaxFusedPasses) histogram(ws_, out->keys.get(), n, passes, s_);
   if (sort_after_histogram(ws_, out->keys.get(), out->vals.get(), n, lay.nbits, s_, &kt_,
@@ -159,6 +160,37 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
   res->xl = xl;
   res->yl = yl;
 
+  // ---- Y free-key rank compression (shorter product keys) ----------------
+  // kyf: width of the column field inside Y keys and product keys from here
+  // on; ytab maps a rank back to the Y free key when the field is a rank.
+  int kyf = ynf;
+  const uint64_t* ykeys = Y.keys.get();
+  DevBuf<uint64_t> ykeys_rank, ytab;
+  if (ynf > 0 && ny > 1) {
+    DevBuf<uint64_t> fk(ny);
+    DevBuf<float> fv(ny);
+    launch_yfree_extract(Y.keys.get(), ny, low_mask(ynf), fk.get(), s_);
+    const int rbf = rb_ ? rb_ : choose_radix_bits(ynf);
+    ws_.ensure(ny, rbf, s_);
+    ws_.prepare(radix_passes(ynf, rbf), s_);
+    histogram(ws_, fk.get(), ny, radix_passes(ynf, rbf), s_);
+    const bool alt = sort_after_histogram(ws_, fk.get(), fv.get(), ny, ynf, s_);
+    ytab.reset(ny);
+    DevBuf<U2> tmp(scan_tmp_count(ny) + 1);
+    run_unique(alt ? ws_.alt_keys.get() : fk.get(), ny, ytab.get(), tmp.get(), s_);
+    const uint64_t U = read_total(tmp.get() + scan_tmp_count(ny)).a;
+    const int yrb = mode_bits(U);
+    if (yrb < ynf) {
+      ykeys_rank.reset(ny);
+      launch_yrank(Y.keys.get(), ny, ynf, low_mask(ynf), ytab.get(), U, yrb, ykeys_rank.get(),
+                   s_);
+      ykeys = ykeys_rank.get();
+      kyf = yrb;
+    } else {
+      ytab.release();
+    }
+  }
+
   // ---- (c) index match ---------------------------------------------------
   DevBuf<uint64_t> tlo, thi, ukey, ustart;
   YTable tab;
@@ -169,7 +201,7 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
     thi.reset(K);
     fill_u64(tlo.get(), 0, K, s_);
     fill_u64(thi.get(), 0, K, s_);
-    launch_ytable_direct(Y.keys.get(), ny, ynf, K, tlo.get(), thi.get(), s_);
+    launch_ytable_direct(ykeys, ny, kyf, K, tlo.get(), thi.get(), s_);
     tab.direct = 1;
     tab.lo = tlo.get();
     tab.hi = thi.get();
@@ -177,7 +209,7 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
     ukey.reset(ny + 1);
     ustart.reset(ny + 1);
     DevBuf<U2> tmp(scan_tmp_count(ny) + 1);
-    run_yunique(Y.keys.get(), ny, ynf, ukey.get(), ustart.get(), tmp.get(), s_);
+    run_yunique(ykeys, ny, kyf, ukey.get(), ustart.get(), tmp.get(), s_);
     tab.direct = 0;
     tab.ukey = ukey.get();
     tab.ustart = ustart.get();
@@ -237,7 +269,7 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
     const uint64_t max_row = host_u64_[0];
     const uint64_t live = DevicePool::get().live_bytes();
     const uint64_t free_b = budget_ > live ? budget_ - live : 0;
-    const uint64_t per_prod = 12 + SortWorkspace::bytes_per_key(rb_) + 1;
+    const uint64_t per_prod = 12 + SortWorkspace::bytes_per_key(10) + 1;
     const uint64_t out_per = o_.count_only ? 0 : 16;
     const uint64_t slack = 64ull << 20;
     uint64_t cap;
@@ -245,12 +277,14 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       cap = (free_b - slack - out_per * ir) / per_prod;
     else
       cap = free_b > slack ? (free_b - slack) / (per_prod + out_per) : 0;
+    // coarse granularity keeps the chunking identical across repeated calls
+    if (cap > (1ull << 26)) cap &= ~((1ull << 26) - 1);
     cap = std::min<uint64_t>(cap, std::min<uint64_t>(ir, 4000000000ull));
     if (cap < max_row) {
       msg = "one X row's products exceed the memory budget";
       return SPTC_ERR_OOM_BUDGET;
     }
-    const uint64_t rmax = ynf >= 32 ? (ynf >= 64 ? 1 : (1ull << (64 - ynf))) : (1ull << 32);
+    const uint64_t rmax = kyf >= 32 ? (kyf >= 64 ? 1 : (1ull << (64 - kyf))) : (1ull << 32);
     uint64_t r0 = 0;
     while (r0 < R) {
       launch_chunk_end(row_prod.get(), row_act.get(), R, r0, rmax, cap, scratch.get(), s_);
@@ -275,15 +309,22 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
     DevBuf<uint64_t> kA(chunk_cap);
     DevBuf<float> vA(chunk_cap);
     DevBuf<U2> tmp(scan_tmp_count(chunk_cap) + 1);
-    ws_.ensure(chunk_cap, rb_, s_);
-    const uint64_t fmask = low_mask(ynf);
+    int rb_max = 8;
+    for (size_t c = 1; c < bounds_r.size(); ++c) {
+      const int b = mode_bits(bounds_r[c] - bounds_r[c - 1]) + kyf;
+      rb_max = std::max(rb_max, rb_ ? rb_ : choose_radix_bits(b));
+    }
+    ws_.ensure(chunk_cap, rb_max, s_);
+    const uint64_t fmask = low_mask(kyf);
     for (size_t c = 1; c < bounds_r.size(); ++c) {
       const uint64_t r0c = bounds_r[c - 1], r1c = bounds_r[c];
       const uint64_t P0 = bounds_p[c - 1], P1 = bounds_p[c];
       const uint64_t n = P1 - P0;
       if (!n) continue;
-      const int bits = mode_bits(r1c - r0c) + ynf;
-      const int passes = radix_passes(bits, rb_);
+      const int bits = mode_bits(r1c - r0c) + kyf;
+      const int rb = rb_ ? rb_ : choose_radix_bits(bits);
+      const int passes = radix_passes(bits, rb);
+      ws_.ensure(n, rb, s_);
       ws_.prepare(passes, s_);
       ExpandArgs ea;
       ea.act_prefix = act_prefix.get();
@@ -295,9 +336,9 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       ea.P0 = P0;
       ea.P1 = P1;
       ea.r0 = (uint32_t)r0c;
-      ea.ykeys = Y.keys.get();
+      ea.ykeys = ykeys;
       ea.yvals = Y.vals.get();
-      ea.ynf = ynf;
+      ea.ynf = kyf;
       ea.fmask = fmask;
       ea.out_keys = kA.get();
       ea.out_vals = vA.get();
@@ -305,7 +346,7 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       ea.passes = passes <= kMaxFusedPasses ? passes : 0;
       ea.vsum = vsum.get();
       int tok = kt_.begin(s_);
-      launch_expand(ea, rb_, s_);
+      launch_expand(ea, rb, s_);
       // product (key, value) written, matched Y (key, value) read
       kt_.end(tok, SPTC_K_EXPAND, 24.0 * (double)n, s_);
       if (passes > kMaxFusedPasses) histogram(ws_, kA.get(), n, passes, s_);
@@ -332,8 +373,8 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       oc.col.reset(out_n);
       oc.val.reset(out_n);
       tok = kt_.begin(s_);
-      run_reduce_emit(sk, sv, n, oc.row.get(), oc.col.get(), oc.val.get(), r0c, ynf, fmask,
-                      tmp.get(), s_);
+      run_reduce_emit(sk, sv, n, oc.row.get(), oc.col.get(), oc.val.get(), r0c, kyf, fmask,
+                      ytab.size() ? ytab.get() : nullptr, tmp.get(), s_);
       kt_.end(tok, SPTC_K_REDUCE, 12.0 * (double)n + 16.0 * (double)out_n, s_);
       res->chunks.push_back(std::move(oc));
     }

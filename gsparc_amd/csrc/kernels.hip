@@ -71,17 +71,25 @@ void launch_linearize(const LinArgs& a, int idx_bytes, uint64_t n, uint64_t* key
   if (!n) return;
   uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(n, kBlock * 4), 2048);
   if (passes > kMaxFusedPasses) passes = 0;  // caller falls back to histogram()
+#define SPTC_LIN(T, RBV)                                                                   \
+  hipLaunchKernelGGL((k_linearize<T, RBV>), dim3(grid), dim3(kBlock), 0, s, a, n, keys, hist, \
+                     passes)
   if (idx_bytes == 4) {
-    if (rb == 8)
-      hipLaunchKernelGGL((k_linearize<uint32_t, 8>), dim3(grid), dim3(kBlock), 0, s, a, n, keys, hist, passes);
-    else
-      hipLaunchKernelGGL((k_linearize<uint32_t, 11>), dim3(grid), dim3(kBlock), 0, s, a, n, keys, hist, passes);
+    switch (rb) {
+      case 9: SPTC_LIN(uint32_t, 9); break;
+      case 10: SPTC_LIN(uint32_t, 10); break;
+      case 11: SPTC_LIN(uint32_t, 11); break;
+      default: SPTC_LIN(uint32_t, 8); break;
+    }
   } else {
-    if (rb == 8)
-      hipLaunchKernelGGL((k_linearize<uint64_t, 8>), dim3(grid), dim3(kBlock), 0, s, a, n, keys, hist, passes);
-    else
-      hipLaunchKernelGGL((k_linearize<uint64_t, 11>), dim3(grid), dim3(kBlock), 0, s, a, n, keys, hist, passes);
+    switch (rb) {
+      case 9: SPTC_LIN(uint64_t, 9); break;
+      case 10: SPTC_LIN(uint64_t, 10); break;
+      case 11: SPTC_LIN(uint64_t, 11); break;
+      default: SPTC_LIN(uint64_t, 8); break;
+    }
   }
+#undef SPTC_LIN
 }
 
 // --------------------------------------------------------------------------
@@ -228,6 +236,67 @@ void run_exclusive_scan(const uint64_t* in, uint64_t* out, uint64_t n, U2* tmp,
                         hipStream_t s) {
   ExclScanOp op{in, out, n};
   run_scan(op, n, tmp, s);
+}
+
+// --------------------------------------------------------------------------
+// Y free-key rank compression.  Y's free field (ynf bits) usually takes far
+// fewer distinct values than 2^ynf; replacing it by its rank among Y's
+// distinct free keys shortens every product key (fewer radix passes).  The
+// map is monotone, so Y stays sorted by (c, rank).
+
+__global__ __launch_bounds__(kBlock) void k_yfree_extract(const uint64_t* __restrict__ ykeys,
+                                                          uint64_t ny, uint64_t fmask,
+                                                          uint64_t* __restrict__ out) {
+  uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (; i < ny; i += stride) out[i] = ykeys[i] & fmask;
+}
+
+struct UniqueOp {
+  const uint64_t* keys;  // sorted
+  uint64_t* out;
+  __device__ U2 value(uint64_t i) const { return mk2(i == 0 || keys[i] != keys[i - 1], 0); }
+  __device__ void emit(uint64_t i, U2 excl, U2 v) const {
+    if (v.a) out[excl.a] = keys[i];
+  }
+};
+
+__global__ __launch_bounds__(kBlock) void k_yrank(const uint64_t* __restrict__ ykeys, uint64_t ny,
+                                                  int ynf, uint64_t fmask,
+                                                  const uint64_t* __restrict__ ytab, uint64_t U,
+                                                  int yrb, uint64_t* __restrict__ out) {
+  uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (; i < ny; i += stride) {
+    const uint64_t k = ykeys[i];
+    const uint64_t f = k & fmask;
+    uint64_t lo = 0, hi = U;
+    while (lo < hi) {
+      uint64_t mid = (lo + hi) >> 1;
+      if (ytab[mid] < f) lo = mid + 1; else hi = mid;
+    }
+    out[i] = shl64(shr64(k, ynf), yrb) | lo;
+  }
+}
+
+void launch_yfree_extract(const uint64_t* ykeys, uint64_t ny, uint64_t fmask, uint64_t* out,
+                          hipStream_t s) {
+  if (!ny) return;
+  uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(ny, kBlock), 4096);
+  hipLaunchKernelGGL(k_yfree_extract, dim3(grid), dim3(kBlock), 0, s, ykeys, ny, fmask, out);
+}
+
+void run_unique(const uint64_t* sorted, uint64_t n, uint64_t* out, U2* tmp, hipStream_t s) {
+  UniqueOp op{sorted, out};
+  run_scan(op, n, tmp, s);
+}
+
+void launch_yrank(const uint64_t* ykeys, uint64_t ny, int ynf, uint64_t fmask,
+                  const uint64_t* ytab, uint64_t U, int yrb, uint64_t* out, hipStream_t s) {
+  if (!ny) return;
+  uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(ny, kBlock), 4096);
+  hipLaunchKernelGGL(k_yrank, dim3(grid), dim3(kBlock), 0, s, ykeys, ny, ynf, fmask, ytab, U,
+                     yrb, out);
 }
 
 // --------------------------------------------------------------------------
@@ -392,10 +461,12 @@ void launch_expand(const ExpandArgs& a, int rb, hipStream_t s) {
   const uint64_t nprod = a.P1 - a.P0;
   if (!nprod) return;
   uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(nprod, kExpandTile), 2048);
-  if (rb == 8)
-    hipLaunchKernelGGL(k_expand<8>, dim3(grid), dim3(kBlock), 0, s, a);
-  else
-    hipLaunchKernelGGL(k_expand<11>, dim3(grid), dim3(kBlock), 0, s, a);
+  switch (rb) {
+    case 9: hipLaunchKernelGGL(k_expand<9>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 10: hipLaunchKernelGGL(k_expand<10>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 11: hipLaunchKernelGGL(k_expand<11>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_expand<8>, dim3(grid), dim3(kBlock), 0, s, a); break;
+  }
 }
 
 // --------------------------------------------------------------------------
@@ -410,8 +481,9 @@ struct ReduceOp {
   uint64_t* out_col;
   float* out_val;
   uint64_t r0;
-  int ynf;
-  uint64_t fmask;
+  int ynf;             // width of the key's column field
+  uint64_t fmask;      // its mask
+  const uint64_t* ytab;  // rank -> Y free key (NULL: the field is the key)
   __device__ U2 value(uint64_t i) const {
     return mk2(i == 0 || keys[i] != keys[i - 1], 0);
   }
@@ -429,7 +501,7 @@ struct ReduceOp {
 };
 
 void run_reduce_count(const uint64_t* keys, uint64_t n, U2* tmp, hipStream_t s) {
-  ReduceOp op{keys, nullptr, n, nullptr, nullptr, nullptr, 0, 0, 0};
+  ReduceOp op{keys, nullptr, n, nullptr, nullptr, nullptr, 0, 0, 0, nullptr};
   run_scan(op, n, tmp, s, /*emit=*/false);
 }
 
@@ -481,7 +553,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_emit(ReduceOp op, const U2* t
     if (jj == cnt) {
       for (uint64_t g = base + cnt; g < op.n && op.keys[g] == k; ++g) acc.add(op.vals[g]);
     }
-    s_ocol[o] = k & op.fmask;
+    s_ocol[o] = op.ytab ? op.ytab[k & op.fmask] : (k & op.fmask);
     s_orow[o] = (uint32_t)(op.r0 + shr64(k, op.ynf));
     s_oval[o] = acc.value();
     ++o;
@@ -497,9 +569,9 @@ __global__ __launch_bounds__(kBlock) void k_reduce_emit(ReduceOp op, const U2* t
 
 void run_reduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32_t* out_row,
                      uint64_t* out_col, float* out_val, uint64_t r0, int ynf,
-                     uint64_t fmask, const U2* tmp, hipStream_t s) {
+                     uint64_t fmask, const uint64_t* ytab, const U2* tmp, hipStream_t s) {
   if (!n) return;
-  ReduceOp op{keys, vals, n, out_row, out_col, out_val, r0, ynf, fmask};
+  ReduceOp op{keys, vals, n, out_row, out_col, out_val, r0, ynf, fmask, ytab};
   hipLaunchKernelGGL(k_reduce_emit, dim3((uint32_t)scan_tmp_count(n)), dim3(kBlock), 0, s, op,
                      tmp);
 }

@@ -76,7 +76,12 @@ void launch_expand(const ExpandArgs& a, int rb, hipStream_t s);
 void run_reduce_count(const uint64_t* keys, uint64_t n, U2* tmp, hipStream_t s);
 void run_reduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32_t* out_row,
                      uint64_t* out_col, float* out_val, uint64_t r0, int ynf,
-                     uint64_t fmask, const U2* tmp, hipStream_t s);
+                     uint64_t fmask, const uint64_t* ytab, const U2* tmp, hipStream_t s);
+void launch_yfree_extract(const uint64_t* ykeys, uint64_t ny, uint64_t fmask, uint64_t* out,
+                          hipStream_t s);
+void run_unique(const uint64_t* sorted, uint64_t n, uint64_t* out, U2* tmp, hipStream_t s);
+void launch_yrank(const uint64_t* ykeys, uint64_t ny, int ynf, uint64_t fmask,
+                  const uint64_t* ytab, uint64_t U, int yrb, uint64_t* out, hipStream_t s);
 void launch_pack_keys(const uint32_t* row, const uint64_t* col, const uint64_t* row_key,
                       uint64_t n, int ynf, uint64_t* out, hipStream_t s);
 void launch_gather_rowkey(const uint32_t* row, const uint64_t* row_key, uint64_t n,

@@ -99,11 +99,11 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(
   constexpr int R = 1 << RB;
   constexpr int TILE = kBlock * IPT;
   constexpr int NW = kBlock / kWave;
-  __shared__ uint32_t s_cnt[NW][R];     // per-wave running counts, then prefixes
+  static_assert(TILE < 65536, "16-bit tile offsets");
+  __shared__ uint16_t s_cnt[NW][R];     // per-wave running counts, then prefixes
   __shared__ uint64_t s_gbase[R];       // global start of this tile's digit run
-  __shared__ uint32_t s_dstart[R];      // tile-local start of each digit run
-  __shared__ uint64_t s_keys[TILE];
-  __shared__ float s_vals[TILE];
+  __shared__ uint16_t s_dstart[R];      // tile-local start of each digit run
+  __shared__ uint64_t s_buf[TILE];      // staged keys, then staged values
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_scan[NW];
 
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(
 
   uint64_t k[IPT];
   float v[IPT];
-  uint32_t rank[IPT];
+  uint32_t pos[IPT];
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {
     uint64_t idx = wbase + (uint64_t)i * kWave + lane;
@@ -143,12 +143,9 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(
     uint32_t before = 0;
     if (valid) before = s_cnt[wave][d];
     __builtin_amdgcn_wave_barrier();
-    if (valid) {
-      rank[i] = before + popc_below(peers);
-      // highest lane of the peer group advances the counter
-      bool last = (peers >> lane) == 1ull;
-      if (last) s_cnt[wave][d] = before + (uint32_t)__popcll(peers);
-    }
+    pos[i] = before + popc_below(peers);  // rank within the wave's digit run
+    if (valid && (peers >> lane) == 1ull)  // highest lane of the group advances
+      s_cnt[wave][d] = (uint16_t)(before + (uint32_t)__popcll(peers));
     __builtin_amdgcn_wave_barrier();
   }
   __syncthreads();
@@ -159,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
       uint32_t c = s_cnt[w][d];
-      s_cnt[w][d] = acc;
+      s_cnt[w][d] = (uint16_t)acc;
       acc += c;
     }
     const uint64_t total = acc;
@@ -172,22 +169,22 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(
       int64_t j = (int64_t)tile - 1;
       uint32_t spins = 0;
       while (j >= 0) {
-        uint64_t s = ld_agent(status + (uint64_t)j * R + d);
-        uint32_t se = (uint32_t)((s >> 48) & 0x3FFF);
-        uint64_t fl = s & (3ull << 62);
+        uint64_t st = ld_agent(status + (uint64_t)j * R + d);
+        uint32_t se = (uint32_t)((st >> 48) & 0x3FFF);
+        uint64_t fl = st & (3ull << 62);
         if (se != epoch || fl == 0) {
           if (++spins > kSpinLimit) { atomicOr(err, 1u); break; }
           __builtin_amdgcn_s_sleep(1);
           continue;
         }
-        excl += s & kValMask;
+        excl += st & kValMask;
         if (fl == kFlagPre) break;
         --j;
       }
       st_agent(my, pack_status(kFlagPre, epoch, excl + total));
     }
     s_gbase[d] = digit_base[d] + excl;
-    s_dstart[d] = acc;  // temporarily the tile total
+    s_dstart[d] = (uint16_t)acc;  // temporarily the tile total
   }
   __syncthreads();
   // Exclusive scan of tile totals over digits -> s_dstart.
@@ -204,35 +201,50 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(
     uint32_t ex = block_excl_scan<uint32_t>(sum, s_scan, &tot);
     for (int j = 0; j < PER; ++j) {
       int d = tid * PER + j;
-      if (d < R) s_dstart[d] = ex;
+      if (d < R) s_dstart[d] = (uint16_t)ex;
       ex += loc[j];
     }
   }
   __syncthreads();
-  // Stage in LDS in digit order.
+  // Phase 1: stage keys in digit order, write them out in coalesced runs.
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {
     uint64_t idx = wbase + (uint64_t)i * kWave + lane;
-    if (idx < n) {
-      uint32_t d = (uint32_t)(k[i] >> shift) & (R - 1);
-      uint32_t pos = s_dstart[d] + s_cnt[wave][d] + rank[i];
-      s_keys[pos] = k[i];
-      s_vals[pos] = v[i];
-    }
+    uint32_t d = (uint32_t)(k[i] >> shift) & (R - 1);
+    pos[i] += (uint32_t)s_dstart[d] + (uint32_t)s_cnt[wave][d];
+    if (idx < n) s_buf[pos[i]] = k[i];
   }
   __syncthreads();
   const uint32_t cnt = (uint32_t)((n - tile_begin) < (uint64_t)TILE ? (n - tile_begin) : TILE);
-  for (uint32_t j = tid; j < cnt; j += kBlock) {
-    uint64_t key = s_keys[j];
-    uint32_t d = (uint32_t)(key >> shift) & (R - 1);
-    uint64_t dst = s_gbase[d] + (j - s_dstart[d]);
-    kout[dst] = key;
-    vout[dst] = s_vals[j];
+  uint64_t dst[IPT];
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    const uint32_t j = tid + (uint32_t)i * kBlock;
+    if (j < cnt) {
+      const uint64_t key = s_buf[j];
+      const uint32_t d = (uint32_t)(key >> shift) & (R - 1);
+      dst[i] = s_gbase[d] + (j - s_dstart[d]);
+      kout[dst[i]] = key;
+    }
+  }
+  __syncthreads();
+  // Phase 2: the values through the same buffer, same destinations.
+  float* s_vbuf = reinterpret_cast<float*>(s_buf);
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+    if (idx < n) s_vbuf[pos[i]] = v[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    const uint32_t j = tid + (uint32_t)i * kBlock;
+    if (j < cnt) vout[dst[i]] = s_vbuf[j];
   }
 }
 
 template <int RB>
-constexpr int ipt_for() { return RB <= 8 ? 16 : 12; }
+constexpr int ipt_for() { return 12; }
 
 }  // namespace
 
@@ -251,10 +263,24 @@ int radix_passes(int nbits, int rb) { return nbits <= 0 ? 0 : (nbits + rb - 1) /
 
 uint64_t sort_tile_items(int rb) {
   switch (rb) {
-    case 8: return (uint64_t)kBlock * ipt_for<8>();
+    case 9: return (uint64_t)kBlock * ipt_for<9>();
+    case 10: return (uint64_t)kBlock * ipt_for<10>();
     case 11: return (uint64_t)kBlock * ipt_for<11>();
     default: return (uint64_t)kBlock * ipt_for<8>();
   }
+}
+
+bool radix_bits_supported(int rb) { return rb >= 8 && rb <= 11; }
+
+// Fewest passes with digits of at most 10 bits; the narrowest digit among
+// those that reach that pass count.
+int choose_radix_bits(int nbits) {
+  int best = 8, best_p = radix_passes(nbits, 8);
+  for (int rb = 9; rb <= 10; ++rb) {
+    int p = radix_passes(nbits, rb);
+    if (p < best_p) { best = rb; best_p = p; }
+  }
+  return best;
 }
 
 uint64_t SortWorkspace::bytes_per_key(int rb) {
@@ -296,12 +322,16 @@ void histogram(const SortWorkspace& ws, const uint64_t* keys, uint64_t n,
                int passes, hipStream_t s) {
   if (!n || !passes) return;
   uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(n, kBlock * 8), 2048);
-  if (ws.rb == 8)
-    hipLaunchKernelGGL(k_histogram<8>, dim3(grid), dim3(kBlock), 0, s, keys, n,
-                       passes, ws.hist.get());
-  else
-    hipLaunchKernelGGL(k_histogram<11>, dim3(grid), dim3(kBlock), 0, s, keys, n,
-                       passes, ws.hist.get());
+  switch (ws.rb) {
+    case 9: hipLaunchKernelGGL(k_histogram<9>, dim3(grid), dim3(kBlock), 0, s, keys, n, passes,
+                               ws.hist.get()); break;
+    case 10: hipLaunchKernelGGL(k_histogram<10>, dim3(grid), dim3(kBlock), 0, s, keys, n, passes,
+                                ws.hist.get()); break;
+    case 11: hipLaunchKernelGGL(k_histogram<11>, dim3(grid), dim3(kBlock), 0, s, keys, n, passes,
+                                ws.hist.get()); break;
+    default: hipLaunchKernelGGL(k_histogram<8>, dim3(grid), dim3(kBlock), 0, s, keys, n, passes,
+                                ws.hist.get()); break;
+  }
 }
 
 // Sorts keys/vals (n <= ws.cap) on bits [0, nbits) assuming ws.hist already
@@ -314,12 +344,16 @@ bool sort_after_histogram(SortWorkspace& ws, uint64_t* keys, float* vals,
   const int passes = radix_passes(nbits, rb);
   if (!n || !passes) return false;
   const uint64_t R = 1ull << rb;
-  if (rb == 8)
-    hipLaunchKernelGGL(k_digit_scan<8>, dim3(passes), dim3(kBlock), 0, s, ws.hist.get(),
-                       ws.base.get());
-  else
-    hipLaunchKernelGGL(k_digit_scan<11>, dim3(passes), dim3(kBlock), 0, s, ws.hist.get(),
-                       ws.base.get());
+  switch (rb) {
+    case 9: hipLaunchKernelGGL(k_digit_scan<9>, dim3(passes), dim3(kBlock), 0, s, ws.hist.get(),
+                               ws.base.get()); break;
+    case 10: hipLaunchKernelGGL(k_digit_scan<10>, dim3(passes), dim3(kBlock), 0, s, ws.hist.get(),
+                                ws.base.get()); break;
+    case 11: hipLaunchKernelGGL(k_digit_scan<11>, dim3(passes), dim3(kBlock), 0, s, ws.hist.get(),
+                                ws.base.get()); break;
+    default: hipLaunchKernelGGL(k_digit_scan<8>, dim3(passes), dim3(kBlock), 0, s, ws.hist.get(),
+                                ws.base.get()); break;
+  }
   const uint64_t tiles = ceil_div(n, sort_tile_items(rb));
   uint64_t* ka = keys;
   float* va = vals;
@@ -332,16 +366,17 @@ bool sort_after_histogram(SortWorkspace& ws, uint64_t* keys, float* vals,
       ws.epoch = 1;
     }
     const int tok = kt ? kt->begin(s) : -1;
-    if (rb == 8)
-      hipLaunchKernelGGL((k_onesweep<8, ipt_for<8>()>), dim3((uint32_t)tiles),
-                         dim3(kBlock), 0, s, ka, va, kb, vb, n, p * rb,
-                         ws.base.get() + p * R, ws.status.get(), ws.counters.get() + p,
-                         ws.epoch, ws.err());
-    else
-      hipLaunchKernelGGL((k_onesweep<11, ipt_for<11>()>), dim3((uint32_t)tiles),
-                         dim3(kBlock), 0, s, ka, va, kb, vb, n, p * rb,
-                         ws.base.get() + p * R, ws.status.get(), ws.counters.get() + p,
-                         ws.epoch, ws.err());
+#define SPTC_ONESWEEP(RBV)                                                                \
+  hipLaunchKernelGGL((k_onesweep<RBV, ipt_for<RBV>()>), dim3((uint32_t)tiles), dim3(kBlock), \
+                     0, s, ka, va, kb, vb, n, p * rb, ws.base.get() + p * R,               \
+                     ws.status.get(), ws.counters.get() + p, ws.epoch, ws.err())
+    switch (rb) {
+      case 9: SPTC_ONESWEEP(9); break;
+      case 10: SPTC_ONESWEEP(10); break;
+      case 11: SPTC_ONESWEEP(11); break;
+      default: SPTC_ONESWEEP(8); break;
+    }
+#undef SPTC_ONESWEEP
     if (kt) kt->end(tok, cls, 24.0 * (double)n, s);
     std::swap(ka, kb);
     std::swap(va, vb);

@@ -30,6 +30,9 @@ struct SortWorkspace {
 
 int radix_passes(int nbits, int rb);
 uint64_t sort_tile_items(int rb);
+bool radix_bits_supported(int rb);
+// Digit width for a sort over nbits key bits (fewest passes, digits <= 10).
+int choose_radix_bits(int nbits);
 void histogram(const SortWorkspace& ws, const uint64_t* keys, uint64_t n,
                int passes, hipStream_t s);
 // Sorts (keys, vals) on bits [0, nbits), ws.hist holding the histograms of

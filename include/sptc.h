@@ -65,7 +65,7 @@ typedef struct {
   uint64_t mem_budget_bytes; /* device bytes this call may hold at once
                                 (operands, workspaces, output); 0: free
                                 device memory minus 1 GiB               */
-  int32_t radix_bits;        /* 0: autotune; else 8 or 11                 */
+  int32_t radix_bits;        /* 0: per-sort choice; else 8..11            */
   int32_t verbose;           /* 1: phase log on stderr                    */
   int32_t kernel_timing;     /* 1: HIP-event time every hot-kernel launch
                                 (see sptc_result_kernel_stats)          */

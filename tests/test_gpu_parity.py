@@ -63,7 +63,8 @@ def test_linearize_matches_oracle(gpu, seed):
 @pytest.mark.parametrize("n,bits,rb", [(0, 8, 8), (1, 3, 8), (4095, 20, 8), (4096, 20, 8),
                                        (4097, 33, 8), (100_000, 43, 8), (1_000_003, 48, 8),
                                        (2_000_000, 64, 8), (300_000, 31, 11),
-                                       (1_500_000, 44, 11)])
+                                       (1_500_000, 44, 11), (700_000, 36, 9),
+                                       (900_001, 41, 10), (3_000_000, 36, 0)])
 def test_sort_pairs_stable_bit_exact(gpu, n, bits, rb):
     import torch
     rng = np.random.default_rng(n + bits)
