@@ -373,8 +373,8 @@ sptc_status sptc_linearize(int32_t order, uint64_t nnz, const void* const* idx, 
 
 sptc_status sptc_sort_pairs(uint64_t* keys, float* vals, uint64_t n, int32_t nbits,
                             int32_t radix_bits, void* stream) {
-  if (nbits < 0 || nbits > 64 || (n && (!keys || !vals)))
-    return fail(SPTC_ERR_BAD_ARGS, "sptc_sort_pairs: bad arguments");
+  if (nbits < 0 || nbits > 64 || (n && (!keys || !vals)) || n >= (1ull << 32))
+    return fail(SPTC_ERR_BAD_ARGS, "sptc_sort_pairs: bad arguments (n must be below 2^32)");
   if (radix_bits != 0 && !radix_bits_supported(radix_bits))
     return fail(SPTC_ERR_BAD_ARGS, "radix_bits must be 0 or 8..11");
   if (!device_ok()) return fail(SPTC_ERR_NO_DEVICE, "no GPU");

@@ -17,6 +17,10 @@
 //     [61:48] epoch, [47:0] count.
 #include "common.hpp"
 #include "sort.hpp"
+#include "scan.hpp"
+
+#include <cstdlib>
+#include <string>
 
 namespace sptc {
 
@@ -243,6 +247,159 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(
   }
 }
 
+// ---- reduce-then-scan pass (no inter-tile serialization) -----------------
+// upsweep: per-tile digit counts into a digit-major matrix cnt[d][tile];
+// an exclusive scan of that matrix, read in memory order, is exactly each
+// tile's global start for each digit; downsweep ranks, stages and scatters
+// like the onesweep kernel but takes its offsets from the scanned matrix.
+template <int RB, int IPT>
+__global__ __launch_bounds__(kBlock) void k_rts_upsweep(const uint64_t* __restrict__ kin,
+                                                        uint64_t n, int shift,
+                                                        uint32_t* __restrict__ cnt,
+                                                        uint64_t ntiles) {
+  constexpr int R = 1 << RB;
+  constexpr int TILE = kBlock * IPT;
+  __shared__ uint32_t sh[R];
+  for (int i = threadIdx.x; i < R; i += kBlock) sh[i] = 0;
+  __syncthreads();
+  const uint64_t tile = blockIdx.x;
+  const uint64_t base = tile * TILE;
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    const uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
+    const bool valid = idx < n;
+    const uint64_t k = valid ? kin[idx] : 0;
+    hist_add_wave<RB>(sh, k >> shift, valid, 1);
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < R; d += kBlock) cnt[(uint64_t)d * ntiles + tile] = sh[d];
+}
+
+template <int RB, int IPT>
+__global__ __launch_bounds__(kBlock) void k_rts_downsweep(
+    const uint64_t* __restrict__ kin, const float* __restrict__ vin,
+    uint64_t* __restrict__ kout, float* __restrict__ vout, uint64_t n, int shift,
+    const uint32_t* __restrict__ offs, uint64_t ntiles) {
+  constexpr int R = 1 << RB;
+  constexpr int TILE = kBlock * IPT;
+  constexpr int NW = kBlock / kWave;
+  __shared__ uint16_t s_cnt[NW][R];
+  __shared__ uint64_t s_gbase[R];
+  __shared__ uint16_t s_dstart[R];
+  __shared__ uint64_t s_buf[TILE];
+  __shared__ uint32_t s_scan[NW];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t wave = tid / kWave;
+  const uint32_t lane = lane_id();
+  for (int i = tid; i < NW * R; i += kBlock) (&s_cnt[0][0])[i] = 0;
+  const uint64_t tile = blockIdx.x;
+  for (int d = tid; d < R; d += kBlock) s_gbase[d] = offs[(uint64_t)d * ntiles + tile];
+  __syncthreads();
+  const uint64_t tile_begin = tile * TILE;
+  const uint64_t wbase = tile_begin + (uint64_t)wave * kWave * IPT;
+  uint64_t k[IPT];
+  float v[IPT];
+  uint32_t pos[IPT];
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+    bool valid = idx < n;
+    k[i] = valid ? kin[idx] : 0;
+    v[i] = valid ? vin[idx] : 0.f;
+  }
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+    bool valid = idx < n;
+    uint32_t d = (uint32_t)(k[i] >> shift) & (R - 1);
+    uint64_t peers = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+      bool bit = (d >> b) & 1u;
+      uint64_t bb = ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    uint32_t before = 0;
+    if (valid) before = s_cnt[wave][d];
+    __builtin_amdgcn_wave_barrier();
+    pos[i] = before + popc_below(peers);
+    if (valid && (peers >> lane) == 1ull)
+      s_cnt[wave][d] = (uint16_t)(before + (uint32_t)__popcll(peers));
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  for (int d = tid; d < R; d += kBlock) {
+    uint32_t acc = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      uint32_t c = s_cnt[w][d];
+      s_cnt[w][d] = (uint16_t)acc;
+      acc += c;
+    }
+    s_dstart[d] = (uint16_t)acc;
+  }
+  __syncthreads();
+  {
+    constexpr int PER = (R + kBlock - 1) / kBlock;
+    uint32_t loc[PER];
+    uint32_t sum = 0;
+    for (int j = 0; j < PER; ++j) {
+      int d = tid * PER + j;
+      loc[j] = d < R ? s_dstart[d] : 0;
+      sum += loc[j];
+    }
+    uint32_t tot;
+    uint32_t ex = block_excl_scan<uint32_t>(sum, s_scan, &tot);
+    for (int j = 0; j < PER; ++j) {
+      int d = tid * PER + j;
+      if (d < R) s_dstart[d] = (uint16_t)ex;
+      ex += loc[j];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+    uint32_t d = (uint32_t)(k[i] >> shift) & (R - 1);
+    pos[i] += (uint32_t)s_dstart[d] + (uint32_t)s_cnt[wave][d];
+    if (idx < n) s_buf[pos[i]] = k[i];
+  }
+  __syncthreads();
+  const uint32_t cnt = (uint32_t)((n - tile_begin) < (uint64_t)TILE ? (n - tile_begin) : TILE);
+  uint64_t dst[IPT];
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    const uint32_t j = tid + (uint32_t)i * kBlock;
+    if (j < cnt) {
+      const uint64_t key = s_buf[j];
+      const uint32_t d = (uint32_t)(key >> shift) & (R - 1);
+      dst[i] = s_gbase[d] + (j - s_dstart[d]);
+      kout[dst[i]] = key;
+    }
+  }
+  __syncthreads();
+  float* s_vbuf = reinterpret_cast<float*>(s_buf);
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+    if (idx < n) s_vbuf[pos[i]] = v[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    const uint32_t j = tid + (uint32_t)i * kBlock;
+    if (j < cnt) vout[dst[i]] = s_vbuf[j];
+  }
+}
+
+// Exclusive scan of the uint32 count matrix into uint32 offsets (n < 2^32).
+struct CountScanOp {
+  const uint32_t* in;
+  uint32_t* out;
+  __device__ U2 value(uint64_t i) const { return mk2(in[i], 0); }
+  __device__ void emit(uint64_t i, U2 excl, U2) const { out[i] = (uint32_t)excl.a; }
+};
+
 template <int RB>
 constexpr int ipt_for() { return 12; }
 
@@ -284,7 +441,16 @@ int choose_radix_bits(int nbits) {
 }
 
 uint64_t SortWorkspace::bytes_per_key(int rb) {
-  return sizeof(uint64_t) + sizeof(float) + (8ull << rb) / sort_tile_items(rb) + 1;
+  // alt keys + values, look-back words, and the two count matrices
+  return sizeof(uint64_t) + sizeof(float) + (16ull << rb) / sort_tile_items(rb) + 1;
+}
+
+static bool use_onesweep() {
+  static const bool v = [] {
+    const char* e = getenv("SPTC_SORT");
+    return e && std::string(e) == "onesweep";
+  }();
+  return v;
 }
 
 void SortWorkspace::ensure(uint64_t n, int rb_, hipStream_t s) {
@@ -306,6 +472,11 @@ void SortWorkspace::ensure(uint64_t n, int rb_, hipStream_t s) {
     status.reset(tiles * R);
     fill_u64(status.get(), 0, tiles * R, s);
     epoch = 0;
+  }
+  if (rts_cnt.size() < tiles * R) {
+    rts_cnt.reset(tiles * R);
+    rts_off.reset(tiles * R);
+    rts_tmp.reset(scan_tmp_count(tiles * R) + 1);
   }
   rb = rb_;
   cap = alt_keys.size();
@@ -359,6 +530,40 @@ bool sort_after_histogram(SortWorkspace& ws, uint64_t* keys, float* vals,
   float* va = vals;
   uint64_t* kb = ws.alt_keys.get();
   float* vb = ws.alt_vals.get();
+  if (!use_onesweep()) {
+    const uint64_t m = tiles * R;
+    for (int p = 0; p < passes; ++p) {
+      const int shift = p * rb;
+      int tok = kt ? kt->begin(s) : -1;
+#define SPTC_RTS_UP(RBV)                                                                       \
+  hipLaunchKernelGGL((k_rts_upsweep<RBV, ipt_for<RBV>()>), dim3((uint32_t)tiles), dim3(kBlock), \
+                     0, s, ka, n, shift, ws.rts_cnt.get(), tiles)
+      switch (rb) {
+        case 9: SPTC_RTS_UP(9); break;
+        case 10: SPTC_RTS_UP(10); break;
+        case 11: SPTC_RTS_UP(11); break;
+        default: SPTC_RTS_UP(8); break;
+      }
+#undef SPTC_RTS_UP
+      run_scan(CountScanOp{ws.rts_cnt.get(), ws.rts_off.get()}, m, ws.rts_tmp.get(), s);
+      if (kt) kt->end(tok, SPTC_K_SORT_COUNT, 8.0 * (double)n + 12.0 * (double)m, s);
+      tok = kt ? kt->begin(s) : -1;
+#define SPTC_RTS_DOWN(RBV)                                                              \
+  hipLaunchKernelGGL((k_rts_downsweep<RBV, ipt_for<RBV>()>), dim3((uint32_t)tiles),      \
+                     dim3(kBlock), 0, s, ka, va, kb, vb, n, shift, ws.rts_off.get(), tiles)
+      switch (rb) {
+        case 9: SPTC_RTS_DOWN(9); break;
+        case 10: SPTC_RTS_DOWN(10); break;
+        case 11: SPTC_RTS_DOWN(11); break;
+        default: SPTC_RTS_DOWN(8); break;
+      }
+#undef SPTC_RTS_DOWN
+      if (kt) kt->end(tok, cls, 24.0 * (double)n, s);
+      std::swap(ka, kb);
+      std::swap(va, vb);
+    }
+    return (passes & 1) != 0;
+  }
   for (int p = 0; p < passes; ++p) {
     ws.epoch = (ws.epoch + 1) & 0x3FFF;
     if (ws.epoch == 0) {

@@ -3,6 +3,7 @@
 #include "common.hpp"
 #include "device.hpp"
 #include "ktimer.hpp"
+#include "scan.hpp"
 
 namespace sptc {
 
@@ -18,6 +19,9 @@ struct SortWorkspace {
   DevBuf<uint64_t> hist;        // [kMaxPasses][R] digit counts
   DevBuf<uint64_t> base;        // [kMaxPasses][R] digit starts
   DevBuf<uint32_t> counters;    // [kMaxPasses] tile counters, [32] error flag
+  DevBuf<uint32_t> rts_cnt;     // [R][tiles] per-tile digit counts
+  DevBuf<uint32_t> rts_off;     // [R][tiles] their exclusive scan
+  DevBuf<U2> rts_tmp;           // scan scratch
   uint32_t epoch = 0;
 
   // Grows the workspace to hold `n` keys at `rb` radix bits.

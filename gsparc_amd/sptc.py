@@ -43,7 +43,7 @@ class _Opts(Structure):
 
 
 KERNEL_CLASSES = ["linearize", "sort_input", "match", "expand", "sort_products", "reduce",
-                  "dense"]
+                  "dense", "sort_count"]
 
 
 class _Info(Structure):

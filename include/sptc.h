@@ -146,13 +146,14 @@ sptc_status sptc_result_decode(const sptc_result* r, uint32_t* const* mode_idx,
 /* Kernel classes timed when opts.kernel_timing is set. */
 typedef enum {
   SPTC_K_LINEARIZE = 0,   /* linearize + fused digit histograms           */
-  SPTC_K_SORT_INPUT = 1,  /* radix pass over operand keys                 */
+  SPTC_K_SORT_INPUT = 1,  /* radix pass (scatter) over operand keys       */
   SPTC_K_MATCH = 2,       /* index match (reduce + emit)                  */
   SPTC_K_EXPAND = 3,      /* ESC product expansion                        */
-  SPTC_K_SORT_PROD = 4,   /* radix pass over product keys                 */
+  SPTC_K_SORT_PROD = 4,   /* radix pass (scatter) over product keys       */
   SPTC_K_REDUCE = 5,      /* reduce-by-key (count + emit)                 */
   SPTC_K_DENSE = 6,       /* dense per-row accumulation                   */
-  SPTC_K_COUNT = 7
+  SPTC_K_SORT_COUNT = 7,  /* radix pass digit counting + offset scan      */
+  SPTC_K_COUNT = 8
 } sptc_kernel_class;
 
 /* Launch count, summed HIP-event time (ms) and algorithmic HBM bytes of one
