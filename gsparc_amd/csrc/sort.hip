@@ -275,7 +275,7 @@ __global__ __launch_bounds__(kBlock) void k_rts_upsweep(const uint64_t* __restri
   for (int d = threadIdx.x; d < R; d += kBlock) cnt[(uint64_t)d * ntiles + tile] = sh[d];
 }
 
-template <int RB, int IPT>
+template <int RB, int IPT, bool STAGE>
 __global__ __launch_bounds__(kBlock) void k_rts_downsweep(
     const uint64_t* __restrict__ kin, const float* __restrict__ vin,
     uint64_t* __restrict__ kout, float* __restrict__ vout, uint64_t n, int shift,
@@ -339,6 +339,21 @@ __global__ __launch_bounds__(kBlock) void k_rts_downsweep(
     s_dstart[d] = (uint16_t)acc;
   }
   __syncthreads();
+  if constexpr (!STAGE) {
+    // Direct scatter from registers: destination = tile's digit start +
+    // the wave prefix within the tile + the rank within the wave.
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) {
+      uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+      if (idx < n) {
+        uint32_t d = (uint32_t)(k[i] >> shift) & (R - 1);
+        uint64_t o = s_gbase[d] + s_cnt[wave][d] + pos[i];
+        kout[o] = k[i];
+        vout[o] = v[i];
+      }
+    }
+    return;
+  }
   {
     constexpr int PER = (R + kBlock - 1) / kBlock;
     uint32_t loc[PER];
@@ -403,6 +418,34 @@ struct CountScanOp {
 template <int RB>
 constexpr int ipt_for() { return 12; }
 
+// Reduce-then-scan variant for 8-bit digits (SPTC_RTS): tile items per
+// thread and LDS staging ("12s" default, "16s", "8s", "12d" = direct scatter).
+struct RtsVariant {
+  int ipt = 12;
+  bool stage = true;
+};
+static bool use_onesweep() {
+  static const bool v = [] {
+    const char* e = getenv("SPTC_SORT");
+    return e && std::string(e) == "onesweep";
+  }();
+  return v;
+}
+
+static RtsVariant rts_variant() {
+  static const RtsVariant v = [] {
+    RtsVariant r;
+    const char* e = getenv("SPTC_RTS");
+    if (!e || use_onesweep()) return r;
+    std::string t(e);
+    if (t == "16s") r.ipt = 16;
+    if (t == "8s") r.ipt = 8;
+    if (t == "12d") r.stage = false;
+    return r;
+  }();
+  return v;
+}
+
 }  // namespace
 
 void fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t s) {
@@ -419,6 +462,7 @@ void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
 int radix_passes(int nbits, int rb) { return nbits <= 0 ? 0 : (nbits + rb - 1) / rb; }
 
 uint64_t sort_tile_items(int rb) {
+  if (rb == 8) return (uint64_t)kBlock * rts_variant().ipt;
   switch (rb) {
     case 9: return (uint64_t)kBlock * ipt_for<9>();
     case 10: return (uint64_t)kBlock * ipt_for<10>();
@@ -445,13 +489,7 @@ uint64_t SortWorkspace::bytes_per_key(int rb) {
   return sizeof(uint64_t) + sizeof(float) + (16ull << rb) / sort_tile_items(rb) + 1;
 }
 
-static bool use_onesweep() {
-  static const bool v = [] {
-    const char* e = getenv("SPTC_SORT");
-    return e && std::string(e) == "onesweep";
-  }();
-  return v;
-}
+
 
 void SortWorkspace::ensure(uint64_t n, int rb_, hipStream_t s) {
   const uint64_t R = 1ull << rb_;
@@ -535,27 +573,37 @@ bool sort_after_histogram(SortWorkspace& ws, uint64_t* keys, float* vals,
     for (int p = 0; p < passes; ++p) {
       const int shift = p * rb;
       int tok = kt ? kt->begin(s) : -1;
-#define SPTC_RTS_UP(RBV)                                                                       \
-  hipLaunchKernelGGL((k_rts_upsweep<RBV, ipt_for<RBV>()>), dim3((uint32_t)tiles), dim3(kBlock), \
-                     0, s, ka, n, shift, ws.rts_cnt.get(), tiles)
+#define SPTC_RTS_UP(RBV, IPTV)                                                          \
+  hipLaunchKernelGGL((k_rts_upsweep<RBV, IPTV>), dim3((uint32_t)tiles), dim3(kBlock), 0, s, ka, \
+                     n, shift, ws.rts_cnt.get(), tiles)
+      const RtsVariant var = rts_variant();
       switch (rb) {
-        case 9: SPTC_RTS_UP(9); break;
-        case 10: SPTC_RTS_UP(10); break;
-        case 11: SPTC_RTS_UP(11); break;
-        default: SPTC_RTS_UP(8); break;
+        case 9: SPTC_RTS_UP(9, 12); break;
+        case 10: SPTC_RTS_UP(10, 12); break;
+        case 11: SPTC_RTS_UP(11, 12); break;
+        default:
+          if (var.ipt == 16) SPTC_RTS_UP(8, 16);
+          else if (var.ipt == 8) SPTC_RTS_UP(8, 8);
+          else SPTC_RTS_UP(8, 12);
+          break;
       }
 #undef SPTC_RTS_UP
       run_scan(CountScanOp{ws.rts_cnt.get(), ws.rts_off.get()}, m, ws.rts_tmp.get(), s);
       if (kt) kt->end(tok, SPTC_K_SORT_COUNT, 8.0 * (double)n + 12.0 * (double)m, s);
       tok = kt ? kt->begin(s) : -1;
-#define SPTC_RTS_DOWN(RBV)                                                              \
-  hipLaunchKernelGGL((k_rts_downsweep<RBV, ipt_for<RBV>()>), dim3((uint32_t)tiles),      \
-                     dim3(kBlock), 0, s, ka, va, kb, vb, n, shift, ws.rts_off.get(), tiles)
+#define SPTC_RTS_DOWN(RBV, IPTV, ST)                                                     \
+  hipLaunchKernelGGL((k_rts_downsweep<RBV, IPTV, ST>), dim3((uint32_t)tiles), dim3(kBlock), 0, \
+                     s, ka, va, kb, vb, n, shift, ws.rts_off.get(), tiles)
       switch (rb) {
-        case 9: SPTC_RTS_DOWN(9); break;
-        case 10: SPTC_RTS_DOWN(10); break;
-        case 11: SPTC_RTS_DOWN(11); break;
-        default: SPTC_RTS_DOWN(8); break;
+        case 9: SPTC_RTS_DOWN(9, 12, true); break;
+        case 10: SPTC_RTS_DOWN(10, 12, true); break;
+        case 11: SPTC_RTS_DOWN(11, 12, true); break;
+        default:
+          if (var.ipt == 16) SPTC_RTS_DOWN(8, 16, true);
+          else if (var.ipt == 8) SPTC_RTS_DOWN(8, 8, true);
+          else if (!var.stage) SPTC_RTS_DOWN(8, 12, false);
+          else SPTC_RTS_DOWN(8, 12, true);
+          break;
       }
 #undef SPTC_RTS_DOWN
       if (kt) kt->end(tok, cls, 24.0 * (double)n, s);
