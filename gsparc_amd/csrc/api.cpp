@@ -385,7 +385,7 @@ sptc_status sptc_sort_pairs(uint64_t* keys, float* vals, uint64_t n, int32_t nbi
     ws.ensure(n, rb, s);
     const int passes = radix_passes(nbits, rb);
     ws.prepare(passes, s);
-    histogram(ws, keys, n, passes, s);
+    if (sort_needs_histogram()) histogram(ws, keys, n, passes, s);
     if (sort_after_histogram(ws, keys, vals, n, nbits, s)) {
       SPTC_HIP(hipMemcpyAsync(keys, ws.alt_keys.get(), n * 8, hipMemcpyDeviceToDevice, s));
       SPTC_HIP(hipMemcpyAsync(vals, ws.alt_vals.get(), n * 4, hipMemcpyDeviceToDevice, s));

@@ -135,10 +135,11 @@ void Engine::prepare(const StagedCOO& t, const Layout& lay, Operand* out) {
   ws_.ensure(n, rb, s_);
   ws_.prepare(passes, s_);
   int tok = kt_.begin(s_);
+  const bool hist = sort_needs_histogram();
   launch_linearize(a, t.idx_bytes, n, out->keys.get(), ws_.hist.get(),
-                   passes <= kMaxFusedPasses ? passes : 0, rb, s_);
+                   hist && passes <= kMaxFusedPasses ? passes : 0, rb, s_);
   kt_.end(tok, SPTC_K_LINEARIZE, (double)n * (lay.order * t.idx_bytes + 8), s_);
-  if (passes > kMaxFusedPasses) histogram(ws_, out->keys.get(), n, passes, s_);
+  if (hist && passes > kMaxFusedPasses) histogram(ws_, out->keys.get(), n, passes, s_);
   if (sort_after_histogram(ws_, out->keys.get(), out->vals.get(), n, lay.nbits, s_, &kt_,
                            SPTC_K_SORT_INPUT)) {
     std::swap(out->keys, ws_.alt_keys);
@@ -173,7 +174,7 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
     const int rbf = rb_ ? rb_ : choose_radix_bits(ynf);
     ws_.ensure(ny, rbf, s_);
     ws_.prepare(radix_passes(ynf, rbf), s_);
-    histogram(ws_, fk.get(), ny, radix_passes(ynf, rbf), s_);
+    if (sort_needs_histogram()) histogram(ws_, fk.get(), ny, radix_passes(ynf, rbf), s_);
     const bool alt = sort_after_histogram(ws_, fk.get(), fv.get(), ny, ynf, s_);
     ytab.reset(ny);
     DevBuf<U2> tmp(scan_tmp_count(ny) + 1);
@@ -343,13 +344,14 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       ea.out_keys = kA.get();
       ea.out_vals = vA.get();
       ea.hist = ws_.hist.get();
-      ea.passes = passes <= kMaxFusedPasses ? passes : 0;
+      const bool hist = sort_needs_histogram();
+      ea.passes = hist && passes <= kMaxFusedPasses ? passes : 0;
       ea.vsum = vsum.get();
       int tok = kt_.begin(s_);
       launch_expand(ea, rb, s_);
       // product (key, value) written, matched Y (key, value) read
       kt_.end(tok, SPTC_K_EXPAND, 24.0 * (double)n, s_);
-      if (passes > kMaxFusedPasses) histogram(ws_, kA.get(), n, passes, s_);
+      if (hist && passes > kMaxFusedPasses) histogram(ws_, kA.get(), n, passes, s_);
       const bool alt = sort_after_histogram(ws_, kA.get(), vA.get(), n, bits, s_, &kt_,
                                             SPTC_K_SORT_PROD);
       const uint64_t* sk = alt ? ws_.alt_keys.get() : kA.get();

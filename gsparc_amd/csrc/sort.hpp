@@ -35,6 +35,9 @@ struct SortWorkspace {
 int radix_passes(int nbits, int rb);
 uint64_t sort_tile_items(int rb);
 bool radix_bits_supported(int rb);
+// True when the configured pass algorithm needs up-front digit histograms
+// (onesweep); reduce-then-scan counts per tile itself.
+bool sort_needs_histogram();
 // Digit width for a sort over nbits key bits (fewest passes, digits <= 10).
 int choose_radix_bits(int nbits);
 void histogram(const SortWorkspace& ws, const uint64_t* keys, uint64_t n,
