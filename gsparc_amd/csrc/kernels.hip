@@ -514,9 +514,6 @@ void run_reduce_count(const uint64_t* keys, uint64_t n, U2* tmp, hipStream_t s) 
 __global__ __launch_bounds__(kBlock) void k_reduce_emit(ReduceOp op, const U2* tile_offs) {
   __shared__ uint64_t s_k[kScanTile];
   __shared__ float s_v[kScanTile];
-  __shared__ uint64_t s_ocol[kScanTile];
-  __shared__ float s_oval[kScanTile];
-  __shared__ uint32_t s_orow[kScanTile];
   __shared__ U2 scratch[kBlock / kWave];
   const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
   const uint32_t cnt = (uint32_t)(op.n - base < kScanTile ? op.n - base : kScanTile);
@@ -526,23 +523,17 @@ __global__ __launch_bounds__(kBlock) void k_reduce_emit(ReduceOp op, const U2* t
   }
   const uint64_t prev_key = base ? op.keys[base - 1] : 0;
   __syncthreads();
+  // Thread t owns items [t*8, t*8+8); its runs' sums stay in registers.
   const uint32_t q0 = threadIdx.x * kScanItems;
-  uint32_t heads = 0;
+  uint64_t ok[kScanItems];
+  float ov[kScanItems];
+  uint32_t nh = 0;
 #pragma unroll
   for (int e = 0; e < kScanItems; ++e) {
     const uint32_t i = q0 + e;
-    if (i < cnt) {
-      const uint64_t pk = i ? s_k[i - 1] : prev_key;
-      heads += (base + i == 0 || s_k[i] != pk) ? 1u : 0u;
-    }
-  }
-  U2 tot;
-  uint32_t o = (uint32_t)block_excl_scan_u2(mk2(heads, 0), scratch, &tot).a;
-  const uint32_t nout = (uint32_t)tot.a;
-#pragma unroll 1
-  for (int e = 0; e < kScanItems; ++e) {
-    const uint32_t i = q0 + e;
-    if (i >= cnt) break;
+    ok[e] = 0;
+    ov[e] = 0.f;
+    if (i >= cnt) continue;
     const uint64_t k = s_k[i];
     const uint64_t pk = i ? s_k[i - 1] : prev_key;
     if (!(base + i == 0 || k != pk)) continue;
@@ -553,17 +544,28 @@ __global__ __launch_bounds__(kBlock) void k_reduce_emit(ReduceOp op, const U2* t
     if (jj == cnt) {
       for (uint64_t g = base + cnt; g < op.n && op.keys[g] == k; ++g) acc.add(op.vals[g]);
     }
-    s_ocol[o] = op.ytab ? op.ytab[k & op.fmask] : (k & op.fmask);
-    s_orow[o] = (uint32_t)(op.r0 + shr64(k, op.ynf));
-    s_oval[o] = acc.value();
-    ++o;
+    ok[nh] = k;
+    ov[nh] = acc.value();
+    ++nh;
+  }
+  U2 tot;
+  const uint32_t o0 = (uint32_t)block_excl_scan_u2(mk2(nh, 0), scratch, &tot).a;
+  const uint32_t nout = (uint32_t)tot.a;
+  // (the scan's barriers separate the reads above from the staging below)
+#pragma unroll
+  for (int e = 0; e < kScanItems; ++e) {
+    if ((uint32_t)e < nh) {
+      s_k[o0 + e] = ok[e];
+      s_v[o0 + e] = ov[e];
+    }
   }
   __syncthreads();
   const uint64_t obase = tile_offs[blockIdx.x].a;
   for (uint32_t i = threadIdx.x; i < nout; i += kBlock) {
-    op.out_row[obase + i] = s_orow[i];
-    op.out_col[obase + i] = s_ocol[i];
-    op.out_val[obase + i] = s_oval[i];
+    const uint64_t k = s_k[i];
+    op.out_row[obase + i] = (uint32_t)(op.r0 + shr64(k, op.ynf));
+    op.out_col[obase + i] = op.ytab ? op.ytab[k & op.fmask] : (k & op.fmask);
+    op.out_val[obase + i] = s_v[i];
   }
 }
 

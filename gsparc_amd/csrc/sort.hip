@@ -367,12 +367,12 @@ struct CountScanOp {
 
 // ---- tile configuration ---------------------------------------------------
 // SPTC_SORT selects the pass algorithm ("rts" default, "onesweep");
-// SPTC_SORT_TILE the 8-bit-digit tile as threads x items ("512x16" default,
-// "256x16", "256x12", "512x12").  Wider digits always use 256x12.
+// SPTC_SORT_TILE the 8-bit-digit tile as threads x items ("512x12" default,
+// "512x16", "256x16", "256x12").  9-bit digits use 512x12, wider ones 256x12.
 struct SortCfg {
   bool onesweep = false;
   int nt = 512;
-  int ipt = 16;
+  int ipt = 12;
 };
 const SortCfg& sort_cfg() {
   static const SortCfg c = [] {
@@ -382,7 +382,7 @@ const SortCfg& sort_cfg() {
       const std::string t(e);
       if (t == "256x16") { r.nt = 256; r.ipt = 16; }
       if (t == "256x12") { r.nt = 256; r.ipt = 12; }
-      if (t == "512x12") { r.nt = 512; r.ipt = 12; }
+      if (t == "512x16") { r.nt = 512; r.ipt = 16; }
     }
     return r;
   }();
@@ -394,7 +394,7 @@ template <class F>
 void with_tile(int rb, F&& f) {
   const SortCfg& c = sort_cfg();
   switch (rb) {
-    case 9: f.template run<9, 12, 256>(); return;
+    case 9: f.template run<9, 12, 512>(); return;
     case 10: f.template run<10, 12, 256>(); return;
     case 11: f.template run<11, 12, 256>(); return;
     default:
@@ -468,6 +468,7 @@ int radix_passes(int nbits, int rb) { return nbits <= 0 ? 0 : (nbits + rb - 1) /
 
 uint64_t sort_tile_items(int rb) {
   if (rb == 8) return (uint64_t)sort_cfg().nt * sort_cfg().ipt;
+  if (rb == 9) return 512ull * 12;
   return (uint64_t)kBlock * 12;
 }
 
@@ -476,7 +477,11 @@ bool radix_bits_supported(int rb) { return rb >= 8 && rb <= 11; }
 // 8-bit digits: the wider-digit passes run on smaller tiles and measured
 // slower per pass than the pass they save.
 int choose_radix_bits(int nbits) {
-  (void)nbits;
+  static const int wide = [] {
+    const char* e = getenv("SPTC_SORT_RB9");
+    return e && e[0] == '1' ? 1 : 0;
+  }();
+  if (wide && radix_passes(nbits, 9) < radix_passes(nbits, 8)) return 9;
   return 8;
 }
 
