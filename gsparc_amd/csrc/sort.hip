@@ -474,12 +474,13 @@ uint64_t sort_tile_items(int rb) {
 
 bool radix_bits_supported(int rb) { return rb >= 8 && rb <= 11; }
 
-// 8-bit digits: the wider-digit passes run on smaller tiles and measured
-// slower per pass than the pass they save.
+// 8-bit digits, or 9-bit ones when that saves a pass (a 9-bit pass on the
+// 512x12 tile measured ~10% slower than an 8-bit one, a pass saved is 20-25%
+// of the sort).  SPTC_SORT_RB9=0 keeps 8-bit digits throughout.
 int choose_radix_bits(int nbits) {
   static const int wide = [] {
     const char* e = getenv("SPTC_SORT_RB9");
-    return e && e[0] == '1' ? 1 : 0;
+    return e && e[0] == '0' ? 0 : 1;
   }();
   if (wide && radix_passes(nbits, 9) < radix_passes(nbits, 8)) return 9;
   return 8;
