@@ -245,6 +245,7 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
   host_u64_[1] = A;
   SPTC_HIP(hipMemcpyAsync(row_prod.get() + R, &host_u64_[0], 8, hipMemcpyHostToDevice, s_));
   SPTC_HIP(hipMemcpyAsync(row_act.get() + R, &host_u64_[1], 8, hipMemcpyHostToDevice, s_));
+  SPTC_HIP(hipMemcpyAsync(act_prefix.get() + A, &host_u64_[0], 8, hipMemcpyHostToDevice, s_));
   tlo.release();
   thi.release();
   ukey.release();
@@ -254,15 +255,79 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
   info.nrows = R;
   res->row_key = std::move(row_key);
 
-  // ---- (e) partition: chunk the X rows so ESC buffers fit the budget -------
-  if (o_.dense_accumulator == 1) {
-    msg = "dense accumulator is not available for this contraction";
-    return SPTC_ERR_BAD_ARGS;
-  }
-  std::vector<uint64_t> bounds_r{0}, bounds_p{0}, bounds_a{0};
+  // ---- accumulator choice (the reference's -d flag, sptc.cuh:201-218) ------
+  // Dense applies when the column space (ranks of Y's free keys) fits an LDS
+  // row and Y has no duplicate coordinates; automatically chosen when rows
+  // are dense enough for the LDS row to pay off.
   DevBuf<double> vsum(1);
   fill_u64(reinterpret_cast<uint64_t*>(vsum.get()), 0, 1, s_);
-  if (ir) {
+  bool use_dense = false;
+  if (o_.dense_accumulator >= 0 && ir && kyf <= kDenseMaxBits) {
+    DevBuf<uint32_t> flag(1);
+    launch_any_dup(Y.keys.get(), ny, flag.get(), s_);
+    SPTC_HIP(hipMemcpyAsync(host_u64_, flag.get(), 4, hipMemcpyDeviceToHost, s_));
+    SPTC_HIP(hipStreamSynchronize(s_));
+    const bool dups = *reinterpret_cast<uint32_t*>(host_u64_) != 0;
+    const uint64_t E = 1ull << (kyf < 8 ? 8 : kyf);
+    // count-only runs stay on ESC, whose expansion also yields the value sum
+    if (!dups)
+      use_dense = o_.dense_accumulator == 1 || (!o_.count_only && ir * 32 >= R * E);
+  }
+  if (o_.dense_accumulator == 1 && ir && !use_dense) {
+    msg = "dense accumulator does not apply (column space above 2^13 or duplicate Y "
+          "coordinates)";
+    return SPTC_ERR_BAD_ARGS;
+  }
+  info.accumulator = use_dense ? 1 : 0;
+  std::vector<uint64_t> bounds_r{0}, bounds_p{0}, bounds_a{0};
+  if (use_dense) {
+    record(4);
+    const int eb = kyf <= 8 ? 8 : (kyf <= 10 ? 10 : 13);
+    DevBuf<uint32_t> row_cnt(R);
+    DevBuf<uint64_t> out_off(R + 1);
+    DevBuf<U2> tmp(scan_tmp_count(R) + 1);
+    DenseArgs da;
+    std::memset(&da, 0, sizeof(da));
+    da.act_prefix = act_prefix.get();
+    da.act_ypos = act_ypos.get();
+    da.act_xval = act_xval.get();
+    da.row_act = row_act.get();
+    da.r0 = 0;
+    da.ykeys = ykeys;
+    da.yvals = Y.vals.get();
+    da.fmask = low_mask(kyf);
+    da.ytab = ytab.size() ? ytab.get() : nullptr;
+    da.row_cnt = row_cnt.get();
+    da.out_off = out_off.get();
+    da.vsum = vsum.get();
+    int tok = kt_.begin(s_);
+    launch_dense_count(da, eb, R, s_);
+    kt_.end(tok, SPTC_K_DENSE, 8.0 * (double)ir, s_);
+    run_row_count_scan(row_cnt.get(), out_off.get(), R, tmp.get(), s_);
+    const uint64_t out_n = read_total(tmp.get() + scan_tmp_count(R)).a;
+    info.nnz = out_n;
+    info.nchunks = 1;
+    if (!o_.count_only) {
+      if (DevicePool::get().live_bytes() + out_n * 16 > budget_) {
+        msg = "output does not fit the memory budget";
+        return SPTC_ERR_OOM_BUDGET;
+      }
+      OutChunk oc;
+      oc.nnz = out_n;
+      oc.r0 = 0;
+      oc.r1 = R;
+      oc.row.reset(out_n);
+      oc.col.reset(out_n);
+      oc.val.reset(out_n);
+      da.out_row = oc.row.get();
+      da.out_col = oc.col.get();
+      da.out_val = oc.val.get();
+      tok = kt_.begin(s_);
+      launch_dense_numeric(da, eb, R, s_);
+      kt_.end(tok, SPTC_K_DENSE, 12.0 * (double)ir + 16.0 * (double)out_n, s_);
+      res->chunks.push_back(std::move(oc));
+    }
+  } else if (ir) {
     DevBuf<uint64_t> scratch(4);
     launch_max_row(row_prod.get(), R, scratch.get(), s_);
     SPTC_HIP(hipMemcpyAsync(host_u64_, scratch.get(), 8, hipMemcpyDeviceToHost, s_));

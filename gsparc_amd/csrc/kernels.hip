@@ -579,6 +579,165 @@ void run_reduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32
 }
 
 // --------------------------------------------------------------------------
+// (d) Dense per-row accumulation for small column spaces (E = 2^EB slots,
+// the rank space of Y's free keys).  One workgroup per X row.  The row's
+// active X nonzeros are taken in order; each one's Y segment is spread over
+// the threads and added into the dense LDS row with compensated FP32
+// arithmetic, a barrier separating segments, so every slot sums its products
+// in the same order as the ESC path's stable sort (Y free keys are distinct
+// within a segment when Y has no duplicate coordinates, which the caller
+// checks).  A first, symbolic kernel counts each row's touched slots so the
+// numeric kernel writes its outputs, in column order, straight to their
+// final place.
+
+template <int EB>
+__device__ __forceinline__ void dense_row_range(const DenseArgs& a, uint64_t r, uint64_t& j0,
+                                                uint64_t& j1) {
+  j0 = a.row_act[r];
+  j1 = a.row_act[r + 1];
+}
+
+template <int EB>
+__global__ __launch_bounds__(kBlock) void k_dense_count(DenseArgs a) {
+  constexpr int E = 1 << EB;
+  constexpr int W = E / 32;
+  __shared__ uint32_t bm[W];
+  __shared__ uint32_t s_red[kBlock / kWave];
+  const uint64_t r = a.r0 + blockIdx.x;
+  for (int i = threadIdx.x; i < W; i += kBlock) bm[i] = 0;
+  __syncthreads();
+  uint64_t j0, j1;
+  dense_row_range<EB>(a, r, j0, j1);
+  for (uint64_t j = j0; j < j1; ++j) {
+    const uint64_t y0 = a.act_ypos[j];
+    const uint64_t c = a.act_prefix[j + 1] - a.act_prefix[j];
+    for (uint64_t q = threadIdx.x; q < c; q += kBlock) {
+      const uint32_t rank = (uint32_t)(a.ykeys[y0 + q] & a.fmask);
+      atomicOr(&bm[rank >> 5], 1u << (rank & 31));
+    }
+  }
+  __syncthreads();
+  uint32_t cnt = 0;
+  for (int i = threadIdx.x; i < W; i += kBlock) cnt += __popc(bm[i]);
+  for (int off = kWave / 2; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, kWave);
+  if (lane_id() == 0) s_red[threadIdx.x / kWave] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) t += s_red[w];
+    a.row_cnt[blockIdx.x] = t;
+  }
+}
+
+template <int EB>
+__global__ __launch_bounds__(kBlock) void k_dense_numeric(DenseArgs a) {
+  constexpr int E = 1 << EB;
+  constexpr int W = E / 32;
+  constexpr int S = E / kBlock;  // slots per thread in the compaction (1..32)
+  static_assert(E >= kBlock && E % kBlock == 0, "dense row width");
+  __shared__ float s_sum[E];
+  __shared__ float s_cmp[E];
+  __shared__ uint32_t bm[W];
+  __shared__ uint32_t s_scan[kBlock / kWave];
+  const uint64_t r = a.r0 + blockIdx.x;
+  for (int i = threadIdx.x; i < E; i += kBlock) {
+    s_sum[i] = 0.f;
+    s_cmp[i] = 0.f;
+  }
+  for (int i = threadIdx.x; i < W; i += kBlock) bm[i] = 0;
+  __syncthreads();
+  uint64_t j0, j1;
+  dense_row_range<EB>(a, r, j0, j1);
+  double vsum = 0.0;
+  for (uint64_t j = j0; j < j1; ++j) {
+    const uint64_t y0 = a.act_ypos[j];
+    const uint64_t c = a.act_prefix[j + 1] - a.act_prefix[j];
+    const float xv = a.act_xval[j];
+    for (uint64_t q = threadIdx.x; q < c; q += kBlock) {
+      const uint64_t y = y0 + q;
+      const uint32_t rank = (uint32_t)(a.ykeys[y] & a.fmask);
+      const float v = xv * a.yvals[y];
+      vsum += (double)v;
+      // Neumaier step on the slot (one writer per slot within a segment)
+      const float sm = s_sum[rank];
+      const float t = sm + v;
+      if (fabsf(sm) >= fabsf(v)) s_cmp[rank] += (sm - t) + v;
+      else s_cmp[rank] += (v - t) + sm;
+      s_sum[rank] = t;
+      atomicOr(&bm[rank >> 5], 1u << (rank & 31));
+    }
+    __syncthreads();
+  }
+  // Compaction in column order: thread t owns slots [t*S, t*S+S).
+  const uint32_t first = threadIdx.x * S;
+  const uint32_t word = bm[first >> 5];
+  uint32_t bits;
+  if constexpr (S == 32) bits = word;
+  else bits = (word >> (first & 31)) & ((1u << S) - 1);
+  uint32_t tot;
+  uint32_t o = block_excl_scan<uint32_t>((uint32_t)__popc(bits), s_scan, &tot);
+  const uint64_t obase = a.out_off[blockIdx.x];
+  for (int e = 0; e < S; ++e) {
+    if ((bits >> e) & 1u) {
+      const uint32_t rank = first + e;
+      a.out_row[obase + o] = (uint32_t)r;
+      a.out_col[obase + o] = a.ytab ? a.ytab[rank] : rank;
+      a.out_val[obase + o] = s_sum[rank] + s_cmp[rank];
+      ++o;
+    }
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1) vsum += __shfl_xor(vsum, off, kWave);
+  if (lane_id() == 0 && vsum != 0.0) atomicAdd(a.vsum, vsum);
+}
+
+void launch_dense_count(const DenseArgs& a, int eb, uint64_t rows, hipStream_t s) {
+  if (!rows) return;
+  switch (eb) {
+    case 8: hipLaunchKernelGGL(k_dense_count<8>, dim3((uint32_t)rows), dim3(kBlock), 0, s, a); break;
+    case 10: hipLaunchKernelGGL(k_dense_count<10>, dim3((uint32_t)rows), dim3(kBlock), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_dense_count<13>, dim3((uint32_t)rows), dim3(kBlock), 0, s, a); break;
+  }
+}
+
+void launch_dense_numeric(const DenseArgs& a, int eb, uint64_t rows, hipStream_t s) {
+  if (!rows) return;
+  switch (eb) {
+    case 8: hipLaunchKernelGGL(k_dense_numeric<8>, dim3((uint32_t)rows), dim3(kBlock), 0, s, a); break;
+    case 10: hipLaunchKernelGGL(k_dense_numeric<10>, dim3((uint32_t)rows), dim3(kBlock), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_dense_numeric<13>, dim3((uint32_t)rows), dim3(kBlock), 0, s, a); break;
+  }
+}
+
+// Any two equal adjacent keys in a sorted array (duplicate coordinates).
+__global__ __launch_bounds__(kBlock) void k_any_dup(const uint64_t* __restrict__ keys, uint64_t n,
+                                                    uint32_t* flag) {
+  uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x + 1;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  bool dup = false;
+  for (; i < n; i += stride) dup |= keys[i] == keys[i - 1];
+  if (__ballot(dup) && lane_id() == 0) atomicOr(flag, 1u);
+}
+
+void launch_any_dup(const uint64_t* keys, uint64_t n, uint32_t* flag, hipStream_t s) {
+  fill_u32(flag, 0, 1, s);
+  if (n < 2) return;
+  uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(n, kBlock), 2048);
+  hipLaunchKernelGGL(k_any_dup, dim3(grid), dim3(kBlock), 0, s, keys, n, flag);
+}
+
+// Exclusive scan of per-row uint32 counts into uint64 offsets (+ total).
+struct RowCountScanOp {
+  const uint32_t* in;
+  uint64_t* out;
+  __device__ U2 value(uint64_t i) const { return mk2(in[i], 0); }
+  __device__ void emit(uint64_t i, U2 excl, U2) const { out[i] = excl.a; }
+};
+
+void run_row_count_scan(const uint32_t* cnt, uint64_t* off, uint64_t n, U2* tmp, hipStream_t s) {
+  run_scan(RowCountScanOp{cnt, off}, n, tmp, s);
+}
+
+// --------------------------------------------------------------------------
 // Result decode helpers.
 __global__ __launch_bounds__(kBlock) void k_pack_keys(const uint32_t* row, const uint64_t* col,
                                                       const uint64_t* row_key, uint64_t n,

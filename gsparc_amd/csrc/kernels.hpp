@@ -58,6 +58,31 @@ struct ExpandArgs {
   double* vsum;
 };
 
+struct DenseArgs {
+  const uint64_t* act_prefix;  // [A+1], act_prefix[A] = ir
+  const uint64_t* act_ypos;
+  const float* act_xval;
+  const uint64_t* row_act;     // [R+1]
+  uint64_t r0;                 // first row of the chunk
+  const uint64_t* ykeys;
+  const float* yvals;
+  uint64_t fmask;              // column field mask
+  const uint64_t* ytab;        // rank -> Y free key (NULL: field is the key)
+  uint32_t* row_cnt;           // [rows] symbolic counts
+  const uint64_t* out_off;     // [rows] output offsets
+  uint32_t* out_row;
+  uint64_t* out_col;
+  float* out_val;
+  double* vsum;
+};
+
+constexpr int kDenseMaxBits = 13;  // dense LDS rows up to 8192 slots
+
+void launch_dense_count(const DenseArgs& a, int eb, uint64_t rows, hipStream_t s);
+void launch_dense_numeric(const DenseArgs& a, int eb, uint64_t rows, hipStream_t s);
+void launch_any_dup(const uint64_t* keys, uint64_t n, uint32_t* flag, hipStream_t s);
+void run_row_count_scan(const uint32_t* cnt, uint64_t* off, uint64_t n, U2* tmp, hipStream_t s);
+
 void launch_linearize(const LinArgs& a, int idx_bytes, uint64_t n, uint64_t* keys,
                       uint64_t* hist, int passes, int rb, hipStream_t s);
 void launch_ytable_direct(const uint64_t* ykeys, uint64_t ny, int ynf, uint64_t K,

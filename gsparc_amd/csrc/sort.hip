@@ -254,7 +254,9 @@ __device__ __forceinline__ uint32_t wave_prefixes(TileSmem<RB, IPT, NT>& sm, int
   return acc;
 }
 
-template <int RB, int IPT, int NT>
+// TAG only separates kernel symbols (0: operand sorts, 1: product sorts) so
+// profiles attribute time to the product sort on its own.
+template <int RB, int IPT, int NT, int TAG>
 __global__ __launch_bounds__(NT) void k_onesweep(
     const uint64_t* __restrict__ kin, const float* __restrict__ vin,
     uint64_t* __restrict__ kout, float* __restrict__ vout, uint64_t n, int shift,
@@ -331,7 +333,7 @@ __global__ __launch_bounds__(NT) void k_rts_upsweep(const uint64_t* __restrict__
   for (int d = threadIdx.x; d < R; d += NT) cnt[(uint64_t)d * ntiles + tile] = sh[d];
 }
 
-template <int RB, int IPT, int NT>
+template <int RB, int IPT, int NT, int TAG>
 __global__ __launch_bounds__(NT) void k_rts_downsweep(
     const uint64_t* __restrict__ kin, const float* __restrict__ vin,
     uint64_t* __restrict__ kout, float* __restrict__ vout, uint64_t n, int shift,
@@ -406,6 +408,7 @@ void with_tile(int rb, F&& f) {
 }
 
 struct PassArgs {
+  int tag;
   SortWorkspace* ws;
   uint64_t* ka;
   float* va;
@@ -425,10 +428,16 @@ struct OnesweepPass {
   void run() {
     const uint64_t R = 1ull << RB;
     const int tok = a.kt ? a.kt->begin(a.s) : -1;
-    hipLaunchKernelGGL((k_onesweep<RB, IPT, NT>), dim3((uint32_t)a.tiles), dim3(NT), 0, a.s,
-                       a.ka, a.va, a.kb, a.vb, a.n, a.pass * RB, a.ws->base.get() + a.pass * R,
-                       a.ws->status.get(), a.ws->counters.get() + a.pass, a.ws->epoch,
-                       a.ws->err());
+    if (a.tag)
+      hipLaunchKernelGGL((k_onesweep<RB, IPT, NT, 1>), dim3((uint32_t)a.tiles), dim3(NT), 0, a.s,
+                         a.ka, a.va, a.kb, a.vb, a.n, a.pass * RB, a.ws->base.get() + a.pass * R,
+                         a.ws->status.get(), a.ws->counters.get() + a.pass, a.ws->epoch,
+                         a.ws->err());
+    else
+      hipLaunchKernelGGL((k_onesweep<RB, IPT, NT, 0>), dim3((uint32_t)a.tiles), dim3(NT), 0, a.s,
+                         a.ka, a.va, a.kb, a.vb, a.n, a.pass * RB, a.ws->base.get() + a.pass * R,
+                         a.ws->status.get(), a.ws->counters.get() + a.pass, a.ws->epoch,
+                         a.ws->err());
     if (a.kt) a.kt->end(tok, a.cls, 24.0 * (double)a.n, a.s);
   }
 };
@@ -445,8 +454,12 @@ struct RtsPass {
     run_scan(CountScanOp{a.ws->rts_cnt.get(), a.ws->rts_off.get()}, m, a.ws->rts_tmp.get(), a.s);
     if (a.kt) a.kt->end(tok, SPTC_K_SORT_COUNT, 8.0 * (double)a.n + 12.0 * (double)m, a.s);
     tok = a.kt ? a.kt->begin(a.s) : -1;
-    hipLaunchKernelGGL((k_rts_downsweep<RB, IPT, NT>), dim3((uint32_t)a.tiles), dim3(NT), 0,
-                       a.s, a.ka, a.va, a.kb, a.vb, a.n, shift, a.ws->rts_off.get(), a.tiles);
+    if (a.tag)
+      hipLaunchKernelGGL((k_rts_downsweep<RB, IPT, NT, 1>), dim3((uint32_t)a.tiles), dim3(NT), 0,
+                         a.s, a.ka, a.va, a.kb, a.vb, a.n, shift, a.ws->rts_off.get(), a.tiles);
+    else
+      hipLaunchKernelGGL((k_rts_downsweep<RB, IPT, NT, 0>), dim3((uint32_t)a.tiles), dim3(NT), 0,
+                         a.s, a.ka, a.va, a.kb, a.vb, a.n, shift, a.ws->rts_off.get(), a.tiles);
     if (a.kt) a.kt->end(tok, a.cls, 24.0 * (double)a.n, a.s);
   }
 };
@@ -570,7 +583,10 @@ bool sort_after_histogram(SortWorkspace& ws, uint64_t* keys, float* vals,
   }
   const uint64_t R = 1ull << rb;
   const uint64_t tiles = ceil_div(n, sort_tile_items(rb));
-  PassArgs a{&ws, keys, vals, ws.alt_keys.get(), ws.alt_vals.get(), n, tiles, 0, s, kt, cls};
+  const int tag = (cls == SPTC_K_SORT_PROD || cls == SPTC_K_SORT_PROD_WIDE) ? 1 : 0;
+  const int kcls = (cls == SPTC_K_SORT_PROD && rb > 8) ? SPTC_K_SORT_PROD_WIDE : cls;
+  PassArgs a{tag, &ws, keys, vals, ws.alt_keys.get(), ws.alt_vals.get(), n, tiles, 0, s, kt,
+             kcls};
   for (int p = 0; p < passes; ++p) {
     a.pass = p;
     if (onesweep) {

@@ -149,11 +149,14 @@ typedef enum {
   SPTC_K_SORT_INPUT = 1,  /* radix pass (scatter) over operand keys       */
   SPTC_K_MATCH = 2,       /* index match (reduce + emit)                  */
   SPTC_K_EXPAND = 3,      /* ESC product expansion                        */
-  SPTC_K_SORT_PROD = 4,   /* radix pass (scatter) over product keys       */
+  SPTC_K_SORT_PROD = 4,   /* radix pass (scatter) over product keys,
+                             8-bit digits                                */
   SPTC_K_REDUCE = 5,      /* reduce-by-key (count + emit)                 */
   SPTC_K_DENSE = 6,       /* dense per-row accumulation                   */
   SPTC_K_SORT_COUNT = 7,  /* radix pass digit counting + offset scan      */
-  SPTC_K_COUNT = 8
+  SPTC_K_SORT_PROD_WIDE = 8, /* radix pass (scatter) over product keys,
+                                digits wider than 8 bits                 */
+  SPTC_K_COUNT = 9
 } sptc_kernel_class;
 
 /* Launch count, summed HIP-event time (ms) and algorithmic HBM bytes of one

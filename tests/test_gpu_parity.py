@@ -270,3 +270,56 @@ def test_empty_and_disjoint(gpu):
                   np.zeros(0, np.float32))
     res = sptc.SpTC(E, Y, [1], [0])
     assert res.nnz == 0
+
+
+# ---- dense per-row accumulator ----------------------------------------------
+
+SMALL_COL = [c for c in FIXTURES if c["name"] in ("self_o3_c2", "self_o4_c23", "spgemm_o2",
+                                                  "pow2_dims", "zipf_heavy", "dim1_mode",
+                                                  "mixed_order_o4_o3", "outer_nc0",
+                                                  "scalar_full", "xy_o3_c21_c02")]
+
+
+@pytest.mark.parametrize("case", SMALL_COL, ids=[c["name"] for c in SMALL_COL])
+def test_dense_matches_golden(gpu, case):
+    res = sptc.SpTC(case["X"], case["Y"], case["cx"], case["cy"], dense_accumulator=1)
+    assert res.info.accumulator == 1
+    coords, v = _gpu_coords(res)
+    assert_same_coo(coords, v, case["zcoords"], case["zval"], rtol=REL_TOL)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_dense_equals_esc_bitwise(gpu, seed):
+    """Both accumulators sum each output's products in the same order with the
+    same compensated FP32 steps: identical indices and identical bits."""
+    rng = np.random.default_rng(500 + seed)
+    X = random_coo(rng, (400, 300, 50), 80_000, positive=bool(seed % 2))
+    Y = random_coo(rng, (50, 60, 90), 60_000, positive=bool(seed % 2))
+    d = sptc.SpTC(X, Y, [2], [0], dense_accumulator=1)
+    e = sptc.SpTC(X, Y, [2], [0], dense_accumulator=-1)
+    assert d.info.accumulator == 1 and e.info.accumulator == 0
+    for a, b in zip(d.to_host(), e.to_host()):
+        np.testing.assert_array_equal(a, b)
+    _compare_with_oracle(X, Y, [2], [0], d)
+
+
+def test_dense_rejected_when_it_cannot_apply(gpu):
+    rng = np.random.default_rng(9)
+    X = random_coo(rng, (300, 200, 40), 30_000)  # Y free space 300x200 > 2^13 slots
+    with pytest.raises(sptc.SpTCError) as e:
+        sptc.SpTC(X, X, [2], [2], dense_accumulator=1)
+    assert e.value.status == sptc.ERR_BAD_ARGS
+
+
+def test_duplicate_coordinates_sum_like_the_oracle(gpu):
+    """The .tns reader keeps duplicate entries (io_manager.tpp:210-240); ESC
+    sums their products like any equal keys, dense refuses them."""
+    rng = np.random.default_rng(10)
+    X = random_coo(rng, (30, 20, 10), 1500)
+    dup = COOTensor(X.dims, [np.concatenate([a, a[:200]]) for a in X.idx],
+                    np.concatenate([X.val, X.val[:200] * 0.5]))
+    res = sptc.SpTC(dup, dup, [2], [2])
+    assert res.info.accumulator == 0
+    _compare_with_oracle(dup, dup, [2], [2], res)
+    with pytest.raises(sptc.SpTCError):
+        sptc.SpTC(dup, dup, [2], [2], dense_accumulator=1)
