@@ -207,11 +207,11 @@ def test_contract_random_matches_oracle(gpu, seed):
 def test_chunked_equals_unchunked(gpu):
     rng = np.random.default_rng(7)
     X = random_coo(rng, (60, 50, 40), 60_000)
-    full = sptc.SpTC(X, X, [2], [2])
+    full = sptc.SpTC(X, X, [2], [2], dense_accumulator=-1)
     assert full.info.nchunks == 1
     # room for the output plus about a third of the products' ESC buffers:
     # forces several accumulation chunks
-    small = sptc.SpTC(X, X, [2], [2],
+    small = sptc.SpTC(X, X, [2], [2], dense_accumulator=-1,
                       mem_budget_bytes=full.nnz * 16 + full.ir_nnz * 8 + (128 << 20))
     assert small.info.nchunks > 1
     a = full.to_host()
@@ -220,6 +220,10 @@ def test_chunked_equals_unchunked(gpu):
         np.testing.assert_array_equal(x, y)
     cnt = sptc.SpTC(X, X, [2], [2], count_only=True)
     assert cnt.nnz == full.nnz and cnt.ir_nnz == full.ir_nnz
+    auto = sptc.SpTC(X, X, [2], [2])  # narrow column space: the planner picks dense
+    assert auto.info.accumulator == 1
+    for x, y in zip(a, auto.to_host()):
+        np.testing.assert_array_equal(x, y)
 
 
 def test_plain_small_config(gpu):
