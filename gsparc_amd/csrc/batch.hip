@@ -1,0 +1,677 @@
+// Batched (multi-contraction) kernels for gfx950; see batch.hpp.
+#include "batch.hpp"
+#include "sort_core.hpp"
+
+#include <algorithm>
+#include <stdexcept>
+
+namespace sptc {
+
+namespace {
+
+// Segment of global index i: the s with off[s] <= i < off[s+1] (off is
+// non-decreasing with nseg+1 entries; empty segments are skipped).
+__device__ __forceinline__ uint32_t find_seg(const uint64_t* off, uint32_t nseg, uint64_t i) {
+  uint32_t lo = 0, hi = nseg;  // last s with off[s] <= i
+  while (lo + 1 < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= i) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// ---- segmented reduce-then-scan radix pass (8-bit digits, 512 x 12 tiles) --
+constexpr int kSegRB = 8;
+constexpr int kSegIPT = 12;
+constexpr int kSegNT = 512;
+constexpr uint64_t kSegTile = (uint64_t)kSegIPT * kSegNT;
+
+__global__ __launch_bounds__(kSegNT) void k_seg_upsweep(const uint64_t* __restrict__ kin,
+                                                        int shift, const SegTile* __restrict__ tiles,
+                                                        uint32_t* __restrict__ cnt) {
+  constexpr int R = 1 << kSegRB;
+  __shared__ uint32_t sh[R];
+  for (int i = threadIdx.x; i < R; i += kSegNT) sh[i] = 0;
+  __syncthreads();
+  const SegTile t = tiles[blockIdx.x];
+  const uint64_t end = t.start + t.count;
+#pragma unroll
+  for (int i = 0; i < kSegIPT; ++i) {
+    const uint64_t idx = t.start + (uint64_t)i * kSegNT + threadIdx.x;
+    const bool valid = idx < end;
+    const uint64_t k = valid ? kin[idx] : 0;
+    hist_add_wave<kSegRB>(sh, k >> shift, valid, 1);
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < R; d += kSegNT) cnt[t.cbase + (uint64_t)d * t.cstride] = sh[d];
+}
+
+__global__ __launch_bounds__(kSegNT) void k_seg_downsweep(
+    const uint64_t* __restrict__ kin, const float* __restrict__ vin, uint64_t* __restrict__ kout,
+    float* __restrict__ vout, int shift, const SegTile* __restrict__ tiles,
+    const uint32_t* __restrict__ offs) {
+  constexpr int R = 1 << kSegRB;
+  __shared__ TileSmem<kSegRB, kSegIPT, kSegNT> sm;
+  const uint32_t tid = threadIdx.x;
+  const SegTile t = tiles[blockIdx.x];
+  for (int i = tid; i < (kSegNT / kWave) * R; i += kSegNT) (&sm.cnt[0][0])[i] = 0;
+  for (int d = tid; d < R; d += kSegNT) sm.gbase[d] = offs[t.cbase + (uint64_t)d * t.cstride];
+  __syncthreads();
+  const uint64_t end = t.start + t.count;
+  const uint64_t wbase = t.start + (uint64_t)(tid / kWave) * kWave * kSegIPT;
+  uint64_t k[kSegIPT];
+  float v[kSegIPT];
+  uint32_t pos[kSegIPT];
+  tile_load_rank<kSegRB, kSegIPT, kSegNT>(sm, kin, vin, end, shift, wbase, k, v, pos);
+  __syncthreads();
+  for (int d = tid; d < R; d += kSegNT)
+    sm.dstart[d] = (uint16_t)wave_prefixes<kSegRB, kSegIPT, kSegNT>(sm, d);
+  __syncthreads();
+  tile_stage_scatter<kSegRB, kSegIPT, kSegNT>(sm, end, shift, t.start, wbase, k, v, pos, kout,
+                                              vout);
+}
+
+struct SegCountScanOp {
+  const uint32_t* in;
+  uint32_t* out;
+  __device__ U2 value(uint64_t i) const { return mk2(in[i], 0); }
+  __device__ void emit(uint64_t i, U2 excl, U2) const { out[i] = (uint32_t)excl.a; }
+};
+
+// ---- batched linearize -------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_blinearize(const BLinArgs* __restrict__ args,
+                                                       const uint64_t* __restrict__ seg_off,
+                                                       uint32_t nseg, uint64_t n,
+                                                       uint64_t* __restrict__ keys,
+                                                       float* __restrict__ vals) {
+  uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (; i < n; i += stride) {
+    const uint32_t sgm = find_seg(seg_off, nseg, i);
+    const BLinArgs& a = args[sgm];
+    const uint64_t li = i - seg_off[sgm];
+    uint64_t k = 0;
+    for (int m = 0; m < a.order; ++m) {
+      const uint64_t x = a.idx_bytes == 4 ? (uint64_t) static_cast<const uint32_t*>(a.idx[m])[li]
+                                          : static_cast<const uint64_t*>(a.idx[m])[li];
+      k |= shl64(x & low_mask(a.width[m]), a.shift[m]);
+    }
+    keys[i] = k;
+    vals[i] = a.val[li];
+  }
+}
+
+// ---- Y free-key rank compression --------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_bfree_extract(const uint64_t* __restrict__ ykeys,
+                                                          const uint64_t* __restrict__ yoff,
+                                                          uint32_t B,
+                                                          const uint64_t* __restrict__ fmask,
+                                                          uint64_t n, uint64_t* __restrict__ out) {
+  uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (; i < n; i += stride) out[i] = ykeys[i] & fmask[find_seg(yoff, B, i)];
+}
+
+struct BUniqueOp {
+  const uint64_t* keys;
+  const uint64_t* seg_off;
+  uint32_t nseg;
+  uint64_t* table;
+  uint64_t* seg_first;
+  __device__ U2 value(uint64_t i) const {
+    const uint32_t s = find_seg(seg_off, nseg, i);
+    return mk2(i == seg_off[s] || keys[i] != keys[i - 1], 0);
+  }
+  __device__ void emit(uint64_t i, U2 excl, U2 v) const {
+    const uint32_t s = find_seg(seg_off, nseg, i);
+    if (i == seg_off[s]) seg_first[s] = excl.a;
+    if (v.a) table[excl.a] = keys[i];
+  }
+};
+
+__global__ __launch_bounds__(kBlock) void k_brank(const uint64_t* __restrict__ ykeys,
+                                                  const uint64_t* __restrict__ yoff, uint32_t B,
+                                                  const int* __restrict__ ynf,
+                                                  const int* __restrict__ kyf,
+                                                  const uint64_t* __restrict__ ytab,
+                                                  const uint64_t* __restrict__ yt_off, uint64_t n,
+                                                  uint64_t* __restrict__ out) {
+  uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (; i < n; i += stride) {
+    const uint32_t s = find_seg(yoff, B, i);
+    const uint64_t k = ykeys[i];
+    const uint64_t f = k & low_mask(ynf[s]);
+    uint64_t lo = yt_off[s], hi = yt_off[s + 1];
+    const uint64_t base = lo;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (ytab[mid] < f) lo = mid + 1; else hi = mid;
+    }
+    out[i] = shl64(shr64(k, ynf[s]), kyf[s]) | (lo - base);
+  }
+}
+
+struct BContractOp {
+  const uint64_t* ykeys;
+  const uint64_t* yoff;
+  uint32_t B;
+  const int* kyf;
+  uint64_t n;
+  uint64_t* ukey;
+  uint64_t* ustart;
+  uint64_t* u_off;
+  __device__ U2 value(uint64_t i) const {
+    const uint32_t s = find_seg(yoff, B, i);
+    return mk2(i == yoff[s] || shr64(ykeys[i], kyf[s]) != shr64(ykeys[i - 1], kyf[s]), 0);
+  }
+  __device__ void emit(uint64_t i, U2 excl, U2 v) const {
+    const uint32_t s = find_seg(yoff, B, i);
+    if (i == yoff[s]) u_off[s] = excl.a;
+    if (v.a) {
+      ukey[excl.a] = shr64(ykeys[i], kyf[s]);
+      ustart[excl.a] = i;
+    }
+    if (i == n - 1) ustart[excl.a + v.a] = n;
+  }
+};
+
+// value: a = match count; b = (active << 32) | row_head
+struct BMatchOp {
+  const uint64_t* xkeys;
+  const float* xvals;
+  const uint64_t* xoff;
+  uint32_t B;
+  const BInst* inst;
+  const uint64_t* ukey;
+  const uint64_t* ustart;
+  const uint64_t* u_off;  // [B+1]
+  BMatchOut o;
+
+  __device__ void lookup(uint32_t s, uint64_t c, uint64_t& pos, uint64_t& cnt) const {
+    uint64_t lo = u_off[s], hi = u_off[s + 1];
+    const uint64_t end = hi;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (ukey[mid] < c) lo = mid + 1; else hi = mid;
+    }
+    if (lo < end && ukey[lo] == c) {
+      pos = ustart[lo];
+      cnt = ustart[lo + 1] - pos;
+    } else {
+      pos = 0;
+      cnt = 0;
+    }
+  }
+  __device__ U2 value(uint64_t i) const {
+    const uint32_t s = find_seg(xoff, B, i);
+    const BInst in = inst[s];
+    const uint64_t k = xkeys[i];
+    uint64_t pos, cnt;
+    lookup(s, k & in.cmask, pos, cnt);
+    const uint64_t head =
+        (i == xoff[s]) || (shr64(xkeys[i - 1], in.ncbits) != shr64(k, in.ncbits));
+    return mk2(cnt, ((uint64_t)(cnt != 0) << 32) | head);
+  }
+  __device__ void emit(uint64_t i, U2 excl, U2 v) const {
+    const uint32_t s = find_seg(xoff, B, i);
+    const BInst in = inst[s];
+    const uint64_t k = xkeys[i];
+    uint64_t pos, cnt;
+    lookup(s, k & in.cmask, pos, cnt);
+    const uint64_t head = v.b & 1;
+    const uint64_t row = (excl.b & 0xFFFFFFFFull) + head - 1;
+    if (i == xoff[s]) {
+      o.inst_P[s] = excl.a;
+      o.inst_A[s] = excl.b >> 32;
+      o.inst_R[s] = excl.b & 0xFFFFFFFFull;
+    }
+    if (head) {
+      o.row_key[row] = shr64(k, in.ncbits);
+      o.row_prod[row] = excl.a;
+      o.row_act[row] = excl.b >> 32;
+    }
+    if (cnt) {
+      const uint64_t j = excl.b >> 32;
+      o.act_prefix[j] = excl.a;
+      o.act_ypos[j] = pos;
+      o.act_xval[j] = xvals[i];
+      o.act_row[j] = (uint32_t)row;
+    }
+  }
+};
+
+// ---- batched ESC expand ----------------------------------------------------
+constexpr int kBExpandItems = 8;
+constexpr uint32_t kBExpandTile = kBlock * kBExpandItems;
+
+__global__ __launch_bounds__(kBlock) void k_bexpand(BExpandArgs b) {
+  const ExpandArgs& a = b.e;
+  __shared__ uint64_t s_pref[kBExpandTile + 1];
+  __shared__ uint64_t s_key[kBExpandTile];
+  __shared__ float s_val[kBExpandTile];
+  __shared__ uint64_t s_ja;
+  __shared__ double s_sum[kBlock / kWave];
+  double vsum = 0.0;
+  const uint64_t nprod = a.P1 - a.P0;
+  const uint64_t ntiles = ceil_div(nprod, kBExpandTile);
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint64_t pb = a.P0 + tile * kBExpandTile;
+    const uint64_t pe = pb + kBExpandTile < a.P1 ? pb + kBExpandTile : a.P1;
+    const uint32_t cnt = (uint32_t)(pe - pb);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint64_t lo = a.A0, hi = a.A1;
+      while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (a.act_prefix[mid] <= pb) lo = mid + 1; else hi = mid;
+      }
+      s_ja = lo - 1;
+    }
+    __syncthreads();
+    const uint64_t ja = s_ja;
+    const uint64_t avail = a.A1 - ja;
+    const uint32_t m = (uint32_t)(avail < kBExpandTile + 1 ? avail : kBExpandTile + 1);
+    for (uint32_t k = threadIdx.x; k < m; k += kBlock) s_pref[k] = a.act_prefix[ja + k];
+    __syncthreads();
+    const uint32_t q0 = threadIdx.x * kBExpandItems;
+    if (q0 < cnt) {
+      const uint64_t p0 = pb + q0;
+      uint32_t lo = 0, hi = m;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_pref[mid] <= p0) lo = mid + 1; else hi = mid;
+      }
+      uint32_t k = lo - 1;
+      uint64_t j = ja + k;
+      auto load_owner = [&](uint64_t jj, uint32_t kk, uint64_t& ybase, float& xv,
+                            uint64_t& rbits, uint64_t& cmask) {
+        ybase = a.act_ypos[jj] - s_pref[kk];
+        xv = a.act_xval[jj];
+        const uint32_t row = a.act_row[jj];
+        rbits = shl64((uint64_t)(row - a.r0), a.ynf);
+        cmask = low_mask(b.inst[find_seg(b.inst_R, b.B, row)].kyf);
+      };
+      uint64_t ybase, rbits, cmask;
+      float xv;
+      load_owner(j, k, ybase, xv, rbits, cmask);
+      uint64_t next = k + 1 < m ? s_pref[k + 1] : ~0ull;
+#pragma unroll
+      for (int e = 0; e < kBExpandItems; ++e) {
+        const uint64_t p = p0 + e;
+        if (q0 + e < cnt) {
+          while (p >= next) {
+            ++k;
+            j = ja + k;
+            load_owner(j, k, ybase, xv, rbits, cmask);
+            next = k + 1 < m ? s_pref[k + 1] : ~0ull;
+          }
+          const uint64_t y = ybase + p;
+          const float v = xv * a.yvals[y];
+          s_key[q0 + e] = rbits | (a.ykeys[y] & cmask);
+          s_val[q0 + e] = v;
+          vsum += (double)v;
+        }
+      }
+    }
+    __syncthreads();
+    const uint64_t obase = pb - a.P0;
+    for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) {
+      a.out_keys[obase + i] = s_key[i];
+      a.out_vals[obase + i] = s_val[i];
+    }
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1) vsum += __shfl_xor(vsum, off, kWave);
+  if (lane_id() == 0) s_sum[threadIdx.x / kWave] = vsum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) t += s_sum[w];
+    atomicAdd(a.vsum, t);
+  }
+}
+
+// ---- batched reduce emit ---------------------------------------------------
+struct BReduceOp {
+  const uint64_t* keys;
+  const float* vals;
+  uint64_t n;
+  uint32_t* out_row;
+  uint64_t* out_col;
+  float* out_val;
+  uint64_t r0;
+  int K;
+  BReduceArgs b;
+};
+
+__global__ __launch_bounds__(kBlock) void k_breduce_emit(BReduceOp op, const U2* tile_offs) {
+  __shared__ uint64_t s_k[kScanTile];
+  __shared__ float s_v[kScanTile];
+  __shared__ U2 scratch[kBlock / kWave];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  const uint32_t cnt = (uint32_t)(op.n - base < kScanTile ? op.n - base : kScanTile);
+  for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) {
+    s_k[i] = op.keys[base + i];
+    s_v[i] = op.vals[base + i];
+  }
+  const uint64_t prev_key = base ? op.keys[base - 1] : 0;
+  __syncthreads();
+  const uint32_t q0 = threadIdx.x * kScanItems;
+  uint64_t ok[kScanItems];
+  float ov[kScanItems];
+  uint32_t nh = 0;
+#pragma unroll
+  for (int e = 0; e < kScanItems; ++e) {
+    const uint32_t i = q0 + e;
+    ok[e] = 0;
+    ov[e] = 0.f;
+    if (i >= cnt) continue;
+    const uint64_t k = s_k[i];
+    const uint64_t pk = i ? s_k[i - 1] : prev_key;
+    if (!(base + i == 0 || k != pk)) continue;
+    KahanF32 acc;
+    acc.add(s_v[i]);
+    uint32_t jj = i + 1;
+    while (jj < cnt && s_k[jj] == k) acc.add(s_v[jj++]);
+    if (jj == cnt) {
+      for (uint64_t g = base + cnt; g < op.n && op.keys[g] == k; ++g) acc.add(op.vals[g]);
+    }
+    ok[nh] = k;
+    ov[nh] = acc.value();
+    ++nh;
+  }
+  U2 tot;
+  const uint32_t o0 = (uint32_t)block_excl_scan_u2(mk2(nh, 0), scratch, &tot).a;
+  const uint32_t nout = (uint32_t)tot.a;
+#pragma unroll
+  for (int e = 0; e < kScanItems; ++e) {
+    if ((uint32_t)e < nh) {
+      s_k[o0 + e] = ok[e];
+      s_v[o0 + e] = ov[e];
+    }
+  }
+  __syncthreads();
+  const uint64_t obase = tile_offs[blockIdx.x].a;
+  const uint64_t cm = low_mask(op.K);
+  for (uint32_t i = threadIdx.x; i < nout; i += kBlock) {
+    const uint64_t k = s_k[i];
+    const uint64_t row = op.r0 + shr64(k, op.K);
+    const BInst in = op.b.inst[find_seg(op.b.inst_R, op.b.B, row)];
+    op.out_row[obase + i] = (uint32_t)row;
+    op.out_col[obase + i] = op.b.ytab[in.yt_off + (k & cm)];
+    op.out_val[obase + i] = s_v[i];
+  }
+}
+
+// ---- batched dense ---------------------------------------------------------
+// Block b handles the b-th row of the group: group entry g (binary search in
+// grp_rows), global row grp_row0[g] + (b - grp_rows[g]).
+__device__ __forceinline__ void bdense_row(const BDenseArgs& a, uint64_t b, uint64_t& row,
+                                           BInst& in) {
+  const uint32_t g = find_seg(a.grp_rows, a.G, b);
+  row = a.grp_row0[g] + (b - a.grp_rows[g]);
+  in = a.inst[a.grp_inst[g]];
+}
+
+template <int EB>
+__global__ __launch_bounds__(kBlock) void k_bdense_count(BDenseArgs a) {
+  constexpr int E = 1 << EB;
+  constexpr int W = E / 32;
+  __shared__ uint32_t bm[W];
+  __shared__ uint32_t s_red[kBlock / kWave];
+  uint64_t r;
+  BInst in;
+  bdense_row(a, blockIdx.x, r, in);
+  const uint64_t cm = low_mask(in.kyf);
+  for (int i = threadIdx.x; i < W; i += kBlock) bm[i] = 0;
+  __syncthreads();
+  const uint64_t j0 = a.d.row_act[r], j1 = a.d.row_act[r + 1];
+  for (uint64_t j = j0; j < j1; ++j) {
+    const uint64_t y0 = a.d.act_ypos[j];
+    const uint64_t c = a.d.act_prefix[j + 1] - a.d.act_prefix[j];
+    for (uint64_t q = threadIdx.x; q < c; q += kBlock) {
+      const uint32_t rank = (uint32_t)(a.d.ykeys[y0 + q] & cm);
+      atomicOr(&bm[rank >> 5], 1u << (rank & 31));
+    }
+  }
+  __syncthreads();
+  uint32_t cnt = 0;
+  for (int i = threadIdx.x; i < W; i += kBlock) cnt += __popc(bm[i]);
+  for (int off = kWave / 2; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, kWave);
+  if (lane_id() == 0) s_red[threadIdx.x / kWave] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) t += s_red[w];
+    a.d.row_cnt[blockIdx.x] = t;
+  }
+}
+
+template <int EB>
+__global__ __launch_bounds__(kBlock) void k_bdense_numeric(BDenseArgs a) {
+  constexpr int E = 1 << EB;
+  constexpr int W = E / 32;
+  constexpr int S = E / kBlock;
+  __shared__ float s_sum[E];
+  __shared__ float s_cmp[E];
+  __shared__ uint32_t bm[W];
+  __shared__ uint32_t s_scan[kBlock / kWave];
+  uint64_t r;
+  BInst in;
+  bdense_row(a, blockIdx.x, r, in);
+  const uint64_t cm = low_mask(in.kyf);
+  for (int i = threadIdx.x; i < E; i += kBlock) {
+    s_sum[i] = 0.f;
+    s_cmp[i] = 0.f;
+  }
+  for (int i = threadIdx.x; i < W; i += kBlock) bm[i] = 0;
+  __syncthreads();
+  const uint64_t j0 = a.d.row_act[r], j1 = a.d.row_act[r + 1];
+  double vsum = 0.0;
+  for (uint64_t j = j0; j < j1; ++j) {
+    const uint64_t y0 = a.d.act_ypos[j];
+    const uint64_t c = a.d.act_prefix[j + 1] - a.d.act_prefix[j];
+    const float xv = a.d.act_xval[j];
+    for (uint64_t q = threadIdx.x; q < c; q += kBlock) {
+      const uint64_t y = y0 + q;
+      const uint32_t rank = (uint32_t)(a.d.ykeys[y] & cm);
+      const float v = xv * a.d.yvals[y];
+      vsum += (double)v;
+      const float sm = s_sum[rank];
+      const float t = sm + v;
+      if (fabsf(sm) >= fabsf(v)) s_cmp[rank] += (sm - t) + v;
+      else s_cmp[rank] += (v - t) + sm;
+      s_sum[rank] = t;
+      atomicOr(&bm[rank >> 5], 1u << (rank & 31));
+    }
+    __syncthreads();
+  }
+  const uint32_t first = threadIdx.x * S;
+  const uint32_t word = bm[first >> 5];
+  uint32_t bits;
+  if constexpr (S == 32) bits = word;
+  else bits = (word >> (first & 31)) & ((1u << S) - 1);
+  uint32_t tot;
+  uint32_t o = block_excl_scan<uint32_t>((uint32_t)__popc(bits), s_scan, &tot);
+  const uint64_t obase = a.d.out_off[blockIdx.x];
+  for (int e = 0; e < S; ++e) {
+    if ((bits >> e) & 1u) {
+      const uint32_t rank = first + e;
+      a.d.out_row[obase + o] = (uint32_t)r;
+      a.d.out_col[obase + o] = a.ytab[in.yt_off + rank];
+      a.d.out_val[obase + o] = s_sum[rank] + s_cmp[rank];
+      ++o;
+    }
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1) vsum += __shfl_xor(vsum, off, kWave);
+  if (lane_id() == 0 && vsum != 0.0) atomicAdd(a.d.vsum, vsum);
+}
+
+__global__ void k_row_bounds(const uint32_t* rows, uint64_t n, const uint64_t* q, uint32_t nq,
+                             uint64_t* out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const uint64_t v = q[i];
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if ((uint64_t)rows[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  out[i] = lo;
+}
+
+uint32_t grid_for(uint64_t n, uint32_t cap = 4096) {
+  return (uint32_t)std::min<uint64_t>(ceil_div(n ? n : 1, kBlock), cap);
+}
+
+}  // namespace
+
+// ---- host launchers --------------------------------------------------------
+
+void SegSortPlan::build(const std::vector<uint64_t>& seg_off, hipStream_t s) {
+  constexpr uint64_t R = 1ull << kSegRB;
+  std::vector<SegTile> tv;
+  uint64_t tiles_before = 0;
+  for (size_t g = 0; g + 1 < seg_off.size(); ++g) {
+    const uint64_t len = seg_off[g + 1] - seg_off[g];
+    const uint64_t nt = ceil_div(len, kSegTile);
+    for (uint64_t t = 0; t < nt; ++t) {
+      SegTile d;
+      d.start = seg_off[g] + t * kSegTile;
+      d.count = (uint32_t)std::min<uint64_t>(kSegTile, len - t * kSegTile);
+      d.cbase = R * tiles_before + t;
+      d.cstride = nt;
+      d.pad = 0;
+      tv.push_back(d);
+    }
+    tiles_before += nt;
+  }
+  n = seg_off.empty() ? 0 : seg_off.back();
+  if (n >= (1ull << 32)) throw std::length_error("segmented sort of 2^32 or more keys");
+  ntiles = tv.size();
+  tiles.reset(ntiles ? ntiles : 1);
+  if (ntiles)
+    SPTC_HIP(hipMemcpyAsync(tiles.get(), tv.data(), ntiles * sizeof(SegTile),
+                            hipMemcpyHostToDevice, s));
+  SPTC_HIP(hipStreamSynchronize(s));  // tv is released on return
+}
+
+void segmented_sort_reserve(SortWorkspace& ws, const SegSortPlan& plan, hipStream_t s) {
+  const uint64_t m = plan.ntiles << kSegRB;
+  ws.ensure(plan.n ? plan.n : 1, kSegRB, s);
+  if (ws.rts_cnt.size() < m) {
+    ws.rts_cnt.reset(m);
+    ws.rts_off.reset(m);
+  }
+  if (ws.rts_tmp.size() < scan_tmp_count(m) + 1) ws.rts_tmp.reset(scan_tmp_count(m) + 1);
+}
+
+bool segmented_sort(SortWorkspace& ws, const SegSortPlan& plan, uint64_t* keys, float* vals,
+                    int nbits, hipStream_t s, KTimer* kt, int cls) {
+  const uint64_t n = plan.n;
+  const int passes = radix_passes(nbits, kSegRB);
+  if (!n || !passes) return false;
+  const uint64_t m = plan.ntiles << kSegRB;
+  if (ws.rts_cnt.size() < m || ws.alt_keys.size() < n)
+    throw std::logic_error("segmented sort workspace not reserved");
+  uint64_t* ka = keys;
+  float* va = vals;
+  uint64_t* kb = ws.alt_keys.get();
+  float* vb = ws.alt_vals.get();
+  for (int p = 0; p < passes; ++p) {
+    const int shift = p * kSegRB;
+    int tok = kt ? kt->begin(s) : -1;
+    hipLaunchKernelGGL(k_seg_upsweep, dim3((uint32_t)plan.ntiles), dim3(kSegNT), 0, s, ka,
+                       shift, plan.tiles.get(), ws.rts_cnt.get());
+    run_scan(SegCountScanOp{ws.rts_cnt.get(), ws.rts_off.get()}, m, ws.rts_tmp.get(), s);
+    if (kt) kt->end(tok, SPTC_K_SORT_COUNT, 8.0 * (double)n + 12.0 * (double)m, s);
+    tok = kt ? kt->begin(s) : -1;
+    hipLaunchKernelGGL(k_seg_downsweep, dim3((uint32_t)plan.ntiles), dim3(kSegNT), 0, s, ka, va,
+                       kb, vb, shift, plan.tiles.get(), ws.rts_off.get());
+    if (kt) kt->end(tok, cls, 24.0 * (double)n, s);
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  return (passes & 1) != 0;
+}
+
+void launch_blinearize(const BLinArgs* args, const uint64_t* seg_off, uint32_t nseg,
+                       uint64_t n, uint64_t* keys, float* vals, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_blinearize, dim3(grid_for(n)), dim3(kBlock), 0, s, args, seg_off, nseg,
+                     n, keys, vals);
+}
+
+void launch_bfree_extract(const uint64_t* ykeys, const uint64_t* yoff, uint32_t B,
+                          const uint64_t* fmask, uint64_t n, uint64_t* out, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_bfree_extract, dim3(grid_for(n)), dim3(kBlock), 0, s, ykeys, yoff, B,
+                     fmask, n, out);
+}
+
+void run_bunique(const uint64_t* sorted, const uint64_t* seg_off, uint32_t nseg, uint64_t n,
+                 uint64_t* table, uint64_t* seg_first, U2* tmp, hipStream_t s) {
+  run_scan(BUniqueOp{sorted, seg_off, nseg, table, seg_first}, n, tmp, s);
+}
+
+void launch_brank(const uint64_t* ykeys, const uint64_t* yoff, uint32_t B, const int* ynf,
+                  const int* kyf, const uint64_t* ytab, const uint64_t* yt_off, uint64_t n,
+                  uint64_t* out, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_brank, dim3(grid_for(n)), dim3(kBlock), 0, s, ykeys, yoff, B, ynf, kyf,
+                     ytab, yt_off, n, out);
+}
+
+void run_bycontract(const uint64_t* ykeys, const uint64_t* yoff, uint32_t B, const int* kyf,
+                    uint64_t n, uint64_t* ukey, uint64_t* ustart, uint64_t* u_off, U2* tmp,
+                    hipStream_t s) {
+  run_scan(BContractOp{ykeys, yoff, B, kyf, n, ukey, ustart, u_off}, n, tmp, s);
+}
+
+void run_bmatch(const uint64_t* xkeys, const float* xvals, const uint64_t* xoff, uint32_t B,
+                const BInst* inst, const uint64_t* ukey, const uint64_t* ustart,
+                const uint64_t* u_off, uint64_t nx, const BMatchOut& o, U2* tmp, hipStream_t s) {
+  run_scan(BMatchOp{xkeys, xvals, xoff, B, inst, ukey, ustart, u_off, o}, nx, tmp, s);
+}
+
+void launch_bexpand(const BExpandArgs& a, hipStream_t s) {
+  const uint64_t nprod = a.e.P1 - a.e.P0;
+  if (!nprod) return;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(nprod, kBExpandTile), 2048);
+  hipLaunchKernelGGL(k_bexpand, dim3(grid), dim3(kBlock), 0, s, a);
+}
+
+void run_breduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32_t* out_row,
+                      uint64_t* out_col, float* out_val, uint64_t r0, int K,
+                      const BReduceArgs& b, const U2* tmp, hipStream_t s) {
+  if (!n) return;
+  BReduceOp op{keys, vals, n, out_row, out_col, out_val, r0, K, b};
+  hipLaunchKernelGGL(k_breduce_emit, dim3((uint32_t)scan_tmp_count(n)), dim3(kBlock), 0, s, op,
+                     tmp);
+}
+
+void launch_bdense_count(const BDenseArgs& a, int eb, uint64_t rows, hipStream_t s) {
+  if (!rows) return;
+  switch (eb) {
+    case 8: hipLaunchKernelGGL(k_bdense_count<8>, dim3((uint32_t)rows), dim3(kBlock), 0, s, a); break;
+    case 10: hipLaunchKernelGGL(k_bdense_count<10>, dim3((uint32_t)rows), dim3(kBlock), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_bdense_count<13>, dim3((uint32_t)rows), dim3(kBlock), 0, s, a); break;
+  }
+}
+
+void launch_bdense_numeric(const BDenseArgs& a, int eb, uint64_t rows, hipStream_t s) {
+  if (!rows) return;
+  switch (eb) {
+    case 8: hipLaunchKernelGGL(k_bdense_numeric<8>, dim3((uint32_t)rows), dim3(kBlock), 0, s, a); break;
+    case 10: hipLaunchKernelGGL(k_bdense_numeric<10>, dim3((uint32_t)rows), dim3(kBlock), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_bdense_numeric<13>, dim3((uint32_t)rows), dim3(kBlock), 0, s, a); break;
+  }
+}
+
+void launch_row_bounds(const uint32_t* rows, uint64_t n, const uint64_t* q, uint32_t nq,
+                       uint64_t* out, hipStream_t s) {
+  if (!nq) return;
+  hipLaunchKernelGGL(k_row_bounds, dim3(ceil_div(nq, 256)), dim3(256), 0, s, rows, n, q, nq, out);
+}
+
+}  // namespace sptc

@@ -1,0 +1,139 @@
+// Batched (multi-contraction) kernels and their host launchers.
+//
+// A batch concatenates the operands of B contractions into shared arrays
+// with per-instance offset tables; every phase is one launch over all
+// instances.  Row ids are global across the batch, so product keys
+// (row << K | column rank) sort every instance's products with one
+// unsegmented radix sort; operand sorts are segmented (one segment per
+// operand, tiles never straddle segments).
+#pragma once
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace sptc {
+
+constexpr int kBatchMaxInstances = 4096;
+
+// One tile of a segmented sort.
+struct SegTile {
+  uint64_t start;    // first key (global index)
+  uint64_t cbase;    // count-matrix index of digit 0 of this tile
+  uint64_t cstride;  // distance between digits (tiles in the segment)
+  uint32_t count;    // keys in the tile
+  uint32_t pad;
+};
+
+// Per-operand linearize arguments (device table, one per segment).
+struct BLinArgs {
+  const void* idx[SPTC_MAX_ORDER];
+  int shift[SPTC_MAX_ORDER];
+  int width[SPTC_MAX_ORDER];
+  int order;
+  int idx_bytes;
+  const float* val;
+};
+
+// Per-instance parameters used by the match/accumulate kernels.
+struct BInst {
+  uint64_t cmask;    // X contract field mask
+  int ncbits;        // X contract field width
+  int kyf;           // Y column (rank) field width
+  uint64_t yt_off;   // first entry of this instance's rank -> free-key table
+};
+
+// Tile table of a segmented sort: each segment [seg_off[s], seg_off[s+1])
+// is cut into its own tiles; the count matrix is segment-major and digit-
+// major within a segment, so one flat exclusive scan yields every tile's
+// global digit offsets.  Built once per batch shape (one host->device copy).
+struct SegSortPlan {
+  DevBuf<SegTile> tiles;
+  uint64_t ntiles = 0;
+  uint64_t n = 0;
+  void build(const std::vector<uint64_t>& seg_off, hipStream_t s);
+};
+// Sorts each segment of (keys, vals) on bits [0, nbits) with 8-bit
+// reduce-then-scan passes; ws must hold plan.n keys and the plan's count
+// matrix (segmented_sort_reserve).  Launches only (no allocation or host
+// synchronization), so it can be replayed and captured.  Returns true when
+// the result ended in the workspace's alternate buffers.
+void segmented_sort_reserve(SortWorkspace& ws, const SegSortPlan& plan, hipStream_t s);
+bool segmented_sort(SortWorkspace& ws, const SegSortPlan& plan, uint64_t* keys, float* vals,
+                    int nbits, hipStream_t s, KTimer* kt = nullptr, int cls = SPTC_K_SORT_INPUT);
+
+void launch_blinearize(const BLinArgs* args, const uint64_t* seg_off, uint32_t nseg,
+                       uint64_t n, uint64_t* keys, float* vals, hipStream_t s);
+
+// Y free keys (key & fmask of the segment's instance) for rank compression.
+void launch_bfree_extract(const uint64_t* ykeys, const uint64_t* yoff, uint32_t B,
+                          const uint64_t* fmask, uint64_t n, uint64_t* out, hipStream_t s);
+// Distinct values per segment of a segmented-sorted array: table[] gets
+// them, seg_first[s] the table index of segment s's first distinct value.
+void run_bunique(const uint64_t* sorted, const uint64_t* seg_off, uint32_t nseg, uint64_t n,
+                 uint64_t* table, uint64_t* seg_first, U2* tmp, hipStream_t s);
+// Y key (c << ynf | f) -> (c << kyf | rank of f in its instance's table).
+void launch_brank(const uint64_t* ykeys, const uint64_t* yoff, uint32_t B, const int* ynf,
+                  const int* kyf, const uint64_t* ytab, const uint64_t* yt_off, uint64_t n,
+                  uint64_t* out, hipStream_t s);
+// Distinct contract values per Y instance: ukey/ustart (global Y index),
+// u_off[s] = first table entry of instance s; ustart[total] = n.
+void run_bycontract(const uint64_t* ykeys, const uint64_t* yoff, uint32_t B, const int* kyf,
+                    uint64_t n, uint64_t* ukey, uint64_t* ustart, uint64_t* u_off, U2* tmp,
+                    hipStream_t s);
+
+struct BMatchOut {
+  uint64_t* act_prefix;
+  uint64_t* act_ypos;
+  float* act_xval;
+  uint32_t* act_row;
+  uint64_t* row_key;
+  uint64_t* row_prod;
+  uint64_t* row_act;
+  uint64_t* inst_P;  // [B] first product / active entry / row of instance s
+  uint64_t* inst_A;
+  uint64_t* inst_R;
+};
+void run_bmatch(const uint64_t* xkeys, const float* xvals, const uint64_t* xoff, uint32_t B,
+                const BInst* inst, const uint64_t* ukey, const uint64_t* ustart,
+                const uint64_t* u_off, uint64_t nx, const BMatchOut& o, U2* tmp, hipStream_t s);
+
+// ESC expand over global rows [r0, r1) (ESC instances only): product key =
+// (row - r0) << K | column rank; inst_R (B+1, global row starts) maps rows
+// to instances for their column field width.
+struct BExpandArgs {
+  ExpandArgs e;         // e.ynf = K, e.fmask unused
+  const uint64_t* inst_R;
+  const BInst* inst;
+  uint32_t B;
+};
+void launch_bexpand(const BExpandArgs& a, hipStream_t s);
+
+// Reduce-by-key emit with per-instance column tables.
+struct BReduceArgs {
+  const uint64_t* inst_R;
+  const BInst* inst;
+  const uint64_t* ytab;
+  uint32_t B;
+};
+void run_breduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32_t* out_row,
+                      uint64_t* out_col, float* out_val, uint64_t r0, int K,
+                      const BReduceArgs& b, const U2* tmp, hipStream_t s);
+
+// Dense accumulation over a group of instances sharing one slot width.
+struct BDenseArgs {
+  DenseArgs d;                 // d.r0 unused; d.fmask/d.ytab unused
+  const uint64_t* grp_rows;    // [G+1] prefix of the group's row counts
+  const uint64_t* grp_row0;    // [G] first global row of each group instance
+  const uint32_t* grp_inst;    // [G] instance id
+  const BInst* inst;
+  const uint64_t* ytab;
+  uint32_t G;
+};
+void launch_bdense_count(const BDenseArgs& a, int eb, uint64_t rows, hipStream_t s);
+void launch_bdense_numeric(const BDenseArgs& a, int eb, uint64_t rows, hipStream_t s);
+
+// For each of nq sorted query rows, lower_bound in a chunk's row array.
+void launch_row_bounds(const uint32_t* rows, uint64_t n, const uint64_t* q, uint32_t nq,
+                       uint64_t* out, hipStream_t s);
+
+}  // namespace sptc

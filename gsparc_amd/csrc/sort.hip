@@ -22,6 +22,7 @@
 #include "common.hpp"
 #include "sort.hpp"
 #include "scan.hpp"
+#include "sort_core.hpp"
 
 #include <cstdlib>
 #include <stdexcept>
@@ -98,164 +99,6 @@ __global__ __launch_bounds__(kBlock) void k_digit_scan(const uint64_t* __restric
   }
 }
 
-// Block-wide exclusive scan of uint32 for NT threads.
-template <int NT>
-__device__ __forceinline__ uint32_t block_scan_u32(uint32_t v, uint32_t* scratch,
-                                                   uint32_t* total) {
-  const uint32_t lane = lane_id();
-  const uint32_t wave = threadIdx.x / kWave;
-  uint32_t inc = wave_incl_scan32(v);
-  if (lane == kWave - 1) scratch[wave] = inc;
-  __syncthreads();
-  uint32_t off = 0, tot = 0;
-#pragma unroll
-  for (int w = 0; w < NT / kWave; ++w) {
-    uint32_t s = scratch[w];
-    if ((uint32_t)w < wave) off += s;
-    tot += s;
-  }
-  __syncthreads();
-  *total = tot;
-  return off + inc - v;
-}
-
-template <int RB, int IPT, int NT>
-struct TileSmem {
-  static constexpr int R = 1 << RB;
-  static constexpr int TILE = NT * IPT;
-  static constexpr int NW = NT / kWave;
-  uint16_t cnt[NW][R];     // per-wave running counts, then wave prefixes
-  uint32_t gbase[R];       // global start of this tile's run of each digit
-  uint16_t dstart[R];      // tile-local start of each digit's run
-  uint64_t buf[TILE];      // staged keys, then staged values
-  uint32_t scan[NW];
-  uint32_t tile;
-};
-
-// Loads the tile (wave w owns keys [w*64*IPT, (w+1)*64*IPT) of it, lane-
-// striped) and ranks every key within its wave's run of equal digits.
-// Afterwards sm.cnt[w][d] = number of keys of digit d in wave w.
-template <int RB, int IPT, int NT>
-__device__ __forceinline__ void tile_load_rank(TileSmem<RB, IPT, NT>& sm,
-                                               const uint64_t* __restrict__ kin,
-                                               const float* __restrict__ vin, uint64_t n,
-                                               int shift, uint64_t wbase, uint64_t (&k)[IPT],
-                                               float (&v)[IPT], uint32_t (&pos)[IPT]) {
-  constexpr int R = 1 << RB;
-  const uint32_t lane = lane_id();
-  const uint32_t wave = threadIdx.x / kWave;
-#pragma unroll
-  for (int i = 0; i < IPT; ++i) {
-    const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
-    const bool valid = idx < n;
-    k[i] = valid ? kin[idx] : 0;
-    v[i] = valid ? vin[idx] : 0.f;
-  }
-#pragma unroll
-  for (int i = 0; i < IPT; ++i) {
-    const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
-    const bool valid = idx < n;
-    const uint32_t d = (uint32_t)(k[i] >> shift) & (R - 1);
-    uint64_t peers = ballot(valid);
-#pragma unroll
-    for (int b = 0; b < RB; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bb = ballot(bit);
-      peers &= bit ? bb : ~bb;
-    }
-    uint32_t before = 0;
-    if (valid) before = sm.cnt[wave][d];
-    __builtin_amdgcn_wave_barrier();
-    pos[i] = before + popc_below(peers);  // rank within the wave's digit run
-    if (valid && (peers >> lane) == 1ull)  // highest lane of the group advances
-      sm.cnt[wave][d] = (uint16_t)(before + (uint32_t)__popcll(peers));
-    __builtin_amdgcn_wave_barrier();
-  }
-}
-
-// Given sm.gbase and sm.dstart (tile totals per digit on entry), scans the
-// totals into tile-local run starts, stages keys in digit order, writes them
-// in coalesced runs, then does the same for the values.
-template <int RB, int IPT, int NT>
-__device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT>& sm, uint64_t n,
-                                                   int shift, uint64_t tile_begin,
-                                                   uint64_t wbase, const uint64_t (&k)[IPT],
-                                                   const float (&v)[IPT], uint32_t (&pos)[IPT],
-                                                   uint64_t* __restrict__ kout,
-                                                   float* __restrict__ vout) {
-  constexpr int R = 1 << RB;
-  constexpr int TILE = NT * IPT;
-  constexpr int PER = (R + NT - 1) / NT;
-  const uint32_t tid = threadIdx.x;
-  const uint32_t lane = lane_id();
-  const uint32_t wave = tid / kWave;
-  {
-    uint32_t loc[PER];
-    uint32_t sum = 0;
-    for (int j = 0; j < PER; ++j) {
-      const int d = tid * PER + j;
-      loc[j] = d < R ? sm.dstart[d] : 0;
-      sum += loc[j];
-    }
-    uint32_t tot;
-    uint32_t ex = block_scan_u32<NT>(sum, sm.scan, &tot);
-    for (int j = 0; j < PER; ++j) {
-      const int d = tid * PER + j;
-      if (d < R) sm.dstart[d] = (uint16_t)ex;
-      ex += loc[j];
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < IPT; ++i) {
-    const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
-    const uint32_t d = (uint32_t)(k[i] >> shift) & (R - 1);
-    pos[i] += (uint32_t)sm.dstart[d] + (uint32_t)sm.cnt[wave][d];
-    if (idx < n) sm.buf[pos[i]] = k[i];
-  }
-  __syncthreads();
-  const uint32_t cnt = (uint32_t)((n - tile_begin) < (uint64_t)TILE ? (n - tile_begin) : TILE);
-  uint32_t dst[IPT];
-#pragma unroll
-  for (int i = 0; i < IPT; ++i) {
-    const uint32_t j = tid + (uint32_t)i * NT;
-    if (j < cnt) {
-      const uint64_t key = sm.buf[j];
-      const uint32_t d = (uint32_t)(key >> shift) & (R - 1);
-      dst[i] = sm.gbase[d] + (j - sm.dstart[d]);
-      kout[dst[i]] = key;
-    }
-  }
-  __syncthreads();
-  float* vbuf = reinterpret_cast<float*>(sm.buf);
-#pragma unroll
-  for (int i = 0; i < IPT; ++i) {
-    const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
-    if (idx < n) vbuf[pos[i]] = v[i];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < IPT; ++i) {
-    const uint32_t j = tid + (uint32_t)i * NT;
-    if (j < cnt) vout[dst[i]] = vbuf[j];
-  }
-}
-
-// Per digit: turn per-wave counts into wave prefixes; returns the tile total.
-template <int RB, int IPT, int NT>
-__device__ __forceinline__ uint32_t wave_prefixes(TileSmem<RB, IPT, NT>& sm, int d) {
-  uint32_t acc = 0;
-#pragma unroll
-  for (int w = 0; w < NT / kWave; ++w) {
-    const uint32_t c = sm.cnt[w][d];
-    sm.cnt[w][d] = (uint16_t)acc;
-    acc += c;
-  }
-  return acc;
-}
-
-// TAG only separates kernel symbols (0: operand sorts, 1: product sorts) so
-// profiles attribute time to the product sort on its own.
 template <int RB, int IPT, int NT, int TAG>
 __global__ __launch_bounds__(NT) void k_onesweep(
     const uint64_t* __restrict__ kin, const float* __restrict__ vin,
