@@ -29,7 +29,9 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--workload", default="nips", choices=["nips", "plain_small", "nell2"])
+    p.add_argument("--workload", default="nips",
+                   choices=["nips", "plain_small", "nell2", "batch"])
+    p.add_argument("--batch-instances", type=int, default=256)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-rows", type=int, default=0,
                    help="X free rows in the CPU-baseline sample (0: auto)")
@@ -100,12 +102,84 @@ def cpu_baseline(spec, X, n_rows):
             "sample_out_nnz": int(nnz_z[-1])}
 
 
+def bench_batch(args):
+    """256 independent self-contractions (batch_specs): one step is every
+    instance once -- as 256 plain calls, and as one batch replayed through
+    its captured hipGraph.  Output nnz/s of the batched form is the value."""
+    import torch
+    from gsparc_amd import sptc
+    from gsparc_amd.workloads import BATCH_IR_CAP, batch_specs
+
+    specs = batch_specs(args.batch_instances)
+    tens = [s.make() for s in specs]
+    dev = [to_device(t) for t in tens]
+    items = [(d, d, list(s.cx), list(s.cy)) for d, s in zip(dev, specs)]
+    torch.cuda.synchronize()
+
+    def plain_step():
+        tot = 0
+        for d, _, cx, cy in items:
+            r = sptc.SpTC(d, d, cx, cy, on_device=True)
+            tot += r.nnz
+            del r
+        torch.cuda.synchronize()
+        return tot
+
+    for _ in range(args.warmup):
+        plain_out = plain_step()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plain_out = plain_step()
+    plain_s = (time.perf_counter() - t0) / args.steps
+
+    t0 = time.perf_counter()
+    b = sptc.Batch(items, on_device=True)
+    torch.cuda.synchronize()
+    plan_s = time.perf_counter() - t0
+    info = b.info
+    if info["out_nnz"] != plain_out:
+        raise SystemExit(f"batched output nnz {info['out_nnz']} != plain {plain_out}")
+    res = {}
+    for mode, use_graph in (("eager", False), ("graph", True)):
+        for _ in range(max(1, args.warmup)):
+            b.run(use_graph=use_graph)
+            b.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.run(use_graph=use_graph)
+        b.sync()
+        torch.cuda.synchronize()
+        res[mode] = (time.perf_counter() - t0) / args.steps
+    ir = info["ir_nnz"]
+    out = {
+        "metric": "SpTC output nnz/s (batched)",
+        "value": plain_out / res["graph"],
+        "unit": "nnz/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res["graph"] * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batch{len(specs)}_cap{BATCH_IR_CAP}",
+                   "instances": len(specs), "nnz_in": info["nnz_in"], "ir_nnz": ir,
+                   "out_nnz": plain_out},
+        "plain_ms_per_step": plain_s * 1e3,
+        "batched_eager_ms_per_step": res["eager"] * 1e3,
+        "batched_plan_ms": plan_s * 1e3,
+        "speedup_vs_plain": plain_s / res["graph"],
+        "speedup_eager_vs_plain": plain_s / res["eager"],
+        "batch_info": info,
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
               f"(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        return bench_batch(args)
     import torch
     from gsparc_amd import sptc
     from gsparc_amd.workloads import CONFIGS, NELL2_CAP_BYTES

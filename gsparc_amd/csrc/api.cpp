@@ -4,9 +4,15 @@
 #include <new>
 #include <vector>
 
+#include "batch_engine.hpp"
 #include "engine.hpp"
 
 using namespace sptc;
+
+struct sptc_batch_s {
+  BatchEngine e;
+  KStat kstats[SPTC_K_COUNT];
+};
 
 namespace {
 
@@ -100,23 +106,10 @@ sptc_status sptc_contract(const sptc_coo* X, const sptc_coo* Y, int32_t nc, cons
   sptc_opts o;
   std::memset(&o, 0, sizeof(o));
   if (opts) o = *opts;
-  // One contract-field layout for X and Y: the wider of the two dims per
-  // contracted pair (equal to the reference's layout whenever the dims agree).
-  int32_t cw[SPTC_MAX_ORDER];
-  for (int i = 0; i < nc; ++i) {
-    if (cx[i] < 0 || cx[i] >= X->order || cy[i] < 0 || cy[i] >= Y->order)
-      return fail(SPTC_ERR_BAD_ARGS, "contracted mode out of range");
-    cw[i] = std::max(mode_bits(X->dims[cx[i]]), mode_bits(Y->dims[cy[i]]));
-  }
   Layout xl, yl;
-  st = make_layout(X->order, X->dims, nc, cx, cw, 1, &xl);
-  if (st != SPTC_OK)
-    return fail(st, st == SPTC_ERR_KEY_OVERFLOW ? "X key needs more than 64 bits"
-                                                : "bad X modes");
-  st = make_layout(Y->order, Y->dims, nc, cy, cw, 2, &yl);
-  if (st != SPTC_OK)
-    return fail(st, st == SPTC_ERR_KEY_OVERFLOW ? "Y key needs more than 64 bits"
-                                                : "bad Y modes");
+  std::string lmsg;
+  st = make_pair_layouts(X->order, X->dims, Y->order, Y->dims, nc, cx, cy, &xl, &yl, &lmsg);
+  if (st != SPTC_OK) return fail(st, lmsg);
   sptc_result_s* res = new (std::nothrow) sptc_result_s();
   if (!res) return fail(SPTC_ERR_BAD_ARGS, "host allocation failed");
   std::memset(&res->info, 0, sizeof(res->info));
@@ -173,9 +166,9 @@ sptc_status sptc_result_get_chunk(const sptc_result* r, int32_t i, uint64_t* nnz
     return fail(SPTC_ERR_BAD_ARGS, "bad chunk index");
   const OutChunk& c = r->chunks[i];
   if (nnz) *nnz = c.nnz;
-  if (row) *row = c.row.get();
-  if (col) *col = c.col.get();
-  if (val) *val = c.val.get();
+  if (row) *row = c.R();
+  if (col) *col = c.C();
+  if (val) *val = c.V();
   return SPTC_OK;
 }
 
@@ -187,7 +180,7 @@ sptc_status sptc_result_copy_to_host(const sptc_result* r, uint64_t* keys, uint6
   try {
     std::vector<uint64_t> rk(r->info.nrows);
     if (r->info.nrows)
-      SPTC_HIP(hipMemcpy(rk.data(), r->row_key.get(), r->info.nrows * 8, hipMemcpyDeviceToHost));
+      SPTC_HIP(hipMemcpy(rk.data(), r->RK(), r->info.nrows * 8, hipMemcpyDeviceToHost));
     uint64_t off = 0;
     std::vector<uint32_t> row;
     std::vector<uint64_t> col;
@@ -195,10 +188,10 @@ sptc_status sptc_result_copy_to_host(const sptc_result* r, uint64_t* keys, uint6
       if (!c.nnz) continue;
       row.resize(c.nnz);
       col.resize(c.nnz);
-      SPTC_HIP(hipMemcpy(row.data(), c.row.get(), c.nnz * 4, hipMemcpyDeviceToHost));
-      SPTC_HIP(hipMemcpy(col.data(), c.col.get(), c.nnz * 8, hipMemcpyDeviceToHost));
+      SPTC_HIP(hipMemcpy(row.data(), c.R(), c.nnz * 4, hipMemcpyDeviceToHost));
+      SPTC_HIP(hipMemcpy(col.data(), c.C(), c.nnz * 8, hipMemcpyDeviceToHost));
       if (vals)
-        SPTC_HIP(hipMemcpy(vals + off, c.val.get(), c.nnz * 4, hipMemcpyDeviceToHost));
+        SPTC_HIP(hipMemcpy(vals + off, c.V(), c.nnz * 4, hipMemcpyDeviceToHost));
       const int ynf = r->info.y_nfbits;
       for (uint64_t i = 0; i < c.nnz; ++i) {
         const uint64_t xf = rk[row[i]];
@@ -218,7 +211,7 @@ sptc_status sptc_result_row_keys(const sptc_result* r, uint64_t* row_keys) {
   if (!r || (!row_keys && r->info.nrows)) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");
   try {
     if (r->info.nrows)
-      SPTC_HIP(hipMemcpy(row_keys, r->row_key.get(), r->info.nrows * 8, hipMemcpyDeviceToHost));
+      SPTC_HIP(hipMemcpy(row_keys, r->RK(), r->info.nrows * 8, hipMemcpyDeviceToHost));
   } catch (const HipError& he) {
     return from_hip(he);
   }
@@ -235,7 +228,7 @@ sptc_status sptc_result_row_range(const sptc_result* r, uint64_t row, uint64_t* 
         // lower/upper bound of `row` in the chunk's sorted row array
         auto at = [&](uint64_t i) {
           uint32_t v;
-          SPTC_HIP(hipMemcpy(&v, c.row.get() + i, 4, hipMemcpyDeviceToHost));
+          SPTC_HIP(hipMemcpy(&v, c.R() + i, 4, hipMemcpyDeviceToHost));
           return (uint64_t)v;
         };
         uint64_t lo = 0, hi = c.nnz;
@@ -274,17 +267,17 @@ sptc_status sptc_result_copy_range(const sptc_result* r, uint64_t begin, uint64_
         const uint64_t n = ce - cb, lo = cb - off;
         if (xfree) {
           row.resize(n);
-          SPTC_HIP(hipMemcpy(row.data(), c.row.get() + lo, n * 4, hipMemcpyDeviceToHost));
+          SPTC_HIP(hipMemcpy(row.data(), c.R() + lo, n * 4, hipMemcpyDeviceToHost));
           // rows are ascending within the range: fetch that slice of row_key once
           const uint64_t rlo = row.front(), rhi = row.back();
           std::vector<uint64_t> rk(rhi - rlo + 1);
-          SPTC_HIP(hipMemcpy(rk.data(), r->row_key.get() + rlo, rk.size() * 8,
+          SPTC_HIP(hipMemcpy(rk.data(), r->RK() + rlo, rk.size() * 8,
                              hipMemcpyDeviceToHost));
           for (uint64_t i = 0; i < n; ++i) xfree[o + i] = rk[row[i] - rlo];
         }
         if (yfree)
-          SPTC_HIP(hipMemcpy(yfree + o, c.col.get() + lo, n * 8, hipMemcpyDeviceToHost));
-        if (vals) SPTC_HIP(hipMemcpy(vals + o, c.val.get() + lo, n * 4, hipMemcpyDeviceToHost));
+          SPTC_HIP(hipMemcpy(yfree + o, c.C() + lo, n * 8, hipMemcpyDeviceToHost));
+        if (vals) SPTC_HIP(hipMemcpy(vals + o, c.V() + lo, n * 4, hipMemcpyDeviceToHost));
         o += n;
       }
       off += c.nnz;
@@ -487,5 +480,112 @@ sptc_status sptc_device_info(int32_t* num_cus, uint64_t* free_bytes, uint64_t* t
 }
 
 void sptc_trim_cache(void) { DevicePool::get().trim(); }
+
+// ---- batched contractions -------------------------------------------------
+
+sptc_status sptc_batch_create(int32_t n, const sptc_coo* X, const sptc_coo* Y,
+                              const int32_t* nc, const int32_t* const* cx,
+                              const int32_t* const* cy, const sptc_opts* opts, void* stream,
+                              sptc_batch** out) {
+  if (!out) return fail(SPTC_ERR_BAD_ARGS, "out is NULL");
+  *out = nullptr;
+  if (n < 1 || n > kBatchMaxInstances || !X || !Y || !nc || !cx || !cy)
+    return fail(SPTC_ERR_BAD_ARGS, "bad batch arguments (1 <= n <= 4096, non-NULL arrays)");
+  for (int i = 0; i < n; ++i) {
+    sptc_status st;
+    if ((st = check_coo(&X[i], "X")) != SPTC_OK) return st;
+    if ((st = check_coo(&Y[i], "Y")) != SPTC_OK) return st;
+  }
+  if (!device_ok()) return fail(SPTC_ERR_NO_DEVICE, "no GPU");
+  sptc_opts o;
+  std::memset(&o, 0, sizeof(o));
+  if (opts) o = *opts;
+  sptc_batch_s* b = new (std::nothrow) sptc_batch_s();
+  if (!b) return fail(SPTC_ERR_BAD_ARGS, "host allocation failed");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  try {
+    sptc_status st = b->e.plan(n, X, Y, nc, cx, cy, o, s);
+    if (st != SPTC_OK) {
+      std::string m = b->e.msg;
+      delete b;
+      return fail(st, m);
+    }
+    b->e.ktimer().collect(b->kstats, SPTC_K_COUNT);
+  } catch (const HipError& he) {
+    delete b;
+    return from_hip(he);
+  } catch (const std::bad_alloc&) {
+    delete b;
+    return fail(SPTC_ERR_OOM_BUDGET, "host allocation failed");
+  } catch (const std::exception& ex) {
+    delete b;
+    return fail(SPTC_ERR_INTERNAL, ex.what());
+  }
+  *out = b;
+  return SPTC_OK;
+}
+
+sptc_status sptc_batch_run(sptc_batch* b, int32_t use_graph, void* stream) {
+  if (!b) return fail(SPTC_ERR_BAD_ARGS, "NULL batch");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  try {
+    if (use_graph) return b->e.launch_graph(s);
+    b->e.enqueue(s);
+    SPTC_HIP(hipGetLastError());
+  } catch (const HipError& he) {
+    return from_hip(he);
+  } catch (const std::exception& ex) {
+    return fail(SPTC_ERR_INTERNAL, ex.what());
+  }
+  return SPTC_OK;
+}
+
+sptc_status sptc_batch_sync(sptc_batch* b, void* stream) {
+  if (!b) return fail(SPTC_ERR_BAD_ARGS, "NULL batch");
+  try {
+    sptc_status st = b->e.sync(reinterpret_cast<hipStream_t>(stream));
+    if (st != SPTC_OK) return fail(st, b->e.msg);
+  } catch (const HipError& he) {
+    return from_hip(he);
+  }
+  return SPTC_OK;
+}
+
+sptc_status sptc_batch_get_info(const sptc_batch* b, sptc_batch_info* info) {
+  if (!b || !info) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");
+  *info = b->e.info;
+  return SPTC_OK;
+}
+
+sptc_status sptc_batch_result(sptc_batch* b, int32_t i, sptc_result** out) {
+  if (!b || !out || i < 0 || i >= b->e.info.instances)
+    return fail(SPTC_ERR_BAD_ARGS, "bad instance index");
+  *out = nullptr;
+  sptc_result_s* r = new (std::nothrow) sptc_result_s();
+  if (!r) return fail(SPTC_ERR_BAD_ARGS, "host allocation failed");
+  try {
+    sptc_status st = b->e.instance_result(i, r);
+    if (st != SPTC_OK) {
+      delete r;
+      return fail(st, b->e.msg);
+    }
+  } catch (const HipError& he) {
+    delete r;
+    return from_hip(he);
+  }
+  *out = r;
+  return SPTC_OK;
+}
+
+sptc_status sptc_batch_kernel_stats(const sptc_batch* b, int32_t cls, uint64_t* launches,
+                                    double* total_ms, double* alg_bytes) {
+  if (!b || cls < 0 || cls >= SPTC_K_COUNT) return fail(SPTC_ERR_BAD_ARGS, "bad kernel class");
+  if (launches) *launches = b->kstats[cls].launches;
+  if (total_ms) *total_ms = b->kstats[cls].ms;
+  if (alg_bytes) *alg_bytes = b->kstats[cls].bytes;
+  return SPTC_OK;
+}
+
+void sptc_batch_free(sptc_batch* b) { delete b; }
 
 }  // extern "C"

@@ -126,6 +126,7 @@ struct BUniqueOp {
     const uint32_t s = find_seg(seg_off, nseg, i);
     if (i == seg_off[s]) seg_first[s] = excl.a;
     if (v.a) table[excl.a] = keys[i];
+    if (i == seg_off[nseg] - 1) seg_first[nseg] = excl.a + v.a;
   }
 };
 
@@ -172,7 +173,10 @@ struct BContractOp {
       ukey[excl.a] = shr64(ykeys[i], kyf[s]);
       ustart[excl.a] = i;
     }
-    if (i == n - 1) ustart[excl.a + v.a] = n;
+    if (i == n - 1) {
+      ustart[excl.a + v.a] = n;
+      u_off[B] = excl.a + v.a;
+    }
   }
 };
 
@@ -238,30 +242,43 @@ struct BMatchOp {
       o.act_xval[j] = xvals[i];
       o.act_row[j] = (uint32_t)row;
     }
+    if (i == xoff[B] - 1) {  // totals close every table
+      const uint64_t P = excl.a + cnt;
+      const uint64_t A = (excl.b >> 32) + (cnt != 0);
+      const uint64_t R = (excl.b & 0xFFFFFFFFull) + head;
+      o.inst_P[B] = P;
+      o.inst_A[B] = A;
+      o.inst_R[B] = R;
+      o.act_prefix[A] = P;
+      o.row_prod[R] = P;
+      o.row_act[R] = A;
+    }
   }
 };
 
 // ---- batched ESC expand ----------------------------------------------------
+// One block per tile of the chunk's tile table (grid-stride).  The tile's
+// owners are found by one binary search over act_prefix inside its piece,
+// their prefix run staged in LDS; each thread then walks its 8 consecutive
+// products.  Keys/values are staged in LDS and written coalesced.
 constexpr int kBExpandItems = 8;
-constexpr uint32_t kBExpandTile = kBlock * kBExpandItems;
+static_assert(kBExpandTileItems == kBlock * kBExpandItems, "expand tile");
 
-__global__ __launch_bounds__(kBlock) void k_bexpand(BExpandArgs b) {
-  const ExpandArgs& a = b.e;
-  __shared__ uint64_t s_pref[kBExpandTile + 1];
-  __shared__ uint64_t s_key[kBExpandTile];
-  __shared__ float s_val[kBExpandTile];
+__global__ __launch_bounds__(kBlock) void k_bexpand(BExpandArgs a) {
+  constexpr uint32_t TILE = kBExpandTileItems;
+  __shared__ uint64_t s_pref[TILE + 1];
+  __shared__ uint64_t s_key[TILE];
+  __shared__ float s_val[TILE];
   __shared__ uint64_t s_ja;
-  __shared__ double s_sum[kBlock / kWave];
-  double vsum = 0.0;
-  const uint64_t nprod = a.P1 - a.P0;
-  const uint64_t ntiles = ceil_div(nprod, kBExpandTile);
-  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const uint64_t pb = a.P0 + tile * kBExpandTile;
-    const uint64_t pe = pb + kBExpandTile < a.P1 ? pb + kBExpandTile : a.P1;
-    const uint32_t cnt = (uint32_t)(pe - pb);
+  if (*a.bad) return;
+  for (uint32_t t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+    const BETile tl = a.tiles[t];
+    const BPiece pc = a.pieces[tl.piece];
+    const uint64_t pb = tl.p0;
+    const uint32_t cnt = tl.cnt;
     __syncthreads();
     if (threadIdx.x == 0) {
-      uint64_t lo = a.A0, hi = a.A1;
+      uint64_t lo = pc.A0, hi = pc.A1;  // last j in the piece with prefix <= pb
       while (lo < hi) {
         const uint64_t mid = (lo + hi) >> 1;
         if (a.act_prefix[mid] <= pb) lo = mid + 1; else hi = mid;
@@ -270,8 +287,8 @@ __global__ __launch_bounds__(kBlock) void k_bexpand(BExpandArgs b) {
     }
     __syncthreads();
     const uint64_t ja = s_ja;
-    const uint64_t avail = a.A1 - ja;
-    const uint32_t m = (uint32_t)(avail < kBExpandTile + 1 ? avail : kBExpandTile + 1);
+    const uint64_t avail = pc.A1 - ja;
+    const uint32_t m = (uint32_t)(avail < TILE + 1 ? avail : TILE + 1);
     for (uint32_t k = threadIdx.x; k < m; k += kBlock) s_pref[k] = a.act_prefix[ja + k];
     __syncthreads();
     const uint32_t q0 = threadIdx.x * kBExpandItems;
@@ -283,18 +300,17 @@ __global__ __launch_bounds__(kBlock) void k_bexpand(BExpandArgs b) {
         if (s_pref[mid] <= p0) lo = mid + 1; else hi = mid;
       }
       uint32_t k = lo - 1;
-      uint64_t j = ja + k;
-      auto load_owner = [&](uint64_t jj, uint32_t kk, uint64_t& ybase, float& xv,
-                            uint64_t& rbits, uint64_t& cmask) {
+      uint64_t ybase = 0, rbits = 0, cmask = 0;
+      float xv = 0.f;
+      auto load_owner = [&](uint32_t kk) {
+        const uint64_t jj = ja + kk;
         ybase = a.act_ypos[jj] - s_pref[kk];
         xv = a.act_xval[jj];
         const uint32_t row = a.act_row[jj];
-        rbits = shl64((uint64_t)(row - a.r0), a.ynf);
-        cmask = low_mask(b.inst[find_seg(b.inst_R, b.B, row)].kyf);
+        rbits = shl64(pc.lrow0 + (row - pc.ra), a.K);
+        cmask = low_mask(a.inst[find_seg(a.inst_R, a.B, row)].kyf);
       };
-      uint64_t ybase, rbits, cmask;
-      float xv;
-      load_owner(j, k, ybase, xv, rbits, cmask);
+      load_owner(k);
       uint64_t next = k + 1 < m ? s_pref[k + 1] : ~0ull;
 #pragma unroll
       for (int e = 0; e < kBExpandItems; ++e) {
@@ -302,32 +318,20 @@ __global__ __launch_bounds__(kBlock) void k_bexpand(BExpandArgs b) {
         if (q0 + e < cnt) {
           while (p >= next) {
             ++k;
-            j = ja + k;
-            load_owner(j, k, ybase, xv, rbits, cmask);
+            load_owner(k);
             next = k + 1 < m ? s_pref[k + 1] : ~0ull;
           }
           const uint64_t y = ybase + p;
-          const float v = xv * a.yvals[y];
           s_key[q0 + e] = rbits | (a.ykeys[y] & cmask);
-          s_val[q0 + e] = v;
-          vsum += (double)v;
+          s_val[q0 + e] = xv * a.yvals[y];
         }
       }
     }
     __syncthreads();
-    const uint64_t obase = pb - a.P0;
     for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) {
-      a.out_keys[obase + i] = s_key[i];
-      a.out_vals[obase + i] = s_val[i];
+      a.out_keys[tl.out0 + i] = s_key[i];
+      a.out_vals[tl.out0 + i] = s_val[i];
     }
-  }
-  for (int off = kWave / 2; off > 0; off >>= 1) vsum += __shfl_xor(vsum, off, kWave);
-  if (lane_id() == 0) s_sum[threadIdx.x / kWave] = vsum;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0;
-    for (int w = 0; w < kBlock / kWave; ++w) t += s_sum[w];
-    atomicAdd(a.vsum, t);
   }
 }
 
@@ -339,15 +343,17 @@ struct BReduceOp {
   uint32_t* out_row;
   uint64_t* out_col;
   float* out_val;
-  uint64_t r0;
   int K;
   BReduceArgs b;
 };
 
+// Same run detection and compensated summation as the plain reduce; the
+// chunk-local row maps back to a global row through the piece table.
 __global__ __launch_bounds__(kBlock) void k_breduce_emit(BReduceOp op, const U2* tile_offs) {
   __shared__ uint64_t s_k[kScanTile];
   __shared__ float s_v[kScanTile];
   __shared__ U2 scratch[kBlock / kWave];
+  if (*op.b.bad) return;
   const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
   const uint32_t cnt = (uint32_t)(op.n - base < kScanTile ? op.n - base : kScanTile);
   for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) {
@@ -383,6 +389,7 @@ __global__ __launch_bounds__(kBlock) void k_breduce_emit(BReduceOp op, const U2*
   U2 tot;
   const uint32_t o0 = (uint32_t)block_excl_scan_u2(mk2(nh, 0), scratch, &tot).a;
   const uint32_t nout = (uint32_t)tot.a;
+  __syncthreads();
 #pragma unroll
   for (int e = 0; e < kScanItems; ++e) {
     if ((uint32_t)e < nh) {
@@ -395,7 +402,14 @@ __global__ __launch_bounds__(kBlock) void k_breduce_emit(BReduceOp op, const U2*
   const uint64_t cm = low_mask(op.K);
   for (uint32_t i = threadIdx.x; i < nout; i += kBlock) {
     const uint64_t k = s_k[i];
-    const uint64_t row = op.r0 + shr64(k, op.K);
+    const uint64_t lrow = shr64(k, op.K);
+    uint32_t lo = 0, hi = op.b.npieces;  // last piece with lrow0 <= lrow
+    while (lo + 1 < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (op.b.pieces[mid].lrow0 <= lrow) lo = mid; else hi = mid;
+    }
+    const BPiece pc = op.b.pieces[lo];
+    const uint64_t row = pc.ra + (lrow - pc.lrow0);
     const BInst in = op.b.inst[find_seg(op.b.inst_R, op.b.B, row)];
     op.out_row[obase + i] = (uint32_t)row;
     op.out_col[obase + i] = op.b.ytab[in.yt_off + (k & cm)];
@@ -419,6 +433,7 @@ __global__ __launch_bounds__(kBlock) void k_bdense_count(BDenseArgs a) {
   constexpr int W = E / 32;
   __shared__ uint32_t bm[W];
   __shared__ uint32_t s_red[kBlock / kWave];
+  if (*a.bad) return;
   uint64_t r;
   BInst in;
   bdense_row(a, blockIdx.x, r, in);
@@ -456,6 +471,7 @@ __global__ __launch_bounds__(kBlock) void k_bdense_numeric(BDenseArgs a) {
   __shared__ float s_cmp[E];
   __shared__ uint32_t bm[W];
   __shared__ uint32_t s_scan[kBlock / kWave];
+  if (*a.bad) return;
   uint64_t r;
   BInst in;
   bdense_row(a, blockIdx.x, r, in);
@@ -520,6 +536,68 @@ __global__ void k_row_bounds(const uint32_t* rows, uint64_t n, const uint64_t* q
   out[i] = lo;
 }
 
+__global__ void k_localize_rows(uint32_t* rows, uint64_t n, const uint64_t* bounds,
+                                const uint64_t* inst_R, uint32_t B, const uint32_t* bad) {
+  if (*bad) return;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) rows[i] -= (uint32_t)inst_R[find_seg(bounds, B, i)];
+}
+
+__global__ void k_fix_empty(uint64_t* arr, const uint64_t* seg_off, uint32_t nseg) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int64_t g = (int64_t)nseg - 1; g >= 0; --g)
+    if (seg_off[g] == seg_off[g + 1]) arr[g] = arr[g + 1];
+}
+
+__global__ __launch_bounds__(kBlock) void k_bany_dup(const uint64_t* keys,
+                                                     const uint64_t* seg_off, uint32_t nseg,
+                                                     uint64_t n, uint32_t* flags) {
+  uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x + 1;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (; i < n; i += stride) {
+    if (keys[i] != keys[i - 1]) continue;
+    const uint32_t sg = find_seg(seg_off, nseg, i);
+    if (i > seg_off[sg]) flags[sg] = 1;
+  }
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(kBlock) void k_digest(const uint64_t* keys, uint64_t n,
+                                                   unsigned long long* out) {
+  uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  uint64_t h = 0;
+  for (; i < n; i += stride) h += mix64(keys[i] ^ (i * 0x9E3779B97F4A7C15ull));
+  for (int off = kWave / 2; off > 0; off >>= 1) h += __shfl_xor(h, off, kWave);
+  if (lane_id() == 0 && h) atomicAdd(out, (unsigned long long)h);
+}
+
+__global__ void k_check_digest(const uint64_t* got, const uint64_t* want, uint32_t* bad) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && *got != *want) *bad = 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_f64(const float* v, uint64_t n, double* out) {
+  __shared__ double s_red[kBlock / kWave];
+  uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  double t = 0.0;
+  for (; i < n; i += stride) t += (double)v[i];
+  for (int off = kWave / 2; off > 0; off >>= 1) t += __shfl_xor(t, off, kWave);
+  if (lane_id() == 0) s_red[threadIdx.x / kWave] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double u = 0.0;
+    for (int w = 0; w < kBlock / kWave; ++w) u += s_red[w];
+    if (u != 0.0) atomicAdd(out, u);
+  }
+}
+
 uint32_t grid_for(uint64_t n, uint32_t cap = 4096) {
   return (uint32_t)std::min<uint64_t>(ceil_div(n ? n : 1, kBlock), cap);
 }
@@ -558,7 +636,7 @@ void SegSortPlan::build(const std::vector<uint64_t>& seg_off, hipStream_t s) {
 
 void segmented_sort_reserve(SortWorkspace& ws, const SegSortPlan& plan, hipStream_t s) {
   const uint64_t m = plan.ntiles << kSegRB;
-  ws.ensure(plan.n ? plan.n : 1, kSegRB, s);
+  ws.ensure(1, kSegRB, s);
   if (ws.rts_cnt.size() < m) {
     ws.rts_cnt.reset(m);
     ws.rts_off.reset(m);
@@ -566,18 +644,20 @@ void segmented_sort_reserve(SortWorkspace& ws, const SegSortPlan& plan, hipStrea
   if (ws.rts_tmp.size() < scan_tmp_count(m) + 1) ws.rts_tmp.reset(scan_tmp_count(m) + 1);
 }
 
-bool segmented_sort(SortWorkspace& ws, const SegSortPlan& plan, uint64_t* keys, float* vals,
-                    int nbits, hipStream_t s, KTimer* kt, int cls) {
+int segmented_sort_passes(int nbits) { return radix_passes(nbits, kSegRB); }
+
+bool segmented_sort(SortWorkspace& ws, const SegSortPlan& plan, uint64_t* k0, float* v0,
+                    uint64_t* k1, float* v1, int nbits, hipStream_t s, KTimer* kt, int cls) {
   const uint64_t n = plan.n;
   const int passes = radix_passes(nbits, kSegRB);
   if (!n || !passes) return false;
   const uint64_t m = plan.ntiles << kSegRB;
-  if (ws.rts_cnt.size() < m || ws.alt_keys.size() < n)
+  if (ws.rts_cnt.size() < m || ws.rts_tmp.size() < scan_tmp_count(m) + 1)
     throw std::logic_error("segmented sort workspace not reserved");
-  uint64_t* ka = keys;
-  float* va = vals;
-  uint64_t* kb = ws.alt_keys.get();
-  float* vb = ws.alt_vals.get();
+  uint64_t* ka = k0;
+  float* va = v0;
+  uint64_t* kb = k1;
+  float* vb = v1;
   for (int p = 0; p < passes; ++p) {
     const int shift = p * kSegRB;
     int tok = kt ? kt->begin(s) : -1;
@@ -635,17 +715,16 @@ void run_bmatch(const uint64_t* xkeys, const float* xvals, const uint64_t* xoff,
 }
 
 void launch_bexpand(const BExpandArgs& a, hipStream_t s) {
-  const uint64_t nprod = a.e.P1 - a.e.P0;
-  if (!nprod) return;
-  const uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(nprod, kBExpandTile), 2048);
+  if (!a.ntiles) return;
+  const uint32_t grid = std::min<uint32_t>(a.ntiles, 4096);
   hipLaunchKernelGGL(k_bexpand, dim3(grid), dim3(kBlock), 0, s, a);
 }
 
 void run_breduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32_t* out_row,
-                      uint64_t* out_col, float* out_val, uint64_t r0, int K,
-                      const BReduceArgs& b, const U2* tmp, hipStream_t s) {
+                      uint64_t* out_col, float* out_val, int K, const BReduceArgs& b,
+                      const U2* tmp, hipStream_t s) {
   if (!n) return;
-  BReduceOp op{keys, vals, n, out_row, out_col, out_val, r0, K, b};
+  BReduceOp op{keys, vals, n, out_row, out_col, out_val, K, b};
   hipLaunchKernelGGL(k_breduce_emit, dim3((uint32_t)scan_tmp_count(n)), dim3(kBlock), 0, s, op,
                      tmp);
 }
@@ -672,6 +751,42 @@ void launch_row_bounds(const uint32_t* rows, uint64_t n, const uint64_t* q, uint
                        uint64_t* out, hipStream_t s) {
   if (!nq) return;
   hipLaunchKernelGGL(k_row_bounds, dim3(ceil_div(nq, 256)), dim3(256), 0, s, rows, n, q, nq, out);
+}
+
+void launch_localize_rows(uint32_t* rows, uint64_t n, const uint64_t* bounds,
+                          const uint64_t* inst_R, uint32_t B, const uint32_t* bad,
+                          hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_localize_rows, dim3(grid_for(n)), dim3(kBlock), 0, s, rows, n, bounds,
+                     inst_R, B, bad);
+}
+
+void launch_fix_empty(uint64_t* arr, const uint64_t* seg_off, uint32_t nseg, hipStream_t s) {
+  hipLaunchKernelGGL(k_fix_empty, dim3(1), dim3(64), 0, s, arr, seg_off, nseg);
+}
+
+void launch_bany_dup(const uint64_t* keys, const uint64_t* seg_off, uint32_t nseg, uint64_t n,
+                     uint32_t* flags, hipStream_t s) {
+  fill_u32(flags, 0, nseg, s);
+  if (n < 2) return;
+  hipLaunchKernelGGL(k_bany_dup, dim3(grid_for(n)), dim3(kBlock), 0, s, keys, seg_off, nseg, n,
+                     flags);
+}
+
+void launch_digest(const uint64_t* keys, uint64_t n, uint64_t* out, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_digest, dim3(grid_for(n, 2048)), dim3(kBlock), 0, s, keys, n,
+                     reinterpret_cast<unsigned long long*>(out));
+}
+
+void launch_check_digest(const uint64_t* got, const uint64_t* want, uint32_t* bad,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_check_digest, dim3(1), dim3(64), 0, s, got, want, bad);
+}
+
+void launch_sum_f64(const float* vals, uint64_t n, double* out, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_sum_f64, dim3(grid_for(n, 1024)), dim3(kBlock), 0, s, vals, n, out);
 }
 
 }  // namespace sptc

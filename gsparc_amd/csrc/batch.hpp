@@ -2,10 +2,11 @@
 //
 // A batch concatenates the operands of B contractions into shared arrays
 // with per-instance offset tables; every phase is one launch over all
-// instances.  Row ids are global across the batch, so product keys
-// (row << K | column rank) sort every instance's products with one
-// unsegmented radix sort; operand sorts are segmented (one segment per
-// operand, tiles never straddle segments).
+// instances.  X rows are numbered globally across the batch, so the product
+// keys of an ESC chunk (chunk-local row << K | column rank) sort the
+// products of every instance in the chunk with one unsegmented radix sort;
+// operand sorts are segmented (one segment per operand, tiles never
+// straddle segments).
 #pragma once
 #include <vector>
 
@@ -52,14 +53,16 @@ struct SegSortPlan {
   uint64_t n = 0;
   void build(const std::vector<uint64_t>& seg_off, hipStream_t s);
 };
-// Sorts each segment of (keys, vals) on bits [0, nbits) with 8-bit
-// reduce-then-scan passes; ws must hold plan.n keys and the plan's count
-// matrix (segmented_sort_reserve).  Launches only (no allocation or host
+// Sorts each segment of (k0, v0) on bits [0, nbits) with 8-bit reduce-then-
+// scan passes, ping-ponging with (k1, v1); ws supplies the count matrices
+// (segmented_sort_reserve).  Launches only (no allocation or host
 // synchronization), so it can be replayed and captured.  Returns true when
-// the result ended in the workspace's alternate buffers.
+// the result ended in (k1, v1).
 void segmented_sort_reserve(SortWorkspace& ws, const SegSortPlan& plan, hipStream_t s);
-bool segmented_sort(SortWorkspace& ws, const SegSortPlan& plan, uint64_t* keys, float* vals,
-                    int nbits, hipStream_t s, KTimer* kt = nullptr, int cls = SPTC_K_SORT_INPUT);
+bool segmented_sort(SortWorkspace& ws, const SegSortPlan& plan, uint64_t* k0, float* v0,
+                    uint64_t* k1, float* v1, int nbits, hipStream_t s, KTimer* kt = nullptr,
+                    int cls = SPTC_K_SORT_INPUT);
+int segmented_sort_passes(int nbits);
 
 void launch_blinearize(const BLinArgs* args, const uint64_t* seg_off, uint32_t nseg,
                        uint64_t n, uint64_t* keys, float* vals, hipStream_t s);
@@ -97,27 +100,58 @@ void run_bmatch(const uint64_t* xkeys, const float* xvals, const uint64_t* xoff,
                 const BInst* inst, const uint64_t* ukey, const uint64_t* ustart,
                 const uint64_t* u_off, uint64_t nx, const BMatchOut& o, U2* tmp, hipStream_t s);
 
-// ESC expand over global rows [r0, r1) (ESC instances only): product key =
-// (row - r0) << K | column rank; inst_R (B+1, global row starts) maps rows
-// to instances for their column field width.
+// An ESC chunk is a list of pieces, each a contiguous global row range of
+// ESC instances; dense instances' rows between pieces are skipped.  Chunk-
+// local row = lrow0 + (row - ra); product key = local row << K | rank.
+struct BPiece {
+  uint64_t ra;     // first global row
+  uint64_t lrow0;  // chunk-local row of ra
+  uint64_t A0, A1; // active X entries of the piece
+};
+// Expand tile: products [p0, p0 + cnt) of piece `piece`, written at chunk
+// offset out0.  Tiles never straddle pieces.
+struct BETile {
+  uint64_t p0;
+  uint64_t out0;
+  uint32_t cnt;
+  uint32_t piece;
+};
+constexpr uint32_t kBExpandTileItems = 256 * 8;  // products per expand tile
+
 struct BExpandArgs {
-  ExpandArgs e;         // e.ynf = K, e.fmask unused
-  const uint64_t* inst_R;
+  const uint64_t* act_prefix;
+  const uint64_t* act_ypos;
+  const float* act_xval;
+  const uint32_t* act_row;
+  const uint64_t* ykeys;  // Y region (rank-compressed keys)
+  const float* yvals;
+  const BPiece* pieces;
+  const BETile* tiles;
+  uint32_t ntiles;
+  int K;
+  const uint64_t* inst_R;  // [B+1] global row starts
   const BInst* inst;
   uint32_t B;
+  uint64_t* out_keys;
+  float* out_vals;
+  const uint32_t* bad;     // nonzero: operands changed since planning, skip
 };
 void launch_bexpand(const BExpandArgs& a, hipStream_t s);
 
-// Reduce-by-key emit with per-instance column tables.
+// Reduce-by-key emit with per-instance column tables; rows written as
+// global row ids.
 struct BReduceArgs {
+  const BPiece* pieces;
+  uint32_t npieces;
   const uint64_t* inst_R;
   const BInst* inst;
   const uint64_t* ytab;
   uint32_t B;
+  const uint32_t* bad;
 };
 void run_breduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32_t* out_row,
-                      uint64_t* out_col, float* out_val, uint64_t r0, int K,
-                      const BReduceArgs& b, const U2* tmp, hipStream_t s);
+                      uint64_t* out_col, float* out_val, int K, const BReduceArgs& b,
+                      const U2* tmp, hipStream_t s);
 
 // Dense accumulation over a group of instances sharing one slot width.
 struct BDenseArgs {
@@ -128,6 +162,7 @@ struct BDenseArgs {
   const BInst* inst;
   const uint64_t* ytab;
   uint32_t G;
+  const uint32_t* bad;
 };
 void launch_bdense_count(const BDenseArgs& a, int eb, uint64_t rows, hipStream_t s);
 void launch_bdense_numeric(const BDenseArgs& a, int eb, uint64_t rows, hipStream_t s);
@@ -135,5 +170,24 @@ void launch_bdense_numeric(const BDenseArgs& a, int eb, uint64_t rows, hipStream
 // For each of nq sorted query rows, lower_bound in a chunk's row array.
 void launch_row_bounds(const uint32_t* rows, uint64_t n, const uint64_t* q, uint32_t nq,
                        uint64_t* out, hipStream_t s);
+// Global -> instance-local rows: entry i of a buffer whose instance slices
+// start at bounds[0..B] gets row -= inst_R[instance].
+void launch_localize_rows(uint32_t* rows, uint64_t n, const uint64_t* bounds,
+                          const uint64_t* inst_R, uint32_t B, const uint32_t* bad,
+                          hipStream_t s);
+// arr[s] of every empty segment s (seg_off[s] == seg_off[s+1]) takes the
+// value of the next segment, so per-segment starts are monotone with
+// arr[nseg] = total.
+void launch_fix_empty(uint64_t* arr, const uint64_t* seg_off, uint32_t nseg, hipStream_t s);
+// flags[s] = 1 when segment s of a sorted key array holds equal neighbours.
+void launch_bany_dup(const uint64_t* keys, const uint64_t* seg_off, uint32_t nseg, uint64_t n,
+                     uint32_t* flags, hipStream_t s);
+// Position-dependent 64-bit digest of n keys, added into *out (cleared by
+// the caller); launch_check_digest sets *bad when *got != *want.
+void launch_digest(const uint64_t* keys, uint64_t n, uint64_t* out, hipStream_t s);
+void launch_check_digest(const uint64_t* got, const uint64_t* want, uint32_t* bad,
+                         hipStream_t s);
+// *out += sum of vals[0, n) in FP64.
+void launch_sum_f64(const float* vals, uint64_t n, double* out, hipStream_t s);
 
 }  // namespace sptc

@@ -55,6 +55,34 @@ sptc_status make_layout(int order, const uint64_t* dims, int nc, const int32_t* 
   return L->nbits > 64 ? SPTC_ERR_KEY_OVERFLOW : SPTC_OK;
 }
 
+sptc_status make_pair_layouts(int xorder, const uint64_t* xdims, int yorder,
+                              const uint64_t* ydims, int nc, const int32_t* cx,
+                              const int32_t* cy, Layout* xl, Layout* yl, std::string* msg) {
+  if (nc < 0 || nc > xorder || nc > yorder || (nc && (!cx || !cy))) {
+    *msg = "bad contracted-mode count";
+    return SPTC_ERR_BAD_ARGS;
+  }
+  int32_t cw[SPTC_MAX_ORDER];
+  for (int i = 0; i < nc; ++i) {
+    if (cx[i] < 0 || cx[i] >= xorder || cy[i] < 0 || cy[i] >= yorder) {
+      *msg = "contracted mode out of range";
+      return SPTC_ERR_BAD_ARGS;
+    }
+    cw[i] = std::max(mode_bits(xdims[cx[i]]), mode_bits(ydims[cy[i]]));
+  }
+  sptc_status st = make_layout(xorder, xdims, nc, cx, cw, 1, xl);
+  if (st != SPTC_OK) {
+    *msg = st == SPTC_ERR_KEY_OVERFLOW ? "X key needs more than 64 bits" : "bad X modes";
+    return st;
+  }
+  st = make_layout(yorder, ydims, nc, cy, cw, 2, yl);
+  if (st != SPTC_OK) {
+    *msg = st == SPTC_ERR_KEY_OVERFLOW ? "Y key needs more than 64 bits" : "bad Y modes";
+    return st;
+  }
+  return SPTC_OK;
+}
+
 Engine::Engine(hipStream_t s, const sptc_opts& o) : s_(s), o_(o) {
   for (auto& e : ev_) SPTC_HIP(hipEventCreate(&e));
   SPTC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_total_), sizeof(U2) * 2, 0));

@@ -27,6 +27,13 @@ struct Layout {
 int mode_bits(uint64_t d);
 sptc_status make_layout(int order, const uint64_t* dims, int nc, const int32_t* cmodes,
                         const int32_t* cwidth, int role, Layout* out);
+// X and Y layouts of one contraction with one contract-field layout for
+// both: the wider of the two dims per contracted pair (equal to the
+// reference's layout whenever the dims agree).  msg receives the reason of a
+// failure.
+sptc_status make_pair_layouts(int xorder, const uint64_t* xdims, int yorder,
+                              const uint64_t* ydims, int nc, const int32_t* cx,
+                              const int32_t* cy, Layout* xl, Layout* yl, std::string* msg);
 
 // A linearized, sorted operand resident in device memory.
 struct Operand {
@@ -36,13 +43,20 @@ struct Operand {
   DevBuf<float> vals;
 };
 
-// One accumulation chunk of the output (a contiguous X-row range).
+// One accumulation chunk of the output (a contiguous X-row range).  A chunk
+// either owns its arrays or views a slice of a batch's output buffer.
 struct OutChunk {
   uint64_t nnz = 0;
   uint64_t r0 = 0, r1 = 0;  // X-row range [r0, r1) of the chunk
   DevBuf<uint32_t> row;
   DevBuf<uint64_t> col;
   DevBuf<float> val;
+  const uint32_t* vrow = nullptr;
+  const uint64_t* vcol = nullptr;
+  const float* vval = nullptr;
+  const uint32_t* R() const { return vrow ? vrow : row.get(); }
+  const uint64_t* C() const { return vcol ? vcol : col.get(); }
+  const float* V() const { return vval ? vval : val.get(); }
 };
 
 }  // namespace sptc
@@ -52,7 +66,9 @@ struct sptc_result_s {
   sptc::KStat kstats[SPTC_K_COUNT];
   sptc::Layout xl, yl;
   sptc::DevBuf<uint64_t> row_key;
+  const uint64_t* vrow_key = nullptr;  // view (batch instance results)
   std::vector<sptc::OutChunk> chunks;
+  const uint64_t* RK() const { return vrow_key ? vrow_key : row_key.get(); }
 };
 
 namespace sptc {

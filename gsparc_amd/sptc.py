@@ -55,6 +55,13 @@ class _Info(Structure):
                 ("row_key", c_void_p), ("phase_ms", c_double * 6)]
 
 
+class _BatchInfo(Structure):
+    _fields_ = [("instances", c_int32), ("dense_instances", c_int32),
+                ("dense_classes", c_int32), ("esc_chunks", c_int32), ("nnz_in", c_uint64),
+                ("ir_nnz", c_uint64), ("out_nnz", c_uint64), ("graph_nodes", c_uint64),
+                ("device_bytes", c_uint64), ("plan_ms", c_double)]
+
+
 _bound = False
 
 
@@ -101,6 +108,21 @@ def lib():
         L.sptc_result_kernel_stats.restype = c_int32
         L.sptc_pool_stats.argtypes = [POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64)]
         L.sptc_pool_stats.restype = None
+        L.sptc_batch_create.argtypes = [c_int32, POINTER(_COO), POINTER(_COO),
+                                        POINTER(c_int32), POINTER(POINTER(c_int32)),
+                                        POINTER(POINTER(c_int32)), POINTER(_Opts), c_void_p,
+                                        POINTER(c_void_p)]
+        L.sptc_batch_run.argtypes = [c_void_p, c_int32, c_void_p]
+        L.sptc_batch_sync.argtypes = [c_void_p, c_void_p]
+        L.sptc_batch_get_info.argtypes = [c_void_p, POINTER(_BatchInfo)]
+        L.sptc_batch_result.argtypes = [c_void_p, c_int32, POINTER(c_void_p)]
+        L.sptc_batch_kernel_stats.argtypes = [c_void_p, c_int32, POINTER(c_uint64),
+                                              POINTER(c_double), POINTER(c_double)]
+        L.sptc_batch_free.argtypes = [c_void_p]
+        L.sptc_batch_free.restype = None
+        for name in ("sptc_batch_create", "sptc_batch_run", "sptc_batch_sync",
+                     "sptc_batch_get_info", "sptc_batch_result", "sptc_batch_kernel_stats"):
+            getattr(L, name).restype = c_int32
         for name in ("sptc_contract", "sptc_result_get_info", "sptc_result_get_chunk",
                      "sptc_result_copy_to_host", "sptc_result_decode", "sptc_layout",
                      "sptc_result_row_keys", "sptc_result_row_range", "sptc_result_copy_range",
@@ -158,8 +180,9 @@ class _HostCOO:
 class Result:
     """Device-resident contraction output (freed with the object)."""
 
-    def __init__(self, handle):
+    def __init__(self, handle, owner=None):
         self._h = c_void_p(handle)
+        self._owner = owner  # a Batch whose buffers this result views
         self.info = _Info()
         check(lib().sptc_result_get_info(self._h, byref(self.info)))
 
@@ -271,6 +294,79 @@ def SpTC(X, Y, cmodes_x, cmodes_y, dense_accumulator=0, count_only=False,
                          c_void_p(stream or 0), byref(out))
     check(st)
     return Result(out.value)
+
+
+class Batch:
+    """Many independent contractions planned and run as one launch sequence
+    (sptc_batch_*).  ``items`` is a list of (X, Y, cmodes_x, cmodes_y).
+    Creating the batch runs it once; :meth:`run` re-runs it (on the
+    operands' current values) directly or as a captured hipGraph."""
+
+    def __init__(self, items, dense_accumulator=0, mem_budget_bytes=0, radix_bits=0,
+                 kernel_timing=False, stream=None, on_device=False):
+        L = lib()
+        n = len(items)
+        self._keep = []
+        xs = (_COO * n)()
+        ys = (_COO * n)()
+        ncs = (c_int32 * n)()
+        cxs = (POINTER(c_int32) * n)()
+        cys = (POINTER(c_int32) * n)()
+        for i, (X, Y, cmx, cmy) in enumerate(items):
+            if len(cmx) != len(cmy):
+                raise SpTCError(ERR_BAD_ARGS, "contracted mode lists differ in length")
+            hx = _HostCOO(X, on_device)
+            hy = hx if X is Y else _HostCOO(Y, on_device)
+            cx = (c_int32 * max(1, len(cmx)))(*cmx)
+            cy = (c_int32 * max(1, len(cmy)))(*cmy)
+            self._keep += [hx, hy, cx, cy]
+            xs[i] = hx.s
+            ys[i] = hy.s
+            ncs[i] = len(cmx)
+            cxs[i] = ctypes.cast(cx, POINTER(c_int32))
+            cys[i] = ctypes.cast(cy, POINTER(c_int32))
+        o = _Opts(int(dense_accumulator), 0, int(mem_budget_bytes), int(radix_bits), 0,
+                  int(bool(kernel_timing)))
+        h = c_void_p()
+        check(L.sptc_batch_create(n, xs, ys, ncs, cxs, cys, byref(o), c_void_p(stream or 0),
+                                  byref(h)))
+        self._h = h
+        if not on_device:
+            self._keep = []  # host operands were copied to the device
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().sptc_batch_free(h)
+            self._h = c_void_p(0)
+
+    def run(self, use_graph=True, stream=None):
+        check(lib().sptc_batch_run(self._h, int(bool(use_graph)), c_void_p(stream or 0)))
+
+    def sync(self, stream=None):
+        check(lib().sptc_batch_sync(self._h, c_void_p(stream or 0)))
+
+    @property
+    def info(self):
+        bi = _BatchInfo()
+        check(lib().sptc_batch_get_info(self._h, byref(bi)))
+        return {k: getattr(bi, k) for k, _ in _BatchInfo._fields_}
+
+    def __len__(self):
+        return self.info["instances"]
+
+    def result(self, i):
+        out = c_void_p()
+        check(lib().sptc_batch_result(self._h, int(i), byref(out)))
+        return Result(out.value, owner=self)
+
+    def kernel_stats(self):
+        out = {}
+        for i, name in enumerate(KERNEL_CLASSES):
+            n, ms, b = c_uint64(), c_double(), c_double()
+            check(lib().sptc_batch_kernel_stats(self._h, i, byref(n), byref(ms), byref(b)))
+            out[name] = (n.value, ms.value, b.value)
+        return out
 
 
 def pool_stats():

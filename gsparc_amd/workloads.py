@@ -6,15 +6,20 @@
                 uniform mode 3, lognormal(0,1) values; Z = X x_{{2,3},{2,3}} X.
   nell2         order-3 12092x9184x28818, 7.69e7 nnz, Zipf(1.1) every mode,
                 uniform(0,1] values; Z = X x_{2,2} X under a 64 GB cap.
-  batch         256 independent self-contractions, per-instance seed.
+  batch         256 independent self-contractions, per-instance seed, each
+                capped at 2^24 intermediate products (batch_specs).
   serving       Poisson request stream; 90 % batch-distribution requests,
                 10 % large order-3 requests.
 
 Seeds are 0x5EED0000 + workload number; instance i of a multi-instance
 workload uses splitmix64(seed + i).
 """
+import dataclasses
+import functools
 import math
 from dataclasses import dataclass
+
+import numpy as np
 
 from .coo import (UNIFORM, ZIPF, VAL_LOGNORMAL, VAL_UNIFORM01, generate)
 
@@ -103,6 +108,39 @@ def batch_instance(i, seed=SEED0 + 3, nnz_cap=2_000_000):
     return Spec(f"batch{i}", splitmix64(seed + 1000003 * (i + 1)), dims,
                 (ZIPF,) * order, (1.1,) * order, nnz, VAL_UNIFORM01,
                 tuple(cmodes), tuple(cmodes))
+
+
+BATCH_IR_CAP = 1 << 24
+
+
+def self_products(t, cmodes):
+    """Intermediate products of T x_{c,c} T: sum over contract keys of the
+    squared key count."""
+    ck = np.zeros(t.nnz, np.int64)
+    for m in cmodes:
+        ck = ck * t.dims[m] + t.idx[m].astype(np.int64)
+    _, cnt = np.unique(ck, return_counts=True)
+    return int((cnt.astype(np.float64) ** 2).sum())
+
+
+@functools.lru_cache(maxsize=4)
+def batch_specs(n=256, ir_cap=BATCH_IR_CAP):
+    """The first n batch instances with each one's nnz scaled down until its
+    self-contraction has at most ir_cap intermediate products.  Drawn as
+    stated, the distribution's products are heavy-tailed (256 instances:
+    8.4e11 products in all, one instance alone 1.2e11, a quarter above
+    2.7e8), so a few instances would be the whole workload; 104 of the 256
+    are scaled."""
+    out = []
+    for i in range(n):
+        s = batch_instance(i)
+        for _ in range(8):
+            ir = self_products(s.make(), s.cx)
+            if ir <= ir_cap:
+                break
+            s = dataclasses.replace(s, nnz=max(1000, int(s.nnz * math.sqrt(ir_cap / ir) * 0.9)))
+        out.append(s)
+    return tuple(out)
 
 
 def serving_large(i, seed=SEED0 + 4):

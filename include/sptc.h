@@ -172,6 +172,61 @@ void sptc_pool_stats(uint64_t* live_bytes, uint64_t* reserved_bytes,
 
 void sptc_result_free(sptc_result* r);
 
+/* ---- batched contractions ------------------------------------------------
+ * Many independent contractions through one launch sequence: operands of
+ * all instances are laid out back to back and every phase (linearize,
+ * segmented sort, match, dense accumulation, ESC chunks) runs as one launch
+ * over all of them.  Replaces n separate gsparc::SpTC calls (sptc.cuh:16-26,
+ * each driven as in main.cu:293-376); the reference has no batched form. */
+typedef struct sptc_batch_s sptc_batch;
+
+typedef struct {
+  int32_t instances;
+  int32_t dense_instances;  /* instances on the dense accumulator          */
+  int32_t dense_classes;    /* dense launches (one per LDS slot width)     */
+  int32_t esc_chunks;       /* ESC chunks (each spans many instances)      */
+  uint64_t nnz_in;          /* operand nonzeros, all instances             */
+  uint64_t ir_nnz;          /* intermediate products, all instances        */
+  uint64_t out_nnz;         /* output nonzeros, all instances              */
+  uint64_t graph_nodes;     /* nodes of the captured graph (0 until used)  */
+  uint64_t device_bytes;    /* device bytes the batch holds                */
+  double plan_ms;           /* host wall time of sptc_batch_create         */
+} sptc_batch_info;
+
+/* Plans and runs the batch once: instance i is X[i] x_{cx[i],cy[i]} Y[i]
+ * with nc[i] contracted modes.  Options apply to every instance
+ * (count_only is not supported; dense_accumulator is the per-instance
+ * rule).  Results are available after the call (sptc_batch_result). */
+sptc_status sptc_batch_create(int32_t n, const sptc_coo* X, const sptc_coo* Y,
+                              const int32_t* nc, const int32_t* const* cx,
+                              const int32_t* const* cy, const sptc_opts* opts,
+                              void* stream, sptc_batch** out);
+
+/* Re-runs the batch on the operands' current values (host operands were
+ * copied at creation); index arrays must be unchanged.  use_graph = 1
+ * launches the captured hipGraph of the whole sequence (captured on first
+ * use), 0 enqueues the kernels directly.  Asynchronous: call
+ * sptc_batch_sync before reading results.  Results are overwritten in place. */
+sptc_status sptc_batch_run(sptc_batch* b, int32_t use_graph, void* stream);
+
+/* Waits for the stream; SPTC_ERR_INTERNAL when the operands' index arrays
+ * changed since creation (the run then left the results untouched). */
+sptc_status sptc_batch_sync(sptc_batch* b, void* stream);
+
+sptc_status sptc_batch_get_info(const sptc_batch* b, sptc_batch_info* info);
+
+/* Result of instance i as a view into the batch's buffers (valid until
+ * sptc_batch_free; free the handle with sptc_result_free).  All sptc_result_*
+ * accessors apply; rows index the instance's own row table. */
+sptc_status sptc_batch_result(sptc_batch* b, int32_t i, sptc_result** out);
+
+/* Kernel statistics of the planning run (opts.kernel_timing). */
+sptc_status sptc_batch_kernel_stats(const sptc_batch* b, int32_t cls,
+                                    uint64_t* launches, double* total_ms,
+                                    double* alg_bytes);
+
+void sptc_batch_free(sptc_batch* b);
+
 /* ---- per-phase entry points (device pointers, caller-owned buffers) ---- */
 
 /* Key layout of one operand (get_fmode + compute_nbtis + setup_slitom,
