@@ -30,7 +30,9 @@ def parse():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="nips",
-                   choices=["nips", "plain_small", "nell2", "batch"])
+                   choices=["nips", "plain_small", "nell2", "batch", "serving"])
+    p.add_argument("--requests", type=int, default=200)
+    p.add_argument("--qps", type=float, default=10.0)
     p.add_argument("--batch-instances", type=int, default=256)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-rows", type=int, default=0,
@@ -110,9 +112,11 @@ def bench_batch(args):
     from gsparc_amd import sptc
     from gsparc_amd.workloads import BATCH_IR_CAP, batch_specs
 
+    log("generating the batch instances")
     specs = batch_specs(args.batch_instances)
     tens = [s.make() for s in specs]
     dev = [to_device(t) for t in tens]
+    log("plain calls")
     items = [(d, d, list(s.cx), list(s.cy)) for d, s in zip(dev, specs)]
     torch.cuda.synchronize()
 
@@ -132,6 +136,7 @@ def bench_batch(args):
         plain_out = plain_step()
     plain_s = (time.perf_counter() - t0) / args.steps
 
+    log("batched")
     t0 = time.perf_counter()
     b = sptc.Batch(items, on_device=True)
     torch.cuda.synchronize()
@@ -172,6 +177,155 @@ def bench_batch(args):
     print(json.dumps(out))
 
 
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def _percentiles(lat):
+    a = np.sort(np.asarray(lat, np.float64))
+    return {"p50_ms": float(np.percentile(a, 50)), "p99_ms": float(np.percentile(a, 99)),
+            "mean_ms": float(a.mean()), "max_ms": float(a.max())}
+
+
+def bench_serving(args):
+    """Poisson request stream over resident tensors: 90 % requests drawn from
+    the batched workload's instances, 10 % large (order 3, dims ~1e4, 1e7 nnz,
+    one contracted mode; count-only responses).  The server (continuous
+    batching, HBM-footprint admission under 96 GiB, resident prepared
+    operands) against serial plain calls on the same arrival times, plus a
+    burst (all requests at once) for throughput."""
+    import threading
+    import torch
+    from gsparc_amd import sptc
+    from gsparc_amd.workloads import batch_specs, serving_large
+
+    log("generating tensors")
+    specs = batch_specs(256)
+    small = [s.make() for s in specs]
+    larges = [serving_large(i) for i in range(4)]
+    big = [s.make() for s in larges]
+    log("server: registering and warming")
+    rng = np.random.default_rng(0x5EED0004)
+    n = args.requests
+    kinds = rng.random(n) < 0.1
+    picks = [int(rng.integers(len(big))) if k else int(rng.integers(len(small))) for k in kinds]
+    arrivals = np.cumsum(rng.exponential(1.0 / args.qps, size=n))
+    reqs = []
+    for k, i in zip(kinds, picks):
+        if k:
+            reqs.append(("large", i, list(larges[i].cx)))
+        else:
+            reqs.append(("small", i, list(specs[i].cx)))
+
+    # ---- server ------------------------------------------------------------
+    budget = 96 << 30
+    srv = sptc.Server(budget_bytes=budget, workers=3, large_products=1 << 28,
+                      large_budget_bytes=budget // 3)
+    sid = [srv.add_tensor(t) for t in small]
+    bid = [srv.add_tensor(t) for t in big]
+
+    def tid(q):
+        return bid[q[1]] if q[0] == "large" else sid[q[1]]
+
+    # warm the resident operand cache and the product counts, one request each
+    seen = set()
+    for q in reqs:
+        key = (q[0], q[1])
+        if key in seen:
+            continue
+        seen.add(key)
+        r, _ = srv.wait(srv.submit(tid(q), tid(q), q[2], q[2]), want_result=False)
+    torch.cuda.synchronize()
+
+    def run_server(times):
+        ids = [None] * n
+        t0 = time.perf_counter()
+
+        def submitter():
+            for k, q in enumerate(reqs):
+                dt = times[k] - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                ids[k] = srv.submit(tid(q), tid(q), q[2], q[2])
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        lat, out_nnz, sizes = [], 0, []
+        for k in range(n):
+            r, info = srv.wait(ids[k])
+            lat.append(info["latency_ms"])
+            sizes.append(info["batch_size"])
+            out_nnz += r.nnz if r is not None else 0
+            del r
+        wall = time.perf_counter() - t0
+        return lat, wall, out_nnz, sizes
+
+    log("server: poisson")
+    lat_s, wall_s, out_s, sizes = run_server(arrivals)
+    log("server: burst")
+    burst_s = run_server(np.zeros(n))
+    st = srv.stats()
+    srv.close()
+
+    # ---- serial plain calls --------------------------------------------------
+    dsmall = [to_device(t) for t in small]
+    dbig = [to_device(t) for t in big]
+
+    def plain(q):
+        if q[0] == "large":
+            d = dbig[q[1]]
+            return sptc.SpTC(d, d, q[2], q[2], on_device=True, count_only=True,
+                             mem_budget_bytes=budget // 3)
+        d = dsmall[q[1]]
+        return sptc.SpTC(d, d, q[2], q[2], on_device=True)
+
+    for q in reqs[:20]:
+        del_r = plain(q)
+        del del_r
+    torch.cuda.synchronize()
+
+    def run_serial(times):
+        t0 = time.perf_counter()
+        lat, out_nnz = [], 0
+        for k, q in enumerate(reqs):
+            dt = times[k] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            r = plain(q)
+            torch.cuda.synchronize()
+            lat.append((time.perf_counter() - t0 - times[k]) * 1e3)
+            out_nnz += r.nnz
+            del r
+        return lat, time.perf_counter() - t0, out_nnz
+
+    log("serial: poisson")
+    lat_p, wall_p, out_p = run_serial(arrivals)
+    log("serial: burst")
+    burst_p = run_serial(np.zeros(n))
+    res = {
+        "metric": "SpTC requests/s (served, burst)",
+        "value": n / burst_s[1],
+        "unit": "requests/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": burst_s[1] * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "serving", "requests": n, "qps": args.qps,
+                   "large_fraction": float(kinds.mean()), "budget_bytes": budget},
+        "server_poisson": {**_percentiles(lat_s), "wall_s": wall_s,
+                           "mean_batch": float(np.mean(sizes))},
+        "serial_poisson": {**_percentiles(lat_p), "wall_s": wall_p},
+        "server_burst": {**_percentiles(burst_s[0]), "wall_s": burst_s[1],
+                         "requests_per_s": n / burst_s[1]},
+        "serial_burst": {**_percentiles(burst_p[0]), "wall_s": burst_p[1],
+                         "requests_per_s": n / burst_p[1]},
+        "burst_speedup": burst_p[1] / burst_s[1],
+        "output_nnz_equal": out_s == out_p and burst_s[2] == out_p,
+        "server_stats": st,
+    }
+    print(json.dumps(res))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -180,6 +334,8 @@ def main():
         sys.exit(2)
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "serving":
+        return bench_serving(args)
     import torch
     from gsparc_amd import sptc
     from gsparc_amd.workloads import CONFIGS, NELL2_CAP_BYTES

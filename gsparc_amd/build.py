@@ -16,7 +16,7 @@ CSRC = os.path.join(PKG_DIR, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 HIP_SOURCES = ["sort.hip", "kernels.hip", "batch.hip", "device.cpp", "engine.cpp",
-               "batch_engine.cpp", "api.cpp"]
+               "batch_engine.cpp", "serve.cpp", "api.cpp"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # no packed-FP32 instructions anywhere in the device code

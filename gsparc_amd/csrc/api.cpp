@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "batch_engine.hpp"
+#include "serve.hpp"
 #include "engine.hpp"
 
 using namespace sptc;
@@ -12,6 +13,10 @@ using namespace sptc;
 struct sptc_batch_s {
   BatchEngine e;
   KStat kstats[SPTC_K_COUNT];
+};
+
+struct sptc_server_s {
+  std::unique_ptr<Server> srv;
 };
 
 namespace {
@@ -587,5 +592,67 @@ sptc_status sptc_batch_kernel_stats(const sptc_batch* b, int32_t cls, uint64_t* 
 }
 
 void sptc_batch_free(sptc_batch* b) { delete b; }
+
+// ---- serving ----------------------------------------------------------------
+
+sptc_status sptc_server_create(const sptc_server_opts* opts, sptc_server** out) {
+  if (!out) return fail(SPTC_ERR_BAD_ARGS, "out is NULL");
+  *out = nullptr;
+  if (!device_ok()) return fail(SPTC_ERR_NO_DEVICE, "no GPU");
+  sptc_server_opts o;
+  std::memset(&o, 0, sizeof(o));
+  if (opts) o = *opts;
+  if (o.workers > 16 || o.max_batch < 0 || o.workers < 0)
+    return fail(SPTC_ERR_BAD_ARGS, "workers must be 0..16, max_batch >= 0");
+  try {
+    auto* s = new sptc_server_s();
+    s->srv = std::make_unique<Server>(o);
+    *out = s;
+  } catch (const HipError& he) {
+    return from_hip(he);
+  }
+  return SPTC_OK;
+}
+
+sptc_status sptc_server_add_tensor(sptc_server* s, const sptc_coo* t, int32_t* id) {
+  if (!s || !id) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");
+  sptc_status st = check_coo(t, "tensor");
+  if (st != SPTC_OK) return st;
+  try {
+    std::string m;
+    st = s->srv->add_tensor(*t, id, &m);
+    if (st != SPTC_OK) return fail(st, m);
+  } catch (const HipError& he) {
+    return from_hip(he);
+  }
+  return SPTC_OK;
+}
+
+sptc_status sptc_server_submit(sptc_server* s, int32_t x, int32_t y, int32_t nc,
+                               const int32_t* cx, const int32_t* cy, uint64_t* request) {
+  if (!s || !request || (nc && (!cx || !cy))) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");
+  std::string m;
+  sptc_status st = s->srv->submit(x, y, nc, cx, cy, request, &m);
+  if (st != SPTC_OK) return fail(st, m);
+  return SPTC_OK;
+}
+
+sptc_status sptc_server_wait(sptc_server* s, uint64_t request, sptc_result** out,
+                             sptc_request_info* info) {
+  if (!s) return fail(SPTC_ERR_BAD_ARGS, "NULL server");
+  if (out) *out = nullptr;
+  std::string m;
+  sptc_status st = s->srv->wait(request, out, info, &m);
+  if (st != SPTC_OK) return fail(st, m);
+  return SPTC_OK;
+}
+
+sptc_status sptc_server_get_stats(sptc_server* s, sptc_server_stats* stats) {
+  if (!s || !stats) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");
+  s->srv->stats(stats);
+  return SPTC_OK;
+}
+
+void sptc_server_free(sptc_server* s) { delete s; }
 
 }  // extern "C"

@@ -90,6 +90,11 @@ __global__ __launch_bounds__(kBlock) void k_blinearize(const BLinArgs* __restric
     const uint32_t sgm = find_seg(seg_off, nseg, i);
     const BLinArgs& a = args[sgm];
     const uint64_t li = i - seg_off[sgm];
+    if (a.skeys) {
+      keys[i] = a.skeys[li];
+      vals[i] = a.svals[li];
+      continue;
+    }
     uint64_t k = 0;
     for (int m = 0; m < a.order; ++m) {
       const uint64_t x = a.idx_bytes == 4 ? (uint64_t) static_cast<const uint32_t*>(a.idx[m])[li]

@@ -25,7 +25,9 @@ struct SegTile {
   uint32_t pad;
 };
 
-// Per-operand linearize arguments (device table, one per segment).
+// Per-operand linearize arguments (device table, one per segment).  With
+// skeys set the segment is an already linearized and sorted operand and is
+// copied instead.
 struct BLinArgs {
   const void* idx[SPTC_MAX_ORDER];
   int shift[SPTC_MAX_ORDER];
@@ -33,6 +35,8 @@ struct BLinArgs {
   int order;
   int idx_bytes;
   const float* val;
+  const uint64_t* skeys;
+  const float* svals;
 };
 
 // Per-instance parameters used by the match/accumulate kernels.

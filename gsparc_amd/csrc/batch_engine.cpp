@@ -35,7 +35,7 @@ uint64_t BatchEngine::readback_u64(const uint64_t* dev, hipStream_t s) {
 
 sptc_status BatchEngine::plan(int B, const sptc_coo* X, const sptc_coo* Y, const int32_t* nc,
                               const int32_t* const* cx, const int32_t* const* cy,
-                              const sptc_opts& o, hipStream_t s) {
+                              const sptc_opts& o, hipStream_t s, const Operand* const* pre) {
   const auto t0 = std::chrono::steady_clock::now();
   o_ = o;
   if (o.count_only) {
@@ -72,13 +72,24 @@ sptc_status BatchEngine::plan(int B, const sptc_coo* X, const sptc_coo* Y, const
   staged_.resize(2 * B);
   seg_off_h_.assign(2 * B + 1, 0);
   std::vector<BLinArgs> la(2 * B);
+  presorted_ = pre != nullptr;
   for (int g = 0; g < 2 * B; ++g) {
     const sptc_coo& t = g < B ? X[g] : Y[g - B];
     const Layout& L = g < B ? xl_[g] : yl_[g - B];
-    stager.stage(t, &staged_[g]);
-    seg_off_h_[g + 1] = seg_off_h_[g] + t.nnz;
     BLinArgs& a = la[g];
     std::memset(&a, 0, sizeof(a));
+    seg_off_h_[g + 1] = seg_off_h_[g] + t.nnz;
+    if (pre && pre[g]) {
+      if (pre[g]->nnz != t.nnz || pre[g]->lay.nbits != L.nbits) {
+        msg = "prepared operand does not match its instance";
+        return SPTC_ERR_BAD_ARGS;
+      }
+      a.skeys = pre[g]->keys.get();
+      a.svals = pre[g]->vals.get();
+      continue;
+    }
+    presorted_ = false;
+    stager.stage(t, &staged_[g]);
     a.order = L.order;
     a.idx_bytes = t.idx_bytes;
     a.val = staged_[g].val;
@@ -89,6 +100,11 @@ sptc_status BatchEngine::plan(int B, const sptc_coo* X, const sptc_coo* Y, const
     }
     nbits_all_ = std::max(nbits_all_, L.nbits);
   }
+  if (pre)  // copied segments are sorted already; a mixed batch sorts all
+    for (int g = 0; g < 2 * B; ++g) {
+      const Layout& L = g < B ? xl_[g] : yl_[g - B];
+      nbits_all_ = std::max(nbits_all_, L.nbits);
+    }
   NX_ = seg_off_h_[B];
   N_ = seg_off_h_[2 * B];
   NY_ = N_ - NX_;
@@ -122,7 +138,7 @@ sptc_status BatchEngine::plan(int B, const sptc_coo* X, const sptc_coo* Y, const
   plan_y_.build(yoff_h_, s);
   segmented_sort_reserve(ws_, plan_all_, s);
   segmented_sort_reserve(ws_, plan_y_, s);
-  lin_alt_ = (segmented_sort_passes(nbits_all_) & 1) != 0;
+  lin_alt_ = !presorted_ && (segmented_sort_passes(nbits_all_) & 1) != 0;
   const uint64_t n1 = N_ ? N_ : 1;
   k0_.reset(n1);
   v0_.reset(n1);
@@ -355,7 +371,8 @@ sptc_status BatchEngine::run(hipStream_t s, bool P) {
     int tok = kt ? kt->begin(s) : -1;
     launch_blinearize(lin_.get(), seg_off_.get(), 2 * B, N_, kl, vl, s);
     if (kt) kt->end(tok, SPTC_K_LINEARIZE, (double)N_ * 28, s);
-    segmented_sort(ws_, plan_all_, kl, vl, ko, vo, nbits_all_, s, kt, SPTC_K_SORT_INPUT);
+    if (!presorted_)
+      segmented_sort(ws_, plan_all_, kl, vl, ko, vo, nbits_all_, s, kt, SPTC_K_SORT_INPUT);
   }
   fill_u64(digest_.get(), 0, 1, s);
   launch_digest(k0_.get(), N_, digest_.get(), s);
@@ -628,7 +645,7 @@ sptc_status BatchEngine::sync(hipStream_t s) {
   return SPTC_OK;
 }
 
-sptc_status BatchEngine::instance_result(int i, sptc_result_s* r) {
+sptc_status BatchEngine::instance_result(int i, sptc_result_s* r, hipStream_t s) {
   const Layout& xl = xl_[i];
   const Layout& yl = yl_[i];
   sptc_result_info& in = r->info;
@@ -646,7 +663,7 @@ sptc_status BatchEngine::instance_result(int i, sptc_result_s* r) {
   r->vrow_key = row_key_.get() + iR_[i];
   in.row_key = r->vrow_key;
   DevBuf<double> sum(1);
-  fill_u64(reinterpret_cast<uint64_t*>(sum.get()), 0, 1, nullptr);
+  fill_u64(reinterpret_cast<uint64_t*>(sum.get()), 0, 1, s);
   for (const InstSlice& sl : slices_[i]) {
     const BOut& ob = outs_[sl.out];
     OutChunk c;
@@ -656,12 +673,13 @@ sptc_status BatchEngine::instance_result(int i, sptc_result_s* r) {
     c.vrow = ob.row.get() + sl.b;
     c.vcol = ob.col.get() + sl.b;
     c.vval = ob.val.get() + sl.b;
-    launch_sum_f64(c.vval, c.nnz, sum.get(), nullptr);
+    launch_sum_f64(c.vval, c.nnz, sum.get(), s);
     in.nnz += c.nnz;
     r->chunks.push_back(std::move(c));
   }
   in.nchunks = (int32_t)r->chunks.size();
-  SPTC_HIP(hipMemcpy(&in.value_sum, sum.get(), 8, hipMemcpyDeviceToHost));
+  SPTC_HIP(hipMemcpyAsync(&in.value_sum, sum.get(), 8, hipMemcpyDeviceToHost, s));
+  SPTC_HIP(hipStreamSynchronize(s));
   if (r->chunks.size() == 1) {
     in.row = r->chunks[0].vrow;
     in.col = r->chunks[0].vcol;

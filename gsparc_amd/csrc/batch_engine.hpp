@@ -59,16 +59,19 @@ class BatchEngine {
  public:
   BatchEngine();
   ~BatchEngine();
+  // pre (optional, 2B entries: X_0..X_{B-1}, Y_0..Y_{B-1}) supplies operands
+  // already linearized and sorted with this instance's layout (make_pair_
+  // layouts); the batch then copies them instead of linearizing and sorting.
   sptc_status plan(int B, const sptc_coo* X, const sptc_coo* Y, const int32_t* nc,
                    const int32_t* const* cx, const int32_t* const* cy, const sptc_opts& o,
-                   hipStream_t s);
+                   hipStream_t s, const Operand* const* pre = nullptr);
   // Launch sequence without allocation or synchronization.
   void enqueue(hipStream_t s) { run(s, false); }
   sptc_status capture();
   sptc_status launch_graph(hipStream_t s);
   // Waits for stream s and reports a stale plan.
   sptc_status sync(hipStream_t s);
-  sptc_status instance_result(int i, sptc_result_s* out);
+  sptc_status instance_result(int i, sptc_result_s* out, hipStream_t s = nullptr);
   sptc_batch_info info;
   std::string msg;
   KTimer& ktimer() { return kt_; }
@@ -94,6 +97,7 @@ class BatchEngine {
   SegSortPlan plan_all_, plan_y_;
   SortWorkspace ws_;
   bool lin_alt_ = false;
+  bool presorted_ = false;
   DevBuf<uint64_t> k0_, k1_, fk_;
   DevBuf<float> v0_, v1_, fv_;
   DevBuf<uint64_t> ytab_, yt_off_, ukey_, ustart_, u_off_;

@@ -21,10 +21,19 @@ size_t size_class(size_t b) {
 }
 }  // namespace
 
-DevicePool& DevicePool::get() {
+namespace {
+thread_local DevicePool* tl_pool = nullptr;
+}
+
+DevicePool& DevicePool::global() {
   static DevicePool* pool = new DevicePool();  // lives until process exit
   return *pool;
 }
+
+DevicePool& DevicePool::get() { return tl_pool ? *tl_pool : global(); }
+
+DevicePool::Scope::Scope(DevicePool* p) : prev_(tl_pool) { tl_pool = p; }
+DevicePool::Scope::~Scope() { tl_pool = prev_; }
 
 // Best fit over every slab's free ranges; ranges split on allocation and
 // coalesce with their neighbours on release.
@@ -169,6 +178,10 @@ size_t device_free_bytes() {
   size_t fr = 0, tot = 0;
   SPTC_HIP(hipMemGetInfo(&fr, &tot));
   return fr + DevicePool::get().cached_bytes();
+}
+
+DevicePool::~DevicePool() {
+  for (auto& sl : slabs_) (void)hipFree(sl.first);
 }
 
 }  // namespace sptc

@@ -18,7 +18,22 @@ namespace sptc {
 // status here instead of exit(1) on exhaustion.
 class DevicePool {
  public:
+  // The calling thread's pool: the process-wide one, unless a Scope made
+  // another current (each serving worker allocates from its own pool, so a
+  // block freed while its stream still runs is only reused on that stream).
   static DevicePool& get();
+  static DevicePool& global();
+  class Scope {
+   public:
+    explicit Scope(DevicePool* p);
+    ~Scope();
+   private:
+    DevicePool* prev_;
+  };
+  DevicePool() = default;
+  ~DevicePool();
+  DevicePool(const DevicePool&) = delete;
+  DevicePool& operator=(const DevicePool&) = delete;
   void* alloc(size_t bytes);  // throws HipError on failure
   void free(void* p);
   void trim();                // return fully free slabs to the runtime
@@ -51,18 +66,31 @@ class DevBuf {
   ~DevBuf() { release(); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_), pool_(o.pool_) {
+    o.p_ = nullptr;
+    o.n_ = 0;
+  }
   DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+    if (this != &o) {
+      release();
+      p_ = o.p_;
+      n_ = o.n_;
+      pool_ = o.pool_;
+      o.p_ = nullptr;
+      o.n_ = 0;
+    }
     return *this;
   }
   void reset(uint64_t n) {
     release();
     n_ = n;
-    p_ = n ? static_cast<T*>(DevicePool::get().alloc(n * sizeof(T))) : nullptr;
+    if (n) {
+      pool_ = &DevicePool::get();
+      p_ = static_cast<T*>(pool_->alloc(n * sizeof(T)));
+    }
   }
   void release() {
-    if (p_) DevicePool::get().free(p_);
+    if (p_) pool_->free(p_);
     p_ = nullptr;
     n_ = 0;
   }
@@ -73,6 +101,7 @@ class DevBuf {
  private:
   T* p_ = nullptr;
   uint64_t n_ = 0;
+  DevicePool* pool_ = nullptr;
 };
 
 struct DeviceInfo {

@@ -83,6 +83,45 @@ sptc_status make_pair_layouts(int xorder, const uint64_t* xdims, int yorder,
   return SPTC_OK;
 }
 
+uint64_t count_products(const Operand& X, const Operand& Y, hipStream_t s) {
+  const uint64_t nx = X.nnz, ny = Y.nnz;
+  if (!nx || !ny) return 0;
+  const int ncb = X.lay.ncbits, ynf = Y.lay.nfbits;
+  DevBuf<uint64_t> tlo, thi, ukey, ustart;
+  YTable tab;
+  std::memset(&tab, 0, sizeof(tab));
+  U2 host;
+  if (ncb <= kDirectBits) {
+    const uint64_t K = 1ull << ncb;
+    tlo.reset(K);
+    thi.reset(K);
+    fill_u64(tlo.get(), 0, K, s);
+    fill_u64(thi.get(), 0, K, s);
+    launch_ytable_direct(Y.keys.get(), ny, ynf, K, tlo.get(), thi.get(), s);
+    tab.direct = 1;
+    tab.lo = tlo.get();
+    tab.hi = thi.get();
+  } else {
+    ukey.reset(ny + 1);
+    ustart.reset(ny + 1);
+    DevBuf<U2> tmp(scan_tmp_count(ny) + 1);
+    run_yunique(Y.keys.get(), ny, ynf, ukey.get(), ustart.get(), tmp.get(), s);
+    SPTC_HIP(hipMemcpyAsync(&host, tmp.get() + scan_tmp_count(ny), sizeof(U2),
+                            hipMemcpyDeviceToHost, s));
+    SPTC_HIP(hipStreamSynchronize(s));
+    tab.direct = 0;
+    tab.ukey = ukey.get();
+    tab.ustart = ustart.get();
+    tab.U = host.a;
+  }
+  DevBuf<U2> tmp(scan_tmp_count(nx) + 1);
+  run_match_count(X.keys.get(), nx, ncb, tab, tmp.get(), s);
+  SPTC_HIP(hipMemcpyAsync(&host, tmp.get() + scan_tmp_count(nx), sizeof(U2),
+                          hipMemcpyDeviceToHost, s));
+  SPTC_HIP(hipStreamSynchronize(s));
+  return host.a;
+}
+
 Engine::Engine(hipStream_t s, const sptc_opts& o) : s_(s), o_(o) {
   for (auto& e : ev_) SPTC_HIP(hipEventCreate(&e));
   SPTC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_total_), sizeof(U2) * 2, 0));

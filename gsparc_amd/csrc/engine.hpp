@@ -2,6 +2,7 @@
 // memory-aware chunking and ESC accumulation.  Host orchestration only; the
 // work runs in the kernels of kernels.hip and sort.hip.
 #pragma once
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -67,6 +68,7 @@ struct sptc_result_s {
   sptc::Layout xl, yl;
   sptc::DevBuf<uint64_t> row_key;
   const uint64_t* vrow_key = nullptr;  // view (batch instance results)
+  std::shared_ptr<void> keep;          // owner of the viewed buffers
   std::vector<sptc::OutChunk> chunks;
   const uint64_t* RK() const { return vrow_key ? vrow_key : row_key.get(); }
 };
@@ -86,6 +88,11 @@ struct StagedCOO {
   const float* val = nullptr;
   std::vector<DevBuf<uint8_t>> own;
 };
+
+// Intermediate products of X x Y for two prepared operands (one reduce scan
+// over X against a contract table of Y): the partitioner's output-size
+// bound, used for admission.  Synchronizes the stream.
+uint64_t count_products(const Operand& X, const Operand& Y, hipStream_t s);
 
 class Engine {
  public:

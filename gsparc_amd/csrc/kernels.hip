@@ -221,6 +221,13 @@ void run_match(const uint64_t* xkeys, const float* xvals, uint64_t nx, int ncbit
   run_scan(op, nx, tmp, s);
 }
 
+void run_match_count(const uint64_t* xkeys, uint64_t nx, int ncbits, const YTable& t, U2* tmp,
+                     hipStream_t s) {
+  MatchOut o{};
+  MatchOp op{xkeys, nullptr, nx, low_mask(ncbits), ncbits, t, o, 0};
+  run_scan(op, nx, tmp, s, /*emit=*/false);
+}
+
 struct ExclScanOp {
   const uint64_t* in;
   uint64_t* out;

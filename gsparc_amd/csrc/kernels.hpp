@@ -91,6 +91,9 @@ void run_yunique(const uint64_t* ykeys, uint64_t ny, int ynf, uint64_t* ukey,
                  uint64_t* ustart, U2* tmp, hipStream_t s);
 void run_match(const uint64_t* xkeys, const float* xvals, uint64_t nx, int ncbits,
                const YTable& t, const MatchOut& o, int api, U2* tmp, hipStream_t s);
+// Reduce half of run_match: tmp[scan_tmp_count(nx)] = {ir, A << 32 | R}.
+void run_match_count(const uint64_t* xkeys, uint64_t nx, int ncbits, const YTable& t, U2* tmp,
+                     hipStream_t s);
 void run_exclusive_scan(const uint64_t* in, uint64_t* out, uint64_t n, U2* tmp,
                         hipStream_t s);
 void launch_chunk_end(const uint64_t* row_prod, const uint64_t* row_act, uint64_t R,

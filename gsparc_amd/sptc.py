@@ -62,6 +62,23 @@ class _BatchInfo(Structure):
                 ("device_bytes", c_uint64), ("plan_ms", c_double)]
 
 
+class _ServerOpts(Structure):
+    _fields_ = [("budget_bytes", c_uint64), ("workers", c_int32), ("max_batch", c_int32),
+                ("large_products", c_uint64), ("large_budget_bytes", c_uint64)]
+
+
+class _ReqInfo(Structure):
+    _fields_ = [("id", c_uint64), ("status", c_int32), ("count_only", c_int32),
+                ("batch_size", c_int32), ("worker", c_int32), ("ir_nnz", c_uint64),
+                ("queue_ms", c_double), ("latency_ms", c_double)]
+
+
+class _ServerStats(Structure):
+    _fields_ = [(n, c_uint64) for n in ("requests_done", "batches", "batched_requests",
+                                        "large_requests", "resident_bytes",
+                                        "peak_in_use_bytes", "in_use_bytes", "budget_bytes")]
+
+
 _bound = False
 
 
@@ -120,6 +137,17 @@ def lib():
                                               POINTER(c_double), POINTER(c_double)]
         L.sptc_batch_free.argtypes = [c_void_p]
         L.sptc_batch_free.restype = None
+        L.sptc_server_create.argtypes = [POINTER(_ServerOpts), POINTER(c_void_p)]
+        L.sptc_server_add_tensor.argtypes = [c_void_p, POINTER(_COO), POINTER(c_int32)]
+        L.sptc_server_submit.argtypes = [c_void_p, c_int32, c_int32, c_int32, POINTER(c_int32),
+                                         POINTER(c_int32), POINTER(c_uint64)]
+        L.sptc_server_wait.argtypes = [c_void_p, c_uint64, POINTER(c_void_p), POINTER(_ReqInfo)]
+        L.sptc_server_get_stats.argtypes = [c_void_p, POINTER(_ServerStats)]
+        L.sptc_server_free.argtypes = [c_void_p]
+        L.sptc_server_free.restype = None
+        for name in ("sptc_server_create", "sptc_server_add_tensor", "sptc_server_submit",
+                     "sptc_server_wait", "sptc_server_get_stats"):
+            getattr(L, name).restype = c_int32
         for name in ("sptc_batch_create", "sptc_batch_run", "sptc_batch_sync",
                      "sptc_batch_get_info", "sptc_batch_result", "sptc_batch_kernel_stats"):
             getattr(L, name).restype = c_int32
@@ -367,6 +395,58 @@ class Batch:
             check(lib().sptc_batch_kernel_stats(self._h, i, byref(n), byref(ms), byref(b)))
             out[name] = (n.value, ms.value, b.value)
         return out
+
+
+class Server:
+    """Request loop over resident tensors (sptc_server_*): HBM-footprint
+    admission, continuous batching of small requests, large requests served
+    alone and count-only."""
+
+    def __init__(self, budget_bytes=0, workers=0, max_batch=0, large_products=0,
+                 large_budget_bytes=0):
+        o = _ServerOpts(int(budget_bytes), int(workers), int(max_batch), int(large_products),
+                        int(large_budget_bytes))
+        h = c_void_p()
+        check(lib().sptc_server_create(byref(o), byref(h)))
+        self._h = h
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().sptc_server_free(h)
+            self._h = c_void_p(0)
+
+    __del__ = close
+
+    def add_tensor(self, t, on_device=False):
+        ht = _HostCOO(t, on_device)
+        tid = c_int32()
+        check(lib().sptc_server_add_tensor(self._h, byref(ht.s), byref(tid)))
+        return tid.value
+
+    def submit(self, x, y, cmodes_x, cmodes_y):
+        nc = len(cmodes_x)
+        cx = (c_int32 * max(1, nc))(*cmodes_x)
+        cy = (c_int32 * max(1, nc))(*cmodes_y)
+        rid = c_uint64()
+        check(lib().sptc_server_submit(self._h, int(x), int(y), nc, cx, cy, byref(rid)))
+        return rid.value
+
+    def wait(self, request, want_result=True):
+        """(Result or None, info dict); raises SpTCError when the request
+        failed."""
+        out = c_void_p()
+        info = _ReqInfo()
+        st = lib().sptc_server_wait(self._h, int(request), byref(out) if want_result else None,
+                                    byref(info))
+        check(st)
+        d = {k: getattr(info, k) for k, _ in _ReqInfo._fields_}
+        return (Result(out.value) if out.value else None), d
+
+    def stats(self):
+        st = _ServerStats()
+        check(lib().sptc_server_get_stats(self._h, byref(st)))
+        return {k: getattr(st, k) for k, _ in _ServerStats._fields_}
 
 
 def pool_stats():

@@ -227,6 +227,76 @@ sptc_status sptc_batch_kernel_stats(const sptc_batch* b, int32_t cls,
 
 void sptc_batch_free(sptc_batch* b);
 
+/* ---- serving ----------------------------------------------------------------
+ * A request loop over tensors kept resident on the device.  Requests
+ * (contractions of two registered tensors) are planned on a planner thread
+ * -- operands linearized and sorted once per tensor and layout and kept
+ * resident, intermediate products counted -- then admitted in arrival order
+ * while their estimated device footprint fits the budget.  Small requests
+ * that are ready together run as one batch (sptc_batch_* machinery); a
+ * request above the large-product threshold runs alone, count-only (nnz and
+ * value checksum: the reference's plain path likewise returns only the nnz,
+ * main.cu:375).  Several workers, each with its own stream, overlap their
+ * launches.  No reference counterpart. */
+typedef struct sptc_server_s sptc_server;
+
+typedef struct {
+  uint64_t budget_bytes;       /* device bytes the server may hold: resident
+                                  tensors, prepared operands and admitted
+                                  work; 0: 96 GiB                         */
+  int32_t workers;             /* concurrent launch streams; 0: 3         */
+  int32_t max_batch;           /* requests per batched launch; 0: 256     */
+  uint64_t large_products;     /* requests with more intermediate products
+                                  run alone, count-only; 0: 2^28          */
+  uint64_t large_budget_bytes; /* device bytes one large request may hold;
+                                  0: budget / 3                           */
+} sptc_server_opts;
+
+typedef struct {
+  uint64_t id;
+  int32_t status;              /* sptc_status of the request             */
+  int32_t count_only;          /* 1: result holds nnz and value_sum only */
+  int32_t batch_size;          /* requests in the launch that served it  */
+  int32_t worker;
+  uint64_t ir_nnz;
+  double queue_ms;             /* submit -> its launch started           */
+  double latency_ms;           /* submit -> result ready                 */
+} sptc_request_info;
+
+typedef struct {
+  uint64_t requests_done;
+  uint64_t batches;            /* batched launches                       */
+  uint64_t batched_requests;   /* requests they served                   */
+  uint64_t large_requests;     /* requests served alone, count-only      */
+  uint64_t resident_bytes;     /* tensors + prepared operands            */
+  uint64_t peak_in_use_bytes;  /* admitted footprints, peak              */
+  uint64_t in_use_bytes;
+  uint64_t budget_bytes;
+} sptc_server_stats;
+
+sptc_status sptc_server_create(const sptc_server_opts* opts, sptc_server** out);
+
+/* Copies a COO tensor (host or device arrays) into the server; it stays
+ * resident until sptc_server_free.  *id names it in requests. */
+sptc_status sptc_server_add_tensor(sptc_server* s, const sptc_coo* t, int32_t* id);
+
+/* Queues Z = T[x] x_{cx,cy} T[y]; returns at once with the request id. */
+sptc_status sptc_server_submit(sptc_server* s, int32_t x, int32_t y, int32_t nc,
+                               const int32_t* cx, const int32_t* cy, uint64_t* request);
+
+/* Blocks until the request is done.  *out (may be NULL) receives its result
+ * (free with sptc_result_free; a batched request's result keeps its batch's
+ * buffers -- and their share of the budget -- until freed).  info (may be
+ * NULL) receives status, batch size and latency.  Returns the request's
+ * status. */
+sptc_status sptc_server_wait(sptc_server* s, uint64_t request, sptc_result** out,
+                             sptc_request_info* info);
+
+sptc_status sptc_server_get_stats(sptc_server* s, sptc_server_stats* stats);
+
+/* Stops the workers (queued requests are dropped) and frees the tensors. */
+void sptc_server_free(sptc_server* s);
+
 /* ---- per-phase entry points (device pointers, caller-owned buffers) ---- */
 
 /* Key layout of one operand (get_fmode + compute_nbtis + setup_slitom,
