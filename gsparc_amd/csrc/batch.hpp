@@ -11,19 +11,11 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "segsort.hpp"
 
 namespace sptc {
 
 constexpr int kBatchMaxInstances = 4096;
-
-// One tile of a segmented sort.
-struct SegTile {
-  uint64_t start;    // first key (global index)
-  uint64_t cbase;    // count-matrix index of digit 0 of this tile
-  uint64_t cstride;  // distance between digits (tiles in the segment)
-  uint32_t count;    // keys in the tile
-  uint32_t pad;
-};
 
 // Per-operand linearize arguments (device table, one per segment).  With
 // skeys set the segment is an already linearized and sorted operand and is
@@ -46,27 +38,6 @@ struct BInst {
   int kyf;           // Y column (rank) field width
   uint64_t yt_off;   // first entry of this instance's rank -> free-key table
 };
-
-// Tile table of a segmented sort: each segment [seg_off[s], seg_off[s+1])
-// is cut into its own tiles; the count matrix is segment-major and digit-
-// major within a segment, so one flat exclusive scan yields every tile's
-// global digit offsets.  Built once per batch shape (one host->device copy).
-struct SegSortPlan {
-  DevBuf<SegTile> tiles;
-  uint64_t ntiles = 0;
-  uint64_t n = 0;
-  void build(const std::vector<uint64_t>& seg_off, hipStream_t s);
-};
-// Sorts each segment of (k0, v0) on bits [0, nbits) with 8-bit reduce-then-
-// scan passes, ping-ponging with (k1, v1); ws supplies the count matrices
-// (segmented_sort_reserve).  Launches only (no allocation or host
-// synchronization), so it can be replayed and captured.  Returns true when
-// the result ended in (k1, v1).
-void segmented_sort_reserve(SortWorkspace& ws, const SegSortPlan& plan, hipStream_t s);
-bool segmented_sort(SortWorkspace& ws, const SegSortPlan& plan, uint64_t* k0, float* v0,
-                    uint64_t* k1, float* v1, int nbits, hipStream_t s, KTimer* kt = nullptr,
-                    int cls = SPTC_K_SORT_INPUT);
-int segmented_sort_passes(int nbits);
 
 void launch_blinearize(const BLinArgs* args, const uint64_t* seg_off, uint32_t nseg,
                        uint64_t n, uint64_t* keys, float* vals, hipStream_t s);

@@ -68,7 +68,7 @@ sptc_status BatchEngine::plan(int B, const sptc_coo* X, const sptc_coo* Y, const
   kt_.on = o.kernel_timing != 0;
 
   // ---- operand layout: X_0..X_{B-1}, then Y_0..Y_{B-1} ----------------------
-  Engine stager(s, o);
+  std::unique_ptr<Engine> stager;  // host operands only
   staged_.resize(2 * B);
   seg_off_h_.assign(2 * B + 1, 0);
   std::vector<BLinArgs> la(2 * B);
@@ -89,7 +89,16 @@ sptc_status BatchEngine::plan(int B, const sptc_coo* X, const sptc_coo* Y, const
       continue;
     }
     presorted_ = false;
-    stager.stage(t, &staged_[g]);
+    if (!t.on_device && t.nnz) {
+      if (!stager) stager = std::make_unique<Engine>(s, o);
+      stager->stage(t, &staged_[g]);
+    } else {
+      staged_[g].nnz = t.nnz;
+      staged_[g].order = t.order;
+      staged_[g].idx_bytes = t.idx_bytes;
+      for (int m = 0; m < t.order; ++m) staged_[g].idx[m] = t.idx[m];
+      staged_[g].val = t.val;
+    }
     a.order = L.order;
     a.idx_bytes = t.idx_bytes;
     a.val = staged_[g].val;
@@ -316,6 +325,7 @@ sptc_status BatchEngine::build_chunks(hipStream_t s) {
       pc.A1 = a1;
       const uint32_t pid = (uint32_t)cur.hp.size();
       cur.hp.push_back(pc);
+      cur.hr.push_back(RowRange{r, r1 - r, cur.lrows, cur.n});
       for (uint64_t q = p; q < p1; q += kBExpandTileItems) {
         BETile t;
         t.p0 = q;
@@ -336,10 +346,31 @@ sptc_status BatchEngine::build_chunks(hipStream_t s) {
     i = j;
   }
   close();
-  uint64_t chunk_cap = 0;
+  uint64_t chunk_cap = 0, max_lrows = 0;
+  for (const EscChunk& c : esc_) max_lrows = std::max(max_lrows, c.lrows);
+  DevBuf<uint64_t> tcnt(max_lrows + 1), toff(max_lrows + 1);
+  if (tmp_.size() < scan_tmp_count(max_lrows) + 1) tmp_.reset(scan_tmp_count(max_lrows) + 1);
   for (EscChunk& c : esc_) {
     c.bits = mode_bits(c.lrows) + c.K;
     c.rb = o_.radix_bits ? o_.radix_bits : choose_radix_bits(c.bits);
+    if (c.K > 0 && c.K < c.bits) {  // row-segmented sort over the column bits?
+      DevBuf<RowRange> rr(c.hr.size());
+      SPTC_HIP(hipMemcpyAsync(rr.get(), c.hr.data(), c.hr.size() * sizeof(RowRange),
+                              hipMemcpyHostToDevice, s));
+      launch_row_tile_count(rr.get(), (uint32_t)c.hr.size(), c.lrows, row_prod_.get(),
+                            tcnt.get(), s);
+      run_exclusive_scan(tcnt.get(), toff.get(), c.lrows, tmp_.get(), s);
+      const uint64_t nt = readback_u64(toff.get() + c.lrows, s);
+      if (prefer_row_sort(c.n, c.bits, c.K, nt, o_.radix_bits)) {
+        c.rowsort = true;
+        c.rnt = nt;
+        c.rtiles.reset(nt);
+        launch_row_tiles(rr.get(), (uint32_t)c.hr.size(), c.lrows, row_prod_.get(), tcnt.get(),
+                         toff.get(), c.rtiles.get(), s);
+        segmented_sort_reserve(ws_, nt, s);
+        SPTC_HIP(hipStreamSynchronize(s));  // rr is released on scope exit
+      }
+    }
     c.pieces.reset(c.hp.size());
     c.tiles.reset(c.ht.size());
     SPTC_HIP(hipMemcpyAsync(c.pieces.get(), c.hp.data(), c.hp.size() * sizeof(BPiece),
@@ -527,12 +558,17 @@ sptc_status BatchEngine::run(hipStream_t s, bool P) {
     int tok = kt ? kt->begin(s) : -1;
     launch_bexpand(ea, s);
     if (kt) kt->end(tok, SPTC_K_EXPAND, 24.0 * (double)c.n, s);
-    ws_.ensure(c.n, c.rb, s);
-    const int passes = radix_passes(c.bits, c.rb);
-    ws_.prepare(passes, s);
-    if (sort_needs_histogram()) histogram(ws_, kA_.get(), c.n, passes, s);
-    const bool alt =
-        sort_after_histogram(ws_, kA_.get(), vA_.get(), c.n, c.bits, s, kt, SPTC_K_SORT_PROD);
+    bool alt;
+    if (c.rowsort) {
+      alt = segmented_sort(ws_, c.rtiles.get(), c.rnt, c.n, kA_.get(), vA_.get(),
+                           ws_.alt_keys.get(), ws_.alt_vals.get(), c.K, s, kt, SPTC_K_SORT_PROD);
+    } else {
+      ws_.ensure(c.n, c.rb, s);
+      const int passes = radix_passes(c.bits, c.rb);
+      ws_.prepare(passes, s);
+      if (sort_needs_histogram()) histogram(ws_, kA_.get(), c.n, passes, s);
+      alt = sort_after_histogram(ws_, kA_.get(), vA_.get(), c.n, c.bits, s, kt, SPTC_K_SORT_PROD);
+    }
     const uint64_t* sk = alt ? ws_.alt_keys.get() : kA_.get();
     const float* sv = alt ? ws_.alt_vals.get() : vA_.get();
     tok = kt ? kt->begin(s) : -1;

@@ -44,8 +44,12 @@ struct EscChunk {
   int K = 0, bits = 0, rb = 8;
   std::vector<BPiece> hp;
   std::vector<BETile> ht;
+  std::vector<RowRange> hr;  // the pieces as row ranges (row-segmented sort)
   DevBuf<BPiece> pieces;
   DevBuf<BETile> tiles;
+  bool rowsort = false;
+  uint64_t rnt = 0;
+  DevBuf<SegTile> rtiles;
   int out = -1;
 };
 

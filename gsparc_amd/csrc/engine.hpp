@@ -9,6 +9,7 @@
 #include "device.hpp"
 #include "kernels.hpp"
 #include "ktimer.hpp"
+#include "segsort.hpp"
 
 namespace sptc {
 
@@ -93,6 +94,15 @@ struct StagedCOO {
 // over X against a contract table of Y): the partitioner's output-size
 // bound, used for admission.  Synchronizes the stream.
 uint64_t count_products(const Operand& X, const Operand& Y, hipStream_t s);
+
+// Product-sort choice for a chunk of n products (key = local row << kyf |
+// column): one sort over all key bits, or a row-segmented sort over the
+// column bits only (the expansion leaves products grouped by row), whose
+// tiles stop at row ends (ntiles_rows of them).  Cost model: per pass, the
+// keys plus a fixed cost per tile of about 1024 keys (block start, digit
+// counts, LDS setup).  SPTC_PROD_SORT=full|rows forces one.
+bool prefer_row_sort(uint64_t n, int full_bits, int col_bits, uint64_t ntiles_rows,
+                     int radix_bits);
 
 class Engine {
  public:

@@ -1,0 +1,82 @@
+// Segmented LSD radix sort: every segment of a key array is sorted on its
+// own, with tiles that never straddle a segment.  Used for the operand sorts
+// of a batch (one segment per operand) and for product sorts (one segment per
+// X row, so only the column bits need sorting: the rows are already in
+// order after the expansion).
+#pragma once
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace sptc {
+
+constexpr int kSegRB = 8;     // digit bits
+constexpr int kSegIPT = 12;   // keys per thread
+constexpr int kSegNT = 512;   // threads per tile
+constexpr uint64_t kSegTile = (uint64_t)kSegIPT * kSegNT;
+
+// One tile of a segmented sort.
+struct SegTile {
+  uint64_t start;    // first key (global index)
+  uint64_t cbase;    // count-matrix index of digit 0 of this tile
+  uint64_t cstride;  // distance between digits (tiles in the segment)
+  uint32_t count;    // keys in the tile
+  uint32_t pad;
+};
+
+// Segment of global index i: the s with off[s] <= i < off[s+1] (off is
+// non-decreasing with nseg+1 entries; empty segments are skipped).
+__device__ __forceinline__ uint32_t find_seg(const uint64_t* off, uint32_t nseg, uint64_t i) {
+  uint32_t lo = 0, hi = nseg;  // last s with off[s] <= i
+  while (lo + 1 < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= i) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Tile table of a segmented sort: each segment [seg_off[s], seg_off[s+1])
+// is cut into its own tiles; the count matrix is segment-major and digit-
+// major within a segment, so one flat exclusive scan yields every tile's
+// global digit offsets.  Built once per batch shape (one host->device copy).
+struct SegSortPlan {
+  DevBuf<SegTile> tiles;
+  uint64_t ntiles = 0;
+  uint64_t n = 0;
+  void build(const std::vector<uint64_t>& seg_off, hipStream_t s);
+};
+// Sorts each segment of (k0, v0) on bits [0, nbits) with 8-bit reduce-then-
+// scan passes, ping-ponging with (k1, v1); ws supplies the count matrices
+// (segmented_sort_reserve).  Launches only (no allocation or host
+// synchronization), so it can be replayed and captured.  Returns true when
+// the result ended in (k1, v1).
+void segmented_sort_reserve(SortWorkspace& ws, uint64_t ntiles, hipStream_t s);
+void segmented_sort_reserve(SortWorkspace& ws, const SegSortPlan& plan, hipStream_t s);
+bool segmented_sort(SortWorkspace& ws, const SegTile* tiles, uint64_t ntiles, uint64_t n,
+                    uint64_t* k0, float* v0, uint64_t* k1, float* v1, int nbits, hipStream_t s,
+                    KTimer* kt = nullptr, int cls = SPTC_K_SORT_INPUT);
+bool segmented_sort(SortWorkspace& ws, const SegSortPlan& plan, uint64_t* k0, float* v0,
+                    uint64_t* k1, float* v1, int nbits, hipStream_t s, KTimer* kt = nullptr,
+                    int cls = SPTC_K_SORT_INPUT);
+int segmented_sort_passes(int nbits);
+
+// Row-segmented tiles over a chunk's products.  The chunk's rows come in
+// ranges of consecutive global rows (one range for a plain chunk; one per
+// piece of a batched chunk); range q covers global rows [ra, ra + rows) and
+// its first product sits at chunk offset out0.  Products of global row r
+// number row_prod[r+1] - row_prod[r].
+struct RowRange {
+  uint64_t ra;
+  uint64_t rows;
+  uint64_t lrow0;  // chunk-local row of ra (prefix of rows)
+  uint64_t out0;   // chunk offset of ra's first product
+};
+// Tile counts per chunk-local row (lrows rows): cnt[l] = ceil(products/tile).
+void launch_row_tile_count(const RowRange* ranges, uint32_t nranges, uint64_t lrows,
+                           const uint64_t* row_prod, uint64_t* cnt, hipStream_t s);
+// tiles[off[l] + t] for every row l (off = exclusive scan of cnt).
+void launch_row_tiles(const RowRange* ranges, uint32_t nranges, uint64_t lrows,
+                      const uint64_t* row_prod, const uint64_t* cnt, const uint64_t* off,
+                      SegTile* tiles, hipStream_t s);
+
+}  // namespace sptc

@@ -298,8 +298,8 @@ void Server::worker(int w) {
         big->batch_size = 1;
         need = big->footprint;
       }
+      ledger_->charge(need);  // under mu_: the next worker sees this charge
     }
-    ledger_->charge(need);
     try {
       if (big) run_large(w, big);
       else run_batch(w, pick);
@@ -348,6 +348,11 @@ void Server::run_large(int w, std::shared_ptr<Request> r) {
   std::memset(&o, 0, sizeof(o));
   o.count_only = 1;
   o.mem_budget_bytes = o_.large_budget_bytes;
+  struct Release {  // the charge goes back however this returns
+    Ledger* l;
+    uint64_t b;
+    ~Release() { l->release(b); }
+  } rel{ledger_.get(), r->footprint};
   auto res = std::make_shared<sptc_result_s>();
   std::memset(&res->info, 0, sizeof(res->info));
   sptc_status st;
@@ -357,7 +362,6 @@ void Server::run_large(int w, std::shared_ptr<Request> r) {
     SPTC_HIP(hipStreamSynchronize(s));
     if (st != SPTC_OK) r->msg = e.msg;
   }
-  ledger_->release(r->footprint);
   r->st = st;
   if (st == SPTC_OK) r->result = res;
 }
