@@ -138,7 +138,6 @@ sptc_status sptc_contract(const sptc_coo* X, const sptc_coo* Y, int32_t nc, cons
     std::memcpy(res->info.phase_ms, e.times.ms, sizeof(res->info.phase_ms));
     // operand preparation ran before contract() collected: add its launches
     e.collect_kernel_stats(res->kstats);
-    res->info.accumulator = 0;
     if (o.verbose) {
       std::fprintf(stderr,
                    "sptc: ir_nnz %llu out_nnz %llu chunks %d | linearize+sort %.3f ms, "
