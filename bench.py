@@ -144,6 +144,12 @@ def bench_batch(args):
     torch.cuda.synchronize()
     plan_s = time.perf_counter() - t0
     info = b.info
+    # kernel breakdown of one planning run (per-launch events; not timed)
+    bt = sptc.Batch(items, on_device=True, kernel_timing=True)
+    kern = {k: {"launches": n, "total_ms": ms, "achieved_gbs": (by / (ms * 1e-3) / 1e9)
+                if ms else None}
+            for k, (n, ms, by) in bt.kernel_stats().items() if n}
+    del bt
     if info["out_nnz"] != plain_out:
         raise SystemExit(f"batched output nnz {info['out_nnz']} != plain {plain_out}")
     res = {}
@@ -175,6 +181,7 @@ def bench_batch(args):
         "speedup_vs_plain": plain_s / res["graph"],
         "speedup_eager_vs_plain": plain_s / res["eager"],
         "batch_info": info,
+        "plan_run_kernels": kern,
     }
     print(json.dumps(out))
 

@@ -21,6 +21,10 @@ HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # no packed-FP32 instructions anywhere in the device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # products are rounded to FP32 before they are summed, on every path (an
+    # FMA of product and running sum would make the dense and ESC
+    # accumulators round differently)
+    "-ffp-contract=off",
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
     "-I" + os.path.join(REPO_DIR, "include"),
 ]

@@ -153,6 +153,7 @@ struct BMatchOp {
     return mk2(cnt, ((uint64_t)(cnt != 0) << 32) | head);
   }
   __device__ void emit(uint64_t i, U2 excl, U2 v) const {
+    if (*o.bad) return;  // the row table is part of the planned results
     const uint32_t s = find_seg(xoff, B, i);
     const BInst in = inst[s];
     const uint64_t k = xkeys[i];

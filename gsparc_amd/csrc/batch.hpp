@@ -70,6 +70,7 @@ struct BMatchOut {
   uint64_t* inst_P;  // [B] first product / active entry / row of instance s
   uint64_t* inst_A;
   uint64_t* inst_R;
+  const uint32_t* bad;  // set: operands changed since planning; write nothing
 };
 void run_bmatch(const uint64_t* xkeys, const float* xvals, const uint64_t* xoff, uint32_t B,
                 const BInst* inst, const uint64_t* ukey, const uint64_t* ustart,

@@ -463,6 +463,7 @@ sptc_status BatchEngine::run(hipStream_t s, bool P) {
   mo.inst_P = iP;
   mo.inst_A = iA;
   mo.inst_R = iR;
+  mo.bad = bad_.get();
   {
     int tok = kt ? kt->begin(s) : -1;
     run_bmatch(k0_.get(), v0_.get(), seg_off_.get(), B, inst_.get(), ukey_.get(), ustart_.get(),

@@ -93,7 +93,8 @@ def test_batch_chunked_pieces_equal_plain(gpu):
     plain = [sptc.SpTC(X, Y, cx, cy, dense_accumulator=-1) for X, Y, cx, cy in items]
     ir = sum(p.ir_nnz for p in plain)
     out = sum(p.nnz for p in plain)
-    b = sptc.Batch(items, dense_accumulator=-1, mem_budget_bytes=out * 16 + ir * 6 + (256 << 20))
+    b = sptc.Batch(items, dense_accumulator=-1,
+                   mem_budget_bytes=out * 16 + ir * 10 + (256 << 20))
     assert b.info["esc_chunks"] > 1
     for i, p in enumerate(plain):
         r = b.result(i)

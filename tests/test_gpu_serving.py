@@ -72,7 +72,9 @@ def test_results_same_alone_and_in_the_mix(gpu):
         else:
             k = reqs.index(q)
             _same(r, alone[k])
-            assert r.info.value_sum == alone[k].info.value_sum
+            # value_sum is an FP64 checksum summed in no fixed order
+            assert abs(r.info.value_sum - alone[k].info.value_sum) <= \
+                1e-12 * abs(alone[k].info.value_sum)
         del r
     st = srv.stats()
     assert st["requests_done"] == len(reqs) + 1 + len(order)
