@@ -59,6 +59,20 @@ def to_device(t):
     return d
 
 
+def measured_traffic(workload, kernel_class):
+    """HBM bytes per launch of the dominant kernel class, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload (calibrated
+    against a copy of known size; see the summary), or None."""
+    path = os.path.join(HERE, "profiles", f"{workload}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"^bench-traffic: (\w+) ([0-9.e+]+)\s*$", line)
+        if m and m.group(1) == kernel_class:
+            return float(m.group(2))
+    return None
+
+
 def cpu_baseline(spec, X, n_rows):
     """Reference CPU SpTC (Sparta ttt, EXPERIMENT_MODES=3) on a bounded
     sample: the nonzeros of the first `n_rows` X free rows (in free-row key
@@ -479,11 +493,12 @@ def main():
         d = kern[dom]
         res["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": d["achieved_gbs"],
                            "peak": peak, "unit": "GB/s", "frac": d["achieved_gbs"] / peak,
-                           "traffic": None, "avg_launch_ms": d["avg_ms"],
+                           "traffic": measured_traffic(args.workload, dom),
+                           "avg_launch_ms": d["avg_ms"],
                            "alg_bytes_per_launch": d["alg_bytes_per_launch"]}
     res["pool"] = sptc.pool_stats()
     if not args.no_cpu_baseline and args.workload != "nell2":
-        rows = args.cpu_sample_rows or (2000 if args.workload == "nips" else 4000)
+        rows = args.cpu_sample_rows or (1200 if args.workload == "nips" else 4000)
         cb = cpu_baseline(spec, X, rows)
         if cb:
             res["cpu_baseline"] = {
