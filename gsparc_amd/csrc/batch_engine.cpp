@@ -191,8 +191,9 @@ sptc_status BatchEngine::plan(int B, const sptc_coo* X, const sptc_coo* Y, const
 
 // Accumulator per instance, with the plain engine's rule (Engine::contract):
 // dense when the column ranks fit an LDS row (<= 2^13), Y has no duplicate
-// coordinates, and the rows are dense enough (ir * 32 >= rows * slots) or
-// dense is forced.
+// coordinates, the rows are dense enough (ir * 32 >= rows * slots) and the
+// matched Y segments long enough (ir >= kDenseMinRun * active), or dense is
+// forced.
 sptc_status BatchEngine::plan_accumulators(hipStream_t s) {
   const uint32_t B = B_;
   std::vector<uint32_t> dup(B, 0);
@@ -205,7 +206,9 @@ sptc_status BatchEngine::plan_accumulators(hipStream_t s) {
     const int kyf = kyf_h_[i];
     if (o_.dense_accumulator >= 0 && ir && kyf <= kDenseMaxBits && !dup[i]) {
       const uint64_t E = 1ull << (kyf < 8 ? 8 : kyf);
-      acc_[i] = (o_.dense_accumulator == 1 || ir * 32 >= R * E) ? 1 : 0;
+      const uint64_t A = iA_[i + 1] - iA_[i];
+      acc_[i] = (o_.dense_accumulator == 1 || (ir * 32 >= R * E && ir >= kDenseMinRun * A))
+                    ? 1 : 0;
     }
     if (o_.dense_accumulator == 1 && ir && !acc_[i]) {
       msg = "instance " + std::to_string(i) +
@@ -302,8 +305,8 @@ sptc_status BatchEngine::build_chunks(hipStream_t s) {
         p1 = iP_[j];
         a1 = iA_[j];
       } else {
-        launch_chunk_end(row_prod_.get(), row_act_.get(), rend, r, ~0ull, rem, scratch_.get(),
-                         s);
+        launch_chunk_end(row_prod_.get(), row_act_.get(), rend, r, rend - r, rem,
+                         scratch_.get(), s);
         uint64_t h[3];
         readback(h, scratch_.get(), 24, s);
         r1 = h[0];
@@ -348,28 +351,19 @@ sptc_status BatchEngine::build_chunks(hipStream_t s) {
   close();
   uint64_t chunk_cap = 0, max_lrows = 0;
   for (const EscChunk& c : esc_) max_lrows = std::max(max_lrows, c.lrows);
-  DevBuf<uint64_t> tcnt(max_lrows + 1), toff(max_lrows + 1);
   if (tmp_.size() < scan_tmp_count(max_lrows) + 1) tmp_.reset(scan_tmp_count(max_lrows) + 1);
   for (EscChunk& c : esc_) {
     c.bits = mode_bits(c.lrows) + c.K;
     c.rb = o_.radix_bits ? o_.radix_bits : choose_radix_bits(c.bits);
-    if (c.K > 0 && c.K < c.bits) {  // row-segmented sort over the column bits?
+    if (c.K > 0 && c.K < c.bits && prefer_row_sort()) {
       DevBuf<RowRange> rr(c.hr.size());
       SPTC_HIP(hipMemcpyAsync(rr.get(), c.hr.data(), c.hr.size() * sizeof(RowRange),
                               hipMemcpyHostToDevice, s));
-      launch_row_tile_count(rr.get(), (uint32_t)c.hr.size(), c.lrows, row_prod_.get(),
-                            tcnt.get(), s);
-      run_exclusive_scan(tcnt.get(), toff.get(), c.lrows, tmp_.get(), s);
-      const uint64_t nt = readback_u64(toff.get() + c.lrows, s);
-      if (prefer_row_sort(c.n, c.bits, c.K, nt, o_.radix_bits)) {
-        c.rowsort = true;
-        c.rnt = nt;
-        c.rtiles.reset(nt);
-        launch_row_tiles(rr.get(), (uint32_t)c.hr.size(), c.lrows, row_prod_.get(), tcnt.get(),
-                         toff.get(), c.rtiles.get(), s);
-        segmented_sort_reserve(ws_, nt, s);
-        SPTC_HIP(hipStreamSynchronize(s));  // rr is released on scope exit
-      }
+      build_row_segments(rr.get(), (uint32_t)c.hr.size(), c.lrows, row_prod_.get(), &c.rseg,
+                         ws_, tmp_.get(), s);
+      c.rowsort = true;
+      SPTC_HIP(hipStreamSynchronize(s));  // rr and the scratch are released below
+      c.rseg.scratch.release();
     }
     c.pieces.reset(c.hp.size());
     c.tiles.reset(c.ht.size());
@@ -561,8 +555,8 @@ sptc_status BatchEngine::run(hipStream_t s, bool P) {
     if (kt) kt->end(tok, SPTC_K_EXPAND, 24.0 * (double)c.n, s);
     bool alt;
     if (c.rowsort) {
-      alt = segmented_sort(ws_, c.rtiles.get(), c.rnt, c.n, kA_.get(), vA_.get(),
-                           ws_.alt_keys.get(), ws_.alt_vals.get(), c.K, s, kt, SPTC_K_SORT_PROD);
+      alt = sort_row_segments(ws_, c.rseg, c.n, kA_.get(), vA_.get(), ws_.alt_keys.get(),
+                              ws_.alt_vals.get(), c.K, s, kt, SPTC_K_SORT_PROD);
     } else {
       ws_.ensure(c.n, c.rb, s);
       const int passes = radix_passes(c.bits, c.rb);

@@ -48,8 +48,7 @@ struct EscChunk {
   DevBuf<BPiece> pieces;
   DevBuf<BETile> tiles;
   bool rowsort = false;
-  uint64_t rnt = 0;
-  DevBuf<SegTile> rtiles;
+  RowSegPlan rseg;
   int out = -1;
 };
 

@@ -83,23 +83,12 @@ sptc_status make_pair_layouts(int xorder, const uint64_t* xdims, int yorder,
   return SPTC_OK;
 }
 
-bool prefer_row_sort(uint64_t n, int full_bits, int col_bits, uint64_t ntiles_rows,
-                     int radix_bits) {
-  static const int force = [] {
+bool prefer_row_sort() {
+  static const bool rows = [] {
     const char* e = getenv("SPTC_PROD_SORT");
-    if (!e) return 0;
-    if (std::string(e) == "full") return -1;
-    if (std::string(e) == "rows") return 1;
-    return 0;
+    return !(e && std::string(e) == "full");
   }();
-  if (force) return force > 0;
-  if (!n || col_bits <= 0) return false;
-  const int rb = radix_bits ? radix_bits : choose_radix_bits(full_bits);
-  const double fixed = 1024.0;
-  const double full = radix_passes(full_bits, rb) *
-                      ((double)n + fixed * (double)ceil_div(n, sort_tile_items(rb)));
-  const double rows = radix_passes(col_bits, kSegRB) * ((double)n + fixed * (double)ntiles_rows);
-  return rows < full;
+  return rows;
 }
 
 uint64_t count_products(const Operand& X, const Operand& Y, hipStream_t s) {
@@ -357,7 +346,8 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
     const uint64_t E = 1ull << (kyf < 8 ? 8 : kyf);
     // count-only runs stay on ESC, whose expansion also yields the value sum
     if (!dups)
-      use_dense = o_.dense_accumulator == 1 || (!o_.count_only && ir * 32 >= R * E);
+      use_dense = o_.dense_accumulator == 1 ||
+                  (!o_.count_only && ir * 32 >= R * E && ir >= kDenseMinRun * A);
   }
   if (o_.dense_accumulator == 1 && ir && !use_dense) {
     msg = "dense accumulator does not apply (column space above 2^13 or duplicate Y "
@@ -471,10 +461,9 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
     }
     ws_.ensure(chunk_cap, rb_max, s_);
     const uint64_t fmask = low_mask(kyf);
-    // row-segmented product sort: per-row tile counts, their scan, the tiles
-    DevBuf<uint64_t> tcnt(max_rows + 1), toff(max_rows + 1);
+    // row-segmented product sort
     DevBuf<RowRange> rrange(1);
-    DevBuf<SegTile> tiles;
+    RowSegPlan rseg;
     for (size_t c = 1; c < bounds_r.size(); ++c) {
       const uint64_t r0c = bounds_r[c - 1], r1c = bounds_r[c];
       const uint64_t P0 = bounds_p[c - 1], P1 = bounds_p[c];
@@ -510,7 +499,7 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       // product (key, value) written, matched Y (key, value) read
       kt_.end(tok, SPTC_K_EXPAND, 24.0 * (double)n, s_);
       bool sorted = false, alt = false;
-      if (kyf > 0 && kyf < bits) {
+      if (kyf > 0 && kyf < bits && prefer_row_sort()) {
         const uint64_t lrows = r1c - r0c;
         RowRange* h = reinterpret_cast<RowRange*>(host_u64_ + 4);
         h->ra = r0c;
@@ -518,22 +507,10 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
         h->lrow0 = 0;
         h->out0 = 0;
         SPTC_HIP(hipMemcpyAsync(rrange.get(), h, sizeof(RowRange), hipMemcpyHostToDevice, s_));
-        launch_row_tile_count(rrange.get(), 1, lrows, row_prod.get(), tcnt.get(), s_);
-        run_exclusive_scan(tcnt.get(), toff.get(), lrows, tmp.get(), s_);
-        SPTC_HIP(hipMemcpyAsync(host_u64_, toff.get() + lrows, 8, hipMemcpyDeviceToHost, s_));
-        SPTC_HIP(hipStreamSynchronize(s_));
-        const uint64_t nt = host_u64_[0];
-        const uint64_t need = (nt << 11) + nt * sizeof(SegTile);  // count matrices + tiles
-        const uint64_t live = DevicePool::get().live_bytes();
-        if (prefer_row_sort(n, bits, kyf, nt, rb_) && live + need <= budget_) {
-          if (tiles.size() < nt) tiles.reset(nt);
-          segmented_sort_reserve(ws_, nt, s_);
-          launch_row_tiles(rrange.get(), 1, lrows, row_prod.get(), tcnt.get(), toff.get(),
-                           tiles.get(), s_);
-          alt = segmented_sort(ws_, tiles.get(), nt, n, kA.get(), vA.get(), ws_.alt_keys.get(),
-                               ws_.alt_vals.get(), kyf, s_, &kt_, SPTC_K_SORT_PROD);
-          sorted = true;
-        }
+        build_row_segments(rrange.get(), 1, lrows, row_prod.get(), &rseg, ws_, tmp.get(), s_);
+        alt = sort_row_segments(ws_, rseg, n, kA.get(), vA.get(), ws_.alt_keys.get(),
+                                ws_.alt_vals.get(), kyf, s_, &kt_, SPTC_K_SORT_PROD);
+        sorted = true;
       }
       if (!sorted) {
         ws_.ensure(n, rb, s_);

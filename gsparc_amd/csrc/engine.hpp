@@ -95,14 +95,10 @@ struct StagedCOO {
 // bound, used for admission.  Synchronizes the stream.
 uint64_t count_products(const Operand& X, const Operand& Y, hipStream_t s);
 
-// Product-sort choice for a chunk of n products (key = local row << kyf |
-// column): one sort over all key bits, or a row-segmented sort over the
-// column bits only (the expansion leaves products grouped by row), whose
-// tiles stop at row ends (ntiles_rows of them).  Cost model: per pass, the
-// keys plus a fixed cost per tile of about 1024 keys (block start, digit
-// counts, LDS setup).  SPTC_PROD_SORT=full|rows forces one.
-bool prefer_row_sort(uint64_t n, int full_bits, int col_bits, uint64_t ntiles_rows,
-                     int radix_bits);
+// Product sorts are row-segmented (the expansion leaves products grouped by
+// X row, so only the column bits need sorting; see build_row_segments) unless
+// SPTC_PROD_SORT=full asks for one sort over all key bits.
+bool prefer_row_sort();
 
 class Engine {
  public:

@@ -40,7 +40,8 @@ __global__ __launch_bounds__(kSegNT) void k_seg_downsweep(
   const uint32_t tid = threadIdx.x;
   const SegTile t = tiles[blockIdx.x];
   for (int i = tid; i < (kSegNT / kWave) * R; i += kSegNT) (&sm.cnt[0][0])[i] = 0;
-  for (int d = tid; d < R; d += kSegNT) sm.gbase[d] = offs[t.cbase + (uint64_t)d * t.cstride];
+  for (int d = tid; d < R; d += kSegNT)
+    sm.gbase[d] = offs[t.cbase + (uint64_t)d * t.cstride] + t.gshift;
   __syncthreads();
   const uint64_t end = t.start + t.count;
   const uint64_t wbase = t.start + (uint64_t)(tid / kWave) * kWave * kSegIPT;
@@ -63,49 +64,167 @@ struct SegCountScanOp {
   __device__ void emit(uint64_t i, U2 excl, U2) const { out[i] = (uint32_t)excl.a; }
 };
 
-__global__ __launch_bounds__(kBlock) void k_row_tile_count(const RowRange* ranges,
-                                                           uint32_t nranges, uint64_t lrows,
-                                                           const uint64_t* row_prod,
-                                                           uint64_t* cnt) {
+// ---- row segments: small-row groups (LDS) and big rows (multi-tile) --------
+constexpr uint64_t kHalfTile = kSegTile / 2;
+
+__device__ __forceinline__ uint64_t rowseg_locate(const RowRange* ranges, uint32_t nranges,
+                                                  uint64_t l, const uint64_t* row_prod,
+                                                  uint64_t* first) {
+  uint32_t lo = 0, hi = nranges;  // last range with lrow0 <= l
+  while (lo + 1 < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (ranges[mid].lrow0 <= l) lo = mid; else hi = mid;
+  }
+  const RowRange rr = ranges[lo];
+  const uint64_t g = rr.ra + (l - rr.lrow0);
+  *first = rr.out0 + (row_prod[g] - row_prod[rr.ra]);
+  return row_prod[g + 1] - row_prod[g];
+}
+
+// Per row: small-row products (0 for big rows) and big-row tile counts.
+__global__ __launch_bounds__(kBlock) void k_rowseg_sizes(const RowRange* ranges, uint32_t nr,
+                                                         uint64_t lrows, const uint64_t* row_prod,
+                                                         uint64_t* small_n, uint64_t* big_t) {
   uint64_t l = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (; l < lrows; l += stride) {
-    uint32_t lo = 0, hi = nranges;  // last range with lrow0 <= l
-    while (lo + 1 < hi) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (ranges[mid].lrow0 <= l) lo = mid; else hi = mid;
-    }
-    const uint64_t g = ranges[lo].ra + (l - ranges[lo].lrow0);
-    cnt[l] = ceil_div(row_prod[g + 1] - row_prod[g], kSegTile);
+    uint64_t first;
+    const uint64_t n = rowseg_locate(ranges, nr, l, row_prod, &first);
+    const bool big = n > kHalfTile;
+    small_n[l] = big ? 0 : n;
+    big_t[l] = big ? ceil_div(n, kSegTile) : 0;
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_row_tiles(const RowRange* ranges, uint32_t nranges,
-                                                      uint64_t lrows, const uint64_t* row_prod,
-                                                      const uint64_t* cnt, const uint64_t* off,
-                                                      SegTile* tiles) {
+// Per row: 1 when a small row starts a group (after a big row, or where the
+// running small-row total enters a new half-tile window).
+__global__ __launch_bounds__(kBlock) void k_rowseg_heads(uint64_t lrows, const uint64_t* small_n,
+                                                         const uint64_t* big_t,
+                                                         const uint64_t* ps, uint64_t* head) {
+  uint64_t l = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (; l < lrows; l += stride) {
+    uint64_t h = 0;
+    if (!big_t[l])
+      h = l == 0 || big_t[l - 1] != 0 || ps[l] / kHalfTile != ps[l - 1] / kHalfTile;
+    head[l] = h;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_rowseg_emit(
+    const RowRange* ranges, uint32_t nr, uint64_t lrows, const uint64_t* row_prod,
+    const uint64_t* small_n, const uint64_t* big_t, const uint64_t* ps, const uint64_t* bt,
+    const uint64_t* head, const uint64_t* hs, LocalSeg* local, SegTile* big) {
   constexpr uint64_t R = 1ull << kSegRB;
   uint64_t l = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (; l < lrows; l += stride) {
-    uint32_t lo = 0, hi = nranges;
-    while (lo + 1 < hi) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (ranges[mid].lrow0 <= l) lo = mid; else hi = mid;
+    uint64_t first;
+    const uint64_t n = rowseg_locate(ranges, nr, l, row_prod, &first);
+    const uint64_t nt = big_t[l];
+    if (nt) {
+      const uint64_t o = bt[l];
+      for (uint64_t t = 0; t < nt; ++t) {
+        SegTile d;
+        d.start = first + t * kSegTile;
+        d.count = (uint32_t)(n - t * kSegTile < kSegTile ? n - t * kSegTile : kSegTile);
+        d.cbase = R * o + t;
+        d.cstride = nt;
+        d.gshift = (uint32_t)ps[l];  // small-row products before this row
+        big[o + t] = d;
+      }
+    } else {
+      // the group holding row l: its head is the last head at or before l
+      const uint64_t gidx = hs[l] + head[l] - 1;
+      if (head[l]) local[gidx].start = first;
+      if (n) atomicAdd(reinterpret_cast<unsigned int*>(&local[gidx].count), (unsigned int)n);
     }
-    const RowRange rr = ranges[lo];
-    const uint64_t g = rr.ra + (l - rr.lrow0);
-    const uint64_t first = rr.out0 + (row_prod[g] - row_prod[rr.ra]);
-    const uint64_t len = row_prod[g + 1] - row_prod[g];
-    const uint64_t nt = cnt[l], o = off[l];
-    for (uint64_t t = 0; t < nt; ++t) {
-      SegTile d;
-      d.start = first + t * kSegTile;
-      d.count = (uint32_t)(len - t * kSegTile < kSegTile ? len - t * kSegTile : kSegTile);
-      d.cbase = R * o + t;
-      d.cstride = nt;
-      d.pad = 0;
-      tiles[o + t] = d;
+  }
+}
+
+__global__ void k_zero_local(LocalSeg* local, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    local[i].count = 0;
+    local[i].pad = 0;
+  }
+}
+
+// One workgroup per small group: load, LSD passes over the bits that vary
+// inside the group entirely in LDS (stable), store.
+__global__ __launch_bounds__(kSegNT) void k_local_sort(const uint64_t* __restrict__ kin,
+                                                       const float* __restrict__ vin,
+                                                       uint64_t* __restrict__ kout,
+                                                       float* __restrict__ vout,
+                                                       const LocalSeg* __restrict__ segs) {
+  constexpr int R = 1 << kSegRB;
+  __shared__ TileSmem<kSegRB, kSegIPT, kSegNT> sm;
+  __shared__ float vbuf[kSegTile];
+  __shared__ uint64_t s_or[kSegNT / kWave];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = lane_id();
+  const uint32_t wave = tid / kWave;
+  const LocalSeg sg = segs[blockIdx.x];
+  const uint64_t n = sg.count;
+  const uint64_t wbase = (uint64_t)wave * kWave * kSegIPT;  // tile-local
+  uint64_t k[kSegIPT];
+  float v[kSegIPT];
+  uint32_t pos[kSegIPT];
+  const uint64_t k0 = n ? kin[sg.start] : 0;
+  uint64_t vary = 0;
+#pragma unroll
+  for (int i = 0; i < kSegIPT; ++i) {
+    const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+    const bool valid = idx < n;
+    k[i] = valid ? kin[sg.start + idx] : 0;
+    v[i] = valid ? vin[sg.start + idx] : 0.f;
+    if (valid) vary |= k[i] ^ k0;
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1) vary |= __shfl_xor(vary, off, kWave);
+  if (lane == 0) s_or[wave] = vary;
+  __syncthreads();
+  vary = 0;
+#pragma unroll
+  for (int w = 0; w < kSegNT / kWave; ++w) vary |= s_or[w];
+  const int passes = vary ? (64 - __builtin_clzll(vary) + kSegRB - 1) / kSegRB : 0;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = p * kSegRB;
+    for (int i = tid; i < (kSegNT / kWave) * R; i += kSegNT) (&sm.cnt[0][0])[i] = 0;
+    __syncthreads();
+    tile_rank<kSegRB, kSegIPT, kSegNT>(sm, n, shift, wbase, k, pos);
+    __syncthreads();
+    for (int d = tid; d < R; d += kSegNT)
+      sm.dstart[d] = (uint16_t)wave_prefixes<kSegRB, kSegIPT, kSegNT>(sm, d);
+    __syncthreads();
+    tile_digit_starts<kSegRB, kSegIPT, kSegNT>(sm);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kSegIPT; ++i) {
+      const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+      if (idx < n) {
+        const uint32_t d = (uint32_t)(k[i] >> shift) & (R - 1);
+        const uint32_t q = pos[i] + (uint32_t)sm.dstart[d] + (uint32_t)sm.cnt[wave][d];
+        sm.buf[q] = k[i];
+        vbuf[q] = v[i];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kSegIPT; ++i) {
+      const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+      if (idx < n) {
+        k[i] = sm.buf[idx];
+        v[i] = vbuf[idx];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < kSegIPT; ++i) {
+    const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+    if (idx < n) {
+      kout[sg.start + idx] = k[i];
+      vout[sg.start + idx] = v[i];
     }
   }
 }
@@ -125,7 +244,7 @@ void SegSortPlan::build(const std::vector<uint64_t>& seg_off, hipStream_t s) {
       d.count = (uint32_t)std::min<uint64_t>(kSegTile, len - t * kSegTile);
       d.cbase = R * tiles_before + t;
       d.cstride = nt;
-      d.pad = 0;
+      d.gshift = 0;
       tv.push_back(d);
     }
     tiles_before += nt;
@@ -192,21 +311,62 @@ bool segmented_sort(SortWorkspace& ws, const SegSortPlan& plan, uint64_t* k0, fl
                         cls);
 }
 
-void launch_row_tile_count(const RowRange* ranges, uint32_t nranges, uint64_t lrows,
-                           const uint64_t* row_prod, uint64_t* cnt, hipStream_t s) {
+void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows,
+                        const uint64_t* row_prod, RowSegPlan* plan, SortWorkspace& ws, U2* tmp,
+                        hipStream_t s) {
+  plan->nlocal = plan->nbig_tiles = 0;
   if (!lrows) return;
+  const uint64_t m = lrows + 1;
+  if (plan->scratch.size() < 6 * m) plan->scratch.reset(6 * m);
+  uint64_t* small_n = plan->scratch.get();
+  uint64_t* big_t = small_n + m;
+  uint64_t* ps = big_t + m;
+  uint64_t* bt = ps + m;
+  uint64_t* head = bt + m;
+  uint64_t* hs = head + m;
   const uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(lrows, kBlock), 4096);
-  hipLaunchKernelGGL(k_row_tile_count, dim3(grid), dim3(kBlock), 0, s, ranges, nranges, lrows,
-                     row_prod, cnt);
+  hipLaunchKernelGGL(k_rowseg_sizes, dim3(grid), dim3(kBlock), 0, s, ranges, nranges, lrows,
+                     row_prod, small_n, big_t);
+  run_exclusive_scan(small_n, ps, lrows, tmp, s);
+  run_exclusive_scan(big_t, bt, lrows, tmp, s);
+  hipLaunchKernelGGL(k_rowseg_heads, dim3(grid), dim3(kBlock), 0, s, lrows, small_n, big_t, ps,
+                     head);
+  run_exclusive_scan(head, hs, lrows, tmp, s);
+  uint64_t tot[3];
+  SPTC_HIP(hipMemcpyAsync(&tot[0], hs + lrows, 8, hipMemcpyDeviceToHost, s));
+  SPTC_HIP(hipMemcpyAsync(&tot[1], bt + lrows, 8, hipMemcpyDeviceToHost, s));
+  SPTC_HIP(hipMemcpyAsync(&tot[2], ps + lrows, 8, hipMemcpyDeviceToHost, s));
+  SPTC_HIP(hipStreamSynchronize(s));
+  plan->nlocal = tot[0];
+  plan->nbig_tiles = tot[1];
+  plan->nsmall = tot[2];
+  if (plan->local.size() < plan->nlocal) plan->local.reset(plan->nlocal);
+  if (plan->big.size() < plan->nbig_tiles) plan->big.reset(plan->nbig_tiles);
+  if (plan->nlocal)
+    hipLaunchKernelGGL(k_zero_local, dim3((uint32_t)ceil_div(plan->nlocal, 256)), dim3(256), 0,
+                       s, plan->local.get(), plan->nlocal);
+  hipLaunchKernelGGL(k_rowseg_emit, dim3(grid), dim3(kBlock), 0, s, ranges, nranges, lrows,
+                     row_prod, small_n, big_t, ps, bt, head, hs, plan->local.get(),
+                     plan->big.get());
+  if (plan->nbig_tiles) segmented_sort_reserve(ws, plan->nbig_tiles, s);
+  if (plan->nlocal >= (1ull << 31) || plan->nbig_tiles >= (1ull << 31))
+    throw std::length_error("row segments");
 }
 
-void launch_row_tiles(const RowRange* ranges, uint32_t nranges, uint64_t lrows,
-                      const uint64_t* row_prod, const uint64_t* cnt, const uint64_t* off,
-                      SegTile* tiles, hipStream_t s) {
-  if (!lrows) return;
-  const uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(lrows, kBlock), 4096);
-  hipLaunchKernelGGL(k_row_tiles, dim3(grid), dim3(kBlock), 0, s, ranges, nranges, lrows,
-                     row_prod, cnt, off, tiles);
+bool sort_row_segments(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, uint64_t* k0,
+                       float* v0, uint64_t* k1, float* v1, int col_bits, hipStream_t s,
+                       KTimer* kt, int cls) {
+  bool alt = false;
+  if (plan.nbig_tiles)
+    alt = segmented_sort(ws, plan.big.get(), plan.nbig_tiles, n - plan.nsmall, k0, v0, k1, v1,
+                         col_bits, s, kt, cls);
+  if (plan.nlocal) {
+    int tok = kt ? kt->begin(s) : -1;
+    hipLaunchKernelGGL(k_local_sort, dim3((uint32_t)plan.nlocal), dim3(kSegNT), 0, s, k0, v0,
+                       alt ? k1 : k0, alt ? v1 : v0, plan.local.get());
+    if (kt) kt->end(tok, SPTC_K_SORT_LOCAL, 24.0 * (double)plan.nsmall, s);
+  }
+  return alt;
 }
 
 }  // namespace sptc

@@ -15,12 +15,21 @@ constexpr int kSegIPT = 12;   // keys per thread
 constexpr int kSegNT = 512;   // threads per tile
 constexpr uint64_t kSegTile = (uint64_t)kSegIPT * kSegNT;
 
-// One tile of a segmented sort.
+// One tile of a segmented sort.  The flat scan of the count matrix places
+// the sorted segments back to back from 0; gshift moves a segment to where
+// it really starts (non-zero when segments sorted elsewhere sit between).
 struct SegTile {
   uint64_t start;    // first key (global index)
   uint64_t cbase;    // count-matrix index of digit 0 of this tile
   uint64_t cstride;  // distance between digits (tiles in the segment)
   uint32_t count;    // keys in the tile
+  uint32_t gshift;   // added to the scanned destination offsets
+};
+
+// A segment small enough to be sorted inside one workgroup's LDS.
+struct LocalSeg {
+  uint64_t start;
+  uint32_t count;  // <= kSegTile
   uint32_t pad;
 };
 
@@ -71,12 +80,29 @@ struct RowRange {
   uint64_t lrow0;  // chunk-local row of ra (prefix of rows)
   uint64_t out0;   // chunk offset of ra's first product
 };
-// Tile counts per chunk-local row (lrows rows): cnt[l] = ceil(products/tile).
-void launch_row_tile_count(const RowRange* ranges, uint32_t nranges, uint64_t lrows,
-                           const uint64_t* row_prod, uint64_t* cnt, hipStream_t s);
-// tiles[off[l] + t] for every row l (off = exclusive scan of cnt).
-void launch_row_tiles(const RowRange* ranges, uint32_t nranges, uint64_t lrows,
-                      const uint64_t* row_prod, const uint64_t* cnt, const uint64_t* off,
-                      SegTile* tiles, hipStream_t s);
+// Row-segmented product sort of a chunk (keys local_row << col_bits | col,
+// grouped by row).  Rows holding more than half a tile of products are
+// segments of their own, sorted on the column bits by the multi-tile passes
+// above; runs of smaller rows are cut into groups of at most one tile
+// (a group starts where the running total of small-row products crosses a
+// multiple of half a tile), each sorted on its varying key bits entirely in
+// one workgroup's LDS.  Every product is read and written once for the
+// groups, ceil(col_bits / 8) times for the big rows.
+struct RowSegPlan {
+  uint64_t nlocal = 0, nbig_tiles = 0, nsmall = 0;  // nsmall: products in groups
+  DevBuf<LocalSeg> local;
+  DevBuf<SegTile> big;
+  DevBuf<uint64_t> scratch;  // 6 x (rows + 1) per-row values and scans
+};
+// Builds the plan (one host synchronization for the counts); tmp holds
+// scan_tmp_count(lrows) + 1 entries.  Reserves ws for the big-row passes.
+void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows,
+                        const uint64_t* row_prod, RowSegPlan* plan, SortWorkspace& ws, U2* tmp,
+                        hipStream_t s);
+// Sorts with a plan; returns true when the result ended in (k1, v1).
+// Launches only (replayable).
+bool sort_row_segments(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, uint64_t* k0,
+                       float* v0, uint64_t* k1, float* v1, int col_bits, hipStream_t s,
+                       KTimer* kt = nullptr, int cls = SPTC_K_SORT_PROD);
 
 }  // namespace sptc

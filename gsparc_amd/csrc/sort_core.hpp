@@ -40,25 +40,17 @@ struct TileSmem {
   uint32_t tile;
 };
 
-// Loads the tile (wave w owns keys [w*64*IPT, (w+1)*64*IPT) of it, lane-
-// striped) and ranks every key within its wave's run of equal digits.
-// Afterwards sm.cnt[w][d] = number of keys of digit d in wave w.
+// Ranks the keys a thread holds (wave w owns keys [w*64*IPT, (w+1)*64*IPT)
+// of the tile, lane-striped: key i of a lane is wbase + i*64 + lane; valid
+// when below n) within its wave's run of equal digits.  sm.cnt must be zero;
+// afterwards sm.cnt[w][d] = number of keys of digit d in wave w.
 template <int RB, int IPT, int NT>
-__device__ __forceinline__ void tile_load_rank(TileSmem<RB, IPT, NT>& sm,
-                                               const uint64_t* __restrict__ kin,
-                                               const float* __restrict__ vin, uint64_t n,
-                                               int shift, uint64_t wbase, uint64_t (&k)[IPT],
-                                               float (&v)[IPT], uint32_t (&pos)[IPT]) {
+__device__ __forceinline__ void tile_rank(TileSmem<RB, IPT, NT>& sm, uint64_t n, int shift,
+                                          uint64_t wbase, const uint64_t (&k)[IPT],
+                                          uint32_t (&pos)[IPT]) {
   constexpr int R = 1 << RB;
   const uint32_t lane = lane_id();
   const uint32_t wave = threadIdx.x / kWave;
-#pragma unroll
-  for (int i = 0; i < IPT; ++i) {
-    const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
-    const bool valid = idx < n;
-    k[i] = valid ? kin[idx] : 0;
-    v[i] = valid ? vin[idx] : 0.f;
-  }
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {
     const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
@@ -81,6 +73,47 @@ __device__ __forceinline__ void tile_load_rank(TileSmem<RB, IPT, NT>& sm,
   }
 }
 
+// Loads the tile from global memory (same layout as tile_rank) and ranks it.
+template <int RB, int IPT, int NT>
+__device__ __forceinline__ void tile_load_rank(TileSmem<RB, IPT, NT>& sm,
+                                               const uint64_t* __restrict__ kin,
+                                               const float* __restrict__ vin, uint64_t n,
+                                               int shift, uint64_t wbase, uint64_t (&k)[IPT],
+                                               float (&v)[IPT], uint32_t (&pos)[IPT]) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+    const bool valid = idx < n;
+    k[i] = valid ? kin[idx] : 0;
+    v[i] = valid ? vin[idx] : 0.f;
+  }
+  tile_rank<RB, IPT, NT>(sm, n, shift, wbase, k, pos);
+}
+
+// sm.dstart holds each digit's tile total: turn it into the tile-local start
+// of each digit's run (block-wide exclusive scan).
+template <int RB, int IPT, int NT>
+__device__ __forceinline__ void tile_digit_starts(TileSmem<RB, IPT, NT>& sm) {
+  constexpr int R = 1 << RB;
+  constexpr int PER = (R + NT - 1) / NT;
+  const uint32_t tid = threadIdx.x;
+  uint32_t loc[PER];
+  uint32_t sum = 0;
+  for (int j = 0; j < PER; ++j) {
+    const int d = tid * PER + j;
+    loc[j] = d < R ? sm.dstart[d] : 0;
+    sum += loc[j];
+  }
+  uint32_t tot;
+  uint32_t ex = block_scan_u32<NT>(sum, sm.scan, &tot);
+  for (int j = 0; j < PER; ++j) {
+    const int d = tid * PER + j;
+    if (d < R) sm.dstart[d] = (uint16_t)ex;
+    ex += loc[j];
+  }
+}
+
 // Given sm.gbase and sm.dstart (tile totals per digit on entry), scans the
 // totals into tile-local run starts, stages keys in digit order, writes them
 // in coalesced runs, then does the same for the values.
@@ -93,26 +126,10 @@ __device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT>& sm, ui
                                                    float* __restrict__ vout) {
   constexpr int R = 1 << RB;
   constexpr int TILE = NT * IPT;
-  constexpr int PER = (R + NT - 1) / NT;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = lane_id();
   const uint32_t wave = tid / kWave;
-  {
-    uint32_t loc[PER];
-    uint32_t sum = 0;
-    for (int j = 0; j < PER; ++j) {
-      const int d = tid * PER + j;
-      loc[j] = d < R ? sm.dstart[d] : 0;
-      sum += loc[j];
-    }
-    uint32_t tot;
-    uint32_t ex = block_scan_u32<NT>(sum, sm.scan, &tot);
-    for (int j = 0; j < PER; ++j) {
-      const int d = tid * PER + j;
-      if (d < R) sm.dstart[d] = (uint16_t)ex;
-      ex += loc[j];
-    }
-  }
+  tile_digit_starts<RB, IPT, NT>(sm);
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {

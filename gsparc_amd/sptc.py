@@ -43,7 +43,7 @@ class _Opts(Structure):
 
 
 KERNEL_CLASSES = ["linearize", "sort_input", "match", "expand", "sort_products", "reduce",
-                  "dense", "sort_count", "sort_products_wide"]
+                  "dense", "sort_count", "sort_products_wide", "sort_local"]
 
 
 class _Info(Structure):

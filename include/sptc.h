@@ -156,7 +156,8 @@ typedef enum {
   SPTC_K_SORT_COUNT = 7,  /* radix pass digit counting + offset scan      */
   SPTC_K_SORT_PROD_WIDE = 8, /* radix pass (scatter) over product keys,
                                 digits wider than 8 bits                 */
-  SPTC_K_COUNT = 9
+  SPTC_K_SORT_LOCAL = 9,  /* product groups of small rows sorted in LDS     */
+  SPTC_K_COUNT = 10
 } sptc_kernel_class;
 
 /* Launch count, summed HIP-event time (ms) and algorithmic HBM bytes of one

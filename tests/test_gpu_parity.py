@@ -327,3 +327,22 @@ def test_duplicate_coordinates_sum_like_the_oracle(gpu):
     _compare_with_oracle(dup, dup, [2], [2], res)
     with pytest.raises(sptc.SpTCError):
         sptc.SpTC(dup, dup, [2], [2], dense_accumulator=1)
+
+
+def test_row_segments_big_and_small_rows(gpu):
+    """Zipf rows: a few X rows with far more than half a sort tile of
+    products (multi-tile column-bit passes) among many small ones (grouped
+    and sorted in LDS), signed values, against the oracle."""
+    from gsparc_amd.coo import UNIFORM, ZIPF, VAL_UNIFORM01, generate
+    X = generate(0x5EED00AA, (300, 200, 48), (ZIPF, ZIPF, UNIFORM), (1.3, 1.3, 1.0), 40_000,
+                 VAL_UNIFORM01)
+    rng = np.random.default_rng(12)
+    X = COOTensor(X.dims, X.idx, X.val * np.where(rng.random(X.nnz) < 0.5, -1, 1).astype(
+        np.float32))
+    res = sptc.SpTC(X, X, [2], [2], dense_accumulator=-1)
+    prod_per_c = np.bincount(X.idx[2].astype(np.int64), minlength=48)
+    rk = X.idx[0].astype(np.int64) * 200 + X.idx[1]
+    per_row = np.bincount(np.unique(rk, return_inverse=True)[1],
+                          weights=prod_per_c[X.idx[2].astype(np.int64)])
+    assert per_row.max() > 3 * 6144 and np.median(per_row) < 3072  # both kinds present
+    _compare_with_oracle(X, X, [2], [2], res)
