@@ -1,5 +1,6 @@
 // Batched (multi-contraction) kernels for gfx950; see batch.hpp.
 #include "batch.hpp"
+#include "dense_core.hpp"
 
 #include <algorithm>
 #include <stdexcept>
@@ -365,98 +366,31 @@ __device__ __forceinline__ void bdense_row(const BDenseArgs& a, uint64_t b, uint
 
 template <int EB>
 __global__ __launch_bounds__(kBlock) void k_bdense_count(BDenseArgs a) {
-  constexpr int E = 1 << EB;
-  constexpr int W = E / 32;
-  __shared__ uint32_t bm[W];
-  __shared__ uint32_t s_red[kBlock / kWave];
+  __shared__ DenseSmem<EB> sm;
+  __shared__ uint64_t s_ja;
   if (*a.bad) return;
   uint64_t r;
   BInst in;
   bdense_row(a, blockIdx.x, r, in);
-  const uint64_t cm = low_mask(in.kyf);
-  for (int i = threadIdx.x; i < W; i += kBlock) bm[i] = 0;
-  __syncthreads();
-  const uint64_t j0 = a.d.row_act[r], j1 = a.d.row_act[r + 1];
-  for (uint64_t j = j0; j < j1; ++j) {
-    const uint64_t y0 = a.d.act_ypos[j];
-    const uint64_t c = a.d.act_prefix[j + 1] - a.d.act_prefix[j];
-    for (uint64_t q = threadIdx.x; q < c; q += kBlock) {
-      const uint32_t rank = (uint32_t)(a.d.ykeys[y0 + q] & cm);
-      atomicOr(&bm[rank >> 5], 1u << (rank & 31));
-    }
-  }
-  __syncthreads();
-  uint32_t cnt = 0;
-  for (int i = threadIdx.x; i < W; i += kBlock) cnt += __popc(bm[i]);
-  for (int off = kWave / 2; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, kWave);
-  if (lane_id() == 0) s_red[threadIdx.x / kWave] = cnt;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t t = 0;
-    for (int w = 0; w < kBlock / kWave; ++w) t += s_red[w];
-    a.d.row_cnt[blockIdx.x] = t;
-  }
+  const uint32_t t = dense_row_count<EB>(a.d, a.d.row_act[r], a.d.row_act[r + 1],
+                                         low_mask(in.kyf), sm, &s_ja);
+  if (threadIdx.x == 0) a.d.row_cnt[blockIdx.x] = t;
 }
 
 template <int EB>
 __global__ __launch_bounds__(kBlock) void k_bdense_numeric(BDenseArgs a) {
   constexpr int E = 1 << EB;
-  constexpr int W = E / 32;
-  constexpr int S = E / kBlock;
   __shared__ float s_sum[E];
   __shared__ float s_cmp[E];
-  __shared__ uint32_t bm[W];
-  __shared__ uint32_t s_scan[kBlock / kWave];
+  __shared__ DenseSmem<EB> sm;
+  __shared__ uint64_t s_ja;
   if (*a.bad) return;
   uint64_t r;
   BInst in;
   bdense_row(a, blockIdx.x, r, in);
-  const uint64_t cm = low_mask(in.kyf);
-  for (int i = threadIdx.x; i < E; i += kBlock) {
-    s_sum[i] = 0.f;
-    s_cmp[i] = 0.f;
-  }
-  for (int i = threadIdx.x; i < W; i += kBlock) bm[i] = 0;
-  __syncthreads();
-  const uint64_t j0 = a.d.row_act[r], j1 = a.d.row_act[r + 1];
-  double vsum = 0.0;
-  for (uint64_t j = j0; j < j1; ++j) {
-    const uint64_t y0 = a.d.act_ypos[j];
-    const uint64_t c = a.d.act_prefix[j + 1] - a.d.act_prefix[j];
-    const float xv = a.d.act_xval[j];
-    for (uint64_t q = threadIdx.x; q < c; q += kBlock) {
-      const uint64_t y = y0 + q;
-      const uint32_t rank = (uint32_t)(a.d.ykeys[y] & cm);
-      const float v = xv * a.d.yvals[y];
-      vsum += (double)v;
-      const float sm = s_sum[rank];
-      const float t = sm + v;
-      if (fabsf(sm) >= fabsf(v)) s_cmp[rank] += (sm - t) + v;
-      else s_cmp[rank] += (v - t) + sm;
-      s_sum[rank] = t;
-      atomicOr(&bm[rank >> 5], 1u << (rank & 31));
-    }
-    __syncthreads();
-  }
-  const uint32_t first = threadIdx.x * S;
-  const uint32_t word = bm[first >> 5];
-  uint32_t bits;
-  if constexpr (S == 32) bits = word;
-  else bits = (word >> (first & 31)) & ((1u << S) - 1);
-  uint32_t tot;
-  uint32_t o = block_excl_scan<uint32_t>((uint32_t)__popc(bits), s_scan, &tot);
-  const uint64_t obase = a.d.out_off[blockIdx.x];
-  for (int e = 0; e < S; ++e) {
-    if ((bits >> e) & 1u) {
-      const uint32_t rank = first + e;
-      a.d.out_row[obase + o] = (uint32_t)r;
-      a.d.out_col[obase + o] = a.ytab[in.yt_off + rank];
-      a.d.out_val[obase + o] = s_sum[rank] + s_cmp[rank];
-      ++o;
-    }
-  }
-  for (int off = kWave / 2; off > 0; off >>= 1) vsum += __shfl_xor(vsum, off, kWave);
-  if (lane_id() == 0 && vsum != 0.0) atomicAdd(a.d.vsum, vsum);
+  dense_row_numeric<EB>(a.d, a.d.row_act[r], a.d.row_act[r + 1], low_mask(in.kyf),
+                        a.ytab + in.yt_off, (uint32_t)r, a.d.out_off[blockIdx.x], s_sum, s_cmp,
+                        sm, &s_ja);
 }
 
 __global__ void k_row_bounds(const uint32_t* rows, uint64_t n, const uint64_t* q, uint32_t nq,

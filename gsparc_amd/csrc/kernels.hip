@@ -1,5 +1,6 @@
 // SpTC phase kernels for gfx950: linearize, index match, ESC expand and
 // reduce.  Host launchers are declared in kernels.hpp.
+#include "dense_core.hpp"
 #include "kernels.hpp"
 
 namespace sptc {
@@ -597,104 +598,26 @@ void run_reduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32
 // numeric kernel writes its outputs, in column order, straight to their
 // final place.
 
-template <int EB>
-__device__ __forceinline__ void dense_row_range(const DenseArgs& a, uint64_t r, uint64_t& j0,
-                                                uint64_t& j1) {
-  j0 = a.row_act[r];
-  j1 = a.row_act[r + 1];
-}
 
 template <int EB>
 __global__ __launch_bounds__(kBlock) void k_dense_count(DenseArgs a) {
-  constexpr int E = 1 << EB;
-  constexpr int W = E / 32;
-  __shared__ uint32_t bm[W];
-  __shared__ uint32_t s_red[kBlock / kWave];
+  __shared__ DenseSmem<EB> sm;
+  __shared__ uint64_t s_ja;
   const uint64_t r = a.r0 + blockIdx.x;
-  for (int i = threadIdx.x; i < W; i += kBlock) bm[i] = 0;
-  __syncthreads();
-  uint64_t j0, j1;
-  dense_row_range<EB>(a, r, j0, j1);
-  for (uint64_t j = j0; j < j1; ++j) {
-    const uint64_t y0 = a.act_ypos[j];
-    const uint64_t c = a.act_prefix[j + 1] - a.act_prefix[j];
-    for (uint64_t q = threadIdx.x; q < c; q += kBlock) {
-      const uint32_t rank = (uint32_t)(a.ykeys[y0 + q] & a.fmask);
-      atomicOr(&bm[rank >> 5], 1u << (rank & 31));
-    }
-  }
-  __syncthreads();
-  uint32_t cnt = 0;
-  for (int i = threadIdx.x; i < W; i += kBlock) cnt += __popc(bm[i]);
-  for (int off = kWave / 2; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, kWave);
-  if (lane_id() == 0) s_red[threadIdx.x / kWave] = cnt;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t t = 0;
-    for (int w = 0; w < kBlock / kWave; ++w) t += s_red[w];
-    a.row_cnt[blockIdx.x] = t;
-  }
+  const uint32_t t = dense_row_count<EB>(a, a.row_act[r], a.row_act[r + 1], a.fmask, sm, &s_ja);
+  if (threadIdx.x == 0) a.row_cnt[blockIdx.x] = t;
 }
 
 template <int EB>
 __global__ __launch_bounds__(kBlock) void k_dense_numeric(DenseArgs a) {
   constexpr int E = 1 << EB;
-  constexpr int W = E / 32;
-  constexpr int S = E / kBlock;  // slots per thread in the compaction (1..32)
-  static_assert(E >= kBlock && E % kBlock == 0, "dense row width");
   __shared__ float s_sum[E];
   __shared__ float s_cmp[E];
-  __shared__ uint32_t bm[W];
-  __shared__ uint32_t s_scan[kBlock / kWave];
+  __shared__ DenseSmem<EB> sm;
+  __shared__ uint64_t s_ja;
   const uint64_t r = a.r0 + blockIdx.x;
-  for (int i = threadIdx.x; i < E; i += kBlock) {
-    s_sum[i] = 0.f;
-    s_cmp[i] = 0.f;
-  }
-  for (int i = threadIdx.x; i < W; i += kBlock) bm[i] = 0;
-  __syncthreads();
-  uint64_t j0, j1;
-  dense_row_range<EB>(a, r, j0, j1);
-  double vsum = 0.0;
-  for (uint64_t j = j0; j < j1; ++j) {
-    const uint64_t y0 = a.act_ypos[j];
-    const uint64_t c = a.act_prefix[j + 1] - a.act_prefix[j];
-    const float xv = a.act_xval[j];
-    for (uint64_t q = threadIdx.x; q < c; q += kBlock) {
-      const uint64_t y = y0 + q;
-      const uint32_t rank = (uint32_t)(a.ykeys[y] & a.fmask);
-      const float v = xv * a.yvals[y];
-      vsum += (double)v;
-      // Neumaier step on the slot (one writer per slot within a segment)
-      const float sm = s_sum[rank];
-      const float t = sm + v;
-      if (fabsf(sm) >= fabsf(v)) s_cmp[rank] += (sm - t) + v;
-      else s_cmp[rank] += (v - t) + sm;
-      s_sum[rank] = t;
-      atomicOr(&bm[rank >> 5], 1u << (rank & 31));
-    }
-    __syncthreads();
-  }
-  // Compaction in column order: thread t owns slots [t*S, t*S+S).
-  const uint32_t first = threadIdx.x * S;
-  const uint32_t word = bm[first >> 5];
-  uint32_t bits;
-  if constexpr (S == 32) bits = word;
-  else bits = (word >> (first & 31)) & ((1u << S) - 1);
-  uint32_t tot;
-  uint32_t o = block_excl_scan<uint32_t>((uint32_t)__popc(bits), s_scan, &tot);
-  const uint64_t obase = a.out_off[blockIdx.x];
-  for (int e = 0; e < S; ++e) {
-    if ((bits >> e) & 1u) {
-      const uint32_t rank = first + e;
-      a.out_row[obase + o] = (uint32_t)r;
-      a.out_col[obase + o] = a.ytab ? a.ytab[rank] : rank;
-      a.out_val[obase + o] = s_sum[rank] + s_cmp[rank];
-      ++o;
-    }
-  }
-  for (int off = kWave / 2; off > 0; off >>= 1) vsum += __shfl_xor(vsum, off, kWave);
-  if (lane_id() == 0 && vsum != 0.0) atomicAdd(a.vsum, vsum);
+  dense_row_numeric<EB>(a, a.row_act[r], a.row_act[r + 1], a.fmask, a.ytab, (uint32_t)r,
+                        a.out_off[blockIdx.x], s_sum, s_cmp, sm, &s_ja);
 }
 
 void launch_dense_count(const DenseArgs& a, int eb, uint64_t rows, hipStream_t s) {
