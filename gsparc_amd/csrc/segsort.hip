@@ -28,7 +28,7 @@ __global__ __launch_bounds__(kSegNT) void k_seg_upsweep(const uint64_t* __restri
     hist_add_wave<kSegRB>(sh, k >> shift, valid, 1);
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < R; d += kSegNT) cnt[t.cbase + (uint64_t)d * t.cstride] = sh[d];
+  for (int d = threadIdx.x; d < R; d += kSegNT) cnt[(uint64_t)blockIdx.x * R + d] = sh[d];
 }
 
 __global__ __launch_bounds__(kSegNT) void k_seg_downsweep(
@@ -40,8 +40,7 @@ __global__ __launch_bounds__(kSegNT) void k_seg_downsweep(
   const uint32_t tid = threadIdx.x;
   const SegTile t = tiles[blockIdx.x];
   for (int i = tid; i < (kSegNT / kWave) * R; i += kSegNT) (&sm.cnt[0][0])[i] = 0;
-  for (int d = tid; d < R; d += kSegNT)
-    sm.gbase[d] = offs[t.cbase + (uint64_t)d * t.cstride] + t.gshift;
+  for (int d = tid; d < R; d += kSegNT) sm.gbase[d] = offs[(uint64_t)blockIdx.x * R + d];
   __syncthreads();
   const uint64_t end = t.start + t.count;
   const uint64_t wbase = t.start + (uint64_t)(tid / kWave) * kWave * kSegIPT;
@@ -57,12 +56,25 @@ __global__ __launch_bounds__(kSegNT) void k_seg_downsweep(
                                               vout);
 }
 
-struct SegCountScanOp {
-  const uint32_t* in;
-  uint32_t* out;
-  __device__ U2 value(uint64_t i) const { return mk2(in[i], 0); }
-  __device__ void emit(uint64_t i, U2 excl, U2) const { out[i] = (uint32_t)excl.a; }
-};
+// Per segment (one workgroup, thread d = digit d): absolute destination of
+// each tile's run of each digit.
+__global__ __launch_bounds__(1 << kSegRB) void k_seg_scan(const SegDesc* __restrict__ segs,
+                                                         const uint32_t* __restrict__ cnt,
+                                                         uint32_t* __restrict__ off) {
+  constexpr int R = 1 << kSegRB;
+  __shared__ uint32_t scratch[R / kWave];
+  const SegDesc g = segs[blockIdx.x];
+  const uint32_t d = threadIdx.x;
+  const uint64_t t0 = g.first_tile, t1 = t0 + g.ntiles;
+  uint32_t tot = 0;
+  for (uint64_t t = t0; t < t1; ++t) tot += cnt[t * R + d];
+  uint32_t all;
+  uint32_t base = (uint32_t)g.start + block_excl_scan<uint32_t>(tot, scratch, &all);
+  for (uint64_t t = t0; t < t1; ++t) {
+    off[t * R + d] = base;
+    base += cnt[t * R + d];
+  }
+}
 
 // ---- row segments: small-row groups (LDS) and big rows (multi-tile) --------
 constexpr uint64_t kHalfTile = kSegTile / 2;
@@ -84,7 +96,8 @@ __device__ __forceinline__ uint64_t rowseg_locate(const RowRange* ranges, uint32
 // Per row: small-row products (0 for big rows) and big-row tile counts.
 __global__ __launch_bounds__(kBlock) void k_rowseg_sizes(const RowRange* ranges, uint32_t nr,
                                                          uint64_t lrows, const uint64_t* row_prod,
-                                                         uint64_t* small_n, uint64_t* big_t) {
+                                                         uint64_t* small_n, uint64_t* big_t,
+                                                         uint64_t* big_f) {
   uint64_t l = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (; l < lrows; l += stride) {
@@ -93,6 +106,7 @@ __global__ __launch_bounds__(kBlock) void k_rowseg_sizes(const RowRange* ranges,
     const bool big = n > kHalfTile;
     small_n[l] = big ? 0 : n;
     big_t[l] = big ? ceil_div(n, kSegTile) : 0;
+    big_f[l] = big ? 1 : 0;
   }
 }
 
@@ -113,9 +127,8 @@ __global__ __launch_bounds__(kBlock) void k_rowseg_heads(uint64_t lrows, const u
 
 __global__ __launch_bounds__(kBlock) void k_rowseg_emit(
     const RowRange* ranges, uint32_t nr, uint64_t lrows, const uint64_t* row_prod,
-    const uint64_t* small_n, const uint64_t* big_t, const uint64_t* ps, const uint64_t* bt,
-    const uint64_t* head, const uint64_t* hs, LocalSeg* local, SegTile* big) {
-  constexpr uint64_t R = 1ull << kSegRB;
+    const uint64_t* big_t, const uint64_t* bt, const uint64_t* bs, const uint64_t* head,
+    const uint64_t* hs, LocalSeg* local, SegTile* big, SegDesc* bigseg) {
   uint64_t l = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (; l < lrows; l += stride) {
@@ -124,13 +137,16 @@ __global__ __launch_bounds__(kBlock) void k_rowseg_emit(
     const uint64_t nt = big_t[l];
     if (nt) {
       const uint64_t o = bt[l];
+      SegDesc g;
+      g.start = first;
+      g.first_tile = (uint32_t)o;
+      g.ntiles = (uint32_t)nt;
+      bigseg[bs[l]] = g;
       for (uint64_t t = 0; t < nt; ++t) {
         SegTile d;
         d.start = first + t * kSegTile;
         d.count = (uint32_t)(n - t * kSegTile < kSegTile ? n - t * kSegTile : kSegTile);
-        d.cbase = R * o + t;
-        d.cstride = nt;
-        d.gshift = (uint32_t)ps[l];  // small-row products before this row
+        d.pad = 0;
         big[o + t] = d;
       }
     } else {
@@ -232,31 +248,34 @@ __global__ __launch_bounds__(kSegNT) void k_local_sort(const uint64_t* __restric
 }  // namespace
 
 void SegSortPlan::build(const std::vector<uint64_t>& seg_off, hipStream_t s) {
-  constexpr uint64_t R = 1ull << kSegRB;
   std::vector<SegTile> tv;
-  uint64_t tiles_before = 0;
+  std::vector<SegDesc> sv;
   for (size_t g = 0; g + 1 < seg_off.size(); ++g) {
     const uint64_t len = seg_off[g + 1] - seg_off[g];
     const uint64_t nt = ceil_div(len, kSegTile);
+    if (!nt) continue;
+    sv.push_back(SegDesc{seg_off[g], (uint32_t)tv.size(), (uint32_t)nt});
     for (uint64_t t = 0; t < nt; ++t) {
       SegTile d;
       d.start = seg_off[g] + t * kSegTile;
       d.count = (uint32_t)std::min<uint64_t>(kSegTile, len - t * kSegTile);
-      d.cbase = R * tiles_before + t;
-      d.cstride = nt;
-      d.gshift = 0;
+      d.pad = 0;
       tv.push_back(d);
     }
-    tiles_before += nt;
   }
   n = seg_off.empty() ? 0 : seg_off.back();
   if (n >= (1ull << 32)) throw std::length_error("segmented sort of 2^32 or more keys");
   ntiles = tv.size();
+  nsegs = sv.size();
   tiles.reset(ntiles ? ntiles : 1);
-  if (ntiles)
+  segs.reset(nsegs ? nsegs : 1);
+  if (ntiles) {
     SPTC_HIP(hipMemcpyAsync(tiles.get(), tv.data(), ntiles * sizeof(SegTile),
                             hipMemcpyHostToDevice, s));
-  SPTC_HIP(hipStreamSynchronize(s));  // tv is released on return
+    SPTC_HIP(hipMemcpyAsync(segs.get(), sv.data(), nsegs * sizeof(SegDesc),
+                            hipMemcpyHostToDevice, s));
+  }
+  SPTC_HIP(hipStreamSynchronize(s));  // tv, sv are released on return
 }
 
 void segmented_sort_reserve(SortWorkspace& ws, const SegSortPlan& plan, hipStream_t s) {
@@ -270,20 +289,19 @@ void segmented_sort_reserve(SortWorkspace& ws, uint64_t ntiles, hipStream_t s) {
     ws.rts_cnt.reset(m);
     ws.rts_off.reset(m);
   }
-  if (ws.rts_tmp.size() < scan_tmp_count(m) + 1) ws.rts_tmp.reset(scan_tmp_count(m) + 1);
 }
 
 int segmented_sort_passes(int nbits) { return radix_passes(nbits, kSegRB); }
 
-bool segmented_sort(SortWorkspace& ws, const SegTile* tiles, uint64_t ntiles, uint64_t n,
-                    uint64_t* k0, float* v0, uint64_t* k1, float* v1, int nbits, hipStream_t s,
-                    KTimer* kt, int cls) {
+bool segmented_sort(SortWorkspace& ws, const SegTile* tiles, uint64_t ntiles,
+                    const SegDesc* segs, uint64_t nsegs, uint64_t n, uint64_t* k0, float* v0,
+                    uint64_t* k1, float* v1, int nbits, hipStream_t s, KTimer* kt, int cls) {
   const int passes = radix_passes(nbits, kSegRB);
   if (!n || !passes || !ntiles) return false;
   const uint64_t m = ntiles << kSegRB;
-  if (ws.rts_cnt.size() < m || ws.rts_tmp.size() < scan_tmp_count(m) + 1)
-    throw std::logic_error("segmented sort workspace not reserved");
-  if (ntiles >= (1ull << 31)) throw std::length_error("segmented sort tile count");
+  if (ws.rts_cnt.size() < m) throw std::logic_error("segmented sort workspace not reserved");
+  if (ntiles >= (1ull << 31) || nsegs >= (1ull << 31))
+    throw std::length_error("segmented sort tile count");
   uint64_t* ka = k0;
   float* va = v0;
   uint64_t* kb = k1;
@@ -293,8 +311,9 @@ bool segmented_sort(SortWorkspace& ws, const SegTile* tiles, uint64_t ntiles, ui
     int tok = kt ? kt->begin(s) : -1;
     hipLaunchKernelGGL(k_seg_upsweep, dim3((uint32_t)ntiles), dim3(kSegNT), 0, s, ka, shift,
                        tiles, ws.rts_cnt.get());
-    run_scan(SegCountScanOp{ws.rts_cnt.get(), ws.rts_off.get()}, m, ws.rts_tmp.get(), s);
-    if (kt) kt->end(tok, SPTC_K_SORT_COUNT, 8.0 * (double)n + 12.0 * (double)m, s);
+    hipLaunchKernelGGL(k_seg_scan, dim3((uint32_t)nsegs), dim3(1 << kSegRB), 0, s, segs,
+                       ws.rts_cnt.get(), ws.rts_off.get());
+    if (kt) kt->end(tok, SPTC_K_SORT_COUNT, 8.0 * (double)n + 16.0 * (double)m, s);
     tok = kt ? kt->begin(s) : -1;
     hipLaunchKernelGGL(k_seg_downsweep, dim3((uint32_t)ntiles), dim3(kSegNT), 0, s, ka, va, kb,
                        vb, shift, tiles, ws.rts_off.get());
@@ -307,50 +326,56 @@ bool segmented_sort(SortWorkspace& ws, const SegTile* tiles, uint64_t ntiles, ui
 
 bool segmented_sort(SortWorkspace& ws, const SegSortPlan& plan, uint64_t* k0, float* v0,
                     uint64_t* k1, float* v1, int nbits, hipStream_t s, KTimer* kt, int cls) {
-  return segmented_sort(ws, plan.tiles.get(), plan.ntiles, plan.n, k0, v0, k1, v1, nbits, s, kt,
-                        cls);
+  return segmented_sort(ws, plan.tiles.get(), plan.ntiles, plan.segs.get(), plan.nsegs, plan.n,
+                        k0, v0, k1, v1, nbits, s, kt, cls);
 }
 
 void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows,
                         const uint64_t* row_prod, RowSegPlan* plan, SortWorkspace& ws, U2* tmp,
                         hipStream_t s) {
-  plan->nlocal = plan->nbig_tiles = 0;
+  plan->nlocal = plan->nbig_tiles = plan->nbig_rows = plan->nsmall = 0;
   if (!lrows) return;
   const uint64_t m = lrows + 1;
-  if (plan->scratch.size() < 6 * m) plan->scratch.reset(6 * m);
+  if (plan->scratch.size() < 8 * m) plan->scratch.reset(8 * m);
   uint64_t* small_n = plan->scratch.get();
   uint64_t* big_t = small_n + m;
   uint64_t* ps = big_t + m;
   uint64_t* bt = ps + m;
   uint64_t* head = bt + m;
   uint64_t* hs = head + m;
+  uint64_t* big_f = hs + m;
+  uint64_t* bs = big_f + m;
   const uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(lrows, kBlock), 4096);
   hipLaunchKernelGGL(k_rowseg_sizes, dim3(grid), dim3(kBlock), 0, s, ranges, nranges, lrows,
-                     row_prod, small_n, big_t);
+                     row_prod, small_n, big_t, big_f);
   run_exclusive_scan(small_n, ps, lrows, tmp, s);
   run_exclusive_scan(big_t, bt, lrows, tmp, s);
+  run_exclusive_scan(big_f, bs, lrows, tmp, s);
   hipLaunchKernelGGL(k_rowseg_heads, dim3(grid), dim3(kBlock), 0, s, lrows, small_n, big_t, ps,
                      head);
   run_exclusive_scan(head, hs, lrows, tmp, s);
-  uint64_t tot[3];
+  uint64_t tot[4];
   SPTC_HIP(hipMemcpyAsync(&tot[0], hs + lrows, 8, hipMemcpyDeviceToHost, s));
   SPTC_HIP(hipMemcpyAsync(&tot[1], bt + lrows, 8, hipMemcpyDeviceToHost, s));
   SPTC_HIP(hipMemcpyAsync(&tot[2], ps + lrows, 8, hipMemcpyDeviceToHost, s));
+  SPTC_HIP(hipMemcpyAsync(&tot[3], bs + lrows, 8, hipMemcpyDeviceToHost, s));
   SPTC_HIP(hipStreamSynchronize(s));
   plan->nlocal = tot[0];
   plan->nbig_tiles = tot[1];
   plan->nsmall = tot[2];
+  plan->nbig_rows = tot[3];
+  if (plan->nlocal >= (1ull << 31) || plan->nbig_tiles >= (1ull << 31))
+    throw std::length_error("row segments");
   if (plan->local.size() < plan->nlocal) plan->local.reset(plan->nlocal);
   if (plan->big.size() < plan->nbig_tiles) plan->big.reset(plan->nbig_tiles);
+  if (plan->bigseg.size() < plan->nbig_rows) plan->bigseg.reset(plan->nbig_rows);
   if (plan->nlocal)
     hipLaunchKernelGGL(k_zero_local, dim3((uint32_t)ceil_div(plan->nlocal, 256)), dim3(256), 0,
                        s, plan->local.get(), plan->nlocal);
   hipLaunchKernelGGL(k_rowseg_emit, dim3(grid), dim3(kBlock), 0, s, ranges, nranges, lrows,
-                     row_prod, small_n, big_t, ps, bt, head, hs, plan->local.get(),
-                     plan->big.get());
+                     row_prod, big_t, bt, bs, head, hs, plan->local.get(), plan->big.get(),
+                     plan->bigseg.get());
   if (plan->nbig_tiles) segmented_sort_reserve(ws, plan->nbig_tiles, s);
-  if (plan->nlocal >= (1ull << 31) || plan->nbig_tiles >= (1ull << 31))
-    throw std::length_error("row segments");
 }
 
 bool sort_row_segments(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, uint64_t* k0,
@@ -358,8 +383,8 @@ bool sort_row_segments(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, ui
                        KTimer* kt, int cls) {
   bool alt = false;
   if (plan.nbig_tiles)
-    alt = segmented_sort(ws, plan.big.get(), plan.nbig_tiles, n - plan.nsmall, k0, v0, k1, v1,
-                         col_bits, s, kt, cls);
+    alt = segmented_sort(ws, plan.big.get(), plan.nbig_tiles, plan.bigseg.get(), plan.nbig_rows,
+                         n - plan.nsmall, k0, v0, k1, v1, col_bits, s, kt, cls);
   if (plan.nlocal) {
     int tok = kt ? kt->begin(s) : -1;
     hipLaunchKernelGGL(k_local_sort, dim3((uint32_t)plan.nlocal), dim3(kSegNT), 0, s, k0, v0,

@@ -15,15 +15,20 @@ constexpr int kSegIPT = 12;   // keys per thread
 constexpr int kSegNT = 512;   // threads per tile
 constexpr uint64_t kSegTile = (uint64_t)kSegIPT * kSegNT;
 
-// One tile of a segmented sort.  The flat scan of the count matrix places
-// the sorted segments back to back from 0; gshift moves a segment to where
-// it really starts (non-zero when segments sorted elsewhere sit between).
+// One tile of a segmented sort (never straddles a segment).
 struct SegTile {
-  uint64_t start;    // first key (global index)
-  uint64_t cbase;    // count-matrix index of digit 0 of this tile
-  uint64_t cstride;  // distance between digits (tiles in the segment)
-  uint32_t count;    // keys in the tile
-  uint32_t gshift;   // added to the scanned destination offsets
+  uint64_t start;  // first key (global index)
+  uint32_t count;  // keys in the tile
+  uint32_t pad;
+};
+// One segment: its tiles are first_tile .. first_tile + ntiles - 1.  Tile t's
+// digit counts and destination offsets are rows t of tile-major count and
+// offset matrices (256 entries each); one workgroup per segment turns the
+// counts into absolute destinations (start + digit start + earlier tiles).
+struct SegDesc {
+  uint64_t start;
+  uint32_t first_tile;
+  uint32_t ntiles;
 };
 
 // A segment small enough to be sorted inside one workgroup's LDS.
@@ -45,25 +50,26 @@ __device__ __forceinline__ uint32_t find_seg(const uint64_t* off, uint32_t nseg,
 }
 
 // Tile table of a segmented sort: each segment [seg_off[s], seg_off[s+1])
-// is cut into its own tiles; the count matrix is segment-major and digit-
-// major within a segment, so one flat exclusive scan yields every tile's
-// global digit offsets.  Built once per batch shape (one host->device copy).
+// is cut into its own tiles.  Built once per batch shape (one host->device
+// copy).
 struct SegSortPlan {
   DevBuf<SegTile> tiles;
-  uint64_t ntiles = 0;
+  DevBuf<SegDesc> segs;
+  uint64_t ntiles = 0, nsegs = 0;
   uint64_t n = 0;
   void build(const std::vector<uint64_t>& seg_off, hipStream_t s);
 };
 // Sorts each segment of (k0, v0) on bits [0, nbits) with 8-bit reduce-then-
-// scan passes, ping-ponging with (k1, v1); ws supplies the count matrices
-// (segmented_sort_reserve).  Launches only (no allocation or host
-// synchronization), so it can be replayed and captured.  Returns true when
-// the result ended in (k1, v1).
+// scan passes (count per tile, per-segment scan, scatter), ping-ponging with
+// (k1, v1); ws supplies the count matrices (segmented_sort_reserve).
+// Launches only (no allocation or host synchronization), so it can be
+// replayed and captured.  Returns true when the result ended in (k1, v1).
 void segmented_sort_reserve(SortWorkspace& ws, uint64_t ntiles, hipStream_t s);
 void segmented_sort_reserve(SortWorkspace& ws, const SegSortPlan& plan, hipStream_t s);
-bool segmented_sort(SortWorkspace& ws, const SegTile* tiles, uint64_t ntiles, uint64_t n,
-                    uint64_t* k0, float* v0, uint64_t* k1, float* v1, int nbits, hipStream_t s,
-                    KTimer* kt = nullptr, int cls = SPTC_K_SORT_INPUT);
+bool segmented_sort(SortWorkspace& ws, const SegTile* tiles, uint64_t ntiles,
+                    const SegDesc* segs, uint64_t nsegs, uint64_t n, uint64_t* k0, float* v0,
+                    uint64_t* k1, float* v1, int nbits, hipStream_t s, KTimer* kt = nullptr,
+                    int cls = SPTC_K_SORT_INPUT);
 bool segmented_sort(SortWorkspace& ws, const SegSortPlan& plan, uint64_t* k0, float* v0,
                     uint64_t* k1, float* v1, int nbits, hipStream_t s, KTimer* kt = nullptr,
                     int cls = SPTC_K_SORT_INPUT);
@@ -89,10 +95,11 @@ struct RowRange {
 // one workgroup's LDS.  Every product is read and written once for the
 // groups, ceil(col_bits / 8) times for the big rows.
 struct RowSegPlan {
-  uint64_t nlocal = 0, nbig_tiles = 0, nsmall = 0;  // nsmall: products in groups
+  uint64_t nlocal = 0, nbig_tiles = 0, nbig_rows = 0, nsmall = 0;  // nsmall: products in groups
   DevBuf<LocalSeg> local;
   DevBuf<SegTile> big;
-  DevBuf<uint64_t> scratch;  // 6 x (rows + 1) per-row values and scans
+  DevBuf<SegDesc> bigseg;
+  DevBuf<uint64_t> scratch;  // 8 x (rows + 1) per-row values and scans
 };
 // Builds the plan (one host synchronization for the counts); tmp holds
 // scan_tmp_count(lrows) + 1 entries.  Reserves ws for the big-row passes.
