@@ -40,7 +40,7 @@ __global__ __launch_bounds__(kSegNT) void k_seg_downsweep(
   const uint32_t tid = threadIdx.x;
   const SegTile t = tiles[blockIdx.x];
   for (int i = tid; i < (kSegNT / kWave) * R; i += kSegNT) (&sm.cnt[0][0])[i] = 0;
-  for (int d = tid; d < R; d += kSegNT) sm.gbase[d] = offs[(uint64_t)blockIdx.x * R + d];
+  for (int d = tid; d < R; d += kSegNT) sm.gbase[d] = offs[t.obase + (uint64_t)d * t.ostride];
   __syncthreads();
   const uint64_t end = t.start + t.count;
   const uint64_t wbase = t.start + (uint64_t)(tid / kWave) * kWave * kSegIPT;
@@ -70,8 +70,9 @@ __global__ __launch_bounds__(1 << kSegRB) void k_seg_scan(const SegDesc* __restr
   for (uint64_t t = t0; t < t1; ++t) tot += cnt[t * R + d];
   uint32_t all;
   uint32_t base = (uint32_t)g.start + block_excl_scan<uint32_t>(tot, scratch, &all);
+  uint32_t* o = off + R * t0 + (uint64_t)d * g.ntiles;  // digit-major in the segment
   for (uint64_t t = t0; t < t1; ++t) {
-    off[t * R + d] = base;
+    o[t - t0] = base;
     base += cnt[t * R + d];
   }
 }
@@ -146,7 +147,8 @@ __global__ __launch_bounds__(kBlock) void k_rowseg_emit(
         SegTile d;
         d.start = first + t * kSegTile;
         d.count = (uint32_t)(n - t * kSegTile < kSegTile ? n - t * kSegTile : kSegTile);
-        d.pad = 0;
+        d.obase = (1ull << kSegRB) * o + t;
+        d.ostride = (uint32_t)nt;
         big[o + t] = d;
       }
     } else {
@@ -254,12 +256,14 @@ void SegSortPlan::build(const std::vector<uint64_t>& seg_off, hipStream_t s) {
     const uint64_t len = seg_off[g + 1] - seg_off[g];
     const uint64_t nt = ceil_div(len, kSegTile);
     if (!nt) continue;
-    sv.push_back(SegDesc{seg_off[g], (uint32_t)tv.size(), (uint32_t)nt});
+    const uint64_t t0 = tv.size();
+    sv.push_back(SegDesc{seg_off[g], (uint32_t)t0, (uint32_t)nt});
     for (uint64_t t = 0; t < nt; ++t) {
       SegTile d;
       d.start = seg_off[g] + t * kSegTile;
       d.count = (uint32_t)std::min<uint64_t>(kSegTile, len - t * kSegTile);
-      d.pad = 0;
+      d.obase = (1ull << kSegRB) * t0 + t;
+      d.ostride = (uint32_t)nt;
       tv.push_back(d);
     }
   }

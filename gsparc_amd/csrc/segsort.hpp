@@ -15,16 +15,20 @@ constexpr int kSegIPT = 12;   // keys per thread
 constexpr int kSegNT = 512;   // threads per tile
 constexpr uint64_t kSegTile = (uint64_t)kSegIPT * kSegNT;
 
-// One tile of a segmented sort (never straddles a segment).
+// One tile of a segmented sort (never straddles a segment).  Its digit
+// counts are row t of a tile-major count matrix (coalesced writes); its
+// destination offsets sit digit-major within the segment, at
+// obase + d * ostride (spread over memory channels when many tiles start at
+// once).
 struct SegTile {
-  uint64_t start;  // first key (global index)
-  uint32_t count;  // keys in the tile
-  uint32_t pad;
+  uint64_t start;    // first key (global index)
+  uint64_t obase;    // offset-matrix index of digit 0 (256 * first tile + index in segment)
+  uint32_t count;    // keys in the tile
+  uint32_t ostride;  // tiles in the segment
 };
-// One segment: its tiles are first_tile .. first_tile + ntiles - 1.  Tile t's
-// digit counts and destination offsets are rows t of tile-major count and
-// offset matrices (256 entries each); one workgroup per segment turns the
-// counts into absolute destinations (start + digit start + earlier tiles).
+// One segment: its tiles are first_tile .. first_tile + ntiles - 1; one
+// workgroup per segment turns their counts into absolute destinations
+// (start + digit start + earlier tiles of the digit).
 struct SegDesc {
   uint64_t start;
   uint32_t first_tile;
