@@ -195,17 +195,15 @@ struct BMatchOp {
 
 // ---- batched ESC expand ----------------------------------------------------
 // One block per tile of the chunk's tile table (grid-stride).  The tile's
-// owners are found by one binary search over act_prefix inside its piece,
-// their prefix run staged in LDS; each thread then walks its 8 consecutive
-// products.  Keys/values are staged in LDS and written coalesced.
+// owners are found by one binary search over act_prefix inside its piece and
+// their prefix run staged in LDS; products are lane-striped (coalesced Y
+// gathers and stores), each finding its owner by an LDS binary search.
 constexpr int kBExpandItems = 8;
 static_assert(kBExpandTileItems == kBlock * kBExpandItems, "expand tile");
 
 __global__ __launch_bounds__(kBlock) void k_bexpand(BExpandArgs a) {
   constexpr uint32_t TILE = kBExpandTileItems;
   __shared__ uint64_t s_pref[TILE + 1];
-  __shared__ uint64_t s_key[TILE];
-  __shared__ float s_val[TILE];
   __shared__ uint64_t s_ja;
   if (*a.bad) return;
   for (uint32_t t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
@@ -228,46 +226,31 @@ __global__ __launch_bounds__(kBlock) void k_bexpand(BExpandArgs a) {
     const uint32_t m = (uint32_t)(avail < TILE + 1 ? avail : TILE + 1);
     for (uint32_t k = threadIdx.x; k < m; k += kBlock) s_pref[k] = a.act_prefix[ja + k];
     __syncthreads();
-    const uint32_t q0 = threadIdx.x * kBExpandItems;
-    if (q0 < cnt) {
-      const uint64_t p0 = pb + q0;
-      uint32_t lo = 0, hi = m;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (s_pref[mid] <= p0) lo = mid + 1; else hi = mid;
-      }
-      uint32_t k = lo - 1;
-      uint64_t ybase = 0, rbits = 0, cmask = 0;
-      float xv = 0.f;
-      auto load_owner = [&](uint32_t kk) {
-        const uint64_t jj = ja + kk;
-        ybase = a.act_ypos[jj] - s_pref[kk];
-        xv = a.act_xval[jj];
-        const uint32_t row = a.act_row[jj];
-        rbits = shl64(pc.lrow0 + (row - pc.ra), a.K);
-        cmask = low_mask(a.inst[find_seg(a.inst_R, a.B, row)].kyf);
-      };
-      load_owner(k);
-      uint64_t next = k + 1 < m ? s_pref[k + 1] : ~0ull;
+    uint32_t klo = 0, kc = ~0u;
+    uint64_t rbits = 0, cmask = 0;
 #pragma unroll
-      for (int e = 0; e < kBExpandItems; ++e) {
-        const uint64_t p = p0 + e;
-        if (q0 + e < cnt) {
-          while (p >= next) {
-            ++k;
-            load_owner(k);
-            next = k + 1 < m ? s_pref[k + 1] : ~0ull;
-          }
-          const uint64_t y = ybase + p;
-          s_key[q0 + e] = rbits | (a.ykeys[y] & cmask);
-          s_val[q0 + e] = xv * a.yvals[y];
+    for (int e = 0; e < kBExpandItems; ++e) {
+      const uint32_t q = (uint32_t)e * kBlock + threadIdx.x;
+      if (q < cnt) {
+        const uint64_t p = pb + q;
+        uint32_t lo = klo, hi = m;  // last k with s_pref[k] <= p
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (s_pref[mid] <= p) lo = mid + 1; else hi = mid;
         }
+        const uint32_t k = lo - 1;
+        klo = k;
+        const uint64_t j = ja + k;
+        if (k != kc) {  // new owner: its row's key bits and column mask
+          kc = k;
+          const uint32_t row = a.act_row[j];
+          rbits = shl64(pc.lrow0 + (row - pc.ra), a.K);
+          cmask = low_mask(a.inst[find_seg(a.inst_R, a.B, row)].kyf);
+        }
+        const uint64_t y = a.act_ypos[j] + (p - s_pref[k]);
+        a.out_keys[tl.out0 + q] = rbits | (a.ykeys[y] & cmask);
+        a.out_vals[tl.out0 + q] = a.act_xval[j] * a.yvals[y];
       }
-    }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) {
-      a.out_keys[tl.out0 + i] = s_key[i];
-      a.out_vals[tl.out0 + i] = s_val[i];
     }
   }
 }

@@ -371,23 +371,26 @@ void launch_max_row(const uint64_t* row_prod, uint64_t R, uint64_t* out, hipStre
 constexpr int kExpandItems = 8;
 constexpr uint32_t kExpandTile = kBlock * kExpandItems;
 
-template <int RB>
+// Products are lane-striped (product pb + e*kBlock + t goes to thread t), so
+// the Y gathers and the product stores of a wave are contiguous runs; each
+// product finds its owner by a binary search over the staged prefixes,
+// starting from its previous product's owner.  HIST adds the fused digit
+// histograms for the look-back sort (its LDS is only reserved then).
+template <int RB, bool HIST>
 __global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs a) {
   constexpr int R = 1 << RB;
   __shared__ uint64_t s_pref[kExpandTile + 1];
-  __shared__ uint64_t s_key[kExpandTile];
-  __shared__ float s_val[kExpandTile];
-  __shared__ uint32_t sh[kMaxFusedPasses * R];
+  __shared__ uint32_t sh[HIST ? kMaxFusedPasses * R : 1];
   __shared__ uint64_t s_ja;
   __shared__ double s_sum[kBlock / kWave];
-  for (int i = threadIdx.x; i < a.passes * R; i += kBlock) sh[i] = 0;
+  if (HIST)
+    for (int i = threadIdx.x; i < a.passes * R; i += kBlock) sh[i] = 0;
   double vsum = 0.0;
   const uint64_t nprod = a.P1 - a.P0;
   const uint64_t ntiles = ceil_div(nprod, kExpandTile);
   for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const uint64_t pb = a.P0 + tile * kExpandTile;
     const uint64_t pe = pb + kExpandTile < a.P1 ? pb + kExpandTile : a.P1;
-    const uint32_t cnt = (uint32_t)(pe - pb);
     __syncthreads();
     if (threadIdx.x == 0) {
       // last j in [A0, A1) with act_prefix[j] <= pb
@@ -404,57 +407,42 @@ __global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs a) {
     const uint32_t m = (uint32_t)(avail < kExpandTile + 1 ? avail : kExpandTile + 1);
     for (uint32_t k = threadIdx.x; k < m; k += kBlock) s_pref[k] = a.act_prefix[ja + k];
     __syncthreads();
-    // Thread t expands products [t*E, t*E+E) of the tile: one LDS search for
-    // the first, then a forward walk over the owning active nonzeros.
-    const uint32_t q0 = threadIdx.x * kExpandItems;
-    if (q0 < cnt) {
-      const uint64_t p0 = pb + q0;
-      uint32_t lo = 0, hi = m;  // last k with s_pref[k] <= p0
-      while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (s_pref[mid] <= p0) lo = mid + 1; else hi = mid;
-      }
-      uint32_t k = lo - 1;
-      uint64_t j = ja + k;
-      uint64_t ybase = a.act_ypos[j] - s_pref[k];
-      float xv = a.act_xval[j];
-      uint64_t rbits = shl64((uint64_t)(a.act_row[j] - a.r0), a.ynf);
-      uint64_t next = k + 1 < m ? s_pref[k + 1] : ~0ull;
-#pragma unroll
-      for (int e = 0; e < kExpandItems; ++e) {
-        const uint64_t p = p0 + e;
-        if (q0 + e < cnt) {
-          while (p >= next) {
-            ++k;
-            j = ja + k;
-            ybase = a.act_ypos[j] - s_pref[k];
-            xv = a.act_xval[j];
-            rbits = shl64((uint64_t)(a.act_row[j] - a.r0), a.ynf);
-            next = k + 1 < m ? s_pref[k + 1] : ~0ull;
-          }
-          const uint64_t y = ybase + p;
-          const float v = xv * a.yvals[y];
-          s_key[q0 + e] = rbits | (a.ykeys[y] & a.fmask);
-          s_val[q0 + e] = v;
-          vsum += (double)v;
-        }
-      }
-    }
-    __syncthreads();
-    // Coalesced stores of the staged tile, and the digit histograms.
     const uint64_t obase = pb - a.P0;
-    for (uint32_t i = threadIdx.x; i < kExpandTile; i += kBlock) {
-      const bool valid = i < cnt;
-      const uint64_t key = valid ? s_key[i] : 0;
+    uint32_t klo = 0, kc = ~0u;
+    uint64_t ybase = 0, rbits = 0;
+    float xv = 0.f;
+#pragma unroll
+    for (int e = 0; e < kExpandItems; ++e) {
+      const uint64_t p = pb + (uint64_t)e * kBlock + threadIdx.x;
+      const bool valid = p < pe;
+      uint64_t key = 0;
       if (valid) {
-        a.out_keys[obase + i] = key;
-        a.out_vals[obase + i] = s_val[i];
+        uint32_t lo = klo, hi = m;  // last k with s_pref[k] <= p
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (s_pref[mid] <= p) lo = mid + 1; else hi = mid;
+        }
+        const uint32_t k = lo - 1;
+        klo = k;
+        if (k != kc) {  // new owner
+          kc = k;
+          const uint64_t j = ja + k;
+          ybase = a.act_ypos[j] - s_pref[k];
+          xv = a.act_xval[j];
+          rbits = shl64((uint64_t)(a.act_row[j] - a.r0), a.ynf);
+        }
+        const uint64_t y = ybase + p;
+        const float v = xv * a.yvals[y];
+        key = rbits | (a.ykeys[y] & a.fmask);
+        a.out_keys[obase + (p - pb)] = key;
+        a.out_vals[obase + (p - pb)] = v;
+        vsum += (double)v;
       }
-      if (a.passes) hist_add_wave<RB>(sh, key, valid, a.passes);
+      if (HIST) hist_add_wave<RB>(sh, key, valid, a.passes);
     }
   }
   __syncthreads();
-  if (a.passes) hist_flush<RB>(sh, a.hist, a.passes);
+  if (HIST) hist_flush<RB>(sh, a.hist, a.passes);
   for (int off = kWave / 2; off > 0; off >>= 1) vsum += __shfl_xor(vsum, off, kWave);
   if (lane_id() == 0) s_sum[threadIdx.x / kWave] = vsum;
   __syncthreads();
@@ -468,12 +456,16 @@ __global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs a) {
 void launch_expand(const ExpandArgs& a, int rb, hipStream_t s) {
   const uint64_t nprod = a.P1 - a.P0;
   if (!nprod) return;
-  uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(nprod, kExpandTile), 2048);
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(nprod, kExpandTile), 8192);
+  if (!a.passes) {
+    hipLaunchKernelGGL((k_expand<8, false>), dim3(grid), dim3(kBlock), 0, s, a);
+    return;
+  }
   switch (rb) {
-    case 9: hipLaunchKernelGGL(k_expand<9>, dim3(grid), dim3(kBlock), 0, s, a); break;
-    case 10: hipLaunchKernelGGL(k_expand<10>, dim3(grid), dim3(kBlock), 0, s, a); break;
-    case 11: hipLaunchKernelGGL(k_expand<11>, dim3(grid), dim3(kBlock), 0, s, a); break;
-    default: hipLaunchKernelGGL(k_expand<8>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 9: hipLaunchKernelGGL((k_expand<9, true>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 10: hipLaunchKernelGGL((k_expand<10, true>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 11: hipLaunchKernelGGL((k_expand<11, true>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_expand<8, true>), dim3(grid), dim3(kBlock), 0, s, a); break;
   }
 }
 
