@@ -78,14 +78,28 @@ def build_oracle():
         _run(["make", "-s", "-C", odir, "ref"])
 
 
+def build_cli():
+    """bin/gsparc: the reference-compatible command line (host C++ over the
+    C ABI; links lib/libsptc_hip.so through an $ORIGIN-relative rpath)."""
+    bindir = os.path.join(PKG_DIR, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    out = os.path.join(bindir, "gsparc")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(REPO_DIR, "include"),
+          "-o", out, os.path.join(PKG_DIR, "cli", "gsparc_main.cpp"), "-L", LIB_DIR,
+          "-lsptc_hip", "-Wl,-rpath,$ORIGIN/../lib"])
+    return out
+
+
 def build_all():
     build_gen()
     build_hip()
+    build_cli()
     build_oracle()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "gen": build_gen, "hip": build_hip, "oracle": build_oracle}[w]()
+        {"all": build_all, "gen": build_gen, "hip": build_hip, "cli": build_cli,
+         "oracle": build_oracle}[w]()
     print("built:", ", ".join(what))
