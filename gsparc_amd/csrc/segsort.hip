@@ -50,39 +50,27 @@ __device__ __forceinline__ GenTile gen_stage(const ProductGen& g, uint64_t q0, u
   return t;
 }
 
-// Per-thread owner cache: data of the owner of the thread's last product.
-struct GenCache {
-  uint32_t k = ~0u;
-  uint64_t ybase = 0, rbits = 0, cmask = 0;
-  float xv = 0.f;
-};
-
-// Product at tile offset idx: key and (VAL) value.  c.k doubles as the owner
-// search start (offsets increase per thread).
-template <bool VAL = true>
+// Product at tile offset idx: key and value.  kk: owner search start (the
+// thread's previous owner; offsets increase per thread).
 __device__ __forceinline__ void gen_product(const ProductGen& g, const GenTile& t,
                                             const GenPiece& pc, const uint64_t* pref,
-                                            uint64_t idx, GenCache& c, uint64_t& key,
+                                            uint64_t idx, uint32_t& kk, uint64_t& key,
                                             float& val) {
   const uint64_t P = t.P + idx;
-  uint32_t lo = c.k == ~0u ? 0 : c.k, hi = t.m;  // last k with pref[k] <= P
+  uint32_t lo = kk, hi = t.m;  // last k with pref[k] <= P
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
     if (pref[mid] <= P) lo = mid + 1; else hi = mid;
   }
   const uint32_t k = lo - 1;
-  if (k != c.k) {  // new owner: reload its data
-    c.k = k;
-    const uint64_t j = t.ja + k;
-    c.ybase = g.act_ypos[j] - pref[k];
-    c.xv = g.act_xval[j];
-    const uint32_t row = g.act_row[j];
-    c.rbits = shl64(pc.lrow0 + (row - pc.ra), g.K);
-    c.cmask = g.inst_kyf ? low_mask(g.inst_kyf[find_seg(g.inst_R, g.B, row)]) : g.fmask;
-  }
-  const uint64_t y = c.ybase + P;
-  key = c.rbits | (g.ykeys[y] & c.cmask);
-  if (VAL) val = c.xv * g.yvals[y];
+  kk = k;
+  const uint64_t j = t.ja + k;
+  const uint64_t y = g.act_ypos[j] + (P - pref[k]);
+  const uint32_t row = g.act_row[j];
+  const uint64_t cmask =
+      g.inst_kyf ? low_mask(g.inst_kyf[find_seg(g.inst_R, g.B, row)]) : g.fmask;
+  key = shl64(pc.lrow0 + (row - pc.ra), g.K) | (g.ykeys[y] & cmask);
+  val = g.act_xval[j] * g.yvals[y];
 }
 
 __device__ __forceinline__ void gen_add_vsum(double v, double* s_red, double* out) {
@@ -116,7 +104,7 @@ __global__ __launch_bounds__(kSegNT) void k_seg_upsweep(const uint64_t* __restri
     __syncthreads();
   }
   const uint64_t end = t.start + t.count;
-  GenCache kk;
+  uint32_t kk = 0;
 #pragma unroll
   for (int i = 0; i < kSegIPT; ++i) {
     const uint64_t off = (uint64_t)i * kSegNT + threadIdx.x;
@@ -126,7 +114,7 @@ __global__ __launch_bounds__(kSegNT) void k_seg_upsweep(const uint64_t* __restri
     if (valid) {
       if (GEN) {
         float v;
-        gen_product<false>(g, gt, pc, pref, off, kk, k, v);
+        gen_product(g, gt, pc, pref, off, kk, k, v);
       } else {
         k = kin[idx];
       }
@@ -162,7 +150,7 @@ __global__ __launch_bounds__(kSegNT) void k_seg_downsweep(
     // owners staged in the (not yet used) key staging buffer
     const GenTile gt = gen_stage(g, t.start, t.count, sm.buf, &s_t);
     const GenPiece pc = g.pieces[gt.piece];
-    GenCache kk;
+    uint32_t kk = 0;
     double vs = 0.0;
 #pragma unroll
     for (int i = 0; i < kSegIPT; ++i) {
@@ -338,7 +326,7 @@ __global__ __launch_bounds__(kSegNT) void k_local_sort(const uint64_t* __restric
     if (n) {  // uniform per block
       const GenTile gt = gen_stage(g, sg.start, (uint32_t)n, sm.buf, &s_t);
       const GenPiece pc = g.pieces[gt.piece];
-      GenCache kk;
+      uint32_t kk = 0;
       double vs = 0.0;
 #pragma unroll
       for (int i = 0; i < kSegIPT; ++i) {
