@@ -329,7 +329,6 @@ sptc_status BatchEngine::build_chunks(hipStream_t s) {
       const uint32_t pid = (uint32_t)cur.hp.size();
       cur.hp.push_back(pc);
       cur.hr.push_back(RowRange{r, r1 - r, cur.lrows, cur.n});
-      cur.hg.push_back(GenPiece{cur.n, p, a, a1, r, cur.lrows});
       for (uint64_t q = p; q < p1; q += kBExpandTileItems) {
         BETile t;
         t.p0 = q;
@@ -362,9 +361,6 @@ sptc_status BatchEngine::build_chunks(hipStream_t s) {
                               hipMemcpyHostToDevice, s));
       build_row_segments(rr.get(), (uint32_t)c.hr.size(), c.lrows, row_prod_.get(), &c.rseg,
                          ws_, tmp_.get(), s);
-      c.gpieces.reset(c.hg.size());
-      SPTC_HIP(hipMemcpyAsync(c.gpieces.get(), c.hg.data(), c.hg.size() * sizeof(GenPiece),
-                              hipMemcpyHostToDevice, s));
       c.rowsort = true;
       SPTC_HIP(hipStreamSynchronize(s));  // rr and the scratch are released below
       c.rseg.scratch.release();
@@ -536,47 +532,32 @@ sptc_status BatchEngine::run(hipStream_t s, bool P) {
     if (st != SPTC_OK) return st;
   }
   for (EscChunk& c : esc_) {
+    BExpandArgs ea;
+    std::memset(&ea, 0, sizeof(ea));
+    ea.act_prefix = act_prefix_.get();
+    ea.act_ypos = act_ypos_.get();
+    ea.act_xval = act_xval_.get();
+    ea.act_row = act_row_.get();
+    ea.ykeys = yk;
+    ea.yvals = yv;
+    ea.pieces = c.pieces.get();
+    ea.tiles = c.tiles.get();
+    ea.ntiles = (uint32_t)c.ht.size();
+    ea.K = c.K;
+    ea.inst_R = iR;
+    ea.inst = inst_.get();
+    ea.B = B;
+    ea.out_keys = kA_.get();
+    ea.out_vals = vA_.get();
+    ea.bad = bad_.get();
+    int tok = kt ? kt->begin(s) : -1;
+    launch_bexpand(ea, s);
+    if (kt) kt->end(tok, SPTC_K_EXPAND, 24.0 * (double)c.n, s);
     bool alt;
-    if (c.rowsort) {  // products generated by the row-segmented sort's first pass
-      ProductGen g{};
-      g.act_prefix = act_prefix_.get();
-      g.act_ypos = act_ypos_.get();
-      g.act_xval = act_xval_.get();
-      g.act_row = act_row_.get();
-      g.ykeys = yk;
-      g.yvals = yv;
-      g.pieces = c.gpieces.get();
-      g.npieces = (uint32_t)c.hg.size();
-      g.K = c.K;
-      g.inst_R = iR;
-      g.inst_kyf = kyf_.get();
-      g.B = B;
-      g.vsum = vsum_.get();
-      g.bad = bad_.get();
+    if (c.rowsort) {
       alt = sort_row_segments(ws_, c.rseg, c.n, kA_.get(), vA_.get(), ws_.alt_keys.get(),
-                              ws_.alt_vals.get(), c.K, s, kt, SPTC_K_SORT_PROD, &g);
+                              ws_.alt_vals.get(), c.K, s, kt, SPTC_K_SORT_PROD);
     } else {
-      BExpandArgs ea;
-      std::memset(&ea, 0, sizeof(ea));
-      ea.act_prefix = act_prefix_.get();
-      ea.act_ypos = act_ypos_.get();
-      ea.act_xval = act_xval_.get();
-      ea.act_row = act_row_.get();
-      ea.ykeys = yk;
-      ea.yvals = yv;
-      ea.pieces = c.pieces.get();
-      ea.tiles = c.tiles.get();
-      ea.ntiles = (uint32_t)c.ht.size();
-      ea.K = c.K;
-      ea.inst_R = iR;
-      ea.inst = inst_.get();
-      ea.B = B;
-      ea.out_keys = kA_.get();
-      ea.out_vals = vA_.get();
-      ea.bad = bad_.get();
-      int tok = kt ? kt->begin(s) : -1;
-      launch_bexpand(ea, s);
-      if (kt) kt->end(tok, SPTC_K_EXPAND, 24.0 * (double)c.n, s);
       ws_.ensure(c.n, c.rb, s);
       const int passes = radix_passes(c.bits, c.rb);
       ws_.prepare(passes, s);
@@ -585,7 +566,7 @@ sptc_status BatchEngine::run(hipStream_t s, bool P) {
     }
     const uint64_t* sk = alt ? ws_.alt_keys.get() : kA_.get();
     const float* sv = alt ? ws_.alt_vals.get() : vA_.get();
-    int tok = kt ? kt->begin(s) : -1;
+    tok = kt ? kt->begin(s) : -1;
     run_reduce_count(sk, c.n, tmp_.get(), s);
     if (kt) kt->end(tok, SPTC_K_REDUCE, 8.0 * (double)c.n, s);
     if (P) {

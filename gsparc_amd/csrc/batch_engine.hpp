@@ -45,8 +45,6 @@ struct EscChunk {
   std::vector<BPiece> hp;
   std::vector<BETile> ht;
   std::vector<RowRange> hr;  // the pieces as row ranges (row-segmented sort)
-  std::vector<GenPiece> hg;  // and as product-generation pieces
-  DevBuf<GenPiece> gpieces;
   DevBuf<BPiece> pieces;
   DevBuf<BETile> tiles;
   bool rowsort = false;

@@ -463,7 +463,6 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
     const uint64_t fmask = low_mask(kyf);
     // row-segmented product sort
     DevBuf<RowRange> rrange(1);
-    DevBuf<GenPiece> gpiece(1);
     RowSegPlan rseg;
     for (size_t c = 1; c < bounds_r.size(); ++c) {
       const uint64_t r0c = bounds_r[c - 1], r1c = bounds_r[c];
@@ -475,60 +474,52 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       const int passes = radix_passes(bits, rb);
       ws_.ensure(n, rb, s_);
       ws_.prepare(passes, s_);
-      bool alt = false;
+      ExpandArgs ea;
+      ea.act_prefix = act_prefix.get();
+      ea.act_ypos = act_ypos.get();
+      ea.act_xval = act_xval.get();
+      ea.act_row = act_row.get();
+      ea.A0 = bounds_a[c - 1];
+      ea.A1 = bounds_a[c];
+      ea.P0 = P0;
+      ea.P1 = P1;
+      ea.r0 = (uint32_t)r0c;
+      ea.ykeys = ykeys;
+      ea.yvals = Y.vals.get();
+      ea.ynf = kyf;
+      ea.fmask = fmask;
+      ea.out_keys = kA.get();
+      ea.out_vals = vA.get();
+      ea.hist = ws_.hist.get();
+      const bool hist = sort_needs_histogram();
+      ea.passes = hist && passes <= kMaxFusedPasses ? passes : 0;
+      ea.vsum = vsum.get();
+      int tok = kt_.begin(s_);
+      launch_expand(ea, rb, s_);
+      // product (key, value) written, matched Y (key, value) read
+      kt_.end(tok, SPTC_K_EXPAND, 24.0 * (double)n, s_);
+      bool sorted = false, alt = false;
       if (kyf > 0 && kyf < bits && prefer_row_sort()) {
-        // row-segmented sort whose first pass generates the products
         const uint64_t lrows = r1c - r0c;
-        RowRange rh{r0c, lrows, 0, 0};
-        GenPiece gp{0, P0, bounds_a[c - 1], bounds_a[c], r0c, 0};
-        SPTC_HIP(hipMemcpyAsync(rrange.get(), &rh, sizeof(rh), hipMemcpyHostToDevice, s_));
-        SPTC_HIP(hipMemcpyAsync(gpiece.get(), &gp, sizeof(gp), hipMemcpyHostToDevice, s_));
+        RowRange* h = reinterpret_cast<RowRange*>(host_u64_ + 4);
+        h->ra = r0c;
+        h->rows = lrows;
+        h->lrow0 = 0;
+        h->out0 = 0;
+        SPTC_HIP(hipMemcpyAsync(rrange.get(), h, sizeof(RowRange), hipMemcpyHostToDevice, s_));
         build_row_segments(rrange.get(), 1, lrows, row_prod.get(), &rseg, ws_, tmp.get(), s_);
-        ProductGen g{};
-        g.act_prefix = act_prefix.get();
-        g.act_ypos = act_ypos.get();
-        g.act_xval = act_xval.get();
-        g.act_row = act_row.get();
-        g.ykeys = ykeys;
-        g.yvals = Y.vals.get();
-        g.pieces = gpiece.get();
-        g.npieces = 1;
-        g.K = kyf;
-        g.fmask = fmask;
-        g.vsum = vsum.get();
         alt = sort_row_segments(ws_, rseg, n, kA.get(), vA.get(), ws_.alt_keys.get(),
-                                ws_.alt_vals.get(), kyf, s_, &kt_, SPTC_K_SORT_PROD, &g);
-      } else {
-        ExpandArgs ea;
-        ea.act_prefix = act_prefix.get();
-        ea.act_ypos = act_ypos.get();
-        ea.act_xval = act_xval.get();
-        ea.act_row = act_row.get();
-        ea.A0 = bounds_a[c - 1];
-        ea.A1 = bounds_a[c];
-        ea.P0 = P0;
-        ea.P1 = P1;
-        ea.r0 = (uint32_t)r0c;
-        ea.ykeys = ykeys;
-        ea.yvals = Y.vals.get();
-        ea.ynf = kyf;
-        ea.fmask = fmask;
-        ea.out_keys = kA.get();
-        ea.out_vals = vA.get();
-        ea.hist = ws_.hist.get();
-        const bool hist = sort_needs_histogram();
-        ea.passes = hist && passes <= kMaxFusedPasses ? passes : 0;
-        ea.vsum = vsum.get();
-        int tok = kt_.begin(s_);
-        launch_expand(ea, rb, s_);
-        // product (key, value) written, matched Y (key, value) read
-        kt_.end(tok, SPTC_K_EXPAND, 24.0 * (double)n, s_);
+                                ws_.alt_vals.get(), kyf, s_, &kt_, SPTC_K_SORT_PROD);
+        sorted = true;
+      }
+      if (!sorted) {
+        ws_.ensure(n, rb, s_);
         if (hist && passes > kMaxFusedPasses) histogram(ws_, kA.get(), n, passes, s_);
         alt = sort_after_histogram(ws_, kA.get(), vA.get(), n, bits, s_, &kt_, SPTC_K_SORT_PROD);
       }
       const uint64_t* sk = alt ? ws_.alt_keys.get() : kA.get();
       const float* sv = alt ? ws_.alt_vals.get() : vA.get();
-      int tok = kt_.begin(s_);
+      tok = kt_.begin(s_);
       run_reduce_count(sk, n, tmp.get(), s_);
       kt_.end(tok, SPTC_K_REDUCE, 8.0 * (double)n, s_);
       const uint64_t out_n = read_total(tmp.get() + scan_tmp_count(n)).a;

@@ -90,36 +90,6 @@ struct RowRange {
   uint64_t lrow0;  // chunk-local row of ra (prefix of rows)
   uint64_t out0;   // chunk offset of ra's first product
 };
-// Product generation for the first pass of a product sort (the ESC
-// expansion fused into the sort, so products are never written unsorted).
-// Chunk offset q lies in one piece (pieces sorted by out0; no tile or group
-// straddles two): global product P = p0 + (q - out0); its owner j is the last
-// active X nonzero of [A0, A1) with act_prefix[j] <= P; y = act_ypos[j] + P -
-// act_prefix[j]; key = (lrow0 + act_row[j] - ra) << K | (ykeys[y] & column
-// mask); value = act_xval[j] * yvals[y].  The column mask is fmask, or in a
-// batch that of the row's instance (inst_kyf[instance], instances by
-// inst_R).  bad (optional) set: skip everything (stale batch operands).
-struct GenPiece {
-  uint64_t out0, p0, A0, A1, ra, lrow0;
-};
-struct ProductGen {
-  const uint64_t* act_prefix;
-  const uint64_t* act_ypos;
-  const float* act_xval;
-  const uint32_t* act_row;
-  const uint64_t* ykeys;
-  const float* yvals;
-  const GenPiece* pieces;
-  uint32_t npieces;
-  int K;
-  uint64_t fmask;
-  const uint64_t* inst_R;
-  const int* inst_kyf;
-  uint32_t B;
-  double* vsum;           // += FP64 sum of all product values
-  const uint32_t* bad;
-};
-
 // Row-segmented product sort of a chunk (keys local_row << col_bits | col,
 // grouped by row).  Rows holding more than half a tile of products are
 // segments of their own, sorted on the column bits by the multi-tile passes
@@ -137,18 +107,13 @@ struct RowSegPlan {
 };
 // Builds the plan (one host synchronization for the counts); tmp holds
 // scan_tmp_count(lrows) + 1 entries.  Reserves ws for the big-row passes.
-// The first row of every range starts a group (groups never straddle
-// ranges, so a group's products are one run of one piece).
 void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows,
                         const uint64_t* row_prod, RowSegPlan* plan, SortWorkspace& ws, U2* tmp,
                         hipStream_t s);
-// Sorts with a plan; returns true when the result ended in (k1, v1).  With
-// gen (host struct of device pointers, passed to the kernels by value) the
-// products are generated by the first pass and the local groups instead of
-// being read from (k0, v0).  Launches only (replayable).
+// Sorts with a plan; returns true when the result ended in (k1, v1).
+// Launches only (replayable).
 bool sort_row_segments(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, uint64_t* k0,
                        float* v0, uint64_t* k1, float* v1, int col_bits, hipStream_t s,
-                       KTimer* kt = nullptr, int cls = SPTC_K_SORT_PROD,
-                       const ProductGen* gen = nullptr);
+                       KTimer* kt = nullptr, int cls = SPTC_K_SORT_PROD);
 
 }  // namespace sptc
