@@ -383,7 +383,7 @@ def bench_nell2(args):
     for _ in range(max(1, args.steps)):
         t0 = time.perf_counter()
         res = sptc.SpTC(dS, dX, [2], [2], on_device=True, count_only=True,
-                        mem_budget_bytes=NELL2_CAP_BYTES, kernel_timing=False)
+                        mem_budget_bytes=NELL2_CAP_BYTES, kernel_timing=True)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
         log(f"step {len(times)}: {times[-1]:.2f} s, {res.info.nchunks} chunks")
@@ -405,6 +405,8 @@ def bench_nell2(args):
         "full_ir_nnz": ir_total,
         "full_time_s_extrapolated": ir_total / (res.ir_nnz / t),
         "value_sum": res.info.value_sum,
+        "kernels": {k: {"launches": n, "avg_ms": ms / n, "achieved_gbs": by / (ms * 1e-3) / 1e9}
+                    for k, (n, ms, by) in res.kernel_stats().items() if n and ms},
     }
     print(json.dumps(out))
 
