@@ -323,17 +323,18 @@ __global__ __launch_bounds__(kBlock) void k_breduce_emit(BReduceOp op, const U2*
   for (uint32_t i = threadIdx.x; i < nout; i += kBlock) {
     const uint64_t k = s_k[i];
     const uint64_t lrow = shr64(k, op.K);
-    uint32_t lo = 0, hi = op.b.npieces;  // last piece with lrow0 <= lrow
-    while (lo + 1 < hi) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (op.b.pieces[mid].lrow0 <= lrow) lo = mid; else hi = mid;
-    }
-    const BPiece pc = op.b.pieces[lo];
-    const uint64_t row = pc.ra + (lrow - pc.lrow0);
-    const BInst in = op.b.inst[find_seg(op.b.inst_R, op.b.B, row)];
-    op.out_row[obase + i] = (uint32_t)row;
+    const uint32_t row = op.b.map_row[lrow];
+    const uint32_t s = op.b.map_inst[lrow];
+    const BInst in = op.b.inst[s];
+    op.out_row[obase + i] = row - (uint32_t)op.b.inst_R[s];
     op.out_col[obase + i] = op.b.ytab[in.yt_off + (k & cm)];
     op.out_val[obase + i] = s_v[i];
+    if (!op.b.bounds) continue;
+    // first output of every instance in (previous output's instance, s]
+    int64_t sp = -1;
+    if (i > 0) sp = op.b.map_inst[shr64(s_k[i - 1], op.K)];
+    else if (base > 0) sp = op.b.map_inst[shr64(prev_key, op.K)];
+    for (int64_t t = sp + 1; t <= (int64_t)s; ++t) op.b.bounds[t] = obase + i;
   }
 }
 
@@ -371,30 +372,43 @@ __global__ __launch_bounds__(kBlock) void k_bdense_numeric(BDenseArgs a) {
   uint64_t r;
   BInst in;
   bdense_row(a, blockIdx.x, r, in);
+  const uint32_t g = find_seg(a.grp_rows, a.G, blockIdx.x);
+  // instance-local row id
   dense_row_numeric<EB>(a.d, a.d.row_act[r], a.d.row_act[r + 1], low_mask(in.kyf),
-                        a.ytab + in.yt_off, (uint32_t)r, a.d.out_off[blockIdx.x], s_sum, s_cmp,
-                        sm, &s_ja);
+                        a.ytab + in.yt_off, (uint32_t)(blockIdx.x - a.grp_rows[g]),
+                        a.d.out_off[blockIdx.x], s_sum, s_cmp, sm, &s_ja);
 }
 
-__global__ void k_row_bounds(const uint32_t* rows, uint64_t n, const uint64_t* q, uint32_t nq,
-                             uint64_t* out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nq) return;
-  const uint64_t v = q[i];
-  uint64_t lo = 0, hi = n;
-  while (lo < hi) {
-    const uint64_t mid = (lo + hi) >> 1;
-    if ((uint64_t)rows[mid] < v) lo = mid + 1; else hi = mid;
+__global__ __launch_bounds__(kBlock) void k_chunk_row_maps(const BPiece* pieces,
+                                                           uint32_t npieces, uint64_t lrows,
+                                                           const uint64_t* inst_R, uint32_t B,
+                                                           uint32_t* map_row,
+                                                           uint32_t* map_inst) {
+  uint64_t l = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (; l < lrows; l += stride) {
+    uint32_t lo = 0, hi = npieces;  // last piece with lrow0 <= l
+    while (lo + 1 < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (pieces[mid].lrow0 <= l) lo = mid; else hi = mid;
+    }
+    const uint64_t g = pieces[lo].ra + (l - pieces[lo].lrow0);
+    map_row[l] = (uint32_t)g;
+    map_inst[l] = find_seg(inst_R, B, g);
   }
-  out[i] = lo;
 }
 
-__global__ void k_localize_rows(uint32_t* rows, uint64_t n, const uint64_t* bounds,
-                                const uint64_t* inst_R, uint32_t B, const uint32_t* bad) {
-  if (*bad) return;
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) rows[i] -= (uint32_t)inst_R[find_seg(bounds, B, i)];
+__global__ void k_dense_bounds(const uint64_t* grp_rows, const uint32_t* grp_inst, uint32_t G,
+                               const uint64_t* out_off, uint64_t nnz, uint32_t B,
+                               uint64_t* bounds, const uint32_t* bad) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > B || *bad) return;
+  uint32_t lo = 0, hi = G;  // first group instance >= t
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (grp_inst[mid] < t) lo = mid + 1; else hi = mid;
+  }
+  bounds[t] = lo < G ? out_off[grp_rows[lo]] : nnz;
 }
 
 __global__ void k_fix_empty(uint64_t* arr, const uint64_t* seg_off, uint32_t nseg) {
@@ -531,18 +545,19 @@ void launch_bdense_numeric(const BDenseArgs& a, int eb, uint64_t rows, hipStream
   }
 }
 
-void launch_row_bounds(const uint32_t* rows, uint64_t n, const uint64_t* q, uint32_t nq,
-                       uint64_t* out, hipStream_t s) {
-  if (!nq) return;
-  hipLaunchKernelGGL(k_row_bounds, dim3(ceil_div(nq, 256)), dim3(256), 0, s, rows, n, q, nq, out);
+void launch_chunk_row_maps(const BPiece* pieces, uint32_t npieces, uint64_t lrows,
+                           const uint64_t* inst_R, uint32_t B, uint32_t* map_row,
+                           uint32_t* map_inst, hipStream_t s) {
+  if (!lrows) return;
+  hipLaunchKernelGGL(k_chunk_row_maps, dim3(grid_for(lrows)), dim3(kBlock), 0, s, pieces,
+                     npieces, lrows, inst_R, B, map_row, map_inst);
 }
 
-void launch_localize_rows(uint32_t* rows, uint64_t n, const uint64_t* bounds,
-                          const uint64_t* inst_R, uint32_t B, const uint32_t* bad,
-                          hipStream_t s) {
-  if (!n) return;
-  hipLaunchKernelGGL(k_localize_rows, dim3(grid_for(n)), dim3(kBlock), 0, s, rows, n, bounds,
-                     inst_R, B, bad);
+void launch_dense_bounds(const uint64_t* grp_rows, const uint32_t* grp_inst, uint32_t G,
+                         const uint64_t* out_off, uint64_t nnz, uint32_t B, uint64_t* bounds,
+                         const uint32_t* bad, hipStream_t s) {
+  hipLaunchKernelGGL(k_dense_bounds, dim3(ceil_div(B + 1, 256)), dim3(256), 0, s, grp_rows,
+                     grp_inst, G, out_off, nnz, B, bounds, bad);
 }
 
 void launch_fix_empty(uint64_t* arr, const uint64_t* seg_off, uint32_t nseg, hipStream_t s) {

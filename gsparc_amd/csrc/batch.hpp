@@ -114,17 +114,27 @@ struct BExpandArgs {
 };
 void launch_bexpand(const BExpandArgs& a, hipStream_t s);
 
-// Reduce-by-key emit with per-instance column tables; rows written as
-// global row ids.
+// Reduce-by-key emit with per-instance column tables.  Outputs are written with instance-local rows; bounds[s] (s <= B, filled
+// with the output count beforehand) receives the first output of instance s.
 struct BReduceArgs {
-  const BPiece* pieces;
-  uint32_t npieces;
+  const uint32_t* map_row;   // chunk-local row -> global row
+  const uint32_t* map_inst;  // chunk-local row -> instance
   const uint64_t* inst_R;
   const BInst* inst;
   const uint64_t* ytab;
   uint32_t B;
+  uint64_t* bounds;
   const uint32_t* bad;
 };
+// Chunk-local row -> (global row, instance) for the rows of a chunk's pieces.
+void launch_chunk_row_maps(const BPiece* pieces, uint32_t npieces, uint64_t lrows,
+                           const uint64_t* inst_R, uint32_t B, uint32_t* map_row,
+                           uint32_t* map_inst, hipStream_t s);
+// Per-instance output starts of a dense class buffer (instances not in the
+// class get the next class instance's start, or nnz).
+void launch_dense_bounds(const uint64_t* grp_rows, const uint32_t* grp_inst, uint32_t G,
+                         const uint64_t* out_off, uint64_t nnz, uint32_t B, uint64_t* bounds,
+                         const uint32_t* bad, hipStream_t s);
 void run_breduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32_t* out_row,
                       uint64_t* out_col, float* out_val, int K, const BReduceArgs& b,
                       const U2* tmp, hipStream_t s);
@@ -143,14 +153,6 @@ struct BDenseArgs {
 void launch_bdense_count(const BDenseArgs& a, int eb, uint64_t rows, hipStream_t s);
 void launch_bdense_numeric(const BDenseArgs& a, int eb, uint64_t rows, hipStream_t s);
 
-// For each of nq sorted query rows, lower_bound in a chunk's row array.
-void launch_row_bounds(const uint32_t* rows, uint64_t n, const uint64_t* q, uint32_t nq,
-                       uint64_t* out, hipStream_t s);
-// Global -> instance-local rows: entry i of a buffer whose instance slices
-// start at bounds[0..B] gets row -= inst_R[instance].
-void launch_localize_rows(uint32_t* rows, uint64_t n, const uint64_t* bounds,
-                          const uint64_t* inst_R, uint32_t B, const uint32_t* bad,
-                          hipStream_t s);
 // arr[s] of every empty segment s (seg_off[s] == seg_off[s+1]) takes the
 // value of the next segment, so per-segment starts are monotone with
 // arr[nseg] = total.

@@ -524,6 +524,11 @@ sptc_status BatchEngine::run(hipStream_t s, bool P) {
     tok = kt ? kt->begin(s) : -1;
     launch_bdense_numeric(da, c.eb, c.rows, s);
     if (kt) kt->end(tok, SPTC_K_DENSE, 12.0 * (double)c.ir + 16.0 * (double)ob.nnz, s);
+    if (P) {
+      ob.bounds.reset(B + 1);
+      launch_dense_bounds(c.grp_rows.get(), c.grp_inst.get(), c.G, c.out_off.get(), ob.nnz, B,
+                          ob.bounds.get(), bad_.get(), s);
+    }
   }
 
   // ---- (d)+(e) ESC chunks ------------------------------------------------------
@@ -585,9 +590,20 @@ sptc_status BatchEngine::run(hipStream_t s, bool P) {
       outs_.push_back(std::move(ob));
     }
     BOut& ob = outs_[c.out];
+    if (P) {
+      // chunk-local row -> (global row, instance), and the per-instance
+      // output starts that the emit below fills in
+      c.map_row.reset(c.lrows);
+      c.map_inst.reset(c.lrows);
+      launch_chunk_row_maps(c.pieces.get(), (uint32_t)c.hp.size(), c.lrows, iR, B,
+                            c.map_row.get(), c.map_inst.get(), s);
+      ob.bounds.reset(B + 1);
+      fill_u64(ob.bounds.get(), ob.nnz, B + 1, s);
+    }
     BReduceArgs rb;
-    rb.pieces = c.pieces.get();
-    rb.npieces = (uint32_t)c.hp.size();
+    rb.map_row = c.map_row.get();
+    rb.map_inst = c.map_inst.get();
+    rb.bounds = P ? ob.bounds.get() : nullptr;
     rb.inst_R = iR;
     rb.inst = inst_.get();
     rb.ytab = ytab_.get();
@@ -599,18 +615,12 @@ sptc_status BatchEngine::run(hipStream_t s, bool P) {
     if (kt) kt->end(tok, SPTC_K_REDUCE, 12.0 * (double)c.n + 16.0 * (double)ob.nnz, s);
   }
 
-  // ---- per-instance slices, instance-local rows ---------------------------------
-  for (size_t k = 0; k < outs_.size(); ++k) {
-    BOut& ob = outs_[k];
-    if (P) {
-      ob.bounds.reset(B + 1);
-      launch_row_bounds(ob.row.get(), ob.nnz, iR, B + 1, ob.bounds.get(), s);
+  // ---- per-instance slices (rows are already instance-local) ---------------------
+  if (P) {
+    for (BOut& ob : outs_) {
       ob.hb.assign(B + 1, 0);
       readback(ob.hb.data(), ob.bounds.get(), (B + 1) * 8, s);
     }
-    launch_localize_rows(ob.row.get(), ob.nnz, ob.bounds.get(), iR, B, bad_.get(), s);
-  }
-  if (P) {
     uint64_t out_nnz = 0;
     auto add_slice = [&](uint32_t i, int k, uint64_t r0, uint64_t r1) {
       const BOut& ob = outs_[k];

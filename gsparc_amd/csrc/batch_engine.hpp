@@ -22,7 +22,7 @@ struct BOut {  // one output buffer: a dense slot-width class or an ESC chunk
   DevBuf<uint32_t> row;
   DevBuf<uint64_t> col;
   DevBuf<float> val;
-  DevBuf<uint64_t> bounds;     // [B+1] first entry of every instance
+  DevBuf<uint64_t> bounds;     // [B+1] first entry of every instance (plan only)
   std::vector<uint64_t> hb;
 };
 
@@ -47,6 +47,7 @@ struct EscChunk {
   std::vector<RowRange> hr;  // the pieces as row ranges (row-segmented sort)
   DevBuf<BPiece> pieces;
   DevBuf<BETile> tiles;
+  DevBuf<uint32_t> map_row, map_inst;  // chunk-local row -> global row, instance
   bool rowsort = false;
   RowSegPlan rseg;
   int out = -1;
