@@ -228,6 +228,7 @@ __global__ __launch_bounds__(kBlock) void k_bexpand(BExpandArgs a) {
     __syncthreads();
     uint32_t klo = 0, kc = ~0u;
     uint64_t rbits = 0, cmask = 0;
+    bool big = false;
 #pragma unroll
     for (int e = 0; e < kBExpandItems; ++e) {
       const uint32_t q = (uint32_t)e * kBlock + threadIdx.x;
@@ -244,12 +245,21 @@ __global__ __launch_bounds__(kBlock) void k_bexpand(BExpandArgs a) {
         if (k != kc) {  // new owner: its row's key bits and column mask
           kc = k;
           const uint32_t row = a.act_row[j];
-          rbits = shl64(pc.lrow0 + (row - pc.ra), a.K);
+          const uint64_t lr = pc.lrow0 + (row - pc.ra);
+          rbits = shl64(lr, a.K);
           cmask = low_mask(a.inst[find_seg(a.inst_R, a.B, row)].kyf);
+          big = a.row_big && a.row_big[lr] != 0;
         }
         const uint64_t y = a.act_ypos[j] + (p - s_pref[k]);
-        a.out_keys[tl.out0 + q] = rbits | (a.ykeys[y] & cmask);
-        a.out_vals[tl.out0 + q] = a.act_xval[j] * a.yvals[y];
+        const uint64_t col = a.ykeys[y] & cmask;
+        const float v = a.act_xval[j] * a.yvals[y];
+        if (big) {
+          a.big_col[tl.out0 + q] = (uint32_t)col;
+          a.big_val[tl.out0 + q] = v;
+        } else {
+          a.out_keys[tl.out0 + q] = rbits | col;
+          a.out_vals[tl.out0 + q] = v;
+        }
       }
     }
   }

@@ -110,6 +110,9 @@ struct BExpandArgs {
   uint32_t B;
   uint64_t* out_keys;
   float* out_vals;
+  const uint64_t* row_big;  // chunk-local row -> big segment (products to big_col/big_val)
+  uint32_t* big_col;
+  float* big_val;
   const uint32_t* bad;     // nonzero: operands changed since planning, skip
 };
 void launch_bexpand(const BExpandArgs& a, hipStream_t s);

@@ -355,13 +355,17 @@ sptc_status BatchEngine::build_chunks(hipStream_t s) {
   for (EscChunk& c : esc_) {
     c.bits = mode_bits(c.lrows) + c.K;
     c.rb = o_.radix_bits ? o_.radix_bits : choose_radix_bits(c.bits);
-    if (c.K > 0 && c.K < c.bits && prefer_row_sort()) {
+    if (c.K > 0 && c.K < c.bits && c.K <= 32 && prefer_row_sort()) {
       DevBuf<RowRange> rr(c.hr.size());
       SPTC_HIP(hipMemcpyAsync(rr.get(), c.hr.data(), c.hr.size() * sizeof(RowRange),
                               hipMemcpyHostToDevice, s));
-      build_row_segments(rr.get(), (uint32_t)c.hr.size(), c.lrows, row_prod_.get(), &c.rseg,
-                         ws_, tmp_.get(), s);
+      build_row_segments(rr.get(), (uint32_t)c.hr.size(), c.lrows, row_prod_.get(), c.K,
+                         &c.rseg, ws_, tmp_.get(), s);
       c.rowsort = true;
+      // keep the per-row big-segment flags the expansion reads, release the rest
+      c.row_big.reset(c.lrows);
+      SPTC_HIP(hipMemcpyAsync(c.row_big.get(), c.rseg.row_big(), c.lrows * 8,
+                              hipMemcpyDeviceToDevice, s));
       SPTC_HIP(hipStreamSynchronize(s));  // rr and the scratch are released below
       c.rseg.scratch.release();
     }
@@ -554,14 +558,21 @@ sptc_status BatchEngine::run(hipStream_t s, bool P) {
     ea.B = B;
     ea.out_keys = kA_.get();
     ea.out_vals = vA_.get();
+    if (c.rowsort) {
+      const NarrowTarget nt =
+          narrow_target(c.rseg, c.n, vA_.get(), ws_.alt_keys.get(), ws_.alt_vals.get());
+      ea.row_big = c.row_big.get();
+      ea.big_col = nt.col;
+      ea.big_val = nt.val;
+    }
     ea.bad = bad_.get();
     int tok = kt ? kt->begin(s) : -1;
     launch_bexpand(ea, s);
     if (kt) kt->end(tok, SPTC_K_EXPAND, 24.0 * (double)c.n, s);
-    bool alt;
+    bool alt = false;
     if (c.rowsort) {
-      alt = sort_row_segments(ws_, c.rseg, c.n, kA_.get(), vA_.get(), ws_.alt_keys.get(),
-                              ws_.alt_vals.get(), c.K, s, kt, SPTC_K_SORT_PROD);
+      sort_row_segments(ws_, c.rseg, c.n, kA_.get(), vA_.get(), ws_.alt_keys.get(),
+                        ws_.alt_vals.get(), s, kt, SPTC_K_SORT_PROD);
     } else {
       ws_.ensure(c.n, c.rb, s);
       const int passes = radix_passes(c.bits, c.rb);

@@ -50,6 +50,7 @@ struct EscChunk {
   DevBuf<uint32_t> map_row, map_inst;  // chunk-local row -> global row, instance
   bool rowsort = false;
   RowSegPlan rseg;
+  DevBuf<uint64_t> row_big;  // chunk-local row -> big segment (row-segmented sort)
   int out = -1;
 };
 

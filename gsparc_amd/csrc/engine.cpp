@@ -474,6 +474,19 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       const int passes = radix_passes(bits, rb);
       ws_.ensure(n, rb, s_);
       ws_.prepare(passes, s_);
+      // row-segmented sort when the column bits leave row bits to skip
+      const bool rowsort = kyf > 0 && kyf < bits && kyf <= 32 && prefer_row_sort();
+      if (rowsort) {
+        const uint64_t lrows = r1c - r0c;
+        RowRange* h = reinterpret_cast<RowRange*>(host_u64_ + 4);
+        h->ra = r0c;
+        h->rows = lrows;
+        h->lrow0 = 0;
+        h->out0 = 0;
+        SPTC_HIP(hipMemcpyAsync(rrange.get(), h, sizeof(RowRange), hipMemcpyHostToDevice, s_));
+        build_row_segments(rrange.get(), 1, lrows, row_prod.get(), kyf, &rseg, ws_, tmp.get(),
+                           s_);
+      }
       ExpandArgs ea;
       ea.act_prefix = act_prefix.get();
       ea.act_ypos = act_ypos.get();
@@ -490,6 +503,16 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       ea.fmask = fmask;
       ea.out_keys = kA.get();
       ea.out_vals = vA.get();
+      ea.row_big = nullptr;
+      ea.big_col = nullptr;
+      ea.big_val = nullptr;
+      if (rowsort) {
+        const NarrowTarget nt =
+            narrow_target(rseg, n, vA.get(), ws_.alt_keys.get(), ws_.alt_vals.get());
+        ea.row_big = rseg.row_big();
+        ea.big_col = nt.col;
+        ea.big_val = nt.val;
+      }
       ea.hist = ws_.hist.get();
       const bool hist = sort_needs_histogram();
       ea.passes = hist && passes <= kMaxFusedPasses ? passes : 0;
@@ -498,21 +521,11 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       launch_expand(ea, rb, s_);
       // product (key, value) written, matched Y (key, value) read
       kt_.end(tok, SPTC_K_EXPAND, 24.0 * (double)n, s_);
-      bool sorted = false, alt = false;
-      if (kyf > 0 && kyf < bits && prefer_row_sort()) {
-        const uint64_t lrows = r1c - r0c;
-        RowRange* h = reinterpret_cast<RowRange*>(host_u64_ + 4);
-        h->ra = r0c;
-        h->rows = lrows;
-        h->lrow0 = 0;
-        h->out0 = 0;
-        SPTC_HIP(hipMemcpyAsync(rrange.get(), h, sizeof(RowRange), hipMemcpyHostToDevice, s_));
-        build_row_segments(rrange.get(), 1, lrows, row_prod.get(), &rseg, ws_, tmp.get(), s_);
-        alt = sort_row_segments(ws_, rseg, n, kA.get(), vA.get(), ws_.alt_keys.get(),
-                                ws_.alt_vals.get(), kyf, s_, &kt_, SPTC_K_SORT_PROD);
-        sorted = true;
-      }
-      if (!sorted) {
+      bool alt = false;
+      if (rowsort) {
+        sort_row_segments(ws_, rseg, n, kA.get(), vA.get(), ws_.alt_keys.get(),
+                          ws_.alt_vals.get(), s_, &kt_, SPTC_K_SORT_PROD);
+      } else {
         ws_.ensure(n, rb, s_);
         if (hist && passes > kMaxFusedPasses) histogram(ws_, kA.get(), n, passes, s_);
         alt = sort_after_histogram(ws_, kA.get(), vA.get(), n, bits, s_, &kt_, SPTC_K_SORT_PROD);

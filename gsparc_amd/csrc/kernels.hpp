@@ -53,6 +53,11 @@ struct ExpandArgs {
   uint64_t fmask;
   uint64_t* out_keys;
   float* out_vals;
+  // row-segmented sort: products of rows with row_big[row - r0] != 0 go to
+  // (big_col, big_val) as 32-bit columns instead (row_big null: none)
+  const uint64_t* row_big;
+  uint32_t* big_col;
+  float* big_val;
   uint64_t* hist;
   int passes;
   double* vsum;

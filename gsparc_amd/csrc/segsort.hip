@@ -11,8 +11,9 @@ namespace {
 
 // ---- segmented reduce-then-scan radix pass (8-bit digits, 512 x 12 tiles) --
 
-__global__ __launch_bounds__(kSegNT) void k_seg_upsweep(const uint64_t* __restrict__ kin,
-                                                        int shift, const SegTile* __restrict__ tiles,
+template <typename KT>
+__global__ __launch_bounds__(kSegNT) void k_seg_upsweep(const KT* __restrict__ kin, int shift,
+                                                        const SegTile* __restrict__ tiles,
                                                         uint32_t* __restrict__ cnt) {
   constexpr int R = 1 << kSegRB;
   __shared__ uint32_t sh[R];
@@ -24,15 +25,18 @@ __global__ __launch_bounds__(kSegNT) void k_seg_upsweep(const uint64_t* __restri
   for (int i = 0; i < kSegIPT; ++i) {
     const uint64_t idx = t.start + (uint64_t)i * kSegNT + threadIdx.x;
     const bool valid = idx < end;
-    const uint64_t k = valid ? kin[idx] : 0;
+    const uint64_t k = valid ? (uint64_t)kin[idx] : 0;
     hist_add_wave<kSegRB>(sh, k >> shift, valid, 1);
   }
   __syncthreads();
   for (int d = threadIdx.x; d < R; d += kSegNT) cnt[(uint64_t)blockIdx.x * R + d] = sh[d];
 }
 
+// KI -> KO: 64 -> 64 (operand sorts), 32 -> 32 (column passes of big rows),
+// 32 -> 64 (their last pass, which ORs in the tile's row bits).
+template <typename KI, typename KO>
 __global__ __launch_bounds__(kSegNT) void k_seg_downsweep(
-    const uint64_t* __restrict__ kin, const float* __restrict__ vin, uint64_t* __restrict__ kout,
+    const KI* __restrict__ kin, const float* __restrict__ vin, KO* __restrict__ kout,
     float* __restrict__ vout, int shift, const SegTile* __restrict__ tiles,
     const uint32_t* __restrict__ offs) {
   constexpr int R = 1 << kSegRB;
@@ -44,7 +48,7 @@ __global__ __launch_bounds__(kSegNT) void k_seg_downsweep(
   __syncthreads();
   const uint64_t end = t.start + t.count;
   const uint64_t wbase = t.start + (uint64_t)(tid / kWave) * kWave * kSegIPT;
-  uint64_t k[kSegIPT];
+  KI k[kSegIPT];
   float v[kSegIPT];
   uint32_t pos[kSegIPT];
   tile_load_rank<kSegRB, kSegIPT, kSegNT>(sm, kin, vin, end, shift, wbase, k, v, pos);
@@ -52,8 +56,9 @@ __global__ __launch_bounds__(kSegNT) void k_seg_downsweep(
   for (int d = tid; d < R; d += kSegNT)
     sm.dstart[d] = (uint16_t)wave_prefixes<kSegRB, kSegIPT, kSegNT>(sm, d);
   __syncthreads();
+  const KO hi = sizeof(KO) > sizeof(KI) ? (KO)t.hi : (KO)0;
   tile_stage_scatter<kSegRB, kSegIPT, kSegNT>(sm, end, shift, t.start, wbase, k, v, pos, kout,
-                                              vout);
+                                              vout, hi);
 }
 
 // Per segment (one workgroup, thread d = digit d): absolute destination of
@@ -127,7 +132,7 @@ __global__ __launch_bounds__(kBlock) void k_rowseg_heads(uint64_t lrows, const u
 }
 
 __global__ __launch_bounds__(kBlock) void k_rowseg_emit(
-    const RowRange* ranges, uint32_t nr, uint64_t lrows, const uint64_t* row_prod,
+    const RowRange* ranges, uint32_t nr, uint64_t lrows, const uint64_t* row_prod, int col_bits,
     const uint64_t* big_t, const uint64_t* bt, const uint64_t* bs, const uint64_t* head,
     const uint64_t* hs, LocalSeg* local, SegTile* big, SegDesc* bigseg) {
   uint64_t l = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -149,6 +154,7 @@ __global__ __launch_bounds__(kBlock) void k_rowseg_emit(
         d.count = (uint32_t)(n - t * kSegTile < kSegTile ? n - t * kSegTile : kSegTile);
         d.obase = (1ull << kSegRB) * o + t;
         d.ostride = (uint32_t)nt;
+        d.hi = shl64(l, col_bits);
         big[o + t] = d;
       }
     } else {
@@ -169,11 +175,8 @@ __global__ void k_zero_local(LocalSeg* local, uint64_t n) {
 }
 
 // One workgroup per small group: load, LSD passes over the bits that vary
-// inside the group entirely in LDS (stable), store.
-__global__ __launch_bounds__(kSegNT) void k_local_sort(const uint64_t* __restrict__ kin,
-                                                       const float* __restrict__ vin,
-                                                       uint64_t* __restrict__ kout,
-                                                       float* __restrict__ vout,
+// inside the group entirely in LDS (stable), store in place.
+__global__ __launch_bounds__(kSegNT) void k_local_sort(uint64_t* keys, float* vals,
                                                        const LocalSeg* __restrict__ segs) {
   constexpr int R = 1 << kSegRB;
   __shared__ TileSmem<kSegRB, kSegIPT, kSegNT> sm;
@@ -188,14 +191,14 @@ __global__ __launch_bounds__(kSegNT) void k_local_sort(const uint64_t* __restric
   uint64_t k[kSegIPT];
   float v[kSegIPT];
   uint32_t pos[kSegIPT];
-  const uint64_t k0 = n ? kin[sg.start] : 0;
+  const uint64_t k0 = n ? keys[sg.start] : 0;
   uint64_t vary = 0;
 #pragma unroll
   for (int i = 0; i < kSegIPT; ++i) {
     const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
     const bool valid = idx < n;
-    k[i] = valid ? kin[sg.start + idx] : 0;
-    v[i] = valid ? vin[sg.start + idx] : 0.f;
+    k[i] = valid ? keys[sg.start + idx] : 0;
+    v[i] = valid ? vals[sg.start + idx] : 0.f;
     if (valid) vary |= k[i] ^ k0;
   }
   for (int off = kWave / 2; off > 0; off >>= 1) vary |= __shfl_xor(vary, off, kWave);
@@ -241,8 +244,8 @@ __global__ __launch_bounds__(kSegNT) void k_local_sort(const uint64_t* __restric
   for (int i = 0; i < kSegIPT; ++i) {
     const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
     if (idx < n) {
-      kout[sg.start + idx] = k[i];
-      vout[sg.start + idx] = v[i];
+      keys[sg.start + idx] = k[i];
+      vals[sg.start + idx] = v[i];
     }
   }
 }
@@ -264,6 +267,7 @@ void SegSortPlan::build(const std::vector<uint64_t>& seg_off, hipStream_t s) {
       d.count = (uint32_t)std::min<uint64_t>(kSegTile, len - t * kSegTile);
       d.obase = (1ull << kSegRB) * t0 + t;
       d.ostride = (uint32_t)nt;
+      d.hi = 0;
       tv.push_back(d);
     }
   }
@@ -313,14 +317,14 @@ bool segmented_sort(SortWorkspace& ws, const SegTile* tiles, uint64_t ntiles,
   for (int p = 0; p < passes; ++p) {
     const int shift = p * kSegRB;
     int tok = kt ? kt->begin(s) : -1;
-    hipLaunchKernelGGL(k_seg_upsweep, dim3((uint32_t)ntiles), dim3(kSegNT), 0, s, ka, shift,
-                       tiles, ws.rts_cnt.get());
+    hipLaunchKernelGGL(k_seg_upsweep<uint64_t>, dim3((uint32_t)ntiles), dim3(kSegNT), 0, s, ka,
+                       shift, tiles, ws.rts_cnt.get());
     hipLaunchKernelGGL(k_seg_scan, dim3((uint32_t)nsegs), dim3(1 << kSegRB), 0, s, segs,
                        ws.rts_cnt.get(), ws.rts_off.get());
     if (kt) kt->end(tok, SPTC_K_SORT_COUNT, 8.0 * (double)n + 16.0 * (double)m, s);
     tok = kt ? kt->begin(s) : -1;
-    hipLaunchKernelGGL(k_seg_downsweep, dim3((uint32_t)ntiles), dim3(kSegNT), 0, s, ka, va, kb,
-                       vb, shift, tiles, ws.rts_off.get());
+    hipLaunchKernelGGL((k_seg_downsweep<uint64_t, uint64_t>), dim3((uint32_t)ntiles),
+                       dim3(kSegNT), 0, s, ka, va, kb, vb, shift, tiles, ws.rts_off.get());
     if (kt) kt->end(tok, cls, 24.0 * (double)n, s);
     std::swap(ka, kb);
     std::swap(va, vb);
@@ -335,9 +339,12 @@ bool segmented_sort(SortWorkspace& ws, const SegSortPlan& plan, uint64_t* k0, fl
 }
 
 void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows,
-                        const uint64_t* row_prod, RowSegPlan* plan, SortWorkspace& ws, U2* tmp,
-                        hipStream_t s) {
+                        const uint64_t* row_prod, int col_bits, RowSegPlan* plan,
+                        SortWorkspace& ws, U2* tmp, hipStream_t s) {
+  if (col_bits > 32) throw std::logic_error("row-segmented sort of more than 32 column bits");
   plan->nlocal = plan->nbig_tiles = plan->nbig_rows = plan->nsmall = 0;
+  plan->lrows = lrows;
+  plan->col_bits = col_bits;
   if (!lrows) return;
   const uint64_t m = lrows + 1;
   if (plan->scratch.size() < 8 * m) plan->scratch.reset(8 * m);
@@ -377,25 +384,66 @@ void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows
     hipLaunchKernelGGL(k_zero_local, dim3((uint32_t)ceil_div(plan->nlocal, 256)), dim3(256), 0,
                        s, plan->local.get(), plan->nlocal);
   hipLaunchKernelGGL(k_rowseg_emit, dim3(grid), dim3(kBlock), 0, s, ranges, nranges, lrows,
-                     row_prod, big_t, bt, bs, head, hs, plan->local.get(), plan->big.get(),
+                     row_prod, col_bits, big_t, bt, bs, head, hs, plan->local.get(), plan->big.get(),
                      plan->bigseg.get());
   if (plan->nbig_tiles) segmented_sort_reserve(ws, plan->nbig_tiles, s);
 }
 
-bool sort_row_segments(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, uint64_t* k0,
-                       float* v0, uint64_t* k1, float* v1, int col_bits, hipStream_t s,
-                       KTimer* kt, int cls) {
-  bool alt = false;
-  if (plan.nbig_tiles)
-    alt = segmented_sort(ws, plan.big.get(), plan.nbig_tiles, plan.bigseg.get(), plan.nbig_rows,
-                         n - plan.nsmall, k0, v0, k1, v1, col_bits, s, kt, cls);
+static int narrow_passes(int col_bits) { return std::max(1, radix_passes(col_bits, kSegRB)); }
+
+// Column-array sets of the big-row passes: set 0 = (k1 as 32-bit words
+// [0, n), v1), set 1 = (k1 words [n, 2n), v).  The passes alternate sets and
+// the last one reads set 0 and writes (k, v), so the expansion starts in set
+// 0 for an odd pass count and in set 1 for an even one.  Set 1 shares v with
+// the groups, at the big rows' positions only.
+NarrowTarget narrow_target(const RowSegPlan& plan, uint64_t n, float* v, uint64_t* k1,
+                           float* v1) {
+  uint32_t* c = reinterpret_cast<uint32_t*>(k1);
+  if (narrow_passes(plan.col_bits) & 1) return NarrowTarget{c, v1};
+  return NarrowTarget{c + n, v};
+}
+
+void sort_row_segments(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, uint64_t* k,
+                       float* v, uint64_t* k1, float* v1, hipStream_t s, KTimer* kt, int cls) {
+  if (plan.nbig_tiles) {
+    const uint64_t ntiles = plan.nbig_tiles;
+    const uint64_t m = ntiles << kSegRB;
+    if (ws.rts_cnt.size() < m) throw std::logic_error("segmented sort workspace not reserved");
+    const uint64_t nb = n - plan.nsmall;  // big-row products
+    uint32_t* col[2] = {reinterpret_cast<uint32_t*>(k1), reinterpret_cast<uint32_t*>(k1) + n};
+    float* val[2] = {v1, v};
+    const int passes = narrow_passes(plan.col_bits);
+    int cur = (passes & 1) ? 0 : 1;
+    const SegTile* tiles = plan.big.get();
+    for (int p = 0; p < passes; ++p) {
+      const int shift = p * kSegRB;
+      int tok = kt ? kt->begin(s) : -1;
+      hipLaunchKernelGGL(k_seg_upsweep<uint32_t>, dim3((uint32_t)ntiles), dim3(kSegNT), 0, s,
+                         col[cur], shift, tiles, ws.rts_cnt.get());
+      hipLaunchKernelGGL(k_seg_scan, dim3((uint32_t)plan.nbig_rows), dim3(1 << kSegRB), 0, s,
+                         plan.bigseg.get(), ws.rts_cnt.get(), ws.rts_off.get());
+      if (kt) kt->end(tok, SPTC_K_SORT_COUNT, 4.0 * (double)nb + 16.0 * (double)m, s);
+      tok = kt ? kt->begin(s) : -1;
+      if (p + 1 < passes) {
+        hipLaunchKernelGGL((k_seg_downsweep<uint32_t, uint32_t>), dim3((uint32_t)ntiles),
+                           dim3(kSegNT), 0, s, col[cur], val[cur], col[1 - cur], val[1 - cur],
+                           shift, tiles, ws.rts_off.get());
+        if (kt) kt->end(tok, cls, 16.0 * (double)nb, s);
+        cur = 1 - cur;
+      } else {
+        hipLaunchKernelGGL((k_seg_downsweep<uint32_t, uint64_t>), dim3((uint32_t)ntiles),
+                           dim3(kSegNT), 0, s, col[cur], val[cur], k, v, shift, tiles,
+                           ws.rts_off.get());
+        if (kt) kt->end(tok, cls, 20.0 * (double)nb, s);
+      }
+    }
+  }
   if (plan.nlocal) {
     int tok = kt ? kt->begin(s) : -1;
-    hipLaunchKernelGGL(k_local_sort, dim3((uint32_t)plan.nlocal), dim3(kSegNT), 0, s, k0, v0,
-                       alt ? k1 : k0, alt ? v1 : v0, plan.local.get());
+    hipLaunchKernelGGL(k_local_sort, dim3((uint32_t)plan.nlocal), dim3(kSegNT), 0, s, k, v,
+                       plan.local.get());
     if (kt) kt->end(tok, SPTC_K_SORT_LOCAL, 24.0 * (double)plan.nsmall, s);
   }
-  return alt;
 }
 
 }  // namespace sptc

@@ -25,6 +25,7 @@ struct SegTile {
   uint64_t obase;    // offset-matrix index of digit 0 (256 * first tile + index in segment)
   uint32_t count;    // keys in the tile
   uint32_t ostride;  // tiles in the segment
+  uint64_t hi;       // key bits OR'ed in when a pass widens 32-bit keys (row << col_bits)
 };
 // One segment: its tiles are first_tile .. first_tile + ntiles - 1; one
 // workgroup per segment turns their counts into absolute destinations
@@ -90,30 +91,47 @@ struct RowRange {
   uint64_t lrow0;  // chunk-local row of ra (prefix of rows)
   uint64_t out0;   // chunk offset of ra's first product
 };
-// Row-segmented product sort of a chunk (keys local_row << col_bits | col,
-// grouped by row).  Rows holding more than half a tile of products are
-// segments of their own, sorted on the column bits by the multi-tile passes
-// above; runs of smaller rows are cut into groups of at most one tile
-// (a group starts where the running total of small-row products crosses a
+// Row-segmented product sort of a chunk, products grouped by row, sorted
+// result as keys local_row << col_bits | col (col_bits <= 32) in (k, v).
+// Rows holding more than half a tile of products are segments of their own:
+// the expansion writes their products narrow -- 32-bit column + value, the
+// row implied by the segment -- and the multi-tile passes above sort them on
+// the column bits, the last pass widening to full keys.  Runs of smaller
+// rows are written as full keys and cut into groups of at most one tile (a
+// group starts where the running total of small-row products crosses a
 // multiple of half a tile), each sorted on its varying key bits entirely in
-// one workgroup's LDS.  Every product is read and written once for the
-// groups, ceil(col_bits / 8) times for the big rows.
+// one workgroup's LDS, in place.  Every product is read and written once for
+// the groups; max(1, ceil(col_bits / 8)) times for the big rows, at 8 bytes
+// per product except the last pass's 12-byte writes.
 struct RowSegPlan {
   uint64_t nlocal = 0, nbig_tiles = 0, nbig_rows = 0, nsmall = 0;  // nsmall: products in groups
+  uint64_t lrows = 0;
+  int col_bits = 0;
   DevBuf<LocalSeg> local;
   DevBuf<SegTile> big;
   DevBuf<SegDesc> bigseg;
   DevBuf<uint64_t> scratch;  // 8 x (rows + 1) per-row values and scans
+  // per chunk-local row: nonzero when the row is a big (narrow) segment
+  const uint64_t* row_big() const { return scratch.get() + (lrows + 1); }
 };
 // Builds the plan (one host synchronization for the counts); tmp holds
 // scan_tmp_count(lrows) + 1 entries.  Reserves ws for the big-row passes.
 void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows,
-                        const uint64_t* row_prod, RowSegPlan* plan, SortWorkspace& ws, U2* tmp,
-                        hipStream_t s);
-// Sorts with a plan; returns true when the result ended in (k1, v1).
+                        const uint64_t* row_prod, int col_bits, RowSegPlan* plan,
+                        SortWorkspace& ws, U2* tmp, hipStream_t s);
+// Where the expansion writes big-row products of an n-product chunk sorted
+// into (k, v) with spare arrays (k1, v1) of the same size: a 32-bit column
+// array and a value array, chosen so that the big-row passes end in (k, v).
+struct NarrowTarget {
+  uint32_t* col;
+  float* val;
+};
+NarrowTarget narrow_target(const RowSegPlan& plan, uint64_t n, float* v, uint64_t* k1,
+                           float* v1);
+// Sorts with a plan: big rows from their narrow target, groups in place.
 // Launches only (replayable).
-bool sort_row_segments(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, uint64_t* k0,
-                       float* v0, uint64_t* k1, float* v1, int col_bits, hipStream_t s,
-                       KTimer* kt = nullptr, int cls = SPTC_K_SORT_PROD);
+void sort_row_segments(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, uint64_t* k,
+                       float* v, uint64_t* k1, float* v1, hipStream_t s, KTimer* kt = nullptr,
+                       int cls = SPTC_K_SORT_PROD);
 
 }  // namespace sptc

@@ -99,6 +99,8 @@ __global__ __launch_bounds__(kBlock) void k_digit_scan(const uint64_t* __restric
   }
 }
 
+// TAG only separates kernel symbols (0: operand sorts, 1: product sorts) so
+// profiles attribute time to the product sort on its own.
 template <int RB, int IPT, int NT, int TAG>
 __global__ __launch_bounds__(NT) void k_onesweep(
     const uint64_t* __restrict__ kin, const float* __restrict__ vin,

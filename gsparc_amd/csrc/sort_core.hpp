@@ -43,10 +43,11 @@ struct TileSmem {
 // Ranks the keys a thread holds (wave w owns keys [w*64*IPT, (w+1)*64*IPT)
 // of the tile, lane-striped: key i of a lane is wbase + i*64 + lane; valid
 // when below n) within its wave's run of equal digits.  sm.cnt must be zero;
-// afterwards sm.cnt[w][d] = number of keys of digit d in wave w.
-template <int RB, int IPT, int NT>
+// afterwards sm.cnt[w][d] = number of keys of digit d in wave w.  Keys are
+// 64-bit, or 32-bit column keys of the row-segmented product sort.
+template <int RB, int IPT, int NT, typename KT>
 __device__ __forceinline__ void tile_rank(TileSmem<RB, IPT, NT>& sm, uint64_t n, int shift,
-                                          uint64_t wbase, const uint64_t (&k)[IPT],
+                                          uint64_t wbase, const KT (&k)[IPT],
                                           uint32_t (&pos)[IPT]) {
   constexpr int R = 1 << RB;
   const uint32_t lane = lane_id();
@@ -74,11 +75,11 @@ __device__ __forceinline__ void tile_rank(TileSmem<RB, IPT, NT>& sm, uint64_t n,
 }
 
 // Loads the tile from global memory (same layout as tile_rank) and ranks it.
-template <int RB, int IPT, int NT>
+template <int RB, int IPT, int NT, typename KT>
 __device__ __forceinline__ void tile_load_rank(TileSmem<RB, IPT, NT>& sm,
-                                               const uint64_t* __restrict__ kin,
+                                               const KT* __restrict__ kin,
                                                const float* __restrict__ vin, uint64_t n,
-                                               int shift, uint64_t wbase, uint64_t (&k)[IPT],
+                                               int shift, uint64_t wbase, KT (&k)[IPT],
                                                float (&v)[IPT], uint32_t (&pos)[IPT]) {
   const uint32_t lane = lane_id();
 #pragma unroll
@@ -116,19 +117,22 @@ __device__ __forceinline__ void tile_digit_starts(TileSmem<RB, IPT, NT>& sm) {
 
 // Given sm.gbase and sm.dstart (tile totals per digit on entry), scans the
 // totals into tile-local run starts, stages keys in digit order, writes them
-// in coalesced runs, then does the same for the values.
-template <int RB, int IPT, int NT>
+// in coalesced runs, then does the same for the values.  A key is written as
+// KO(key) | hi, which widens 32-bit column keys to full product keys on the
+// last pass of a row-segmented sort.
+template <int RB, int IPT, int NT, typename KT, typename KO>
 __device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT>& sm, uint64_t n,
                                                    int shift, uint64_t tile_begin,
-                                                   uint64_t wbase, const uint64_t (&k)[IPT],
+                                                   uint64_t wbase, const KT (&k)[IPT],
                                                    const float (&v)[IPT], uint32_t (&pos)[IPT],
-                                                   uint64_t* __restrict__ kout,
-                                                   float* __restrict__ vout) {
+                                                   KO* __restrict__ kout,
+                                                   float* __restrict__ vout, KO hi = 0) {
   constexpr int R = 1 << RB;
   constexpr int TILE = NT * IPT;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = lane_id();
   const uint32_t wave = tid / kWave;
+  KT* kbuf = reinterpret_cast<KT*>(sm.buf);
   tile_digit_starts<RB, IPT, NT>(sm);
   __syncthreads();
 #pragma unroll
@@ -136,7 +140,7 @@ __device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT>& sm, ui
     const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
     const uint32_t d = (uint32_t)(k[i] >> shift) & (R - 1);
     pos[i] += (uint32_t)sm.dstart[d] + (uint32_t)sm.cnt[wave][d];
-    if (idx < n) sm.buf[pos[i]] = k[i];
+    if (idx < n) kbuf[pos[i]] = k[i];
   }
   __syncthreads();
   const uint32_t cnt = (uint32_t)((n - tile_begin) < (uint64_t)TILE ? (n - tile_begin) : TILE);
@@ -145,10 +149,10 @@ __device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT>& sm, ui
   for (int i = 0; i < IPT; ++i) {
     const uint32_t j = tid + (uint32_t)i * NT;
     if (j < cnt) {
-      const uint64_t key = sm.buf[j];
+      const KT key = kbuf[j];
       const uint32_t d = (uint32_t)(key >> shift) & (R - 1);
       dst[i] = sm.gbase[d] + (j - sm.dstart[d]);
-      kout[dst[i]] = key;
+      kout[dst[i]] = (KO)key | hi;
     }
   }
   __syncthreads();
@@ -178,8 +182,5 @@ __device__ __forceinline__ uint32_t wave_prefixes(TileSmem<RB, IPT, NT>& sm, int
   }
   return acc;
 }
-
-// TAG only separates kernel symbols (0: operand sorts, 1: product sorts) so
-// profiles attribute time to the product sort on its own.
 
 }  // namespace sptc
