@@ -234,12 +234,15 @@ __global__ __launch_bounds__(kBlock) void k_bexpand(BExpandArgs a) {
       const uint32_t q = (uint32_t)e * kBlock + threadIdx.x;
       if (q < cnt) {
         const uint64_t p = pb + q;
-        uint32_t lo = klo, hi = m;  // last k with s_pref[k] <= p
-        while (lo < hi) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (s_pref[mid] <= p) lo = mid + 1; else hi = mid;
+        uint32_t k = klo;  // same owner as the previous product: one LDS read
+        if (k + 1 < m && s_pref[k + 1] <= p) {
+          uint32_t lo = k + 1, hi = m;  // last k with s_pref[k] <= p
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_pref[mid] <= p) lo = mid + 1; else hi = mid;
+          }
+          k = lo - 1;
         }
-        const uint32_t k = lo - 1;
         klo = k;
         const uint64_t j = ja + k;
         if (k != kc) {  // new owner: its row's key bits and column mask

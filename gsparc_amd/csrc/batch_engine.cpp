@@ -568,7 +568,8 @@ sptc_status BatchEngine::run(hipStream_t s, bool P) {
     ea.bad = bad_.get();
     int tok = kt ? kt->begin(s) : -1;
     launch_bexpand(ea, s);
-    if (kt) kt->end(tok, SPTC_K_EXPAND, 24.0 * (double)c.n, s);
+    const double wide = c.rowsort ? (double)c.rseg.nsmall : (double)c.n;
+    if (kt) kt->end(tok, SPTC_K_EXPAND, 20.0 * (double)c.n + 4.0 * wide, s);
     bool alt = false;
     if (c.rowsort) {
       sort_row_segments(ws_, c.rseg, c.n, kA_.get(), vA_.get(), ws_.alt_keys.get(),

@@ -519,8 +519,10 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       ea.vsum = vsum.get();
       int tok = kt_.begin(s_);
       launch_expand(ea, rb, s_);
-      // product (key, value) written, matched Y (key, value) read
-      kt_.end(tok, SPTC_K_EXPAND, 24.0 * (double)n, s_);
+      // matched Y (key, value) read; product (key, value) written, 8 bytes
+      // instead of 12 for the big rows of a row-segmented sort
+      const double wide = rowsort ? (double)rseg.nsmall : (double)n;
+      kt_.end(tok, SPTC_K_EXPAND, 20.0 * (double)n + 4.0 * wide, s_);
       bool alt = false;
       if (rowsort) {
         sort_row_segments(ws_, rseg, n, kA.get(), vA.get(), ws_.alt_keys.get(),

@@ -418,12 +418,15 @@ __global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs a) {
       const bool valid = p < pe;
       uint64_t key = 0;
       if (valid) {
-        uint32_t lo = klo, hi = m;  // last k with s_pref[k] <= p
-        while (lo < hi) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (s_pref[mid] <= p) lo = mid + 1; else hi = mid;
+        uint32_t k = klo;  // same owner as the previous product: one LDS read
+        if (k + 1 < m && s_pref[k + 1] <= p) {
+          uint32_t lo = k + 1, hi = m;  // last k with s_pref[k] <= p
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_pref[mid] <= p) lo = mid + 1; else hi = mid;
+          }
+          k = lo - 1;
         }
-        const uint32_t k = lo - 1;
         klo = k;
         if (k != kc) {  // new owner
           kc = k;
