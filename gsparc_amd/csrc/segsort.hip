@@ -3,6 +3,7 @@
 #include "sort_core.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <stdexcept>
 
 namespace sptc {
@@ -11,22 +12,22 @@ namespace {
 
 // ---- segmented reduce-then-scan radix pass (8-bit digits, 512 x 12 tiles) --
 
-template <typename KT>
+template <typename KT, int RB, int IPT>
 __global__ __launch_bounds__(kSegNT) void k_seg_upsweep(const KT* __restrict__ kin, int shift,
                                                         const SegTile* __restrict__ tiles,
                                                         uint32_t* __restrict__ cnt) {
-  constexpr int R = 1 << kSegRB;
+  constexpr int R = 1 << RB;
   __shared__ uint32_t sh[R];
   for (int i = threadIdx.x; i < R; i += kSegNT) sh[i] = 0;
   __syncthreads();
   const SegTile t = tiles[blockIdx.x];
   const uint64_t end = t.start + t.count;
 #pragma unroll
-  for (int i = 0; i < kSegIPT; ++i) {
+  for (int i = 0; i < IPT; ++i) {
     const uint64_t idx = t.start + (uint64_t)i * kSegNT + threadIdx.x;
     const bool valid = idx < end;
     const uint64_t k = valid ? (uint64_t)kin[idx] : 0;
-    hist_add_wave<kSegRB>(sh, k >> shift, valid, 1);
+    hist_add_wave<RB>(sh, k >> shift, valid, 1);
   }
   __syncthreads();
   for (int d = threadIdx.x; d < R; d += kSegNT) cnt[(uint64_t)blockIdx.x * R + d] = sh[d];
@@ -34,39 +35,40 @@ __global__ __launch_bounds__(kSegNT) void k_seg_upsweep(const KT* __restrict__ k
 
 // KI -> KO: 64 -> 64 (operand sorts), 32 -> 32 (column passes of big rows),
 // 32 -> 64 (their last pass, which ORs in the tile's row bits).
-template <typename KI, typename KO>
+template <typename KI, typename KO, int RB, int IPT>
 __global__ __launch_bounds__(kSegNT) void k_seg_downsweep(
     const KI* __restrict__ kin, const float* __restrict__ vin, KO* __restrict__ kout,
     float* __restrict__ vout, int shift, const SegTile* __restrict__ tiles,
     const uint32_t* __restrict__ offs) {
-  constexpr int R = 1 << kSegRB;
-  __shared__ TileSmem<kSegRB, kSegIPT, kSegNT> sm;
+  constexpr int R = 1 << RB;
+  __shared__ TileSmem<RB, IPT, kSegNT, KI> sm;
   const uint32_t tid = threadIdx.x;
   const SegTile t = tiles[blockIdx.x];
   for (int i = tid; i < (kSegNT / kWave) * R; i += kSegNT) (&sm.cnt[0][0])[i] = 0;
   for (int d = tid; d < R; d += kSegNT) sm.gbase[d] = offs[t.obase + (uint64_t)d * t.ostride];
   __syncthreads();
   const uint64_t end = t.start + t.count;
-  const uint64_t wbase = t.start + (uint64_t)(tid / kWave) * kWave * kSegIPT;
-  KI k[kSegIPT];
-  float v[kSegIPT];
-  uint32_t pos[kSegIPT];
-  tile_load_rank<kSegRB, kSegIPT, kSegNT>(sm, kin, vin, end, shift, wbase, k, v, pos);
+  const uint64_t wbase = t.start + (uint64_t)(tid / kWave) * kWave * IPT;
+  KI k[IPT];
+  float v[IPT];
+  uint32_t pos[IPT];
+  tile_load_rank<RB, IPT, kSegNT>(sm, kin, vin, end, shift, wbase, k, v, pos);
   __syncthreads();
   for (int d = tid; d < R; d += kSegNT)
-    sm.dstart[d] = (uint16_t)wave_prefixes<kSegRB, kSegIPT, kSegNT>(sm, d);
+    sm.dstart[d] = (uint16_t)wave_prefixes<RB, IPT, kSegNT>(sm, d);
   __syncthreads();
   const KO hi = sizeof(KO) > sizeof(KI) ? (KO)t.hi : (KO)0;
-  tile_stage_scatter<kSegRB, kSegIPT, kSegNT>(sm, end, shift, t.start, wbase, k, v, pos, kout,
-                                              vout, hi);
+  tile_stage_scatter<RB, IPT, kSegNT>(sm, end, shift, t.start, wbase, k, v, pos, kout, vout,
+                                      hi);
 }
 
 // Per segment (one workgroup, thread d = digit d): absolute destination of
 // each tile's run of each digit.
-__global__ __launch_bounds__(1 << kSegRB) void k_seg_scan(const SegDesc* __restrict__ segs,
-                                                         const uint32_t* __restrict__ cnt,
-                                                         uint32_t* __restrict__ off) {
-  constexpr int R = 1 << kSegRB;
+template <int RB>
+__global__ __launch_bounds__(1 << RB) void k_seg_scan(const SegDesc* __restrict__ segs,
+                                                     const uint32_t* __restrict__ cnt,
+                                                     uint32_t* __restrict__ off) {
+  constexpr int R = 1 << RB;
   __shared__ uint32_t scratch[R / kWave];
   const SegDesc g = segs[blockIdx.x];
   const uint32_t d = threadIdx.x;
@@ -74,7 +76,7 @@ __global__ __launch_bounds__(1 << kSegRB) void k_seg_scan(const SegDesc* __restr
   uint32_t tot = 0;
   for (uint64_t t = t0; t < t1; ++t) tot += cnt[t * R + d];
   uint32_t all;
-  uint32_t base = (uint32_t)g.start + block_excl_scan<uint32_t>(tot, scratch, &all);
+  uint32_t base = (uint32_t)g.start + block_scan_u32<R>(tot, scratch, &all);
   uint32_t* o = off + R * t0 + (uint64_t)d * g.ntiles;  // digit-major in the segment
   for (uint64_t t = t0; t < t1; ++t) {
     o[t - t0] = base;
@@ -102,8 +104,8 @@ __device__ __forceinline__ uint64_t rowseg_locate(const RowRange* ranges, uint32
 // Per row: small-row products (0 for big rows) and big-row tile counts.
 __global__ __launch_bounds__(kBlock) void k_rowseg_sizes(const RowRange* ranges, uint32_t nr,
                                                          uint64_t lrows, const uint64_t* row_prod,
-                                                         uint64_t* small_n, uint64_t* big_t,
-                                                         uint64_t* big_f) {
+                                                         uint64_t tile, uint64_t* small_n,
+                                                         uint64_t* big_t, uint64_t* big_f) {
   uint64_t l = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (; l < lrows; l += stride) {
@@ -111,7 +113,7 @@ __global__ __launch_bounds__(kBlock) void k_rowseg_sizes(const RowRange* ranges,
     const uint64_t n = rowseg_locate(ranges, nr, l, row_prod, &first);
     const bool big = n > kHalfTile;
     small_n[l] = big ? 0 : n;
-    big_t[l] = big ? ceil_div(n, kSegTile) : 0;
+    big_t[l] = big ? ceil_div(n, tile) : 0;
     big_f[l] = big ? 1 : 0;
   }
 }
@@ -133,6 +135,7 @@ __global__ __launch_bounds__(kBlock) void k_rowseg_heads(uint64_t lrows, const u
 
 __global__ __launch_bounds__(kBlock) void k_rowseg_emit(
     const RowRange* ranges, uint32_t nr, uint64_t lrows, const uint64_t* row_prod, int col_bits,
+    int rb, uint64_t tile,
     const uint64_t* big_t, const uint64_t* bt, const uint64_t* bs, const uint64_t* head,
     const uint64_t* hs, LocalSeg* local, SegTile* big, SegDesc* bigseg) {
   uint64_t l = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -150,9 +153,9 @@ __global__ __launch_bounds__(kBlock) void k_rowseg_emit(
       bigseg[bs[l]] = g;
       for (uint64_t t = 0; t < nt; ++t) {
         SegTile d;
-        d.start = first + t * kSegTile;
-        d.count = (uint32_t)(n - t * kSegTile < kSegTile ? n - t * kSegTile : kSegTile);
-        d.obase = (1ull << kSegRB) * o + t;
+        d.start = first + t * tile;
+        d.count = (uint32_t)(n - t * tile < tile ? n - t * tile : tile);
+        d.obase = (1ull << rb) * o + t;
         d.ostride = (uint32_t)nt;
         d.hi = shl64(l, col_bits);
         big[o + t] = d;
@@ -290,8 +293,8 @@ void segmented_sort_reserve(SortWorkspace& ws, const SegSortPlan& plan, hipStrea
   segmented_sort_reserve(ws, plan.ntiles, s);
 }
 
-void segmented_sort_reserve(SortWorkspace& ws, uint64_t ntiles, hipStream_t s) {
-  const uint64_t m = ntiles << kSegRB;
+void segmented_sort_reserve(SortWorkspace& ws, uint64_t ntiles, hipStream_t s, int rb) {
+  const uint64_t m = ntiles << rb;
   ws.ensure(1, kSegRB, s);
   if (ws.rts_cnt.size() < m) {
     ws.rts_cnt.reset(m);
@@ -317,14 +320,15 @@ bool segmented_sort(SortWorkspace& ws, const SegTile* tiles, uint64_t ntiles,
   for (int p = 0; p < passes; ++p) {
     const int shift = p * kSegRB;
     int tok = kt ? kt->begin(s) : -1;
-    hipLaunchKernelGGL(k_seg_upsweep<uint64_t>, dim3((uint32_t)ntiles), dim3(kSegNT), 0, s, ka,
-                       shift, tiles, ws.rts_cnt.get());
-    hipLaunchKernelGGL(k_seg_scan, dim3((uint32_t)nsegs), dim3(1 << kSegRB), 0, s, segs,
+    hipLaunchKernelGGL((k_seg_upsweep<uint64_t, kSegRB, kSegIPT>), dim3((uint32_t)ntiles),
+                       dim3(kSegNT), 0, s, ka, shift, tiles, ws.rts_cnt.get());
+    hipLaunchKernelGGL(k_seg_scan<kSegRB>, dim3((uint32_t)nsegs), dim3(1 << kSegRB), 0, s, segs,
                        ws.rts_cnt.get(), ws.rts_off.get());
     if (kt) kt->end(tok, SPTC_K_SORT_COUNT, 8.0 * (double)n + 16.0 * (double)m, s);
     tok = kt ? kt->begin(s) : -1;
-    hipLaunchKernelGGL((k_seg_downsweep<uint64_t, uint64_t>), dim3((uint32_t)ntiles),
-                       dim3(kSegNT), 0, s, ka, va, kb, vb, shift, tiles, ws.rts_off.get());
+    hipLaunchKernelGGL((k_seg_downsweep<uint64_t, uint64_t, kSegRB, kSegIPT>),
+                       dim3((uint32_t)ntiles), dim3(kSegNT), 0, s, ka, va, kb, vb, shift, tiles,
+                       ws.rts_off.get());
     if (kt) kt->end(tok, cls, 24.0 * (double)n, s);
     std::swap(ka, kb);
     std::swap(va, vb);
@@ -338,6 +342,30 @@ bool segmented_sort(SortWorkspace& ws, const SegSortPlan& plan, uint64_t* k0, fl
                         k0, v0, k1, v1, nbits, s, kt, cls);
 }
 
+// Digit width of the big-row passes: 10 bits where that saves a pass over
+// 8-bit digits (column bits 9-10, 17-20), else 8 (longer runs per digit).
+static int narrow_rb(int col_bits) {
+  static const int forced = [] {
+    const char* e = getenv("SPTC_SEG_RB");
+    return e ? atoi(e) : 0;
+  }();
+  if (forced == 8) return 8;
+  const int p8 = std::max(1, radix_passes(col_bits, 8));
+  const int p10 = std::max(1, radix_passes(col_bits, 10));
+  return p10 < p8 ? 10 : 8;
+}
+static int narrow_ipt() {
+  static const int ipt = [] {
+    const char* e = getenv("SPTC_SEG_IPT");
+    const int v = e ? atoi(e) : kNarrowIPT;
+    return v == 12 || v == 16 || v == 24 ? v : kNarrowIPT;
+  }();
+  return ipt;
+}
+static int narrow_passes(int col_bits) {
+  return std::max(1, radix_passes(col_bits, narrow_rb(col_bits)));
+}
+
 void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows,
                         const uint64_t* row_prod, int col_bits, RowSegPlan* plan,
                         SortWorkspace& ws, U2* tmp, hipStream_t s) {
@@ -345,6 +373,9 @@ void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows
   plan->nlocal = plan->nbig_tiles = plan->nbig_rows = plan->nsmall = 0;
   plan->lrows = lrows;
   plan->col_bits = col_bits;
+  plan->rb = narrow_rb(col_bits);
+  plan->ipt = narrow_ipt();
+  const uint64_t tile = (uint64_t)plan->ipt * kSegNT;
   if (!lrows) return;
   const uint64_t m = lrows + 1;
   if (plan->scratch.size() < 8 * m) plan->scratch.reset(8 * m);
@@ -358,7 +389,7 @@ void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows
   uint64_t* bs = big_f + m;
   const uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(lrows, kBlock), 4096);
   hipLaunchKernelGGL(k_rowseg_sizes, dim3(grid), dim3(kBlock), 0, s, ranges, nranges, lrows,
-                     row_prod, small_n, big_t, big_f);
+                     row_prod, tile, small_n, big_t, big_f);
   run_exclusive_scan(small_n, ps, lrows, tmp, s);
   run_exclusive_scan(big_t, bt, lrows, tmp, s);
   run_exclusive_scan(big_f, bs, lrows, tmp, s);
@@ -384,12 +415,10 @@ void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows
     hipLaunchKernelGGL(k_zero_local, dim3((uint32_t)ceil_div(plan->nlocal, 256)), dim3(256), 0,
                        s, plan->local.get(), plan->nlocal);
   hipLaunchKernelGGL(k_rowseg_emit, dim3(grid), dim3(kBlock), 0, s, ranges, nranges, lrows,
-                     row_prod, col_bits, big_t, bt, bs, head, hs, plan->local.get(), plan->big.get(),
+                     row_prod, col_bits, plan->rb, tile, big_t, bt, bs, head, hs, plan->local.get(), plan->big.get(),
                      plan->bigseg.get());
-  if (plan->nbig_tiles) segmented_sort_reserve(ws, plan->nbig_tiles, s);
+  if (plan->nbig_tiles) segmented_sort_reserve(ws, plan->nbig_tiles, s, plan->rb);
 }
-
-static int narrow_passes(int col_bits) { return std::max(1, radix_passes(col_bits, kSegRB)); }
 
 // Column-array sets of the big-row passes: set 0 = (k1 as 32-bit words
 // [0, n), v1), set 1 = (k1 words [n, 2n), v).  The passes alternate sets and
@@ -403,39 +432,53 @@ NarrowTarget narrow_target(const RowSegPlan& plan, uint64_t n, float* v, uint64_
   return NarrowTarget{c + n, v};
 }
 
+template <int RB, int IPT>
+static void narrow_sort(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, uint64_t* k,
+                        float* v, uint64_t* k1, float* v1, hipStream_t s, KTimer* kt, int cls) {
+  const uint64_t ntiles = plan.nbig_tiles;
+  const uint64_t m = ntiles << RB;
+  if (ws.rts_cnt.size() < m) throw std::logic_error("segmented sort workspace not reserved");
+  const uint64_t nb = n - plan.nsmall;  // big-row products
+  uint32_t* col[2] = {reinterpret_cast<uint32_t*>(k1), reinterpret_cast<uint32_t*>(k1) + n};
+  float* val[2] = {v1, v};
+  const int passes = narrow_passes(plan.col_bits);
+  int cur = (passes & 1) ? 0 : 1;
+  const SegTile* tiles = plan.big.get();
+  for (int p = 0; p < passes; ++p) {
+    const int shift = p * RB;
+    int tok = kt ? kt->begin(s) : -1;
+    hipLaunchKernelGGL((k_seg_upsweep<uint32_t, RB, IPT>), dim3((uint32_t)ntiles),
+                       dim3(kSegNT), 0, s, col[cur], shift, tiles, ws.rts_cnt.get());
+    hipLaunchKernelGGL(k_seg_scan<RB>, dim3((uint32_t)plan.nbig_rows), dim3(1 << RB), 0, s,
+                       plan.bigseg.get(), ws.rts_cnt.get(), ws.rts_off.get());
+    if (kt) kt->end(tok, SPTC_K_SORT_COUNT, 4.0 * (double)nb + 16.0 * (double)m, s);
+    tok = kt ? kt->begin(s) : -1;
+    if (p + 1 < passes) {
+      hipLaunchKernelGGL((k_seg_downsweep<uint32_t, uint32_t, RB, IPT>),
+                         dim3((uint32_t)ntiles), dim3(kSegNT), 0, s, col[cur], val[cur],
+                         col[1 - cur], val[1 - cur], shift, tiles, ws.rts_off.get());
+      if (kt) kt->end(tok, cls, 16.0 * (double)nb, s);
+      cur = 1 - cur;
+    } else {
+      hipLaunchKernelGGL((k_seg_downsweep<uint32_t, uint64_t, RB, IPT>),
+                         dim3((uint32_t)ntiles), dim3(kSegNT), 0, s, col[cur], val[cur], k, v,
+                         shift, tiles, ws.rts_off.get());
+      if (kt) kt->end(tok, cls, 20.0 * (double)nb, s);
+    }
+  }
+}
+
 void sort_row_segments(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, uint64_t* k,
                        float* v, uint64_t* k1, float* v1, hipStream_t s, KTimer* kt, int cls) {
   if (plan.nbig_tiles) {
-    const uint64_t ntiles = plan.nbig_tiles;
-    const uint64_t m = ntiles << kSegRB;
-    if (ws.rts_cnt.size() < m) throw std::logic_error("segmented sort workspace not reserved");
-    const uint64_t nb = n - plan.nsmall;  // big-row products
-    uint32_t* col[2] = {reinterpret_cast<uint32_t*>(k1), reinterpret_cast<uint32_t*>(k1) + n};
-    float* val[2] = {v1, v};
-    const int passes = narrow_passes(plan.col_bits);
-    int cur = (passes & 1) ? 0 : 1;
-    const SegTile* tiles = plan.big.get();
-    for (int p = 0; p < passes; ++p) {
-      const int shift = p * kSegRB;
-      int tok = kt ? kt->begin(s) : -1;
-      hipLaunchKernelGGL(k_seg_upsweep<uint32_t>, dim3((uint32_t)ntiles), dim3(kSegNT), 0, s,
-                         col[cur], shift, tiles, ws.rts_cnt.get());
-      hipLaunchKernelGGL(k_seg_scan, dim3((uint32_t)plan.nbig_rows), dim3(1 << kSegRB), 0, s,
-                         plan.bigseg.get(), ws.rts_cnt.get(), ws.rts_off.get());
-      if (kt) kt->end(tok, SPTC_K_SORT_COUNT, 4.0 * (double)nb + 16.0 * (double)m, s);
-      tok = kt ? kt->begin(s) : -1;
-      if (p + 1 < passes) {
-        hipLaunchKernelGGL((k_seg_downsweep<uint32_t, uint32_t>), dim3((uint32_t)ntiles),
-                           dim3(kSegNT), 0, s, col[cur], val[cur], col[1 - cur], val[1 - cur],
-                           shift, tiles, ws.rts_off.get());
-        if (kt) kt->end(tok, cls, 16.0 * (double)nb, s);
-        cur = 1 - cur;
-      } else {
-        hipLaunchKernelGGL((k_seg_downsweep<uint32_t, uint64_t>), dim3((uint32_t)ntiles),
-                           dim3(kSegNT), 0, s, col[cur], val[cur], k, v, shift, tiles,
-                           ws.rts_off.get());
-        if (kt) kt->end(tok, cls, 20.0 * (double)nb, s);
-      }
+    switch (plan.rb * 100 + plan.ipt) {
+      case 812: narrow_sort<8, 12>(ws, plan, n, k, v, k1, v1, s, kt, cls); break;
+      case 816: narrow_sort<8, 16>(ws, plan, n, k, v, k1, v1, s, kt, cls); break;
+      case 824: narrow_sort<8, 24>(ws, plan, n, k, v, k1, v1, s, kt, cls); break;
+      case 1012: narrow_sort<10, 12>(ws, plan, n, k, v, k1, v1, s, kt, cls); break;
+      case 1016: narrow_sort<10, 16>(ws, plan, n, k, v, k1, v1, s, kt, cls); break;
+      case 1024: narrow_sort<10, 24>(ws, plan, n, k, v, k1, v1, s, kt, cls); break;
+      default: throw std::logic_error("row-segmented sort tile shape");
     }
   }
   if (plan.nlocal) {

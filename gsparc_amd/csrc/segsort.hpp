@@ -14,6 +14,11 @@ constexpr int kSegRB = 8;     // digit bits
 constexpr int kSegIPT = 12;   // keys per thread
 constexpr int kSegNT = 512;   // threads per tile
 constexpr uint64_t kSegTile = (uint64_t)kSegIPT * kSegNT;
+// Big-row passes of the row-segmented product sort move 32-bit column keys,
+// so more of them fit a tile's LDS staging: kNarrowIPT keys per thread (12,
+// 16 or 24; SPTC_SEG_IPT overrides the default).  16 keeps two 512-thread
+// workgroups per CU (112 VGPRs); 24 drops to one.
+constexpr int kNarrowIPT = 16;
 
 // One tile of a segmented sort (never straddles a segment).  Its digit
 // counts are row t of a tile-major count matrix (coalesced writes); its
@@ -69,7 +74,8 @@ struct SegSortPlan {
 // (k1, v1); ws supplies the count matrices (segmented_sort_reserve).
 // Launches only (no allocation or host synchronization), so it can be
 // replayed and captured.  Returns true when the result ended in (k1, v1).
-void segmented_sort_reserve(SortWorkspace& ws, uint64_t ntiles, hipStream_t s);
+void segmented_sort_reserve(SortWorkspace& ws, uint64_t ntiles, hipStream_t s,
+                            int rb = kSegRB);
 void segmented_sort_reserve(SortWorkspace& ws, const SegSortPlan& plan, hipStream_t s);
 bool segmented_sort(SortWorkspace& ws, const SegTile* tiles, uint64_t ntiles,
                     const SegDesc* segs, uint64_t nsegs, uint64_t n, uint64_t* k0, float* v0,
@@ -96,17 +102,20 @@ struct RowRange {
 // Rows holding more than half a tile of products are segments of their own:
 // the expansion writes their products narrow -- 32-bit column + value, the
 // row implied by the segment -- and the multi-tile passes above sort them on
-// the column bits, the last pass widening to full keys.  Runs of smaller
+// the column bits (digits of 8 bits, or 10 where that saves a pass; tiles of
+// ipt * kSegNT keys), the last pass widening to full keys.  Runs of smaller
 // rows are written as full keys and cut into groups of at most one tile (a
 // group starts where the running total of small-row products crosses a
 // multiple of half a tile), each sorted on its varying key bits entirely in
 // one workgroup's LDS, in place.  Every product is read and written once for
-// the groups; max(1, ceil(col_bits / 8)) times for the big rows, at 8 bytes
+// the groups; once per digit pass (at least one) for the big rows, at 8 bytes
 // per product except the last pass's 12-byte writes.
 struct RowSegPlan {
   uint64_t nlocal = 0, nbig_tiles = 0, nbig_rows = 0, nsmall = 0;  // nsmall: products in groups
   uint64_t lrows = 0;
   int col_bits = 0;
+  int rb = kSegRB;  // digit bits of the big-row passes (8 or 10)
+  int ipt = kNarrowIPT;  // keys per thread of their tiles
   DevBuf<LocalSeg> local;
   DevBuf<SegTile> big;
   DevBuf<SegDesc> bigseg;

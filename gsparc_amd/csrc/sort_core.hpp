@@ -27,7 +27,9 @@ __device__ __forceinline__ uint32_t block_scan_u32(uint32_t v, uint32_t* scratch
   return off + inc - v;
 }
 
-template <int RB, int IPT, int NT>
+// ST: staging element (keys, then values, pass through buf): 64-bit for
+// full keys, 32-bit for column keys.
+template <int RB, int IPT, int NT, typename ST = uint64_t>
 struct TileSmem {
   static constexpr int R = 1 << RB;
   static constexpr int TILE = NT * IPT;
@@ -35,7 +37,7 @@ struct TileSmem {
   uint16_t cnt[NW][R];     // per-wave running counts, then wave prefixes
   uint32_t gbase[R];       // global start of this tile's run of each digit
   uint16_t dstart[R];      // tile-local start of each digit's run
-  uint64_t buf[TILE];      // staged keys, then staged values
+  ST buf[TILE];            // staged keys, then staged values
   uint32_t scan[NW];
   uint32_t tile;
 };
@@ -45,8 +47,8 @@ struct TileSmem {
 // when below n) within its wave's run of equal digits.  sm.cnt must be zero;
 // afterwards sm.cnt[w][d] = number of keys of digit d in wave w.  Keys are
 // 64-bit, or 32-bit column keys of the row-segmented product sort.
-template <int RB, int IPT, int NT, typename KT>
-__device__ __forceinline__ void tile_rank(TileSmem<RB, IPT, NT>& sm, uint64_t n, int shift,
+template <int RB, int IPT, int NT, typename ST, typename KT>
+__device__ __forceinline__ void tile_rank(TileSmem<RB, IPT, NT, ST>& sm, uint64_t n, int shift,
                                           uint64_t wbase, const KT (&k)[IPT],
                                           uint32_t (&pos)[IPT]) {
   constexpr int R = 1 << RB;
@@ -75,8 +77,8 @@ __device__ __forceinline__ void tile_rank(TileSmem<RB, IPT, NT>& sm, uint64_t n,
 }
 
 // Loads the tile from global memory (same layout as tile_rank) and ranks it.
-template <int RB, int IPT, int NT, typename KT>
-__device__ __forceinline__ void tile_load_rank(TileSmem<RB, IPT, NT>& sm,
+template <int RB, int IPT, int NT, typename ST, typename KT>
+__device__ __forceinline__ void tile_load_rank(TileSmem<RB, IPT, NT, ST>& sm,
                                                const KT* __restrict__ kin,
                                                const float* __restrict__ vin, uint64_t n,
                                                int shift, uint64_t wbase, KT (&k)[IPT],
@@ -94,8 +96,8 @@ __device__ __forceinline__ void tile_load_rank(TileSmem<RB, IPT, NT>& sm,
 
 // sm.dstart holds each digit's tile total: turn it into the tile-local start
 // of each digit's run (block-wide exclusive scan).
-template <int RB, int IPT, int NT>
-__device__ __forceinline__ void tile_digit_starts(TileSmem<RB, IPT, NT>& sm) {
+template <int RB, int IPT, int NT, typename ST>
+__device__ __forceinline__ void tile_digit_starts(TileSmem<RB, IPT, NT, ST>& sm) {
   constexpr int R = 1 << RB;
   constexpr int PER = (R + NT - 1) / NT;
   const uint32_t tid = threadIdx.x;
@@ -120,8 +122,8 @@ __device__ __forceinline__ void tile_digit_starts(TileSmem<RB, IPT, NT>& sm) {
 // in coalesced runs, then does the same for the values.  A key is written as
 // KO(key) | hi, which widens 32-bit column keys to full product keys on the
 // last pass of a row-segmented sort.
-template <int RB, int IPT, int NT, typename KT, typename KO>
-__device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT>& sm, uint64_t n,
+template <int RB, int IPT, int NT, typename ST, typename KT, typename KO>
+__device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT, ST>& sm, uint64_t n,
                                                    int shift, uint64_t tile_begin,
                                                    uint64_t wbase, const KT (&k)[IPT],
                                                    const float (&v)[IPT], uint32_t (&pos)[IPT],
@@ -132,6 +134,7 @@ __device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT>& sm, ui
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = lane_id();
   const uint32_t wave = tid / kWave;
+  static_assert(sizeof(KT) <= sizeof(ST), "staging element narrower than the key");
   KT* kbuf = reinterpret_cast<KT*>(sm.buf);
   tile_digit_starts<RB, IPT, NT>(sm);
   __syncthreads();
@@ -171,8 +174,8 @@ __device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT>& sm, ui
 }
 
 // Per digit: turn per-wave counts into wave prefixes; returns the tile total.
-template <int RB, int IPT, int NT>
-__device__ __forceinline__ uint32_t wave_prefixes(TileSmem<RB, IPT, NT>& sm, int d) {
+template <int RB, int IPT, int NT, typename ST>
+__device__ __forceinline__ uint32_t wave_prefixes(TileSmem<RB, IPT, NT, ST>& sm, int d) {
   uint32_t acc = 0;
 #pragma unroll
   for (int w = 0; w < NT / kWave; ++w) {
