@@ -1,6 +1,7 @@
 // Batched (multi-contraction) kernels for gfx950; see batch.hpp.
 #include "batch.hpp"
 #include "dense_core.hpp"
+#include "reduce_core.hpp"
 
 #include <algorithm>
 #include <stdexcept>
@@ -283,69 +284,26 @@ struct BReduceOp {
 // Same run detection and compensated summation as the plain reduce; the
 // chunk-local row maps back to a global row through the piece table.
 __global__ __launch_bounds__(kBlock) void k_breduce_emit(BReduceOp op, const U2* tile_offs) {
-  __shared__ uint64_t s_k[kScanTile];
-  __shared__ float s_v[kScanTile];
-  __shared__ U2 scratch[kBlock / kWave];
+  __shared__ ReduceSmem sm;
   if (*op.b.bad) return;
   const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
-  const uint32_t cnt = (uint32_t)(op.n - base < kScanTile ? op.n - base : kScanTile);
-  for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) {
-    s_k[i] = op.keys[base + i];
-    s_v[i] = op.vals[base + i];
-  }
-  const uint64_t prev_key = base ? op.keys[base - 1] : 0;
-  __syncthreads();
-  const uint32_t q0 = threadIdx.x * kScanItems;
-  uint64_t ok[kScanItems];
-  float ov[kScanItems];
-  uint32_t nh = 0;
-#pragma unroll
-  for (int e = 0; e < kScanItems; ++e) {
-    const uint32_t i = q0 + e;
-    ok[e] = 0;
-    ov[e] = 0.f;
-    if (i >= cnt) continue;
-    const uint64_t k = s_k[i];
-    const uint64_t pk = i ? s_k[i - 1] : prev_key;
-    if (!(base + i == 0 || k != pk)) continue;
-    KahanF32 acc;
-    acc.add(s_v[i]);
-    uint32_t jj = i + 1;
-    while (jj < cnt && s_k[jj] == k) acc.add(s_v[jj++]);
-    if (jj == cnt) {
-      for (uint64_t g = base + cnt; g < op.n && op.keys[g] == k; ++g) acc.add(op.vals[g]);
-    }
-    ok[nh] = k;
-    ov[nh] = acc.value();
-    ++nh;
-  }
-  U2 tot;
-  const uint32_t o0 = (uint32_t)block_excl_scan_u2(mk2(nh, 0), scratch, &tot).a;
-  const uint32_t nout = (uint32_t)tot.a;
-  __syncthreads();
-#pragma unroll
-  for (int e = 0; e < kScanItems; ++e) {
-    if ((uint32_t)e < nh) {
-      s_k[o0 + e] = ok[e];
-      s_v[o0 + e] = ov[e];
-    }
-  }
-  __syncthreads();
+  uint64_t prev_key;
+  const uint32_t nout = reduce_tile(op.keys, op.vals, op.n, sm, &prev_key);
   const uint64_t obase = tile_offs[blockIdx.x].a;
   const uint64_t cm = low_mask(op.K);
   for (uint32_t i = threadIdx.x; i < nout; i += kBlock) {
-    const uint64_t k = s_k[i];
+    const uint64_t k = sm.k[i];
     const uint64_t lrow = shr64(k, op.K);
     const uint32_t row = op.b.map_row[lrow];
     const uint32_t s = op.b.map_inst[lrow];
     const BInst in = op.b.inst[s];
     op.out_row[obase + i] = row - (uint32_t)op.b.inst_R[s];
     op.out_col[obase + i] = op.b.ytab[in.yt_off + (k & cm)];
-    op.out_val[obase + i] = s_v[i];
+    op.out_val[obase + i] = sm.v[i];
     if (!op.b.bounds) continue;
     // first output of every instance in (previous output's instance, s]
     int64_t sp = -1;
-    if (i > 0) sp = op.b.map_inst[shr64(s_k[i - 1], op.K)];
+    if (i > 0) sp = op.b.map_inst[shr64(sm.k[i - 1], op.K)];
     else if (base > 0) sp = op.b.map_inst[shr64(prev_key, op.K)];
     for (int64_t t = sp + 1; t <= (int64_t)s; ++t) op.b.bounds[t] = obase + i;
   }

@@ -2,6 +2,7 @@
 // reduce.  Host launchers are declared in kernels.hpp.
 #include "dense_core.hpp"
 #include "kernels.hpp"
+#include "reduce_core.hpp"
 
 namespace sptc {
 
@@ -496,88 +497,50 @@ struct ReduceOp {
   int ynf;             // width of the key's column field
   uint64_t fmask;      // its mask
   const uint64_t* ytab;  // rank -> Y free key (NULL: the field is the key)
-  __device__ U2 value(uint64_t i) const {
-    return mk2(i == 0 || keys[i] != keys[i - 1], 0);
-  }
-  __device__ void emit(uint64_t i, U2 excl, U2 v) const {
-    if (!v.a) return;
-    const uint64_t k = keys[i];
-    KahanF32 acc;
-    acc.add(vals[i]);
-    for (uint64_t j = i + 1; j < n && keys[j] == k; ++j) acc.add(vals[j]);
-    const uint64_t o = excl.a;
-    out_row[o] = (uint32_t)(r0 + shr64(k, ynf));
-    out_col[o] = k & fmask;
-    out_val[o] = acc.value();
-  }
 };
 
+// Per tile of kScanTile sorted keys: the number of run heads (key differs
+// from its predecessor) in tile_sums[tile].a, the tiling of the emit below.
+// Items are lane-striped (coalesced loads); the predecessor comes from the
+// neighbouring lane.
+__global__ __launch_bounds__(kBlock) void k_reduce_heads(const uint64_t* __restrict__ keys,
+                                                         uint64_t n, U2* __restrict__ tile_sums) {
+  __shared__ U2 scratch[kBlock / kWave];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  uint64_t c = 0;
+#pragma unroll
+  for (int e = 0; e < kScanItems; ++e) {
+    const uint64_t i = base + (uint64_t)e * kBlock + threadIdx.x;
+    const uint64_t k = i < n ? keys[i] : 0;
+    uint64_t pk = __shfl_up(k, 1, kWave);
+    if (lane_id() == 0) pk = i > 0 && i < n ? keys[i - 1] : 0;
+    c += i < n && (i == 0 || k != pk);
+  }
+  U2 tot;
+  block_excl_scan_u2(mk2(c, 0), scratch, &tot);
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
 void run_reduce_count(const uint64_t* keys, uint64_t n, U2* tmp, hipStream_t s) {
-  ReduceOp op{keys, nullptr, n, nullptr, nullptr, nullptr, 0, 0, 0, nullptr};
-  run_scan(op, n, tmp, s, /*emit=*/false);
+  const uint64_t nt = scan_tmp_count(n);
+  if (n)
+    hipLaunchKernelGGL(k_reduce_heads, dim3((uint32_t)nt), dim3(kBlock), 0, s, keys, n, tmp);
+  scan_tiles(tmp, n ? nt : 0, tmp + nt, s);
 }
 
 // Emit for a tile of kScanTile sorted products (same tiling as the count
-// pass, whose scanned tile sums give each tile's first output slot): the
-// tile is staged in LDS with coalesced loads, thread t owns items
-// [t*8, t*8+8), each head sums its run (continuing past the tile end in
-// global memory when the run crosses it), and outputs are staged in LDS and
-// stored coalesced.
+// pass, whose scanned tile sums give each tile's first output slot); the
+// tile's outputs are stored coalesced.
 __global__ __launch_bounds__(kBlock) void k_reduce_emit(ReduceOp op, const U2* tile_offs) {
-  __shared__ uint64_t s_k[kScanTile];
-  __shared__ float s_v[kScanTile];
-  __shared__ U2 scratch[kBlock / kWave];
-  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
-  const uint32_t cnt = (uint32_t)(op.n - base < kScanTile ? op.n - base : kScanTile);
-  for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) {
-    s_k[i] = op.keys[base + i];
-    s_v[i] = op.vals[base + i];
-  }
-  const uint64_t prev_key = base ? op.keys[base - 1] : 0;
-  __syncthreads();
-  // Thread t owns items [t*8, t*8+8); its runs' sums stay in registers.
-  const uint32_t q0 = threadIdx.x * kScanItems;
-  uint64_t ok[kScanItems];
-  float ov[kScanItems];
-  uint32_t nh = 0;
-#pragma unroll
-  for (int e = 0; e < kScanItems; ++e) {
-    const uint32_t i = q0 + e;
-    ok[e] = 0;
-    ov[e] = 0.f;
-    if (i >= cnt) continue;
-    const uint64_t k = s_k[i];
-    const uint64_t pk = i ? s_k[i - 1] : prev_key;
-    if (!(base + i == 0 || k != pk)) continue;
-    KahanF32 acc;
-    acc.add(s_v[i]);
-    uint32_t jj = i + 1;
-    while (jj < cnt && s_k[jj] == k) acc.add(s_v[jj++]);
-    if (jj == cnt) {
-      for (uint64_t g = base + cnt; g < op.n && op.keys[g] == k; ++g) acc.add(op.vals[g]);
-    }
-    ok[nh] = k;
-    ov[nh] = acc.value();
-    ++nh;
-  }
-  U2 tot;
-  const uint32_t o0 = (uint32_t)block_excl_scan_u2(mk2(nh, 0), scratch, &tot).a;
-  const uint32_t nout = (uint32_t)tot.a;
-  // (the scan's barriers separate the reads above from the staging below)
-#pragma unroll
-  for (int e = 0; e < kScanItems; ++e) {
-    if ((uint32_t)e < nh) {
-      s_k[o0 + e] = ok[e];
-      s_v[o0 + e] = ov[e];
-    }
-  }
-  __syncthreads();
+  __shared__ ReduceSmem sm;
+  uint64_t prev;
+  const uint32_t nout = reduce_tile(op.keys, op.vals, op.n, sm, &prev);
   const uint64_t obase = tile_offs[blockIdx.x].a;
   for (uint32_t i = threadIdx.x; i < nout; i += kBlock) {
-    const uint64_t k = s_k[i];
+    const uint64_t k = sm.k[i];
     op.out_row[obase + i] = (uint32_t)(op.r0 + shr64(k, op.ynf));
     op.out_col[obase + i] = op.ytab ? op.ytab[k & op.fmask] : (k & op.fmask);
-    op.out_val[obase + i] = s_v[i];
+    op.out_val[obase + i] = sm.v[i];
   }
 }
 

@@ -1,0 +1,119 @@
+// Reduce-by-key tile core shared by the single-contraction and the batched
+// emit kernels (cub::DeviceReduce::ReduceByKey(Sum) semantics,
+// contraction.cuh:512-537): one output per run of equal sorted keys, the
+// run's values summed in sorted order with compensated FP32 arithmetic.
+#pragma once
+#include "scan.hpp"
+
+namespace sptc {
+
+constexpr int kRedRunsPerThread = (int)(kScanTile / kBlock);  // 8
+
+struct ReduceSmem {
+  uint64_t k[kScanTile];  // the tile's keys, then its outputs' keys
+  float v[kScanTile];     // the tile's values, then its outputs' sums
+  uint8_t head[kScanTile];
+  uint16_t rs[kScanTile + 1];  // tile-local start of each run, rs[nruns] = count
+  U2 scratch[kBlock / kWave];
+  uint64_t tail_end;
+};
+
+// Tile blockIdx.x of kScanTile sorted products.  Runs are found from head
+// flags (key differs from its predecessor), so each run's sum walks a known
+// range: no key compares on the dependent chain, loads issued ahead of the
+// adds.  Runs go to threads round-robin; the run that crosses the tile end
+// continues in global memory up to an end the workgroup finds with
+// coalesced probes.  Leaves the tile's outputs -- one per run starting in
+// it, in key order -- in (sm.k, sm.v)[0 .. nout-1] and returns nout; *prev
+// is the key before the tile (0 for the first tile).
+__device__ __forceinline__ uint32_t reduce_tile(const uint64_t* __restrict__ keys,
+                                                const float* __restrict__ vals, uint64_t n,
+                                                ReduceSmem& sm, uint64_t* prev) {
+  const uint32_t tid = threadIdx.x;
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  const uint32_t cnt = (uint32_t)(n - base < kScanTile ? n - base : kScanTile);
+  for (uint32_t i = tid; i < cnt; i += kBlock) {
+    sm.k[i] = keys[base + i];
+    sm.v[i] = vals[base + i];
+  }
+  const uint64_t prev_key = base ? keys[base - 1] : 0;
+  *prev = prev_key;
+  __syncthreads();
+  for (uint32_t i = tid; i < kScanTile; i += kBlock) {
+    uint8_t h = 0;
+    if (i < cnt) h = (base + i == 0) || sm.k[i] != (i ? sm.k[i - 1] : prev_key);
+    sm.head[i] = h;
+  }
+  __syncthreads();
+  // thread t's flags are bytes [8t, 8t+8): run indices by a block scan
+  const uint64_t f = reinterpret_cast<const uint64_t*>(sm.head)[tid];
+  U2 tot;
+  uint32_t r = (uint32_t)block_excl_scan_u2(mk2((uint64_t)__popcll(f), 0), sm.scratch, &tot).a;
+  const uint32_t nruns = (uint32_t)tot.a;
+  for (uint64_t m = f; m; m &= m - 1) sm.rs[r++] = (uint16_t)(tid * 8 + (__builtin_ctzll(m) >> 3));
+  if (tid == 0) {
+    sm.rs[nruns] = (uint16_t)cnt;
+    sm.tail_end = base + cnt;
+  }
+  __syncthreads();
+  // the last run may continue past the tile: first index with another key
+  if (nruns && base + cnt < n && tid < kWave) {
+    const uint64_t last = sm.k[sm.rs[nruns - 1]];
+    for (uint64_t g0 = base + cnt;; g0 += kWave) {
+      const uint64_t g = g0 + tid;
+      const bool stop = g >= n || keys[g] != last;
+      const uint64_t m = __ballot(stop);
+      if (m) {
+        if (tid == 0) sm.tail_end = g0 + __builtin_ctzll(m);
+        break;
+      }
+    }
+  }
+  __syncthreads();
+  const uint64_t tail_end = sm.tail_end;
+  uint64_t ok[kRedRunsPerThread];
+  float ov[kRedRunsPerThread];
+#pragma unroll
+  for (int q = 0; q < kRedRunsPerThread; ++q) {
+    const uint32_t run = tid + (uint32_t)q * kBlock;
+    ok[q] = 0;
+    ov[q] = 0.f;
+    if (run >= nruns) continue;
+    const uint32_t j0 = sm.rs[run], j1 = sm.rs[run + 1];
+    KahanF32 acc;
+    uint32_t j = j0;
+    for (; j + 4 <= j1; j += 4) {
+      const float a = sm.v[j], b = sm.v[j + 1], c = sm.v[j + 2], d = sm.v[j + 3];
+      acc.add(a);
+      acc.add(b);
+      acc.add(c);
+      acc.add(d);
+    }
+    for (; j < j1; ++j) acc.add(sm.v[j]);
+    if (run + 1 == nruns) {
+      for (uint64_t g = base + cnt; g < tail_end; g += 8) {
+        float t[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) t[e] = g + e < tail_end ? vals[g + e] : 0.f;
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+          if (g + e < tail_end) acc.add(t[e]);
+      }
+    }
+    ok[q] = sm.k[j0];
+    ov[q] = acc.value();
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kRedRunsPerThread; ++q) {
+    const uint32_t run = tid + (uint32_t)q * kBlock;
+    if (run < nruns) {
+      sm.k[run] = ok[q];
+      sm.v[run] = ov[q];
+    }
+  }
+  __syncthreads();
+  return nruns;
+}
+
+}  // namespace sptc
