@@ -149,8 +149,8 @@ typedef enum {
   SPTC_K_SORT_INPUT = 1,  /* radix pass (scatter) over operand keys       */
   SPTC_K_MATCH = 2,       /* index match (reduce + emit)                  */
   SPTC_K_EXPAND = 3,      /* ESC product expansion                        */
-  SPTC_K_SORT_PROD = 4,   /* radix pass (scatter) over product keys,
-                             8-bit digits                                */
+  SPTC_K_SORT_PROD = 4,   /* radix pass (scatter) over product keys:
+                             8-bit digits, or the row-segmented passes  */
   SPTC_K_REDUCE = 5,      /* reduce-by-key (count + emit)                 */
   SPTC_K_DENSE = 6,       /* dense per-row accumulation                   */
   SPTC_K_SORT_COUNT = 7,  /* radix pass digit counting + offset scan      */
