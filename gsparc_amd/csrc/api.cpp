@@ -19,6 +19,12 @@ struct sptc_server_s {
   std::unique_ptr<Server> srv;
 };
 
+struct sptc_plan_s {
+  sptc_opts opts;
+  Operand x, y;
+  double prepare_ms = 0;
+};
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -156,6 +162,124 @@ sptc_status sptc_contract(const sptc_coo* X, const sptc_coo* Y, int32_t nc, cons
   *out = res;
   return SPTC_OK;
 }
+
+sptc_status sptc_plan(const sptc_coo* X, const sptc_coo* Y, int32_t nc, const int32_t* cx,
+                      const int32_t* cy, const sptc_opts* opts, void* stream, sptc_plan_t** out) {
+  if (!out) return fail(SPTC_ERR_BAD_ARGS, "out is NULL");
+  *out = nullptr;
+  sptc_status st;
+  if ((st = check_coo(X, "X")) != SPTC_OK) return st;
+  if ((st = check_coo(Y, "Y")) != SPTC_OK) return st;
+  if (nc < 0 || nc > X->order || nc > Y->order || (nc && (!cx || !cy)))
+    return fail(SPTC_ERR_BAD_ARGS, "bad contracted-mode count");
+  if (!device_ok()) return fail(SPTC_ERR_NO_DEVICE, "no GPU");
+  Layout xl, yl;
+  std::string lmsg;
+  st = make_pair_layouts(X->order, X->dims, Y->order, Y->dims, nc, cx, cy, &xl, &yl, &lmsg);
+  if (st != SPTC_OK) return fail(st, lmsg);
+  sptc_plan_s* p = new (std::nothrow) sptc_plan_s();
+  if (!p) return fail(SPTC_ERR_BAD_ARGS, "host allocation failed");
+  std::memset(&p->opts, 0, sizeof(p->opts));
+  if (opts) p->opts = *opts;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  try {
+    Engine e(s, p->opts);
+    StagedCOO sx, sy;
+    e.record(0);
+    e.stage(*X, &sx);
+    e.stage(*Y, &sy);
+    e.prepare(sx, xl, &p->x);
+    e.prepare(sy, yl, &p->y);
+    e.record(1);
+    SPTC_HIP(hipStreamSynchronize(s));
+    p->prepare_ms = e.elapsed_ms(0, 1);
+  } catch (const HipError& he) {
+    delete p;
+    return from_hip(he);
+  } catch (const std::bad_alloc&) {
+    delete p;
+    return fail(SPTC_ERR_OOM_BUDGET, "host allocation failed");
+  }
+  *out = p;
+  return SPTC_OK;
+}
+
+sptc_status sptc_execute(sptc_plan_t* p, void* stream, sptc_result** out) {
+  if (!p || !out) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");
+  *out = nullptr;
+  sptc_result_s* res = new (std::nothrow) sptc_result_s();
+  if (!res) return fail(SPTC_ERR_BAD_ARGS, "host allocation failed");
+  std::memset(&res->info, 0, sizeof(res->info));
+  try {
+    Engine e(reinterpret_cast<hipStream_t>(stream), p->opts);
+    e.record(0);
+    e.record(1);
+    e.record(2);
+    const sptc_status st = e.contract(p->x, p->y, res);
+    if (st != SPTC_OK) {
+      delete res;
+      return fail(st, e.msg);
+    }
+    std::memcpy(res->info.phase_ms, e.times.ms, sizeof(res->info.phase_ms));
+  } catch (const HipError& he) {
+    delete res;
+    return from_hip(he);
+  } catch (const std::bad_alloc&) {
+    delete res;
+    return fail(SPTC_ERR_OOM_BUDGET, "host allocation failed");
+  }
+  *out = res;
+  return SPTC_OK;
+}
+
+sptc_status sptc_plan_get_info(const sptc_plan_t* p, sptc_plan_info* info) {
+  if (!p || !info) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");
+  std::memset(info, 0, sizeof(*info));
+  info->x_nnz = p->x.nnz;
+  info->y_nnz = p->y.nnz;
+  info->x_nfbits = p->x.lay.nfbits;
+  info->y_nfbits = p->y.lay.nfbits;
+  info->ncbits = p->x.lay.ncbits;
+  info->out_bits = p->x.lay.nfbits + p->y.lay.nfbits;
+  info->out_order = p->x.lay.nf + p->y.lay.nf;
+  info->device_bytes = p->x.keys.size() * 8 + p->x.vals.size() * 4 + p->y.keys.size() * 8 +
+                       p->y.vals.size() * 4;
+  info->prepare_ms = p->prepare_ms;
+  return SPTC_OK;
+}
+
+sptc_status sptc_partition_plan(sptc_plan_t* p, uint64_t budget_bytes, int32_t max_parts,
+                                uint64_t* row_bounds, uint64_t* prod_bounds, int32_t* nparts,
+                                uint64_t* ir_nnz, void* stream) {
+  if (!p || !nparts) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");
+  sptc_opts o = p->opts;
+  if (budget_bytes) o.mem_budget_bytes = budget_bytes;
+  o.dense_accumulator = -1;
+  sptc_result_s res;
+  std::memset(&res.info, 0, sizeof(res.info));
+  try {
+    Engine e(reinterpret_cast<hipStream_t>(stream), o);
+    e.partition_only = true;
+    const sptc_status st = e.contract(p->x, p->y, &res);
+    if (st != SPTC_OK) return fail(st, e.msg);
+  } catch (const HipError& he) {
+    return from_hip(he);
+  } catch (const std::bad_alloc&) {
+    return fail(SPTC_ERR_OOM_BUDGET, "host allocation failed");
+  }
+  const int32_t n = res.part_rows.empty() ? 0 : (int32_t)(res.part_rows.size() - 1);
+  *nparts = n;
+  if (ir_nnz) *ir_nnz = res.info.ir_nnz;
+  if (max_parts >= n) {
+    for (int32_t i = 0; i <= n; ++i) {
+      if (row_bounds) row_bounds[i] = n ? res.part_rows[i] : 0;
+      if (prod_bounds) prod_bounds[i] = n ? res.part_prods[i] : 0;
+    }
+  }
+  return SPTC_OK;
+}
+
+void sptc_plan_destroy(sptc_plan_t* p) { delete p; }
 
 sptc_status sptc_result_get_info(const sptc_result* r, sptc_result_info* info) {
   if (!r || !info) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");

@@ -159,6 +159,14 @@ void Engine::record(int i) {
   ev_used_[i] = true;
 }
 
+double Engine::elapsed_ms(int a, int b) {
+  if (!ev_used_[a] || !ev_used_[b]) return 0.0;
+  SPTC_HIP(hipEventSynchronize(ev_[b]));
+  float t = 0.f;
+  SPTC_HIP(hipEventElapsedTime(&t, ev_[a], ev_[b]));
+  return t;
+}
+
 U2 Engine::read_total(const U2* dev) {
   SPTC_HIP(hipMemcpyAsync(host_total_, dev, sizeof(U2), hipMemcpyDeviceToHost, s_));
   SPTC_HIP(hipStreamSynchronize(s_));
@@ -349,6 +357,7 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       use_dense = o_.dense_accumulator == 1 ||
                   (!o_.count_only && ir * 32 >= R * E && ir >= kDenseMinRun * A);
   }
+  if (partition_only) use_dense = false;  // the plan asked for is ESC's chunking
   if (o_.dense_accumulator == 1 && ir && !use_dense) {
     msg = "dense accumulator does not apply (column space above 2^13 or duplicate Y "
           "coordinates)";
@@ -443,6 +452,14 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       r0 = r1;
     }
     record(4);
+    res->part_rows = bounds_r;
+    res->part_prods = bounds_p;
+    if (partition_only) {  // the plan only: no chunk is accumulated
+      info.nchunks = (int32_t)(bounds_r.size() - 1);
+      bounds_r.resize(1);
+      bounds_p.resize(1);
+      bounds_a.resize(1);
+    }
 
     // ---- (d) ESC accumulate per chunk --------------------------------------
     uint64_t chunk_cap = 0;

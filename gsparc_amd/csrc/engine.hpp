@@ -71,6 +71,9 @@ struct sptc_result_s {
   const uint64_t* vrow_key = nullptr;  // view (batch instance results)
   std::shared_ptr<void> keep;          // owner of the viewed buffers
   std::vector<sptc::OutChunk> chunks;
+  // ESC chunk plan: first X row and first product of each chunk, then the
+  // totals (empty for the dense accumulator)
+  std::vector<uint64_t> part_rows, part_prods;
   const uint64_t* RK() const { return vrow_key ? vrow_key : row_key.get(); }
 };
 
@@ -113,11 +116,15 @@ class Engine {
   uint64_t budget() const { return budget_; }
   hipStream_t stream() const { return s_; }
   void record(int i);
+  // ms between two recorded events (waits for the later one)
+  double elapsed_ms(int a, int b);
   // Collects per-kernel timings (after the stream is synchronized).
   void collect_kernel_stats(sptc::KStat* out) { kt_.collect(out, SPTC_K_COUNT); }
   KTimer& ktimer() { return kt_; }
   PhaseTimes times;
   std::string msg;  // last error detail
+  // contract() stops after the ESC chunk plan (res->part_rows/part_prods)
+  bool partition_only = false;
 
  private:
   U2 read_total(const U2* dev);

@@ -62,6 +62,12 @@ class _BatchInfo(Structure):
                 ("device_bytes", c_uint64), ("plan_ms", c_double)]
 
 
+class _PlanInfo(Structure):
+    _fields_ = [("x_nnz", c_uint64), ("y_nnz", c_uint64), ("x_nfbits", c_int32),
+                ("y_nfbits", c_int32), ("ncbits", c_int32), ("out_bits", c_int32),
+                ("out_order", c_int32), ("device_bytes", c_uint64), ("prepare_ms", c_double)]
+
+
 class _ServerOpts(Structure):
     _fields_ = [("budget_bytes", c_uint64), ("workers", c_int32), ("max_batch", c_int32),
                 ("large_products", c_uint64), ("large_budget_bytes", c_uint64)]
@@ -137,6 +143,16 @@ def lib():
                                               POINTER(c_double), POINTER(c_double)]
         L.sptc_batch_free.argtypes = [c_void_p]
         L.sptc_batch_free.restype = None
+        L.sptc_plan.argtypes = [POINTER(_COO), POINTER(_COO), c_int32, POINTER(c_int32),
+                                POINTER(c_int32), POINTER(_Opts), c_void_p, POINTER(c_void_p)]
+        L.sptc_execute.argtypes = [c_void_p, c_void_p, POINTER(c_void_p)]
+        L.sptc_plan_get_info.argtypes = [c_void_p, POINTER(_PlanInfo)]
+        L.sptc_partition_plan.argtypes = [c_void_p, c_uint64, c_int32, c_void_p, c_void_p,
+                                          POINTER(c_int32), POINTER(c_uint64), c_void_p]
+        L.sptc_plan_destroy.argtypes = [c_void_p]
+        L.sptc_plan_destroy.restype = None
+        for name in ("sptc_plan", "sptc_execute", "sptc_plan_get_info", "sptc_partition_plan"):
+            getattr(L, name).restype = c_int32
         L.sptc_server_create.argtypes = [POINTER(_ServerOpts), POINTER(c_void_p)]
         L.sptc_server_add_tensor.argtypes = [c_void_p, POINTER(_COO), POINTER(c_int32)]
         L.sptc_server_submit.argtypes = [c_void_p, c_int32, c_int32, c_int32, POINTER(c_int32),
@@ -322,6 +338,62 @@ def SpTC(X, Y, cmodes_x, cmodes_y, dense_accumulator=0, count_only=False,
                          c_void_p(stream or 0), byref(out))
     check(st)
     return Result(out.value)
+
+
+class Plan:
+    """Operands laid out, linearized and sorted once (sptc_plan: the
+    reference's ConvertTensor + SortSlitomXY + Partition); :meth:`execute` runs
+    the contraction on them (gsparc::SpTC) as often as wanted and
+    :meth:`partitions` returns the dynamic chunk plan under a budget."""
+
+    def __init__(self, X, Y, cmodes_x, cmodes_y, dense_accumulator=0, count_only=False,
+                 mem_budget_bytes=0, radix_bits=0, kernel_timing=False, stream=None,
+                 on_device=False):
+        L = lib()
+        hx = _HostCOO(X, on_device)
+        hy = X is Y and hx or _HostCOO(Y, on_device)
+        nc = len(cmodes_x)
+        if len(cmodes_y) != nc:
+            raise SpTCError(ERR_BAD_ARGS, "contracted mode lists differ in length")
+        cx = (c_int32 * max(1, nc))(*cmodes_x)
+        cy = (c_int32 * max(1, nc))(*cmodes_y)
+        o = _Opts(int(dense_accumulator), int(bool(count_only)), int(mem_budget_bytes),
+                  int(radix_bits), 0, int(bool(kernel_timing)))
+        h = c_void_p()
+        check(L.sptc_plan(byref(hx.s), byref(hy.s), nc, cx, cy, byref(o),
+                          c_void_p(stream or 0), byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().sptc_plan_destroy(h)
+            self._h = c_void_p(0)
+
+    @property
+    def info(self):
+        pi = _PlanInfo()
+        check(lib().sptc_plan_get_info(self._h, byref(pi)))
+        return {k: getattr(pi, k) for k, _ in _PlanInfo._fields_}
+
+    def execute(self, stream=None):
+        out = c_void_p()
+        check(lib().sptc_execute(self._h, c_void_p(stream or 0), byref(out)))
+        return Result(out.value)
+
+    def partitions(self, budget_bytes=0, stream=None):
+        """(row_bounds, prod_bounds, ir_nnz): first X row and first product
+        of each ESC chunk, then the totals."""
+        L = lib()
+        n, ir = c_int32(), c_uint64()
+        check(L.sptc_partition_plan(self._h, int(budget_bytes), 0, None, None, byref(n),
+                                    byref(ir), c_void_p(stream or 0)))
+        rows = np.zeros(n.value + 1, np.uint64)
+        prods = np.zeros(n.value + 1, np.uint64)
+        check(L.sptc_partition_plan(self._h, int(budget_bytes), n.value,
+                                    rows.ctypes.data, prods.ctypes.data, byref(n), byref(ir),
+                                    c_void_p(stream or 0)))
+        return rows[:n.value + 1], prods[:n.value + 1], ir.value
 
 
 class Batch:

@@ -298,6 +298,53 @@ sptc_status sptc_server_get_stats(sptc_server* s, sptc_server_stats* stats);
 /* Stops the workers (queued requests are dropped) and frees the tensors. */
 void sptc_server_free(sptc_server* s);
 
+/* ---- planned form: operands converted and sorted once -------------------- */
+
+typedef struct sptc_plan_s sptc_plan_t;
+
+typedef struct {
+  uint64_t x_nnz, y_nnz;
+  int32_t x_nfbits, y_nfbits, ncbits, out_bits;
+  int32_t out_order;
+  uint64_t device_bytes;   /* resident operand bytes held by the plan      */
+  double prepare_ms;       /* upload + linearize + sort of both operands   */
+} sptc_plan_info;
+
+/* TensorManager::ConvertTensor (x2) + SortSlitomXY + Partition
+ * (main.cu:293-321, tensor_manager.tpp:212-298,487-522,633-749): lays out,
+ * linearizes and sorts both operands into device memory owned by the plan
+ * (the reference's SLITOM d_indices/d_values).  Host inputs are copied;
+ * device inputs are only read during the call.  The static partitioning of
+ * the reference has no counterpart: chunks are planned per execution from
+ * the product counts (sptc_partition_plan). */
+sptc_status sptc_plan(const sptc_coo* X, const sptc_coo* Y, int32_t nc,
+                      const int32_t* cx, const int32_t* cy,
+                      const sptc_opts* opts, void* stream, sptc_plan_t** out);
+
+/* gsparc::SpTC (sptc.cuh:16-360) on a plan's resident operands: index
+ * match, dynamic partitioning, accumulation.  Each call returns a new
+ * result; phase_ms[0] and [1] are 0 (the operands were prepared by
+ * sptc_plan).  opts.dense_accumulator = 1 / -1 runs the dense / ESC
+ * accumulator (Dense_Contraction / ESC_Contraction, contraction.cuh:
+ * 247-599) alone. */
+sptc_status sptc_execute(sptc_plan_t* p, void* stream, sptc_result** out);
+
+sptc_status sptc_plan_get_info(const sptc_plan_t* p, sptc_plan_info* info);
+
+/* Dynamic_partition (contraction.cuh:143-245) for a plan under a device
+ * budget (0: the plan's): runs the index match and cuts X's rows into
+ * chunks whose products (and, unless count_only, outputs) fit, never
+ * splitting a row.  row_bounds/prod_bounds receive *nparts + 1 entries
+ * (first rows and first products of each chunk, then the totals) when
+ * max_parts is large enough; *nparts is always set.  *ir_nnz receives the
+ * product count.  SPTC_ERR_OOM_BUDGET when one row's products do not fit. */
+sptc_status sptc_partition_plan(sptc_plan_t* p, uint64_t budget_bytes,
+                                int32_t max_parts, uint64_t* row_bounds,
+                                uint64_t* prod_bounds, int32_t* nparts,
+                                uint64_t* ir_nnz, void* stream);
+
+void sptc_plan_destroy(sptc_plan_t* p);
+
 /* ---- per-phase entry points (device pointers, caller-owned buffers) ---- */
 
 /* Key layout of one operand (get_fmode + compute_nbtis + setup_slitom,
