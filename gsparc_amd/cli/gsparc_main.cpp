@@ -157,9 +157,18 @@ int main(int argc, char** argv) {
   uint64_t result_nnz = 0;
   try {
     for (int it = 0; it < iters; ++it) {
-      gsparc::Timers t;
-      gsparc::Result z = gsparc::SpTC(X, Y, nc, cx.data(), cy.data(), dense, &t);
-      result_nnz += z.nnz();
+      // main.cu:293-376's sequence
+      gsparc::TensorManager tm(gpus);
+      tm.FindPartitionNum(X.nnz, Y.nnz, 0, 0);
+      gsparc::SLITOM SX(1), SY(2), SZ;
+      tm.ConvertTensor(&X, &SX, nc, cx.data());
+      tm.ConvertTensor(&Y, &SY, nc, cy.data());
+      tm.SortSlitomXY(&SX, &SY);
+      tm.Partition(&SX);
+      tm.Partition(&SY);
+      gsparc::SpTC(&SX, &SY, &SZ, dense);
+      result_nnz += SZ.nnz;
+      const gsparc::Timers& t = SZ.timers;
       sum.slitom += t.slitom;
       sum.partition += t.partition;
       sum.idxmatch += t.idxmatch;

@@ -204,14 +204,15 @@ sptc_status sptc_plan(const sptc_coo* X, const sptc_coo* Y, int32_t nc, const in
   return SPTC_OK;
 }
 
-sptc_status sptc_execute(sptc_plan_t* p, void* stream, sptc_result** out) {
+sptc_status sptc_execute(sptc_plan_t* p, const sptc_opts* opts, void* stream,
+                         sptc_result** out) {
   if (!p || !out) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");
   *out = nullptr;
   sptc_result_s* res = new (std::nothrow) sptc_result_s();
   if (!res) return fail(SPTC_ERR_BAD_ARGS, "host allocation failed");
   std::memset(&res->info, 0, sizeof(res->info));
   try {
-    Engine e(reinterpret_cast<hipStream_t>(stream), p->opts);
+    Engine e(reinterpret_cast<hipStream_t>(stream), opts ? *opts : p->opts);
     e.record(0);
     e.record(1);
     e.record(2);

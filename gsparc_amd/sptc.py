@@ -145,7 +145,7 @@ def lib():
         L.sptc_batch_free.restype = None
         L.sptc_plan.argtypes = [POINTER(_COO), POINTER(_COO), c_int32, POINTER(c_int32),
                                 POINTER(c_int32), POINTER(_Opts), c_void_p, POINTER(c_void_p)]
-        L.sptc_execute.argtypes = [c_void_p, c_void_p, POINTER(c_void_p)]
+        L.sptc_execute.argtypes = [c_void_p, POINTER(_Opts), c_void_p, POINTER(c_void_p)]
         L.sptc_plan_get_info.argtypes = [c_void_p, POINTER(_PlanInfo)]
         L.sptc_partition_plan.argtypes = [c_void_p, c_uint64, c_int32, c_void_p, c_void_p,
                                           POINTER(c_int32), POINTER(c_uint64), c_void_p]
@@ -363,6 +363,7 @@ class Plan:
         check(L.sptc_plan(byref(hx.s), byref(hy.s), nc, cx, cy, byref(o),
                           c_void_p(stream or 0), byref(h)))
         self._h = h
+        self._opts = o
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -376,9 +377,16 @@ class Plan:
         check(lib().sptc_plan_get_info(self._h, byref(pi)))
         return {k: getattr(pi, k) for k, _ in _PlanInfo._fields_}
 
-    def execute(self, stream=None):
+    def execute(self, dense_accumulator=None, count_only=None, stream=None):
+        """Runs the contraction; the arguments override the plan's options
+        for this call (the reference picks the accumulator per SpTC call)."""
+        o = _Opts.from_buffer_copy(self._opts)
+        if dense_accumulator is not None:
+            o.dense_accumulator = int(dense_accumulator)
+        if count_only is not None:
+            o.count_only = int(bool(count_only))
         out = c_void_p()
-        check(lib().sptc_execute(self._h, c_void_p(stream or 0), byref(out)))
+        check(lib().sptc_execute(self._h, byref(o), c_void_p(stream or 0), byref(out)))
         return Result(out.value)
 
     def partitions(self, budget_bytes=0, stream=None):

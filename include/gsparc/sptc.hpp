@@ -14,6 +14,7 @@
 #ifndef GSPARC_SPTC_HPP_
 #define GSPARC_SPTC_HPP_
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -117,6 +118,111 @@ inline Result SpTC(const SparseTensor<uint64_t, float>& X, const SparseTensor<ui
     timers->total = (ms[0] + ms[5]) * 1e-3;
   }
   return res;
+}
+
+// ---- the reference's call sequence (main.cu:293-376) ----------------------
+//
+//   gsparc::TensorManager tm(1);
+//   tm.FindPartitionNum(X.nnz, Y.nnz, pool_bytes, nbits);
+//   gsparc::SLITOM SX(1), SY(2), SZ;
+//   tm.ConvertTensor(&X, &SX, nc, cmodes_x);
+//   tm.ConvertTensor(&Y, &SY, nc, cmodes_y);
+//   tm.SortSlitomXY(&SX, &SY);
+//   tm.Partition(&SX); tm.Partition(&SY);
+//   gsparc::SpTC(&SX, &SY, &SZ, dense_accumulator);
+//   result_nnz += SZ.nnz;
+//
+// SLITOM (slitom.hpp:27-92) is here an operand handle: after ConvertTensor it
+// names a COO tensor and its contracted modes; SortSlitomXY builds the pair's
+// plan -- both operands linearized and sorted in device memory, shared by SX
+// and SY (sptc_plan); SpTC executes it (sptc_execute) and fills SZ.  There is
+// no static partitioning: the accumulation chunks are planned per execution
+// from the product counts, so FindPartitionNum and Partition only keep the
+// call sequence.
+struct SLITOM {
+  int input_order = 0;  // 1 = X, 2 = Y, 0 = output
+  const SparseTensor<uint64_t, float>* tensor = nullptr;
+  std::vector<int32_t> cmodes;
+  uint64_t nnz = 0;
+  int nprtn = 1;
+  std::shared_ptr<sptc_plan_t> plan;  // set on SX and SY by SortSlitomXY
+  std::shared_ptr<Result> result;     // set on SZ by SpTC
+  Timers timers;                      // SZ: phase times of the SpTC call (s)
+  double slitom_seconds = 0;          // SX, SY: linearize + sort of the pair
+  SLITOM() = default;
+  explicit SLITOM(int order) : input_order(order) {}
+};
+
+class TensorManager {
+ public:
+  explicit TensorManager(int gpu_count = 1) {
+    if (gpu_count != 1) throw Error(SPTC_ERR_BAD_ARGS, "this engine runs on one GPU");
+  }
+  // tensor_manager.tpp:32-61 decides host vs GPU sorting and the static
+  // partition count; operands are always sorted on the GPU here.
+  bool FindPartitionNum(uint64_t, uint64_t, uint64_t, int) const { return true; }
+  // tensor_manager.tpp:212-298: records the tensor and its contracted modes
+  // (linearization happens in SortSlitomXY, on the device).
+  void ConvertTensor(const SparseTensor<uint64_t, float>* t, SLITOM* s, int num_cmodes,
+                     const int* cmodes) const {
+    s->tensor = t;
+    s->cmodes.assign(cmodes, cmodes + num_cmodes);
+    s->nnz = t->nnz;
+  }
+  // tensor_manager.tpp:487-522: lays out, linearizes and sorts both operands.
+  void SortSlitomXY(SLITOM* SX, SLITOM* SY, uint64_t mem_budget_bytes = 0) const {
+    if (!SX->tensor || !SY->tensor || SX->cmodes.size() != SY->cmodes.size())
+      throw Error(SPTC_ERR_BAD_ARGS, "SortSlitomXY: ConvertTensor both operands first");
+    std::vector<const void*> px, py;
+    const sptc_coo cx = as_coo(*SX->tensor, px), cy = as_coo(*SY->tensor, py);
+    sptc_opts o{};
+    o.mem_budget_bytes = mem_budget_bytes;
+    sptc_plan_t* p = nullptr;
+    Result::check(sptc_plan(&cx, &cy, (int32_t)SX->cmodes.size(), SX->cmodes.data(),
+                            SY->cmodes.data(), &o, nullptr, &p));
+    std::shared_ptr<sptc_plan_t> plan(p, sptc_plan_destroy);
+    sptc_plan_info pi{};
+    Result::check(sptc_plan_get_info(p, &pi));
+    SX->plan = SY->plan = plan;
+    SX->slitom_seconds = SY->slitom_seconds = pi.prepare_ms * 1e-3;
+  }
+  // tensor_manager.tpp:633-749: nothing to do (chunks are planned per call).
+  void Partition(SLITOM*) const {}
+
+  static sptc_coo as_coo(const SparseTensor<uint64_t, float>& t, std::vector<const void*>& ptrs) {
+    sptc_coo c{};
+    c.order = t.order;
+    c.nnz = t.nnz;
+    c.dims = t.dims.data();
+    ptrs.resize(t.order);
+    for (int m = 0; m < t.order; ++m) ptrs[m] = t.indices[m].data();
+    c.idx = ptrs.data();
+    c.idx_bytes = 8;
+    c.val = t.values.data();
+    c.on_device = 0;
+    return c;
+  }
+};
+
+// sptc.cuh:16-26 on a SortSlitomXY'd pair: SZ->nnz and SZ->result (the
+// reference's pools, GPU count and per-GPU timers are replaced by the
+// engine's budget and the phase times in SZ->timers).
+inline void SpTC(SLITOM* SX, SLITOM* SY, SLITOM* SZ, int dense_accumulator = 0) {
+  if (!SX->plan || SX->plan != SY->plan)
+    throw Error(SPTC_ERR_BAD_ARGS, "SpTC: SortSlitomXY(SX, SY) first");
+  sptc_opts o{};
+  o.dense_accumulator = dense_accumulator;
+  sptc_result* r = nullptr;
+  Result::check(sptc_execute(SX->plan.get(), &o, nullptr, &r));
+  SZ->result = std::make_shared<Result>(r);
+  SZ->nnz = SZ->result->nnz();
+  const double* ms = SZ->result->info().phase_ms;
+  SZ->timers = Timers{};
+  SZ->timers.slitom = SX->slitom_seconds;
+  SZ->timers.idxmatch = ms[2] * 1e-3;
+  SZ->timers.dynamic_partition = ms[3] * 1e-3;
+  SZ->timers.contraction = ms[4] * 1e-3;
+  SZ->timers.total = SX->slitom_seconds + ms[5] * 1e-3;
 }
 
 }  // namespace gsparc

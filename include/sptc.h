@@ -324,10 +324,12 @@ sptc_status sptc_plan(const sptc_coo* X, const sptc_coo* Y, int32_t nc,
 /* gsparc::SpTC (sptc.cuh:16-360) on a plan's resident operands: index
  * match, dynamic partitioning, accumulation.  Each call returns a new
  * result; phase_ms[0] and [1] are 0 (the operands were prepared by
- * sptc_plan).  opts.dense_accumulator = 1 / -1 runs the dense / ESC
- * accumulator (Dense_Contraction / ESC_Contraction, contraction.cuh:
- * 247-599) alone. */
-sptc_status sptc_execute(sptc_plan_t* p, void* stream, sptc_result** out);
+ * sptc_plan).  opts (NULL: the plan's) apply to this call; as in the
+ * reference the accumulator is chosen here: dense_accumulator = 1 / -1
+ * runs the dense / ESC accumulator (Dense_Contraction / ESC_Contraction,
+ * contraction.cuh:247-599) alone. */
+sptc_status sptc_execute(sptc_plan_t* p, const sptc_opts* opts, void* stream,
+                         sptc_result** out);
 
 sptc_status sptc_plan_get_info(const sptc_plan_t* p, sptc_plan_info* info);
 

@@ -100,3 +100,18 @@ def test_plan_errors(gpu):
     with pytest.raises(sptc.SpTCError) as e:
         sptc.Plan(X, X, [5], [2])
     assert e.value.status == sptc.ERR_BAD_ARGS
+
+
+def test_execute_options_per_call(gpu):
+    """As in the reference (the accumulator is an argument of SpTC, not of
+    the operand preparation), one plan executes with either accumulator."""
+    rng = np.random.default_rng(15)
+    X = random_coo(rng, (40, 12, 30), 9_000)
+    p = sptc.Plan(X, X, [2], [2])
+    for dense in (1, -1, 0):
+        r = p.execute(dense_accumulator=dense)
+        one = sptc.SpTC(X, X, [2], [2], dense_accumulator=dense)
+        assert r.info.accumulator == one.info.accumulator
+        _same(r, one)
+    cnt = p.execute(count_only=True)
+    assert cnt.nnz == one.nnz and cnt.ir_nnz == one.ir_nnz
