@@ -207,7 +207,9 @@ __global__ __launch_bounds__(kBlock) void k_bexpand(BExpandArgs a) {
   __shared__ uint64_t s_pref[TILE + 1];
   __shared__ uint64_t s_ja;
   if (*a.bad) return;
-  for (uint32_t t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+  uint64_t t0, t1, step;
+  xcd_tiles(a.ntiles, a.remap, &t0, &t1, &step);
+  for (uint64_t t = t0; t < t1; t += step) {
     const BETile tl = a.tiles[t];
     const BPiece pc = a.pieces[tl.piece];
     const uint64_t pb = tl.p0;
@@ -286,10 +288,11 @@ struct BReduceOp {
 __global__ __launch_bounds__(kBlock) void k_breduce_emit(BReduceOp op, const U2* tile_offs) {
   __shared__ ReduceSmem sm;
   if (*op.b.bad) return;
-  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  const uint32_t tile = blockIdx.x;
+  const uint64_t base = (uint64_t)tile * kScanTile;
   uint64_t prev_key;
-  const uint32_t nout = reduce_tile(op.keys, op.vals, op.n, sm, &prev_key);
-  const uint64_t obase = tile_offs[blockIdx.x].a;
+  const uint32_t nout = reduce_tile(op.keys, op.vals, op.n, tile, sm, &prev_key);
+  const uint64_t obase = tile_offs[tile].a;
   const uint64_t cm = low_mask(op.K);
   for (uint32_t i = threadIdx.x; i < nout; i += kBlock) {
     const uint64_t k = sm.k[i];
@@ -486,7 +489,9 @@ void run_bmatch(const uint64_t* xkeys, const float* xvals, const uint64_t* xoff,
 void launch_bexpand(const BExpandArgs& a, hipStream_t s) {
   if (!a.ntiles) return;
   const uint32_t grid = std::min<uint32_t>(a.ntiles, 4096);
-  hipLaunchKernelGGL(k_bexpand, dim3(grid), dim3(kBlock), 0, s, a);
+  BExpandArgs b = a;
+  b.remap = xcd_remap_enabled();
+  hipLaunchKernelGGL(k_bexpand, dim3(grid), dim3(kBlock), 0, s, b);
 }
 
 void run_breduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32_t* out_row,

@@ -114,6 +114,7 @@ struct BExpandArgs {
   uint32_t* big_col;
   float* big_val;
   const uint32_t* bad;     // nonzero: operands changed since planning, skip
+  uint32_t remap;          // XCD-contiguous tile order (set by launch_bexpand)
 };
 void launch_bexpand(const BExpandArgs& a, hipStream_t s);
 

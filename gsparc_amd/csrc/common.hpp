@@ -33,6 +33,37 @@ __host__ __device__ __forceinline__ uint64_t low_mask(int bits) {
 }
 
 // ---- device helpers ------------------------------------------------------
+// Workgroups are dispatched to the 8 XCDs round-robin by blockIdx, each XCD
+// with its own L2.  xcd_tile maps block b of n to a tile so that XCD x runs a
+// contiguous range of tiles in order: neighbouring tiles, whose writes meet
+// in the same cache lines, then share one L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
+  const uint32_t per = n / kXcds, rem = n % kXcds;
+  const uint32_t x = b % kXcds, i = b / kXcds;
+  return x * per + (x < rem ? x : rem) + i;
+}
+
+// Persistent-loop form: the tiles of XCD x are one contiguous range that its
+// workgroups walk in turn (tile t0, t0 + step, ... below t1).  With fewer
+// workgroups than XCDs the plain grid-stride order is kept.
+__device__ __forceinline__ void xcd_tiles(uint64_t ntiles, bool remap, uint64_t* t0,
+                                          uint64_t* t1, uint64_t* step) {
+  const uint32_t G = gridDim.x;
+  if (!remap || G < kXcds) {
+    *t0 = blockIdx.x;
+    *t1 = ntiles;
+    *step = G;
+    return;
+  }
+  const uint32_t x = blockIdx.x % kXcds, i = blockIdx.x / kXcds;
+  const uint64_t per = ntiles / kXcds, rem = ntiles % kXcds;
+  const uint64_t lo = x * per + (x < rem ? x : rem);
+  *t0 = lo + i;
+  *t1 = lo + per + (x < rem ? 1 : 0);
+  *step = (G + kXcds - 1 - x) / kXcds;  // workgroups on XCD x
+}
+
 __device__ __forceinline__ uint32_t lane_id() {
   return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
@@ -105,6 +136,9 @@ struct KahanF32 {
   }
   __device__ __forceinline__ float value() const { return s + c; }
 };
+
+// XCD-contiguous tile order on (SPTC_XCD_REMAP=0 turns it off); host side.
+uint32_t xcd_remap_enabled();
 
 // ---- launchers (defined in the .hip files) --------------------------------
 struct SortWorkspace;

@@ -4,6 +4,8 @@
 #include "kernels.hpp"
 #include "reduce_core.hpp"
 
+#include <cstdlib>
+
 namespace sptc {
 
 // --------------------------------------------------------------------------
@@ -389,7 +391,9 @@ __global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs a) {
   double vsum = 0.0;
   const uint64_t nprod = a.P1 - a.P0;
   const uint64_t ntiles = ceil_div(nprod, kExpandTile);
-  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  uint64_t t0, t1, step;
+  xcd_tiles(ntiles, a.remap, &t0, &t1, &step);
+  for (uint64_t tile = t0; tile < t1; tile += step) {
     const uint64_t pb = a.P0 + tile * kExpandTile;
     const uint64_t pe = pb + kExpandTile < a.P1 ? pb + kExpandTile : a.P1;
     __syncthreads();
@@ -466,19 +470,29 @@ __global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs a) {
   }
 }
 
+uint32_t xcd_remap_enabled() {
+  static const uint32_t on = [] {
+    const char* e = getenv("SPTC_XCD_REMAP");
+    return (uint32_t)!(e && std::atoi(e) == 0);
+  }();
+  return on;
+}
+
 void launch_expand(const ExpandArgs& a, int rb, hipStream_t s) {
   const uint64_t nprod = a.P1 - a.P0;
   if (!nprod) return;
   const uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(nprod, kExpandTile), 8192);
+  ExpandArgs b = a;
+  b.remap = xcd_remap_enabled();
   if (!a.passes) {
-    hipLaunchKernelGGL((k_expand<8, false>), dim3(grid), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_expand<8, false>), dim3(grid), dim3(kBlock), 0, s, b);
     return;
   }
   switch (rb) {
-    case 9: hipLaunchKernelGGL((k_expand<9, true>), dim3(grid), dim3(kBlock), 0, s, a); break;
-    case 10: hipLaunchKernelGGL((k_expand<10, true>), dim3(grid), dim3(kBlock), 0, s, a); break;
-    case 11: hipLaunchKernelGGL((k_expand<11, true>), dim3(grid), dim3(kBlock), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_expand<8, true>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 9: hipLaunchKernelGGL((k_expand<9, true>), dim3(grid), dim3(kBlock), 0, s, b); break;
+    case 10: hipLaunchKernelGGL((k_expand<10, true>), dim3(grid), dim3(kBlock), 0, s, b); break;
+    case 11: hipLaunchKernelGGL((k_expand<11, true>), dim3(grid), dim3(kBlock), 0, s, b); break;
+    default: hipLaunchKernelGGL((k_expand<8, true>), dim3(grid), dim3(kBlock), 0, s, b); break;
   }
 }
 
@@ -534,8 +548,9 @@ void run_reduce_count(const uint64_t* keys, uint64_t n, U2* tmp, hipStream_t s) 
 __global__ __launch_bounds__(kBlock) void k_reduce_emit(ReduceOp op, const U2* tile_offs) {
   __shared__ ReduceSmem sm;
   uint64_t prev;
-  const uint32_t nout = reduce_tile(op.keys, op.vals, op.n, sm, &prev);
-  const uint64_t obase = tile_offs[blockIdx.x].a;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t nout = reduce_tile(op.keys, op.vals, op.n, tile, sm, &prev);
+  const uint64_t obase = tile_offs[tile].a;
   for (uint32_t i = threadIdx.x; i < nout; i += kBlock) {
     const uint64_t k = sm.k[i];
     op.out_row[obase + i] = (uint32_t)(op.r0 + shr64(k, op.ynf));

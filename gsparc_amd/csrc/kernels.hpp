@@ -61,6 +61,7 @@ struct ExpandArgs {
   uint64_t* hist;
   int passes;
   double* vsum;
+  uint32_t remap;  // XCD-contiguous tile order (set by launch_expand)
 };
 
 struct DenseArgs {

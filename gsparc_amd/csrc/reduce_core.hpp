@@ -18,7 +18,7 @@ struct ReduceSmem {
   uint64_t tail_end;
 };
 
-// Tile blockIdx.x of kScanTile sorted products.  Runs are found from head
+// Tile `tile` of kScanTile sorted products.  Runs are found from head
 // flags (key differs from its predecessor), so each run's sum walks a known
 // range: no key compares on the dependent chain, loads issued ahead of the
 // adds.  Runs go to threads round-robin; the run that crosses the tile end
@@ -28,9 +28,9 @@ struct ReduceSmem {
 // is the key before the tile (0 for the first tile).
 __device__ __forceinline__ uint32_t reduce_tile(const uint64_t* __restrict__ keys,
                                                 const float* __restrict__ vals, uint64_t n,
-                                                ReduceSmem& sm, uint64_t* prev) {
+                                                uint32_t tile, ReduceSmem& sm, uint64_t* prev) {
   const uint32_t tid = threadIdx.x;
-  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  const uint64_t base = (uint64_t)tile * kScanTile;
   const uint32_t cnt = (uint32_t)(n - base < kScanTile ? n - base : kScanTile);
   for (uint32_t i = tid; i < cnt; i += kBlock) {
     sm.k[i] = keys[base + i];

@@ -39,11 +39,11 @@ template <typename KI, typename KO, int RB, int IPT>
 __global__ __launch_bounds__(kSegNT) void k_seg_downsweep(
     const KI* __restrict__ kin, const float* __restrict__ vin, KO* __restrict__ kout,
     float* __restrict__ vout, int shift, const SegTile* __restrict__ tiles,
-    const uint32_t* __restrict__ offs) {
+    const uint32_t* __restrict__ offs, uint32_t remap) {
   constexpr int R = 1 << RB;
   __shared__ TileSmem<RB, IPT, kSegNT, KI> sm;
   const uint32_t tid = threadIdx.x;
-  const SegTile t = tiles[blockIdx.x];
+  const SegTile t = tiles[remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x];
   for (int i = tid; i < (kSegNT / kWave) * R; i += kSegNT) (&sm.cnt[0][0])[i] = 0;
   for (int d = tid; d < R; d += kSegNT) sm.gbase[d] = offs[t.obase + (uint64_t)d * t.ostride];
   __syncthreads();
@@ -255,6 +255,8 @@ __global__ __launch_bounds__(kSegNT) void k_local_sort(uint64_t* keys, float* va
 
 }  // namespace
 
+static uint32_t xcd_remap() { return xcd_remap_enabled(); }
+
 void SegSortPlan::build(const std::vector<uint64_t>& seg_off, hipStream_t s) {
   std::vector<SegTile> tv;
   std::vector<SegDesc> sv;
@@ -328,7 +330,7 @@ bool segmented_sort(SortWorkspace& ws, const SegTile* tiles, uint64_t ntiles,
     tok = kt ? kt->begin(s) : -1;
     hipLaunchKernelGGL((k_seg_downsweep<uint64_t, uint64_t, kSegRB, kSegIPT>),
                        dim3((uint32_t)ntiles), dim3(kSegNT), 0, s, ka, va, kb, vb, shift, tiles,
-                       ws.rts_off.get());
+                       ws.rts_off.get(), xcd_remap());
     if (kt) kt->end(tok, cls, 24.0 * (double)n, s);
     std::swap(ka, kb);
     std::swap(va, vb);
@@ -456,13 +458,14 @@ static void narrow_sort(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, u
     if (p + 1 < passes) {
       hipLaunchKernelGGL((k_seg_downsweep<uint32_t, uint32_t, RB, IPT>),
                          dim3((uint32_t)ntiles), dim3(kSegNT), 0, s, col[cur], val[cur],
-                         col[1 - cur], val[1 - cur], shift, tiles, ws.rts_off.get());
+                         col[1 - cur], val[1 - cur], shift, tiles, ws.rts_off.get(),
+                         xcd_remap());
       if (kt) kt->end(tok, cls, 16.0 * (double)nb, s);
       cur = 1 - cur;
     } else {
       hipLaunchKernelGGL((k_seg_downsweep<uint32_t, uint64_t, RB, IPT>),
                          dim3((uint32_t)ntiles), dim3(kSegNT), 0, s, col[cur], val[cur], k, v,
-                         shift, tiles, ws.rts_off.get());
+                         shift, tiles, ws.rts_off.get(), xcd_remap());
       if (kt) kt->end(tok, cls, 20.0 * (double)nb, s);
     }
   }

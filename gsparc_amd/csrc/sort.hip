@@ -182,13 +182,13 @@ template <int RB, int IPT, int NT, int TAG>
 __global__ __launch_bounds__(NT) void k_rts_downsweep(
     const uint64_t* __restrict__ kin, const float* __restrict__ vin,
     uint64_t* __restrict__ kout, float* __restrict__ vout, uint64_t n, int shift,
-    const uint32_t* __restrict__ offs, uint64_t ntiles) {
+    const uint32_t* __restrict__ offs, uint64_t ntiles, uint32_t remap) {
   constexpr int R = 1 << RB;
   constexpr int TILE = NT * IPT;
   static_assert(TILE < 65536, "16-bit tile offsets");
   __shared__ TileSmem<RB, IPT, NT> sm;
   const uint32_t tid = threadIdx.x;
-  const uint64_t tile = blockIdx.x;
+  const uint64_t tile = remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
   for (int i = tid; i < (NT / kWave) * R; i += NT) (&sm.cnt[0][0])[i] = 0;
   for (int d = tid; d < R; d += NT) sm.gbase[d] = offs[(uint64_t)d * ntiles + tile];
   __syncthreads();
@@ -301,10 +301,12 @@ struct RtsPass {
     tok = a.kt ? a.kt->begin(a.s) : -1;
     if (a.tag)
       hipLaunchKernelGGL((k_rts_downsweep<RB, IPT, NT, 1>), dim3((uint32_t)a.tiles), dim3(NT), 0,
-                         a.s, a.ka, a.va, a.kb, a.vb, a.n, shift, a.ws->rts_off.get(), a.tiles);
+                         a.s, a.ka, a.va, a.kb, a.vb, a.n, shift, a.ws->rts_off.get(), a.tiles,
+                         xcd_remap_enabled());
     else
       hipLaunchKernelGGL((k_rts_downsweep<RB, IPT, NT, 0>), dim3((uint32_t)a.tiles), dim3(NT), 0,
-                         a.s, a.ka, a.va, a.kb, a.vb, a.n, shift, a.ws->rts_off.get(), a.tiles);
+                         a.s, a.ka, a.va, a.kb, a.vb, a.n, shift, a.ws->rts_off.get(), a.tiles,
+                         xcd_remap_enabled());
     if (a.kt) a.kt->end(tok, a.cls, 24.0 * (double)a.n, a.s);
   }
 };
