@@ -478,6 +478,13 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
     }
     ws_.ensure(chunk_cap, rb_max, s_);
     const uint64_t fmask = low_mask(kyf);
+    // 32-bit column field of Y for the expansion (4 bytes per gathered key
+    // instead of 8)
+    DevBuf<uint32_t> ycol;
+    if (kyf <= 32 && !partition_only) {
+      ycol.reset(ny);
+      launch_ycol(ykeys, ny, fmask, ycol.get(), s_);
+    }
     // row-segmented product sort
     DevBuf<RowRange> rrange(1);
     RowSegPlan rseg;
@@ -523,6 +530,7 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       ea.row_big = nullptr;
       ea.big_col = nullptr;
       ea.big_val = nullptr;
+      ea.ycol = ycol.size() ? ycol.get() : nullptr;
       if (rowsort) {
         const NarrowTarget nt =
             narrow_target(rseg, n, vA.get(), ws_.alt_keys.get(), ws_.alt_vals.get());

@@ -444,7 +444,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs a) {
         }
         const uint64_t y = ybase + p;
         const float v = xv * a.yvals[y];
-        const uint64_t col = a.ykeys[y] & a.fmask;
+        const uint64_t col = a.ycol ? (uint64_t)a.ycol[y] : (a.ykeys[y] & a.fmask);
         key = rbits | col;
         if (big) {
           a.big_col[obase + (p - pb)] = (uint32_t)col;
@@ -468,6 +468,20 @@ __global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs a) {
     for (int w = 0; w < kBlock / kWave; ++w) t += s_sum[w];
     atomicAdd(a.vsum, t);
   }
+}
+
+__global__ void k_ycol(const uint64_t* __restrict__ ykeys, uint64_t ny, uint64_t fmask,
+                       uint32_t* __restrict__ ycol) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < ny; i += stride) ycol[i] = (uint32_t)(ykeys[i] & fmask);
+}
+
+void launch_ycol(const uint64_t* ykeys, uint64_t ny, uint64_t fmask, uint32_t* ycol,
+                 hipStream_t s) {
+  if (!ny) return;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(ny, kBlock), 4096);
+  hipLaunchKernelGGL(k_ycol, dim3(grid), dim3(kBlock), 0, s, ykeys, ny, fmask, ycol);
 }
 
 uint32_t xcd_remap_enabled() {

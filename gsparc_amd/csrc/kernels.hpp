@@ -58,6 +58,8 @@ struct ExpandArgs {
   const uint64_t* row_big;
   uint32_t* big_col;
   float* big_val;
+  // column field of every Y key as 32 bits (NULL: taken from ykeys & fmask)
+  const uint32_t* ycol;
   uint64_t* hist;
   int passes;
   double* vsum;
@@ -119,6 +121,9 @@ void run_reduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32
 void launch_yfree_extract(const uint64_t* ykeys, uint64_t ny, uint64_t fmask, uint64_t* out,
                           hipStream_t s);
 void run_unique(const uint64_t* sorted, uint64_t n, uint64_t* out, U2* tmp, hipStream_t s);
+// ycol[i] = ykeys[i] & fmask (a column field of at most 32 bits).
+void launch_ycol(const uint64_t* ykeys, uint64_t ny, uint64_t fmask, uint32_t* ycol,
+                 hipStream_t s);
 void launch_yrank(const uint64_t* ykeys, uint64_t ny, int ynf, uint64_t fmask,
                   const uint64_t* ytab, uint64_t U, int yrb, uint64_t* out, hipStream_t s);
 void launch_pack_keys(const uint32_t* row, const uint64_t* col, const uint64_t* row_key,
