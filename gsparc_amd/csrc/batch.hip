@@ -262,6 +262,9 @@ __global__ __launch_bounds__(kBlock) void k_bexpand(BExpandArgs a) {
         if (big) {
           a.big_col[tl.out0 + q] = (uint32_t)col;
           a.big_val[tl.out0 + q] = v;
+        } else if (a.row_big) {
+          a.out_col32[tl.out0 + q] = (uint32_t)col;
+          a.out_vals[tl.out0 + q] = v;
         } else {
           a.out_keys[tl.out0 + q] = rbits | col;
           a.out_vals[tl.out0 + q] = v;
@@ -281,34 +284,56 @@ struct BReduceOp {
   float* out_val;
   int K;
   BReduceArgs b;
+  NarrowRows nr;  // narrow form (keys unused)
 };
 
 // Same run detection and compensated summation as the plain reduce; the
 // chunk-local row maps back to a global row through the piece table.
+template <bool NARROW>
 __global__ __launch_bounds__(kBlock) void k_breduce_emit(BReduceOp op, const U2* tile_offs) {
   __shared__ ReduceSmem sm;
   if (*op.b.bad) return;
   const uint32_t tile = blockIdx.x;
   const uint64_t base = (uint64_t)tile * kScanTile;
   uint64_t prev_key;
-  const uint32_t nout = reduce_tile(op.keys, op.vals, op.n, tile, sm, &prev_key);
+  const uint32_t nout = NARROW ? reduce_tile_narrow(op.nr, op.vals, op.n, tile, sm, &prev_key)
+                               : reduce_tile(op.keys, op.vals, op.n, tile, sm, &prev_key);
   const uint64_t obase = tile_offs[tile].a;
   const uint64_t cm = low_mask(op.K);
-  for (uint32_t i = threadIdx.x; i < nout; i += kBlock) {
-    const uint64_t k = sm.k[i];
-    const uint64_t lrow = shr64(k, op.K);
-    const uint32_t row = op.b.map_row[lrow];
-    const uint32_t s = op.b.map_inst[lrow];
-    const BInst in = op.b.inst[s];
-    op.out_row[obase + i] = row - (uint32_t)op.b.inst_R[s];
-    op.out_col[obase + i] = op.b.ytab[in.yt_off + (k & cm)];
+  // the thread's gathers (row and instance maps, then instance and column
+  // tables) are issued a stage at a time for all its outputs, stores last
+  uint64_t key[kRedRunsPerThread], col[kRedRunsPerThread];
+  uint32_t row[kRedRunsPerThread], si[kRedRunsPerThread];
+#pragma unroll
+  for (int q = 0; q < kRedRunsPerThread; ++q) {
+    const uint32_t i = threadIdx.x + (uint32_t)q * kBlock;
+    key[q] = i < nout ? sm.k[i] : 0;
+    const uint64_t lrow = shr64(key[q], op.K);
+    row[q] = i < nout ? op.b.map_row[lrow] : 0;
+    si[q] = i < nout ? op.b.map_inst[lrow] : 0;
+  }
+#pragma unroll
+  for (int q = 0; q < kRedRunsPerThread; ++q) {
+    const uint32_t i = threadIdx.x + (uint32_t)q * kBlock;
+    col[q] = 0;
+    if (i < nout) {
+      col[q] = op.b.ytab[op.b.inst[si[q]].yt_off + (key[q] & cm)];
+      row[q] -= (uint32_t)op.b.inst_R[si[q]];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kRedRunsPerThread; ++q) {
+    const uint32_t i = threadIdx.x + (uint32_t)q * kBlock;
+    if (i >= nout) continue;
+    op.out_row[obase + i] = row[q];
+    op.out_col[obase + i] = col[q];
     op.out_val[obase + i] = sm.v[i];
     if (!op.b.bounds) continue;
     // first output of every instance in (previous output's instance, s]
     int64_t sp = -1;
     if (i > 0) sp = op.b.map_inst[shr64(sm.k[i - 1], op.K)];
     else if (base > 0) sp = op.b.map_inst[shr64(prev_key, op.K)];
-    for (int64_t t = sp + 1; t <= (int64_t)s; ++t) op.b.bounds[t] = obase + i;
+    for (int64_t t = sp + 1; t <= (int64_t)si[q]; ++t) op.b.bounds[t] = obase + i;
   }
 }
 
@@ -498,9 +523,18 @@ void run_breduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint3
                       uint64_t* out_col, float* out_val, int K, const BReduceArgs& b,
                       const U2* tmp, hipStream_t s) {
   if (!n) return;
-  BReduceOp op{keys, vals, n, out_row, out_col, out_val, K, b};
-  hipLaunchKernelGGL(k_breduce_emit, dim3((uint32_t)scan_tmp_count(n)), dim3(kBlock), 0, s, op,
-                     tmp);
+  BReduceOp op{keys, vals, n, out_row, out_col, out_val, K, b, NarrowRows{}};
+  hipLaunchKernelGGL(k_breduce_emit<false>, dim3((uint32_t)scan_tmp_count(n)), dim3(kBlock), 0,
+                     s, op, tmp);
+}
+
+void run_breduce_emit_narrow(const NarrowRows& nr, const float* vals, uint64_t n,
+                             uint32_t* out_row, uint64_t* out_col, float* out_val,
+                             const BReduceArgs& b, const U2* tmp, hipStream_t s) {
+  if (!n) return;
+  BReduceOp op{nullptr, vals, n, out_row, out_col, out_val, 32, b, nr};
+  hipLaunchKernelGGL(k_breduce_emit<true>, dim3((uint32_t)scan_tmp_count(n)), dim3(kBlock), 0,
+                     s, op, tmp);
 }
 
 void launch_bdense_count(const BDenseArgs& a, int eb, uint64_t rows, hipStream_t s) {

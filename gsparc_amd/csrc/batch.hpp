@@ -110,9 +110,12 @@ struct BExpandArgs {
   uint32_t B;
   uint64_t* out_keys;
   float* out_vals;
-  const uint64_t* row_big;  // chunk-local row -> big segment (products to big_col/big_val)
+  // narrow form (row_big non-null): products go out as 32-bit columns, to
+  // (big_col, big_val) for big rows and (out_col32, out_vals) for the others
+  const uint64_t* row_big;  // chunk-local row -> big segment
   uint32_t* big_col;
   float* big_val;
+  uint32_t* out_col32;
   const uint32_t* bad;     // nonzero: operands changed since planning, skip
   uint32_t remap;          // XCD-contiguous tile order (set by launch_bexpand)
 };
@@ -142,6 +145,10 @@ void launch_dense_bounds(const uint64_t* grp_rows, const uint32_t* grp_inst, uin
 void run_breduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32_t* out_row,
                       uint64_t* out_col, float* out_val, int K, const BReduceArgs& b,
                       const U2* tmp, hipStream_t s);
+// The same over a chunk's narrow form (keys: row << 32 | column).
+void run_breduce_emit_narrow(const NarrowRows& nr, const float* vals, uint64_t n,
+                             uint32_t* out_row, uint64_t* out_col, float* out_val,
+                             const BReduceArgs& b, const U2* tmp, hipStream_t s);
 
 // Dense accumulation over a group of instances sharing one slot width.
 struct BDenseArgs {

@@ -362,10 +362,16 @@ sptc_status BatchEngine::build_chunks(hipStream_t s) {
       build_row_segments(rr.get(), (uint32_t)c.hr.size(), c.lrows, row_prod_.get(), c.K,
                          &c.rseg, ws_, tmp_.get(), s);
       c.rowsort = true;
-      // keep the per-row big-segment flags the expansion reads, release the rest
+      // keep the per-row big-segment flags the expansion reads and the row
+      // starts of the narrow form, release the rest
       c.row_big.reset(c.lrows);
       SPTC_HIP(hipMemcpyAsync(c.row_big.get(), c.rseg.row_big(), c.lrows * 8,
                               hipMemcpyDeviceToDevice, s));
+      c.row_start.reset(c.lrows + 1);
+      SPTC_HIP(hipMemcpyAsync(c.row_start.get(), c.rseg.row_start(), (c.lrows + 1) * 8,
+                              hipMemcpyDeviceToDevice, s));
+      c.tile_row.reset(2 * (scan_tmp_count(c.n) + 1));
+      launch_tile_rows(c.row_start.get(), c.lrows, c.tile_row.get(), s);
       SPTC_HIP(hipStreamSynchronize(s));  // rr and the scratch are released below
       c.rseg.scratch.release();
     }
@@ -564,16 +570,17 @@ sptc_status BatchEngine::run(hipStream_t s, bool P) {
       ea.row_big = c.row_big.get();
       ea.big_col = nt.col;
       ea.big_val = nt.val;
+      ea.out_col32 = reinterpret_cast<uint32_t*>(kA_.get());
     }
     ea.bad = bad_.get();
     int tok = kt ? kt->begin(s) : -1;
     launch_bexpand(ea, s);
-    const double wide = c.rowsort ? (double)c.rseg.nsmall : (double)c.n;
-    if (kt) kt->end(tok, SPTC_K_EXPAND, 20.0 * (double)c.n + 4.0 * wide, s);
+    if (kt) kt->end(tok, SPTC_K_EXPAND, (c.rowsort ? 16.0 : 24.0) * (double)c.n, s);
     bool alt = false;
     if (c.rowsort) {
-      sort_row_segments(ws_, c.rseg, c.n, kA_.get(), vA_.get(), ws_.alt_keys.get(),
-                        ws_.alt_vals.get(), s, kt, SPTC_K_SORT_PROD);
+      sort_row_segments(ws_, c.rseg, c.n, reinterpret_cast<uint32_t*>(kA_.get()), vA_.get(),
+                        ws_.alt_keys.get(), ws_.alt_vals.get(), c.row_start.get(), s, kt,
+                        SPTC_K_SORT_PROD);
     } else {
       ws_.ensure(c.n, c.rb, s);
       const int passes = radix_passes(c.bits, c.rb);
@@ -583,9 +590,12 @@ sptc_status BatchEngine::run(hipStream_t s, bool P) {
     }
     const uint64_t* sk = alt ? ws_.alt_keys.get() : kA_.get();
     const float* sv = alt ? ws_.alt_vals.get() : vA_.get();
+    const NarrowRows nr{reinterpret_cast<const uint32_t*>(kA_.get()), c.row_start.get(),
+                        c.lrows, c.tile_row.get()};
     tok = kt ? kt->begin(s) : -1;
-    run_reduce_count(sk, c.n, tmp_.get(), s);
-    if (kt) kt->end(tok, SPTC_K_REDUCE, 8.0 * (double)c.n, s);
+    if (c.rowsort) run_reduce_count_narrow(nr, c.n, tmp_.get(), s);
+    else run_reduce_count(sk, c.n, tmp_.get(), s);
+    if (kt) kt->end(tok, SPTC_K_REDUCE, (c.rowsort ? 4.0 : 8.0) * (double)c.n, s);
     if (P) {
       U2 t;
       readback(&t, tmp_.get() + scan_tmp_count(c.n), sizeof(U2), s);
@@ -622,9 +632,15 @@ sptc_status BatchEngine::run(hipStream_t s, bool P) {
     rb.B = B;
     rb.bad = bad_.get();
     tok = kt ? kt->begin(s) : -1;
-    run_breduce_emit(sk, sv, c.n, ob.row.get(), ob.col.get(), ob.val.get(), c.K, rb, tmp_.get(),
-                     s);
-    if (kt) kt->end(tok, SPTC_K_REDUCE, 12.0 * (double)c.n + 16.0 * (double)ob.nnz, s);
+    if (c.rowsort)
+      run_breduce_emit_narrow(nr, vA_.get(), c.n, ob.row.get(), ob.col.get(), ob.val.get(), rb,
+                              tmp_.get(), s);
+    else
+      run_breduce_emit(sk, sv, c.n, ob.row.get(), ob.col.get(), ob.val.get(), c.K, rb,
+                       tmp_.get(), s);
+    if (kt)
+      kt->end(tok, SPTC_K_REDUCE, (c.rowsort ? 8.0 : 12.0) * (double)c.n + 16.0 * (double)ob.nnz,
+              s);
   }
 
   // ---- per-instance slices (rows are already instance-local) ---------------------

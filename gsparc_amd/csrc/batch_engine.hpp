@@ -50,7 +50,9 @@ struct EscChunk {
   DevBuf<uint32_t> map_row, map_inst;  // chunk-local row -> global row, instance
   bool rowsort = false;
   RowSegPlan rseg;
-  DevBuf<uint64_t> row_big;  // chunk-local row -> big segment (row-segmented sort)
+  DevBuf<uint64_t> row_big;    // chunk-local row -> big segment (row-segmented sort)
+  DevBuf<uint64_t> row_start;  // chunk-local row -> first product; [lrows] = n
+  DevBuf<uint32_t> tile_row;   // rows at the reduce tiles' starts (NarrowRows)
   int out = -1;
 };
 

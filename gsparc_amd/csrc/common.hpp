@@ -140,6 +140,20 @@ struct KahanF32 {
 // XCD-contiguous tile order on (SPTC_XCD_REMAP=0 turns it off); host side.
 uint32_t xcd_remap_enabled();
 
+// Narrow form of a chunk's sorted products: 32-bit columns (+ values)
+// grouped by row, rows given by their first products rstart[0..nrows]
+// (non-decreasing, rstart[nrows] = product count).  A product's key is
+// (row << 32 | column); run heads are where the column changes or a row
+// starts.
+struct NarrowRows {
+  const uint32_t* cols;
+  const uint64_t* rstart;
+  uint64_t nrows;
+  // per reduce tile t (kScanTile products from t * kScanTile): [2t] = row of
+  // its first product, [2t + 1] = row of the product before it
+  const uint32_t* tile_row;
+};
+
 // ---- launchers (defined in the .hip files) --------------------------------
 struct SortWorkspace;
 

@@ -488,6 +488,7 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
     // row-segmented product sort
     DevBuf<RowRange> rrange(1);
     RowSegPlan rseg;
+    DevBuf<uint32_t> trow(2 * (scan_tmp_count(chunk_cap) + 1));
     for (size_t c = 1; c < bounds_r.size(); ++c) {
       const uint64_t r0c = bounds_r[c - 1], r1c = bounds_r[c];
       const uint64_t P0 = bounds_p[c - 1], P1 = bounds_p[c];
@@ -530,13 +531,17 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       ea.row_big = nullptr;
       ea.big_col = nullptr;
       ea.big_val = nullptr;
+      ea.out_col32 = nullptr;
       ea.ycol = ycol.size() ? ycol.get() : nullptr;
+      // narrow form: sorted columns land in kA's first 4n bytes
+      uint32_t* fcol = reinterpret_cast<uint32_t*>(kA.get());
       if (rowsort) {
         const NarrowTarget nt =
             narrow_target(rseg, n, vA.get(), ws_.alt_keys.get(), ws_.alt_vals.get());
         ea.row_big = rseg.row_big();
         ea.big_col = nt.col;
         ea.big_val = nt.val;
+        ea.out_col32 = fcol;
       }
       ea.hist = ws_.hist.get();
       const bool hist = sort_needs_histogram();
@@ -544,14 +549,15 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       ea.vsum = vsum.get();
       int tok = kt_.begin(s_);
       launch_expand(ea, rb, s_);
-      // matched Y (key, value) read; product (key, value) written, 8 bytes
-      // instead of 12 for the big rows of a row-segmented sort
-      const double wide = rowsort ? (double)rseg.nsmall : (double)n;
-      kt_.end(tok, SPTC_K_EXPAND, 20.0 * (double)n + 4.0 * wide, s_);
+      // matched Y (column, value) read; product written: column + value in
+      // the narrow form of a row-segmented sort, 64-bit key + value otherwise
+      kt_.end(tok, SPTC_K_EXPAND, rowsort ? 16.0 * (double)n : 24.0 * (double)n, s_);
       bool alt = false;
+      const NarrowRows nr{fcol, rowsort ? rseg.row_start() : nullptr, r1c - r0c, trow.get()};
       if (rowsort) {
-        sort_row_segments(ws_, rseg, n, kA.get(), vA.get(), ws_.alt_keys.get(),
-                          ws_.alt_vals.get(), s_, &kt_, SPTC_K_SORT_PROD);
+        launch_tile_rows(rseg.row_start(), r1c - r0c, trow.get(), s_);
+        sort_row_segments(ws_, rseg, n, fcol, vA.get(), ws_.alt_keys.get(), ws_.alt_vals.get(),
+                          rseg.row_start(), s_, &kt_, SPTC_K_SORT_PROD);
       } else {
         ws_.ensure(n, rb, s_);
         if (hist && passes > kMaxFusedPasses) histogram(ws_, kA.get(), n, passes, s_);
@@ -560,8 +566,9 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       const uint64_t* sk = alt ? ws_.alt_keys.get() : kA.get();
       const float* sv = alt ? ws_.alt_vals.get() : vA.get();
       tok = kt_.begin(s_);
-      run_reduce_count(sk, n, tmp.get(), s_);
-      kt_.end(tok, SPTC_K_REDUCE, 8.0 * (double)n, s_);
+      if (rowsort) run_reduce_count_narrow(nr, n, tmp.get(), s_);
+      else run_reduce_count(sk, n, tmp.get(), s_);
+      kt_.end(tok, SPTC_K_REDUCE, (rowsort ? 4.0 : 8.0) * (double)n, s_);
       const uint64_t out_n = read_total(tmp.get() + scan_tmp_count(n)).a;
       info.nnz += out_n;
       info.nchunks++;
@@ -578,9 +585,14 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       oc.col.reset(out_n);
       oc.val.reset(out_n);
       tok = kt_.begin(s_);
-      run_reduce_emit(sk, sv, n, oc.row.get(), oc.col.get(), oc.val.get(), r0c, kyf, fmask,
-                      ytab.size() ? ytab.get() : nullptr, tmp.get(), s_);
-      kt_.end(tok, SPTC_K_REDUCE, 12.0 * (double)n + 16.0 * (double)out_n, s_);
+      if (rowsort)
+        run_reduce_emit_narrow(nr, vA.get(), n, oc.row.get(), oc.col.get(), oc.val.get(), r0c,
+                               ytab.size() ? ytab.get() : nullptr, tmp.get(), s_);
+      else
+        run_reduce_emit(sk, sv, n, oc.row.get(), oc.col.get(), oc.val.get(), r0c, kyf, fmask,
+                        ytab.size() ? ytab.get() : nullptr, tmp.get(), s_);
+      kt_.end(tok, SPTC_K_REDUCE, (rowsort ? 8.0 : 12.0) * (double)n + 16.0 * (double)out_n,
+              s_);
       res->chunks.push_back(std::move(oc));
     }
   } else {

@@ -449,6 +449,9 @@ __global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs a) {
         if (big) {
           a.big_col[obase + (p - pb)] = (uint32_t)col;
           a.big_val[obase + (p - pb)] = v;
+        } else if (a.row_big) {
+          a.out_col32[obase + (p - pb)] = (uint32_t)col;
+          a.out_vals[obase + (p - pb)] = v;
         } else {
           a.out_keys[obase + (p - pb)] = key;
           a.out_vals[obase + (p - pb)] = v;
@@ -525,6 +528,7 @@ struct ReduceOp {
   int ynf;             // width of the key's column field
   uint64_t fmask;      // its mask
   const uint64_t* ytab;  // rank -> Y free key (NULL: the field is the key)
+  NarrowRows nr;         // narrow form (keys unused): columns + row starts
 };
 
 // Per tile of kScanTile sorted keys: the number of run heads (key differs
@@ -559,17 +563,35 @@ void run_reduce_count(const uint64_t* keys, uint64_t n, U2* tmp, hipStream_t s) 
 // Emit for a tile of kScanTile sorted products (same tiling as the count
 // pass, whose scanned tile sums give each tile's first output slot); the
 // tile's outputs are stored coalesced.
+template <bool NARROW>
 __global__ __launch_bounds__(kBlock) void k_reduce_emit(ReduceOp op, const U2* tile_offs) {
   __shared__ ReduceSmem sm;
   uint64_t prev;
   const uint32_t tile = blockIdx.x;
-  const uint32_t nout = reduce_tile(op.keys, op.vals, op.n, tile, sm, &prev);
+  const uint32_t nout = NARROW ? reduce_tile_narrow(op.nr, op.vals, op.n, tile, sm, &prev)
+                               : reduce_tile(op.keys, op.vals, op.n, tile, sm, &prev);
   const uint64_t obase = tile_offs[tile].a;
-  for (uint32_t i = threadIdx.x; i < nout; i += kBlock) {
-    const uint64_t k = sm.k[i];
-    op.out_row[obase + i] = (uint32_t)(op.r0 + shr64(k, op.ynf));
-    op.out_col[obase + i] = op.ytab ? op.ytab[k & op.fmask] : (k & op.fmask);
-    op.out_val[obase + i] = sm.v[i];
+  // the thread's column-table gathers are all issued before its stores
+  const uint64_t* __restrict__ ytab = op.ytab;
+  uint32_t* __restrict__ orow = op.out_row + obase;
+  uint64_t* __restrict__ ocol = op.out_col + obase;
+  float* __restrict__ oval = op.out_val + obase;
+  uint64_t key[kRedRunsPerThread], col[kRedRunsPerThread];
+#pragma unroll
+  for (int q = 0; q < kRedRunsPerThread; ++q) {
+    const uint32_t i = threadIdx.x + (uint32_t)q * kBlock;
+    key[q] = i < nout ? sm.k[i] : 0;
+    col[q] = key[q] & op.fmask;
+    if (ytab && i < nout) col[q] = ytab[col[q]];
+  }
+#pragma unroll
+  for (int q = 0; q < kRedRunsPerThread; ++q) {
+    const uint32_t i = threadIdx.x + (uint32_t)q * kBlock;
+    if (i < nout) {
+      orow[i] = (uint32_t)(op.r0 + shr64(key[q], op.ynf));
+      ocol[i] = col[q];
+      oval[i] = sm.v[i];
+    }
   }
 }
 
@@ -577,9 +599,98 @@ void run_reduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32
                      uint64_t* out_col, float* out_val, uint64_t r0, int ynf,
                      uint64_t fmask, const uint64_t* ytab, const U2* tmp, hipStream_t s) {
   if (!n) return;
-  ReduceOp op{keys, vals, n, out_row, out_col, out_val, r0, ynf, fmask, ytab};
-  hipLaunchKernelGGL(k_reduce_emit, dim3((uint32_t)scan_tmp_count(n)), dim3(kBlock), 0, s, op,
-                     tmp);
+  ReduceOp op{keys, vals, n, out_row, out_col, out_val, r0, ynf, fmask, ytab, NarrowRows{}};
+  hipLaunchKernelGGL(k_reduce_emit<false>, dim3((uint32_t)scan_tmp_count(n)), dim3(kBlock), 0,
+                     s, op, tmp);
+}
+
+void run_reduce_emit_narrow(const NarrowRows& nr, const float* vals, uint64_t n,
+                            uint32_t* out_row, uint64_t* out_col, float* out_val, uint64_t r0,
+                            const uint64_t* ytab, const U2* tmp, hipStream_t s) {
+  if (!n) return;
+  ReduceOp op{nullptr, vals, n, out_row, out_col, out_val, r0, 32, 0xFFFFFFFFull, ytab, nr};
+  hipLaunchKernelGGL(k_reduce_emit<true>, dim3((uint32_t)scan_tmp_count(n)), dim3(kBlock), 0,
+                     s, op, tmp);
+}
+
+// Narrow form of k_reduce_heads: a head where the column changes or a row
+// starts (row starts of the tile marked in LDS first; columns staged in LDS
+// so every predecessor is one LDS read).
+__global__ __launch_bounds__(kBlock) void k_reduce_heads_narrow(NarrowRows nr, uint64_t n,
+                                                                U2* __restrict__ tile_sums) {
+  __shared__ U2 scratch[kBlock / kWave];
+  __shared__ uint8_t flag[kScanTile];
+  __shared__ uint32_t kc[kScanTile];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  const uint64_t end = n - base < kScanTile ? n : base + kScanTile;
+  const uint64_t r0 = nr.tile_row[2 * (uint64_t)blockIdx.x];
+  const uint64_t r1 = r0 + 1 + threadIdx.x;
+  const uint64_t st1 = r1 < nr.nrows ? nr.rstart[r1] : ~0ull;
+  const bool at0 = nr.rstart[r0] == base;  // a row starts at the tile's first product
+  uint32_t kreg[kScanItems];
+#pragma unroll
+  for (int e = 0; e < kScanItems; ++e) {
+    const uint64_t i = base + (uint64_t)e * kBlock + threadIdx.x;
+    kreg[e] = i < n ? nr.cols[i] : 0;
+  }
+  const uint32_t pk0 = base > 0 ? nr.cols[base - 1] : 0;
+#pragma unroll
+  for (int e = 0; e < kScanItems; ++e) kc[(uint32_t)e * kBlock + threadIdx.x] = kreg[e];
+  for (uint32_t i = threadIdx.x; i < kScanTile; i += kBlock) flag[i] = 0;
+  // rows starting inside the tile (any shows in thread 0's candidate)
+  if (__syncthreads_or(st1 < end)) {
+    for (uint64_t r = r1; r < nr.nrows; r += kBlock) {
+      const uint64_t st = r == r1 ? st1 : nr.rstart[r];
+      if (st >= end) break;
+      flag[st - base] = 1;
+    }
+  }
+  if (threadIdx.x == 0 && at0) flag[0] = 1;
+  __syncthreads();
+  uint64_t c = 0;
+#pragma unroll
+  for (int e = 0; e < kScanItems; ++e) {
+    const uint32_t j = (uint32_t)e * kBlock + threadIdx.x;
+    const uint64_t i = base + j;
+    const uint32_t k = kreg[e];
+    const uint32_t pk = j ? kc[j - 1] : pk0;
+    c += i < n && (i == 0 || flag[j] || k != pk);
+  }
+  U2 tot;
+  block_excl_scan_u2(mk2(c, 0), scratch, &tot);
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+// One thread per row r with products [s, e), no search: reduce tiles whose
+// first product lies in [s, e) get [2t] = r, those whose preceding product
+// lies in [s, e) get [2t + 1] = r (tile_row holds 2 * (tiles + 1) entries).
+__global__ __launch_bounds__(kBlock) void k_tile_rows(const uint64_t* __restrict__ rstart,
+                                                      uint64_t nrows,
+                                                      uint32_t* __restrict__ tile_row) {
+  uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (; r < nrows; r += stride) {
+    const uint64_t s = rstart[r], e = rstart[r + 1];
+    if (e <= s) continue;
+    for (uint64_t t = ceil_div(s, kScanTile); t * kScanTile < e; ++t) tile_row[2 * t] = (uint32_t)r;
+    for (uint64_t t = ceil_div(s + 1, kScanTile); t * kScanTile < e + 1; ++t)
+      tile_row[2 * t + 1] = (uint32_t)r;
+  }
+}
+
+void launch_tile_rows(const uint64_t* rstart, uint64_t nrows, uint32_t* tile_row,
+                      hipStream_t s) {
+  if (!nrows) return;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(nrows, kBlock), 4096);
+  hipLaunchKernelGGL(k_tile_rows, dim3(grid), dim3(kBlock), 0, s, rstart, nrows, tile_row);
+}
+
+void run_reduce_count_narrow(const NarrowRows& nr, uint64_t n, U2* tmp, hipStream_t s) {
+  const uint64_t nt = scan_tmp_count(n);
+  if (n)
+    hipLaunchKernelGGL(k_reduce_heads_narrow, dim3((uint32_t)nt), dim3(kBlock), 0, s, nr, n,
+                       tmp);
+  scan_tiles(tmp, n ? nt : 0, tmp + nt, s);
 }
 
 // --------------------------------------------------------------------------

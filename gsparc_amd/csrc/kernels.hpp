@@ -53,11 +53,14 @@ struct ExpandArgs {
   uint64_t fmask;
   uint64_t* out_keys;
   float* out_vals;
-  // row-segmented sort: products of rows with row_big[row - r0] != 0 go to
-  // (big_col, big_val) as 32-bit columns instead (row_big null: none)
+  // row-segmented sort (narrow form): products go out as 32-bit columns,
+  // to (big_col, big_val) for rows with row_big[row - r0] != 0 and to
+  // (out_col32, out_vals) for the others (row_big null: 64-bit keys to
+  // out_keys)
   const uint64_t* row_big;
   uint32_t* big_col;
   float* big_val;
+  uint32_t* out_col32;
   // column field of every Y key as 32 bits (NULL: taken from ykeys & fmask)
   const uint32_t* ycol;
   uint64_t* hist;
@@ -118,6 +121,15 @@ void run_reduce_count(const uint64_t* keys, uint64_t n, U2* tmp, hipStream_t s);
 void run_reduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32_t* out_row,
                      uint64_t* out_col, float* out_val, uint64_t r0, int ynf,
                      uint64_t fmask, const uint64_t* ytab, const U2* tmp, hipStream_t s);
+// NarrowRows::tile_row for a chunk of rstart[nrows] products: 2 entries per
+// reduce tile, 2 * (scan_tmp_count(n) + 1) in all.
+void launch_tile_rows(const uint64_t* rstart, uint64_t nrows, uint32_t* tile_row,
+                      hipStream_t s);
+// Narrow-form count and emit (outputs' rows are r0 + chunk-local row).
+void run_reduce_count_narrow(const NarrowRows& nr, uint64_t n, U2* tmp, hipStream_t s);
+void run_reduce_emit_narrow(const NarrowRows& nr, const float* vals, uint64_t n,
+                            uint32_t* out_row, uint64_t* out_col, float* out_val, uint64_t r0,
+                            const uint64_t* ytab, const U2* tmp, hipStream_t s);
 void launch_yfree_extract(const uint64_t* ykeys, uint64_t ny, uint64_t fmask, uint64_t* out,
                           hipStream_t s);
 void run_unique(const uint64_t* sorted, uint64_t n, uint64_t* out, U2* tmp, hipStream_t s);

@@ -33,8 +33,7 @@ __global__ __launch_bounds__(kSegNT) void k_seg_upsweep(const KT* __restrict__ k
   for (int d = threadIdx.x; d < R; d += kSegNT) cnt[(uint64_t)blockIdx.x * R + d] = sh[d];
 }
 
-// KI -> KO: 64 -> 64 (operand sorts), 32 -> 32 (column passes of big rows),
-// 32 -> 64 (their last pass, which ORs in the tile's row bits).
+// KI -> KO: 64 -> 64 (operand sorts), 32 -> 32 (column passes of big rows).
 template <typename KI, typename KO, int RB, int IPT>
 __global__ __launch_bounds__(kSegNT) void k_seg_downsweep(
     const KI* __restrict__ kin, const float* __restrict__ vin, KO* __restrict__ kout,
@@ -57,9 +56,7 @@ __global__ __launch_bounds__(kSegNT) void k_seg_downsweep(
   for (int d = tid; d < R; d += kSegNT)
     sm.dstart[d] = (uint16_t)wave_prefixes<RB, IPT, kSegNT>(sm, d);
   __syncthreads();
-  const KO hi = sizeof(KO) > sizeof(KI) ? (KO)t.hi : (KO)0;
-  tile_stage_scatter<RB, IPT, kSegNT>(sm, end, shift, t.start, wbase, k, v, pos, kout, vout,
-                                      hi);
+  tile_stage_scatter<RB, IPT, kSegNT>(sm, end, shift, t.start, wbase, k, v, pos, kout, vout);
 }
 
 // Per segment (one workgroup, thread d = digit d): absolute destination of
@@ -105,7 +102,8 @@ __device__ __forceinline__ uint64_t rowseg_locate(const RowRange* ranges, uint32
 __global__ __launch_bounds__(kBlock) void k_rowseg_sizes(const RowRange* ranges, uint32_t nr,
                                                          uint64_t lrows, const uint64_t* row_prod,
                                                          uint64_t tile, uint64_t* small_n,
-                                                         uint64_t* big_t, uint64_t* big_f) {
+                                                         uint64_t* big_t, uint64_t* big_f,
+                                                         uint64_t* rstart) {
   uint64_t l = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (; l < lrows; l += stride) {
@@ -115,6 +113,8 @@ __global__ __launch_bounds__(kBlock) void k_rowseg_sizes(const RowRange* ranges,
     small_n[l] = big ? 0 : n;
     big_t[l] = big ? ceil_div(n, tile) : 0;
     big_f[l] = big ? 1 : 0;
+    rstart[l] = first;
+    if (l + 1 == lrows) rstart[lrows] = first + n;
   }
 }
 
@@ -134,8 +134,8 @@ __global__ __launch_bounds__(kBlock) void k_rowseg_heads(uint64_t lrows, const u
 }
 
 __global__ __launch_bounds__(kBlock) void k_rowseg_emit(
-    const RowRange* ranges, uint32_t nr, uint64_t lrows, const uint64_t* row_prod, int col_bits,
-    int rb, uint64_t tile,
+    const RowRange* ranges, uint32_t nr, uint64_t lrows, const uint64_t* row_prod, int rb,
+    uint64_t tile,
     const uint64_t* big_t, const uint64_t* bt, const uint64_t* bs, const uint64_t* head,
     const uint64_t* hs, LocalSeg* local, SegTile* big, SegDesc* bigseg) {
   uint64_t l = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -157,13 +157,15 @@ __global__ __launch_bounds__(kBlock) void k_rowseg_emit(
         d.count = (uint32_t)(n - t * tile < tile ? n - t * tile : tile);
         d.obase = (1ull << rb) * o + t;
         d.ostride = (uint32_t)nt;
-        d.hi = shl64(l, col_bits);
         big[o + t] = d;
       }
     } else {
       // the group holding row l: its head is the last head at or before l
       const uint64_t gidx = hs[l] + head[l] - 1;
-      if (head[l]) local[gidx].start = first;
+      if (head[l]) {
+        local[gidx].start = first;
+        local[gidx].row0 = (uint32_t)l;
+      }
       if (n) atomicAdd(reinterpret_cast<unsigned int*>(&local[gidx].count), (unsigned int)n);
     }
   }
@@ -173,36 +175,81 @@ __global__ void k_zero_local(LocalSeg* local, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     local[i].count = 0;
-    local[i].pad = 0;
+    local[i].row0 = 0;
   }
 }
 
-// One workgroup per small group: load, LSD passes over the bits that vary
-// inside the group entirely in LDS (stable), store in place.
-__global__ __launch_bounds__(kSegNT) void k_local_sort(uint64_t* keys, float* vals,
-                                                       const LocalSeg* __restrict__ segs) {
+// One workgroup per small group of (32-bit column, value) products of the
+// consecutive rows row0..: each product's row within the group comes from
+// the row starts (the last row starting at or before it: marks, then a
+// prefix maximum over the wave-striped layout), the composite key
+// (row in group << col_bits | column) is sorted on the bits that vary inside
+// the group entirely in LDS (stable LSD passes), and columns and values are
+// stored back in place.
+__global__ __launch_bounds__(kSegNT) void k_local_sort(uint32_t* cols, float* vals,
+                                                       const LocalSeg* __restrict__ segs,
+                                                       const uint64_t* __restrict__ rstart,
+                                                       uint64_t lrows, int col_bits) {
   constexpr int R = 1 << kSegRB;
   __shared__ TileSmem<kSegRB, kSegIPT, kSegNT> sm;
   __shared__ float vbuf[kSegTile];
   __shared__ uint64_t s_or[kSegNT / kWave];
+  __shared__ uint32_t s_wmax[kSegNT / kWave];
+  __shared__ uint64_t s_k0;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = lane_id();
   const uint32_t wave = tid / kWave;
   const LocalSeg sg = segs[blockIdx.x];
   const uint64_t n = sg.count;
   const uint64_t wbase = (uint64_t)wave * kWave * kSegIPT;  // tile-local
+  // mark[p]: (r - row0) of the last row r > row0 starting at start + p, else 0
+  uint32_t* mark = reinterpret_cast<uint32_t*>(sm.buf);
+  for (uint32_t i = tid; i < n; i += kSegNT) mark[i] = 0;
+  __syncthreads();
+  for (uint64_t r = (uint64_t)sg.row0 + 1 + tid; r < lrows; r += kSegNT) {
+    const uint64_t st = rstart[r];
+    if (st >= sg.start + n) break;
+    atomicMax(&mark[st - sg.start], (uint32_t)(r - sg.row0));
+  }
+  __syncthreads();
   uint64_t k[kSegIPT];
   float v[kSegIPT];
   uint32_t pos[kSegIPT];
-  const uint64_t k0 = n ? keys[sg.start] : 0;
-  uint64_t vary = 0;
+  uint32_t carry = 0;
+#pragma unroll
+  for (int i = 0; i < kSegIPT; ++i) {
+    const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+    uint32_t m = idx < n ? mark[idx] : 0;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const uint32_t o = __shfl_up(m, d, kWave);
+      if (lane >= (uint32_t)d) m = m > o ? m : o;
+    }
+    pos[i] = m > carry ? m : carry;  // row in group so far (this wave's span)
+    const uint32_t last = __shfl(pos[i], kWave - 1, kWave);
+    carry = last;
+  }
+  if (lane == 0) s_wmax[wave] = carry;
+  __syncthreads();
+  uint32_t before = 0;
+  for (uint32_t w = 0; w < wave; ++w) before = s_wmax[w] > before ? s_wmax[w] : before;
+  const uint64_t cmask = low_mask(col_bits);
 #pragma unroll
   for (int i = 0; i < kSegIPT; ++i) {
     const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
     const bool valid = idx < n;
-    k[i] = valid ? keys[sg.start + idx] : 0;
+    const uint64_t row = pos[i] > before ? pos[i] : before;
+    k[i] = valid ? (shl64(row, col_bits) | cols[sg.start + idx]) : 0;
     v[i] = valid ? vals[sg.start + idx] : 0.f;
-    if (valid) vary |= k[i] ^ k0;
+  }
+  if (tid == 0) s_k0 = k[0];
+  __syncthreads();  // marks read, sm.buf free for the sort
+  const uint64_t k0 = s_k0;
+  uint64_t vary = 0;
+#pragma unroll
+  for (int i = 0; i < kSegIPT; ++i) {
+    const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+    if (idx < n) vary |= k[i] ^ k0;
   }
   for (int off = kWave / 2; off > 0; off >>= 1) vary |= __shfl_xor(vary, off, kWave);
   if (lane == 0) s_or[wave] = vary;
@@ -247,7 +294,7 @@ __global__ __launch_bounds__(kSegNT) void k_local_sort(uint64_t* keys, float* va
   for (int i = 0; i < kSegIPT; ++i) {
     const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
     if (idx < n) {
-      keys[sg.start + idx] = k[i];
+      cols[sg.start + idx] = (uint32_t)(k[i] & cmask);
       vals[sg.start + idx] = v[i];
     }
   }
@@ -272,7 +319,6 @@ void SegSortPlan::build(const std::vector<uint64_t>& seg_off, hipStream_t s) {
       d.count = (uint32_t)std::min<uint64_t>(kSegTile, len - t * kSegTile);
       d.obase = (1ull << kSegRB) * t0 + t;
       d.ostride = (uint32_t)nt;
-      d.hi = 0;
       tv.push_back(d);
     }
   }
@@ -380,7 +426,7 @@ void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows
   const uint64_t tile = (uint64_t)plan->ipt * kSegNT;
   if (!lrows) return;
   const uint64_t m = lrows + 1;
-  if (plan->scratch.size() < 8 * m) plan->scratch.reset(8 * m);
+  if (plan->scratch.size() < 9 * m) plan->scratch.reset(9 * m);
   uint64_t* small_n = plan->scratch.get();
   uint64_t* big_t = small_n + m;
   uint64_t* ps = big_t + m;
@@ -389,9 +435,10 @@ void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows
   uint64_t* hs = head + m;
   uint64_t* big_f = hs + m;
   uint64_t* bs = big_f + m;
+  uint64_t* rstart = bs + m;
   const uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(lrows, kBlock), 4096);
   hipLaunchKernelGGL(k_rowseg_sizes, dim3(grid), dim3(kBlock), 0, s, ranges, nranges, lrows,
-                     row_prod, tile, small_n, big_t, big_f);
+                     row_prod, tile, small_n, big_t, big_f, rstart);
   run_exclusive_scan(small_n, ps, lrows, tmp, s);
   run_exclusive_scan(big_t, bt, lrows, tmp, s);
   run_exclusive_scan(big_f, bs, lrows, tmp, s);
@@ -417,16 +464,16 @@ void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows
     hipLaunchKernelGGL(k_zero_local, dim3((uint32_t)ceil_div(plan->nlocal, 256)), dim3(256), 0,
                        s, plan->local.get(), plan->nlocal);
   hipLaunchKernelGGL(k_rowseg_emit, dim3(grid), dim3(kBlock), 0, s, ranges, nranges, lrows,
-                     row_prod, col_bits, plan->rb, tile, big_t, bt, bs, head, hs, plan->local.get(), plan->big.get(),
-                     plan->bigseg.get());
+                     row_prod, plan->rb, tile, big_t, bt, bs, head, hs, plan->local.get(),
+                     plan->big.get(), plan->bigseg.get());
   if (plan->nbig_tiles) segmented_sort_reserve(ws, plan->nbig_tiles, s, plan->rb);
 }
 
 // Column-array sets of the big-row passes: set 0 = (k1 as 32-bit words
 // [0, n), v1), set 1 = (k1 words [n, 2n), v).  The passes alternate sets and
-// the last one reads set 0 and writes (k, v), so the expansion starts in set
-// 0 for an odd pass count and in set 1 for an even one.  Set 1 shares v with
-// the groups, at the big rows' positions only.
+// the last one reads set 0 and writes (col, v), so the expansion starts in
+// set 0 for an odd pass count and in set 1 for an even one.  Set 1 shares v
+// with the groups, at the big rows' positions only.
 NarrowTarget narrow_target(const RowSegPlan& plan, uint64_t n, float* v, uint64_t* k1,
                            float* v1) {
   uint32_t* c = reinterpret_cast<uint32_t*>(k1);
@@ -435,7 +482,7 @@ NarrowTarget narrow_target(const RowSegPlan& plan, uint64_t n, float* v, uint64_
 }
 
 template <int RB, int IPT>
-static void narrow_sort(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, uint64_t* k,
+static void narrow_sort(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, uint32_t* fcol,
                         float* v, uint64_t* k1, float* v1, hipStream_t s, KTimer* kt, int cls) {
   const uint64_t ntiles = plan.nbig_tiles;
   const uint64_t m = ntiles << RB;
@@ -455,40 +502,34 @@ static void narrow_sort(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, u
                        plan.bigseg.get(), ws.rts_cnt.get(), ws.rts_off.get());
     if (kt) kt->end(tok, SPTC_K_SORT_COUNT, 4.0 * (double)nb + 16.0 * (double)m, s);
     tok = kt ? kt->begin(s) : -1;
-    if (p + 1 < passes) {
-      hipLaunchKernelGGL((k_seg_downsweep<uint32_t, uint32_t, RB, IPT>),
-                         dim3((uint32_t)ntiles), dim3(kSegNT), 0, s, col[cur], val[cur],
-                         col[1 - cur], val[1 - cur], shift, tiles, ws.rts_off.get(),
-                         xcd_remap());
-      if (kt) kt->end(tok, cls, 16.0 * (double)nb, s);
-      cur = 1 - cur;
-    } else {
-      hipLaunchKernelGGL((k_seg_downsweep<uint32_t, uint64_t, RB, IPT>),
-                         dim3((uint32_t)ntiles), dim3(kSegNT), 0, s, col[cur], val[cur], k, v,
-                         shift, tiles, ws.rts_off.get(), xcd_remap());
-      if (kt) kt->end(tok, cls, 20.0 * (double)nb, s);
-    }
+    const bool last = p + 1 == passes;
+    hipLaunchKernelGGL((k_seg_downsweep<uint32_t, uint32_t, RB, IPT>), dim3((uint32_t)ntiles),
+                       dim3(kSegNT), 0, s, col[cur], val[cur], last ? fcol : col[1 - cur],
+                       last ? v : val[1 - cur], shift, tiles, ws.rts_off.get(), xcd_remap());
+    if (kt) kt->end(tok, cls, 16.0 * (double)nb, s);
+    cur = 1 - cur;
   }
 }
 
-void sort_row_segments(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, uint64_t* k,
-                       float* v, uint64_t* k1, float* v1, hipStream_t s, KTimer* kt, int cls) {
+void sort_row_segments(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, uint32_t* col,
+                       float* v, uint64_t* k1, float* v1, const uint64_t* row_start,
+                       hipStream_t s, KTimer* kt, int cls) {
   if (plan.nbig_tiles) {
     switch (plan.rb * 100 + plan.ipt) {
-      case 812: narrow_sort<8, 12>(ws, plan, n, k, v, k1, v1, s, kt, cls); break;
-      case 816: narrow_sort<8, 16>(ws, plan, n, k, v, k1, v1, s, kt, cls); break;
-      case 824: narrow_sort<8, 24>(ws, plan, n, k, v, k1, v1, s, kt, cls); break;
-      case 1012: narrow_sort<10, 12>(ws, plan, n, k, v, k1, v1, s, kt, cls); break;
-      case 1016: narrow_sort<10, 16>(ws, plan, n, k, v, k1, v1, s, kt, cls); break;
-      case 1024: narrow_sort<10, 24>(ws, plan, n, k, v, k1, v1, s, kt, cls); break;
+      case 812: narrow_sort<8, 12>(ws, plan, n, col, v, k1, v1, s, kt, cls); break;
+      case 816: narrow_sort<8, 16>(ws, plan, n, col, v, k1, v1, s, kt, cls); break;
+      case 824: narrow_sort<8, 24>(ws, plan, n, col, v, k1, v1, s, kt, cls); break;
+      case 1012: narrow_sort<10, 12>(ws, plan, n, col, v, k1, v1, s, kt, cls); break;
+      case 1016: narrow_sort<10, 16>(ws, plan, n, col, v, k1, v1, s, kt, cls); break;
+      case 1024: narrow_sort<10, 24>(ws, plan, n, col, v, k1, v1, s, kt, cls); break;
       default: throw std::logic_error("row-segmented sort tile shape");
     }
   }
   if (plan.nlocal) {
     int tok = kt ? kt->begin(s) : -1;
-    hipLaunchKernelGGL(k_local_sort, dim3((uint32_t)plan.nlocal), dim3(kSegNT), 0, s, k, v,
-                       plan.local.get());
-    if (kt) kt->end(tok, SPTC_K_SORT_LOCAL, 24.0 * (double)plan.nsmall, s);
+    hipLaunchKernelGGL(k_local_sort, dim3((uint32_t)plan.nlocal), dim3(kSegNT), 0, s, col, v,
+                       plan.local.get(), row_start, plan.lrows, plan.col_bits);
+    if (kt) kt->end(tok, SPTC_K_SORT_LOCAL, 16.0 * (double)plan.nsmall, s);
   }
 }
 

@@ -30,7 +30,6 @@ struct SegTile {
   uint64_t obase;    // offset-matrix index of digit 0 (256 * first tile + index in segment)
   uint32_t count;    // keys in the tile
   uint32_t ostride;  // tiles in the segment
-  uint64_t hi;       // key bits OR'ed in when a pass widens 32-bit keys (row << col_bits)
 };
 // One segment: its tiles are first_tile .. first_tile + ntiles - 1; one
 // workgroup per segment turns their counts into absolute destinations
@@ -41,11 +40,12 @@ struct SegDesc {
   uint32_t ntiles;
 };
 
-// A segment small enough to be sorted inside one workgroup's LDS.
+// A segment small enough to be sorted inside one workgroup's LDS: the
+// products of consecutive small rows from chunk-local row row0 on.
 struct LocalSeg {
   uint64_t start;
   uint32_t count;  // <= kSegTile
-  uint32_t pad;
+  uint32_t row0;
 };
 
 // Segment of global index i: the s with off[s] <= i < off[s+1] (off is
@@ -97,19 +97,19 @@ struct RowRange {
   uint64_t lrow0;  // chunk-local row of ra (prefix of rows)
   uint64_t out0;   // chunk offset of ra's first product
 };
-// Row-segmented product sort of a chunk, products grouped by row, sorted
-// result as keys local_row << col_bits | col (col_bits <= 32) in (k, v).
-// Rows holding more than half a tile of products are segments of their own:
-// the expansion writes their products narrow -- 32-bit column + value, the
-// row implied by the segment -- and the multi-tile passes above sort them on
-// the column bits (digits of 8 bits, or 10 where that saves a pass; tiles of
-// ipt * kSegNT keys), the last pass widening to full keys.  Runs of smaller
-// rows are written as full keys and cut into groups of at most one tile (a
-// group starts where the running total of small-row products crosses a
-// multiple of half a tile), each sorted on its varying key bits entirely in
-// one workgroup's LDS, in place.  Every product is read and written once for
-// the groups; once per digit pass (at least one) for the big rows, at 8 bytes
-// per product except the last pass's 12-byte writes.
+// Row-segmented product sort of a chunk.  Products are grouped by row and
+// travel narrow -- a 32-bit column (col_bits <= 32) and a value, the row
+// implied by the row starts -- and are sorted by (row, column) into
+// (col, val).  Rows holding more than half a 6144-product tile are segments
+// of their own, sorted on the column bits by the multi-tile passes above
+// (digits of 8 bits, or 10 where that saves a pass; tiles of ipt * kSegNT
+// columns) from their narrow target (narrow_target) into (col, val).  Runs
+// of smaller rows are cut into groups of at most 6144 products (a group
+// starts where the running small-row total crosses a multiple of half a
+// tile); each group is sorted on its (row in group, column) bits entirely in
+// one workgroup's LDS, in place in (col, val).  Every product is read and
+// written once for the groups and once per digit pass (at least one) for
+// the big rows, 8 bytes each way.
 struct RowSegPlan {
   uint64_t nlocal = 0, nbig_tiles = 0, nbig_rows = 0, nsmall = 0;  // nsmall: products in groups
   uint64_t lrows = 0;
@@ -119,9 +119,11 @@ struct RowSegPlan {
   DevBuf<LocalSeg> local;
   DevBuf<SegTile> big;
   DevBuf<SegDesc> bigseg;
-  DevBuf<uint64_t> scratch;  // 8 x (rows + 1) per-row values and scans
-  // per chunk-local row: nonzero when the row is a big (narrow) segment
+  DevBuf<uint64_t> scratch;  // 9 x (rows + 1) per-row values and scans
+  // per chunk-local row: nonzero when the row is a big segment
   const uint64_t* row_big() const { return scratch.get() + (lrows + 1); }
+  // per chunk-local row: its first product; [lrows] = the product count
+  const uint64_t* row_start() const { return scratch.get() + 8 * (lrows + 1); }
 };
 // Builds the plan (one host synchronization for the counts); tmp holds
 // scan_tmp_count(lrows) + 1 entries.  Reserves ws for the big-row passes.
@@ -129,18 +131,20 @@ void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows
                         const uint64_t* row_prod, int col_bits, RowSegPlan* plan,
                         SortWorkspace& ws, U2* tmp, hipStream_t s);
 // Where the expansion writes big-row products of an n-product chunk sorted
-// into (k, v) with spare arrays (k1, v1) of the same size: a 32-bit column
-// array and a value array, chosen so that the big-row passes end in (k, v).
+// into (col, v) with spare arrays (k1, v1) of n 64-bit keys and n values: a
+// 32-bit column array and a value array, chosen so that the big-row passes
+// end in (col, v).
 struct NarrowTarget {
   uint32_t* col;
   float* val;
 };
 NarrowTarget narrow_target(const RowSegPlan& plan, uint64_t n, float* v, uint64_t* k1,
                            float* v1);
-// Sorts with a plan: big rows from their narrow target, groups in place.
-// Launches only (replayable).
-void sort_row_segments(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, uint64_t* k,
-                       float* v, uint64_t* k1, float* v1, hipStream_t s, KTimer* kt = nullptr,
-                       int cls = SPTC_K_SORT_PROD);
+// Sorts with a plan: big rows from their narrow target, groups in place;
+// row_start as RowSegPlan::row_start (kept by the caller when the plan's
+// scratch is released).  Launches only (replayable).
+void sort_row_segments(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, uint32_t* col,
+                       float* v, uint64_t* k1, float* v1, const uint64_t* row_start,
+                       hipStream_t s, KTimer* kt = nullptr, int cls = SPTC_K_SORT_PROD);
 
 }  // namespace sptc
