@@ -447,11 +447,12 @@ __global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs a) {
         const uint64_t col = a.ycol ? (uint64_t)a.ycol[y] : (a.ykeys[y] & a.fmask);
         key = rbits | col;
         if (big) {
-          a.big_col[obase + (p - pb)] = (uint32_t)col;
-          a.big_val[obase + (p - pb)] = v;
+          // streaming stores: Y's columns and values, gathered here, stay cached
+          __builtin_nontemporal_store((uint32_t)col, &a.big_col[obase + (p - pb)]);
+          __builtin_nontemporal_store(v, &a.big_val[obase + (p - pb)]);
         } else if (a.row_big) {
-          a.out_col32[obase + (p - pb)] = (uint32_t)col;
-          a.out_vals[obase + (p - pb)] = v;
+          __builtin_nontemporal_store((uint32_t)col, &a.out_col32[obase + (p - pb)]);
+          __builtin_nontemporal_store(v, &a.out_vals[obase + (p - pb)]);
         } else {
           a.out_keys[obase + (p - pb)] = key;
           a.out_vals[obase + (p - pb)] = v;
@@ -587,10 +588,10 @@ __global__ __launch_bounds__(kBlock) void k_reduce_emit(ReduceOp op, const U2* t
 #pragma unroll
   for (int q = 0; q < kRedRunsPerThread; ++q) {
     const uint32_t i = threadIdx.x + (uint32_t)q * kBlock;
-    if (i < nout) {
-      orow[i] = (uint32_t)(op.r0 + shr64(key[q], op.ynf));
-      ocol[i] = col[q];
-      oval[i] = sm.v[i];
+    if (i < nout) {  // streaming stores, as the loads
+      __builtin_nontemporal_store((uint32_t)(op.r0 + shr64(key[q], op.ynf)), &orow[i]);
+      __builtin_nontemporal_store(col[q], &ocol[i]);
+      __builtin_nontemporal_store(sm.v[i], &oval[i]);
     }
   }
 }

@@ -140,8 +140,10 @@ __device__ __forceinline__ uint32_t reduce_tile_narrow(const NarrowRows& nr,
 #pragma unroll
   for (int e = 0; e < kScanItems; ++e) {
     const uint32_t i = (uint32_t)e * kBlock + tid;
-    if (i < cnt) sm.k[i] = r0 << 32 | nr.cols[base + i];  // columns, coalesced
-    vreg[e] = i < cnt ? vals[base + i] : 0.f;
+    // streaming loads (non-temporal): they should not evict the column
+    // table the emit gathers from
+    if (i < cnt) sm.k[i] = r0 << 32 | __builtin_nontemporal_load(&nr.cols[base + i]);
+    vreg[e] = i < cnt ? __builtin_nontemporal_load(&vals[base + i]) : 0.f;
   }
   // most tiles lie inside one row (thread 0's candidate is the next row):
   // their keys are complete; otherwise rows come from marks at the row

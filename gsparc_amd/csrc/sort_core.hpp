@@ -119,16 +119,14 @@ __device__ __forceinline__ void tile_digit_starts(TileSmem<RB, IPT, NT, ST>& sm)
 
 // Given sm.gbase and sm.dstart (tile totals per digit on entry), scans the
 // totals into tile-local run starts, stages keys in digit order, writes them
-// in coalesced runs, then does the same for the values.  A key is written as
-// KO(key) | hi, which widens 32-bit column keys to full product keys on the
-// last pass of a row-segmented sort.
+// in coalesced runs, then does the same for the values.
 template <int RB, int IPT, int NT, typename ST, typename KT, typename KO>
 __device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT, ST>& sm, uint64_t n,
                                                    int shift, uint64_t tile_begin,
                                                    uint64_t wbase, const KT (&k)[IPT],
                                                    const float (&v)[IPT], uint32_t (&pos)[IPT],
                                                    KO* __restrict__ kout,
-                                                   float* __restrict__ vout, KO hi = 0) {
+                                                   float* __restrict__ vout) {
   constexpr int R = 1 << RB;
   constexpr int TILE = NT * IPT;
   const uint32_t tid = threadIdx.x;
@@ -155,7 +153,7 @@ __device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT, ST>& sm
       const KT key = kbuf[j];
       const uint32_t d = (uint32_t)(key >> shift) & (R - 1);
       dst[i] = sm.gbase[d] + (j - sm.dstart[d]);
-      kout[dst[i]] = (KO)key | hi;
+      kout[dst[i]] = (KO)key;
     }
   }
   __syncthreads();
