@@ -74,10 +74,26 @@ __global__ __launch_bounds__(1 << RB) void k_seg_scan(const SegDesc* __restrict_
   for (uint64_t t = t0; t < t1; ++t) tot += cnt[t * R + d];
   uint32_t all;
   uint32_t base = (uint32_t)g.start + block_scan_u32<R>(tot, scratch, &all);
-  uint32_t* o = off + R * t0 + (uint64_t)d * g.ntiles;  // digit-major in the segment
-  for (uint64_t t = t0; t < t1; ++t) {
-    o[t - t0] = base;
-    base += cnt[t * R + d];
+  // destinations are digit-major in the segment: o[d * nt + (t - t0)].  They
+  // go out through LDS in groups of up to 8 tiles, so the stores are
+  // contiguous (whole segment for nt <= 8, 32-byte pieces otherwise) rather
+  // than one 4-byte store every nt words.
+  __shared__ uint32_t buf[8 * R];
+  uint32_t* o = off + R * t0;
+  const uint64_t nt = g.ntiles;
+  for (uint64_t tg = 0; tg < nt; tg += 8) {
+    const uint32_t m = (uint32_t)(nt - tg < 8 ? nt - tg : 8);
+    for (uint32_t j = 0; j < m; ++j) {
+      buf[j * R + d] = base;
+      base += cnt[(t0 + tg + j) * R + d];
+    }
+    __syncthreads();
+    for (uint32_t k = 0; k < m; ++k) {
+      const uint32_t q = d + k * R;
+      const uint32_t dd = q / m, j = q - dd * m;
+      o[(uint64_t)dd * nt + tg + j] = buf[j * R + dd];
+    }
+    __syncthreads();
   }
 }
 
