@@ -548,7 +548,8 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       ea.passes = hist && passes <= kMaxFusedPasses ? passes : 0;
       ea.vsum = vsum.get();
       int tok = kt_.begin(s_);
-      launch_expand(ea, rb, s_);
+      if (rowsort && ea.ycol) launch_expand_items(ea, rseg, ws_, s_);
+      else launch_expand(ea, rb, s_);
       // matched Y (column, value) read; product written: column + value in
       // the narrow form of a row-segmented sort, 64-bit key + value otherwise
       kt_.end(tok, SPTC_K_EXPAND, rowsort ? 16.0 * (double)n : 24.0 * (double)n, s_);
@@ -557,7 +558,7 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       if (rowsort) {
         launch_tile_rows(rseg.row_start(), r1c - r0c, trow.get(), s_);
         sort_row_segments(ws_, rseg, n, fcol, vA.get(), ws_.alt_keys.get(), ws_.alt_vals.get(),
-                          rseg.row_start(), s_, &kt_, SPTC_K_SORT_PROD);
+                          rseg.row_start(), s_, &kt_, SPTC_K_SORT_PROD, ea.ycol != nullptr);
       } else {
         ws_.ensure(n, rb, s_);
         if (hist && passes > kMaxFusedPasses) histogram(ws_, kA.get(), n, passes, s_);

@@ -316,6 +316,113 @@ __global__ __launch_bounds__(kSegNT) void k_local_sort(uint32_t* cols, float* va
   }
 }
 
+// Expansion by sort work item (row-segmented sort): items are the big-row
+// tiles, then the small-row groups; an item's products are consecutive and
+// are taken in sub-tiles of kExpandSub like the plain expansion (one global
+// search for the sub-tile's first owner, owner prefixes staged in LDS, a
+// product's owner found from its previous one).  Products are written
+// narrow to the big rows' first-pass set or to (out_col32, out_vals); a big
+// tile also counts its first-pass digits into its count row, which takes
+// the place of that pass's upsweep.
+constexpr uint32_t kExpandSub = kBlock * 8;
+
+template <int RB>
+__global__ __launch_bounds__(kBlock) void k_expand_items(ExpandArgs a,
+                                                         const SegTile* __restrict__ big,
+                                                         uint32_t nbig,
+                                                         const LocalSeg* __restrict__ local,
+                                                         uint32_t nlocal,
+                                                         uint32_t* __restrict__ cnt) {
+  constexpr int R = 1 << RB;
+  __shared__ uint64_t s_pref[kExpandSub + 1];
+  __shared__ uint32_t sh[R];
+  __shared__ uint64_t s_ja;
+  __shared__ double s_sum[kBlock / kWave];
+  double vsum = 0.0;
+  uint64_t t0, t1, step;
+  xcd_tiles((uint64_t)nbig + nlocal, a.remap, &t0, &t1, &step);
+  for (uint64_t it = t0; it < t1; it += step) {
+    const bool isbig = it < nbig;
+    uint64_t start;
+    uint32_t count;
+    if (isbig) {
+      start = big[it].start;
+      count = big[it].count;
+    } else {
+      start = local[it - nbig].start;
+      count = local[it - nbig].count;
+    }
+    uint32_t* ocol = isbig ? a.big_col : a.out_col32;
+    float* oval = isbig ? a.big_val : a.out_vals;
+    if (isbig)
+      for (int i = threadIdx.x; i < R; i += kBlock) sh[i] = 0;
+    for (uint32_t sub = 0; sub < count; sub += kExpandSub) {
+      const uint64_t pb = a.P0 + start + sub;
+      const uint64_t pe = pb + (count - sub < kExpandSub ? count - sub : kExpandSub);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        uint64_t lo = a.A0, hi = a.A1;  // last j with act_prefix[j] <= pb
+        while (lo < hi) {
+          const uint64_t mid = (lo + hi) >> 1;
+          if (a.act_prefix[mid] <= pb) lo = mid + 1; else hi = mid;
+        }
+        s_ja = lo - 1;
+      }
+      __syncthreads();
+      const uint64_t ja = s_ja;
+      const uint64_t avail = a.A1 - ja;
+      const uint32_t m = (uint32_t)(avail < kExpandSub + 1 ? avail : kExpandSub + 1);
+      for (uint32_t k = threadIdx.x; k < m; k += kBlock) s_pref[k] = a.act_prefix[ja + k];
+      __syncthreads();
+      uint32_t klo = 0, kc = ~0u;
+      uint64_t ybase = 0;
+      float xv = 0.f;
+#pragma unroll
+      for (int e = 0; e < (int)(kExpandSub / kBlock); ++e) {
+        const uint64_t p = pb + (uint64_t)e * kBlock + threadIdx.x;
+        const bool valid = p < pe;
+        uint32_t col = 0;
+        if (valid) {
+          uint32_t k = klo;  // same owner as the previous product: one LDS read
+          if (k + 1 < m && s_pref[k + 1] <= p) {
+            uint32_t lo = k + 1, hi = m;
+            while (lo < hi) {
+              const uint32_t mid = (lo + hi) >> 1;
+              if (s_pref[mid] <= p) lo = mid + 1; else hi = mid;
+            }
+            k = lo - 1;
+          }
+          klo = k;
+          if (k != kc) {
+            kc = k;
+            ybase = a.act_ypos[ja + k] - s_pref[k];
+            xv = a.act_xval[ja + k];
+          }
+          const uint64_t y = ybase + p;
+          const float v = xv * a.yvals[y];
+          col = a.ycol[y];
+          __builtin_nontemporal_store(col, &ocol[p - a.P0]);
+          __builtin_nontemporal_store(v, &oval[p - a.P0]);
+          vsum += (double)v;
+        }
+        if (isbig) hist_add_wave<RB>(sh, col, valid, 1);
+      }
+    }
+    if (isbig) {
+      __syncthreads();
+      for (int d = threadIdx.x; d < R; d += kBlock) cnt[it * R + d] = sh[d];
+    }
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1) vsum += __shfl_xor(vsum, off, kWave);
+  if (lane_id() == 0) s_sum[threadIdx.x / kWave] = vsum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) t += s_sum[w];
+    atomicAdd(a.vsum, t);
+  }
+}
+
 }  // namespace
 
 static uint32_t xcd_remap() { return xcd_remap_enabled(); }
@@ -499,7 +606,8 @@ NarrowTarget narrow_target(const RowSegPlan& plan, uint64_t n, float* v, uint64_
 
 template <int RB, int IPT>
 static void narrow_sort(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, uint32_t* fcol,
-                        float* v, uint64_t* k1, float* v1, hipStream_t s, KTimer* kt, int cls) {
+                        float* v, uint64_t* k1, float* v1, bool counted, hipStream_t s,
+                        KTimer* kt, int cls) {
   const uint64_t ntiles = plan.nbig_tiles;
   const uint64_t m = ntiles << RB;
   if (ws.rts_cnt.size() < m) throw std::logic_error("segmented sort workspace not reserved");
@@ -512,11 +620,13 @@ static void narrow_sort(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, u
   for (int p = 0; p < passes; ++p) {
     const int shift = p * RB;
     int tok = kt ? kt->begin(s) : -1;
-    hipLaunchKernelGGL((k_seg_upsweep<uint32_t, RB, IPT>), dim3((uint32_t)ntiles),
-                       dim3(kSegNT), 0, s, col[cur], shift, tiles, ws.rts_cnt.get());
+    if (p > 0 || !counted)  // else the expansion counted the first pass
+      hipLaunchKernelGGL((k_seg_upsweep<uint32_t, RB, IPT>), dim3((uint32_t)ntiles),
+                         dim3(kSegNT), 0, s, col[cur], shift, tiles, ws.rts_cnt.get());
     hipLaunchKernelGGL(k_seg_scan<RB>, dim3((uint32_t)plan.nbig_rows), dim3(1 << RB), 0, s,
                        plan.bigseg.get(), ws.rts_cnt.get(), ws.rts_off.get());
-    if (kt) kt->end(tok, SPTC_K_SORT_COUNT, 4.0 * (double)nb + 16.0 * (double)m, s);
+    const double up = (p > 0 || !counted) ? 4.0 * (double)nb : 0.0;
+    if (kt) kt->end(tok, SPTC_K_SORT_COUNT, up + 16.0 * (double)m, s);
     tok = kt ? kt->begin(s) : -1;
     const bool last = p + 1 == passes;
     hipLaunchKernelGGL((k_seg_downsweep<uint32_t, uint32_t, RB, IPT>), dim3((uint32_t)ntiles),
@@ -527,17 +637,35 @@ static void narrow_sort(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, u
   }
 }
 
+void launch_expand_items(const ExpandArgs& a, const RowSegPlan& plan, SortWorkspace& ws,
+                         hipStream_t s) {
+  const uint64_t items = plan.nbig_tiles + plan.nlocal;
+  if (!items) return;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(items, 8192);
+  ExpandArgs b = a;
+  b.remap = xcd_remap_enabled();
+  if (plan.rb == 10)
+    hipLaunchKernelGGL(k_expand_items<10>, dim3(grid), dim3(kBlock), 0, s, b, plan.big.get(),
+                       (uint32_t)plan.nbig_tiles, plan.local.get(), (uint32_t)plan.nlocal,
+                       ws.rts_cnt.get());
+  else
+    hipLaunchKernelGGL(k_expand_items<8>, dim3(grid), dim3(kBlock), 0, s, b, plan.big.get(),
+                       (uint32_t)plan.nbig_tiles, plan.local.get(), (uint32_t)plan.nlocal,
+                       ws.rts_cnt.get());
+}
+
 void sort_row_segments(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, uint32_t* col,
                        float* v, uint64_t* k1, float* v1, const uint64_t* row_start,
-                       hipStream_t s, KTimer* kt, int cls) {
+                       hipStream_t s, KTimer* kt, int cls, bool counted) {
   if (plan.nbig_tiles) {
+    const bool c = counted;
     switch (plan.rb * 100 + plan.ipt) {
-      case 812: narrow_sort<8, 12>(ws, plan, n, col, v, k1, v1, s, kt, cls); break;
-      case 816: narrow_sort<8, 16>(ws, plan, n, col, v, k1, v1, s, kt, cls); break;
-      case 824: narrow_sort<8, 24>(ws, plan, n, col, v, k1, v1, s, kt, cls); break;
-      case 1012: narrow_sort<10, 12>(ws, plan, n, col, v, k1, v1, s, kt, cls); break;
-      case 1016: narrow_sort<10, 16>(ws, plan, n, col, v, k1, v1, s, kt, cls); break;
-      case 1024: narrow_sort<10, 24>(ws, plan, n, col, v, k1, v1, s, kt, cls); break;
+      case 812: narrow_sort<8, 12>(ws, plan, n, col, v, k1, v1, c, s, kt, cls); break;
+      case 816: narrow_sort<8, 16>(ws, plan, n, col, v, k1, v1, c, s, kt, cls); break;
+      case 824: narrow_sort<8, 24>(ws, plan, n, col, v, k1, v1, c, s, kt, cls); break;
+      case 1012: narrow_sort<10, 12>(ws, plan, n, col, v, k1, v1, c, s, kt, cls); break;
+      case 1016: narrow_sort<10, 16>(ws, plan, n, col, v, k1, v1, c, s, kt, cls); break;
+      case 1024: narrow_sort<10, 24>(ws, plan, n, col, v, k1, v1, c, s, kt, cls); break;
       default: throw std::logic_error("row-segmented sort tile shape");
     }
   }
