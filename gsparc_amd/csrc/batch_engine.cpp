@@ -387,7 +387,7 @@ sptc_status BatchEngine::build_chunks(hipStream_t s) {
   }
   kA_.reset(chunk_cap);
   vA_.reset(chunk_cap);
-  if (tmp_.size() < scan_tmp_count(chunk_cap) + 1) tmp_.reset(scan_tmp_count(chunk_cap) + 1);
+  if (tmp_.size() < reduce_tmp_count(chunk_cap)) tmp_.reset(reduce_tmp_count(chunk_cap));
   info.esc_chunks = (int32_t)esc_.size();
   return SPTC_OK;
 }

@@ -470,7 +470,7 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
     uint64_t max_rows = 0;
     for (size_t c = 1; c < bounds_r.size(); ++c)
       max_rows = std::max(max_rows, bounds_r[c] - bounds_r[c - 1]);
-    DevBuf<U2> tmp(scan_tmp_count(std::max(chunk_cap, max_rows)) + 1);
+    DevBuf<U2> tmp(reduce_tmp_count(std::max(chunk_cap, max_rows)));
     int rb_max = 8;
     for (size_t c = 1; c < bounds_r.size(); ++c) {
       const int b = mode_bits(bounds_r[c] - bounds_r[c - 1]) + kyf;

@@ -34,6 +34,81 @@ __global__ __launch_bounds__(kBlock) void k_scan_tiles(U2* sums, uint64_t nt, U2
   if (threadIdx.x == 0) *total = carry;
 }
 
+// Many-workgroup tile-sum scan: block sums, one scan of the block sums,
+// then each block rescans its sums from its offset (1024 threads x 16 sums).
+constexpr int kPartNT = 1024, kPartItems = (int)(kScanSumsPerBlock / kPartNT);
+
+__device__ __forceinline__ U2 block_scan_part(U2 t, U2* scratch, U2* tot) {
+  const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
+  U2 inc = t;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint64_t oa = __shfl_up(inc.a, d, kWave), ob = __shfl_up(inc.b, d, kWave);
+    if (lane >= (uint32_t)d) {
+      inc.a += oa;
+      inc.b += ob;
+    }
+  }
+  if (lane == kWave - 1) scratch[wave] = inc;
+  __syncthreads();
+  U2 off = mk2(0, 0), all = mk2(0, 0);
+  for (int w = 0; w < kPartNT / kWave; ++w) {
+    const U2 x = scratch[w];
+    if ((uint32_t)w < wave) off = off + x;
+    all = all + x;
+  }
+  __syncthreads();
+  *tot = all;
+  return off + inc - t;
+}
+
+__global__ __launch_bounds__(kPartNT) void k_part_sums(const U2* __restrict__ sums, uint64_t nt,
+                                                       U2* __restrict__ parts) {
+  __shared__ U2 scratch[kPartNT / kWave];
+  const uint64_t first = (uint64_t)blockIdx.x * kScanSumsPerBlock + threadIdx.x;
+  U2 t = mk2(0, 0);
+#pragma unroll
+  for (int e = 0; e < kPartItems; ++e) {
+    const uint64_t i = first + (uint64_t)e * kPartNT;
+    if (i < nt) t = t + sums[i];
+  }
+  U2 tot;
+  block_scan_part(t, scratch, &tot);
+  if (threadIdx.x == 0) parts[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kPartNT) void k_part_apply(U2* __restrict__ sums, uint64_t nt,
+                                                        const U2* __restrict__ parts) {
+  __shared__ U2 scratch[kPartNT / kWave];
+  const uint64_t first = (uint64_t)blockIdx.x * kScanSumsPerBlock +
+                         (uint64_t)threadIdx.x * kPartItems;  // blocked: own run of sums
+  U2 v[kPartItems];
+  U2 t = mk2(0, 0);
+#pragma unroll
+  for (int e = 0; e < kPartItems; ++e) {
+    v[e] = first + e < nt ? sums[first + e] : mk2(0, 0);
+    t = t + v[e];
+  }
+  U2 tot;
+  U2 ex = block_scan_part(t, scratch, &tot) + parts[blockIdx.x];
+#pragma unroll
+  for (int e = 0; e < kPartItems; ++e) {
+    if (first + e < nt) sums[first + e] = ex;
+    ex = ex + v[e];
+  }
+}
+
+void scan_tiles_parts(U2* sums, uint64_t nt, U2* total, U2* parts, hipStream_t s) {
+  if (nt <= 2 * kScanSumsPerBlock) {
+    scan_tiles(sums, nt, total, s);
+    return;
+  }
+  const uint32_t nb = (uint32_t)ceil_div(nt, kScanSumsPerBlock);
+  hipLaunchKernelGGL(k_part_sums, dim3(nb), dim3(kPartNT), 0, s, sums, nt, parts);
+  scan_tiles(parts, nb, total, s);  // block offsets in place, grand total
+  hipLaunchKernelGGL(k_part_apply, dim3(nb), dim3(kPartNT), 0, s, sums, nt, parts);
+}
+
 void scan_tiles(U2* sums, uint64_t nt, U2* total, hipStream_t s) {
   hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, s, sums, nt, total);
 }
@@ -558,7 +633,7 @@ void run_reduce_count(const uint64_t* keys, uint64_t n, U2* tmp, hipStream_t s) 
   const uint64_t nt = scan_tmp_count(n);
   if (n)
     hipLaunchKernelGGL(k_reduce_heads, dim3((uint32_t)nt), dim3(kBlock), 0, s, keys, n, tmp);
-  scan_tiles(tmp, n ? nt : 0, tmp + nt, s);
+  scan_tiles_parts(tmp, n ? nt : 0, tmp + nt, tmp + nt + 1, s);
 }
 
 // Emit for a tile of kScanTile sorted products (same tiling as the count
@@ -691,7 +766,7 @@ void run_reduce_count_narrow(const NarrowRows& nr, uint64_t n, U2* tmp, hipStrea
   if (n)
     hipLaunchKernelGGL(k_reduce_heads_narrow, dim3((uint32_t)nt), dim3(kBlock), 0, s, nr, n,
                        tmp);
-  scan_tiles(tmp, n ? nt : 0, tmp + nt, s);
+  scan_tiles_parts(tmp, n ? nt : 0, tmp + nt, tmp + nt + 1, s);
 }
 
 // --------------------------------------------------------------------------

@@ -117,6 +117,8 @@ void launch_chunk_end(const uint64_t* row_prod, const uint64_t* row_act, uint64_
                       hipStream_t s);
 void launch_max_row(const uint64_t* row_prod, uint64_t R, uint64_t* out, hipStream_t s);
 void launch_expand(const ExpandArgs& a, int rb, hipStream_t s);
+// Reduce counts: tmp holds reduce_tmp_count(n) entries (tile sums, total,
+// block partials); the scanned tile sums are the emit's tile offsets.
 void run_reduce_count(const uint64_t* keys, uint64_t n, U2* tmp, hipStream_t s);
 void run_reduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32_t* out_row,
                      uint64_t* out_col, float* out_val, uint64_t r0, int ynf,

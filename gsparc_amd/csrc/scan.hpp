@@ -104,6 +104,18 @@ void scan_tiles(U2* sums, uint64_t nt, U2* total, hipStream_t s);
 // entries; the grand total lands in tmp[scan_tmp_count(n)].
 inline uint64_t scan_tmp_count(uint64_t n) { return ceil_div(n ? n : 1, kScanTile); }
 
+// Long tile-sum arrays (one sum per 2048 products of a reduce count) are
+// scanned by many workgroups, kScanSumsPerBlock sums each, with the block
+// partials kept after the total: reduce_tmp_count(n) entries of tmp.
+constexpr uint64_t kScanSumsPerBlock = 16384;
+inline uint64_t reduce_tmp_count(uint64_t n) {
+  const uint64_t nt = scan_tmp_count(n);
+  return nt + 1 + ceil_div(nt, kScanSumsPerBlock) + 1;
+}
+// Exclusive scan of nt tile sums in place, total -> *total; parts holds
+// ceil(nt / kScanSumsPerBlock) + 1 entries (defined in kernels.hip).
+void scan_tiles_parts(U2* sums, uint64_t nt, U2* total, U2* parts, hipStream_t s);
+
 template <class Op>
 void run_scan(const Op& op, uint64_t n, U2* tmp, hipStream_t s, bool emit = true) {
   const uint64_t nt = scan_tmp_count(n);
