@@ -528,7 +528,6 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       ea.fmask = fmask;
       ea.out_keys = kA.get();
       ea.out_vals = vA.get();
-      ea.row_big = nullptr;
       ea.big_col = nullptr;
       ea.big_val = nullptr;
       ea.out_col32 = nullptr;
@@ -538,7 +537,6 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       if (rowsort) {
         const NarrowTarget nt =
             narrow_target(rseg, n, vA.get(), ws_.alt_keys.get(), ws_.alt_vals.get());
-        ea.row_big = rseg.row_big();
         ea.big_col = nt.col;
         ea.big_val = nt.val;
         ea.out_col32 = fcol;
@@ -548,7 +546,7 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       ea.passes = hist && passes <= kMaxFusedPasses ? passes : 0;
       ea.vsum = vsum.get();
       int tok = kt_.begin(s_);
-      if (rowsort && ea.ycol) launch_expand_items(ea, rseg, ws_, s_);
+      if (rowsort) launch_expand_items(ea, rseg, ws_, s_);  // ycol is set (kyf <= 32)
       else launch_expand(ea, rb, s_);
       // matched Y (column, value) read; product written: column + value in
       // the narrow form of a row-segmented sort, 64-bit key + value otherwise
@@ -558,7 +556,7 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       if (rowsort) {
         launch_tile_rows(rseg.row_start(), r1c - r0c, trow.get(), s_);
         sort_row_segments(ws_, rseg, n, fcol, vA.get(), ws_.alt_keys.get(), ws_.alt_vals.get(),
-                          rseg.row_start(), s_, &kt_, SPTC_K_SORT_PROD, ea.ycol != nullptr);
+                          rseg.row_start(), s_, &kt_, SPTC_K_SORT_PROD, /*counted=*/true);
       } else {
         ws_.ensure(n, rb, s_);
         if (hist && passes > kMaxFusedPasses) histogram(ws_, kA.get(), n, passes, s_);

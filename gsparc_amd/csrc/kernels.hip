@@ -491,7 +491,6 @@ __global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs a) {
     uint32_t klo = 0, kc = ~0u;
     uint64_t ybase = 0, rbits = 0;
     float xv = 0.f;
-    bool big = false;
 #pragma unroll
     for (int e = 0; e < kExpandItems; ++e) {
       const uint64_t p = pb + (uint64_t)e * kBlock + threadIdx.x;
@@ -513,25 +512,14 @@ __global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs a) {
           const uint64_t j = ja + k;
           ybase = a.act_ypos[j] - s_pref[k];
           xv = a.act_xval[j];
-          const uint32_t lr = a.act_row[j] - a.r0;
-          rbits = shl64((uint64_t)lr, a.ynf);
-          big = a.row_big && a.row_big[lr] != 0;
+          rbits = shl64((uint64_t)(a.act_row[j] - a.r0), a.ynf);
         }
         const uint64_t y = ybase + p;
         const float v = xv * a.yvals[y];
         const uint64_t col = a.ycol ? (uint64_t)a.ycol[y] : (a.ykeys[y] & a.fmask);
         key = rbits | col;
-        if (big) {
-          // streaming stores: Y's columns and values, gathered here, stay cached
-          __builtin_nontemporal_store((uint32_t)col, &a.big_col[obase + (p - pb)]);
-          __builtin_nontemporal_store(v, &a.big_val[obase + (p - pb)]);
-        } else if (a.row_big) {
-          __builtin_nontemporal_store((uint32_t)col, &a.out_col32[obase + (p - pb)]);
-          __builtin_nontemporal_store(v, &a.out_vals[obase + (p - pb)]);
-        } else {
-          a.out_keys[obase + (p - pb)] = key;
-          a.out_vals[obase + (p - pb)] = v;
-        }
+        a.out_keys[obase + (p - pb)] = key;
+        a.out_vals[obase + (p - pb)] = v;
         vsum += (double)v;
       }
       if (HIST) hist_add_wave<RB>(sh, key, valid, a.passes);

@@ -53,11 +53,10 @@ struct ExpandArgs {
   uint64_t fmask;
   uint64_t* out_keys;
   float* out_vals;
-  // row-segmented sort (narrow form): products go out as 32-bit columns,
-  // to (big_col, big_val) for rows with row_big[row - r0] != 0 and to
-  // (out_col32, out_vals) for the others (row_big null: 64-bit keys to
-  // out_keys)
-  const uint64_t* row_big;
+  // row-segmented sort (launch_expand_items, narrow form): products go out
+  // as 32-bit columns, to (big_col, big_val) for big-row tiles and to
+  // (out_col32, out_vals) for small-row groups; launch_expand writes 64-bit
+  // keys to out_keys
   uint32_t* big_col;
   float* big_val;
   uint32_t* out_col32;
