@@ -289,15 +289,13 @@ struct BReduceOp {
 
 // Same run detection and compensated summation as the plain reduce; the
 // chunk-local row maps back to a global row through the piece table.
-template <bool NARROW>
 __global__ __launch_bounds__(kBlock) void k_breduce_emit(BReduceOp op, const U2* tile_offs) {
   __shared__ ReduceSmem sm;
   if (*op.b.bad) return;
   const uint32_t tile = blockIdx.x;
   const uint64_t base = (uint64_t)tile * kScanTile;
   uint64_t prev_key;
-  const uint32_t nout = NARROW ? reduce_tile_narrow(op.nr, op.vals, op.n, tile, sm, &prev_key)
-                               : reduce_tile(op.keys, op.vals, op.n, tile, sm, &prev_key);
+  const uint32_t nout = reduce_tile(op.keys, op.vals, op.n, tile, sm, &prev_key);
   const uint64_t obase = tile_offs[tile].a;
   const uint64_t cm = low_mask(op.K);
   // the thread's gathers (row and instance maps, then instance and column
@@ -333,6 +331,53 @@ __global__ __launch_bounds__(kBlock) void k_breduce_emit(BReduceOp op, const U2*
     int64_t sp = -1;
     if (i > 0) sp = op.b.map_inst[shr64(sm.k[i - 1], op.K)];
     else if (base > 0) sp = op.b.map_inst[shr64(prev_key, op.K)];
+    for (int64_t t = sp + 1; t <= (int64_t)si[q]; ++t) op.b.bounds[t] = obase + i;
+  }
+}
+
+// The narrow form, one wave per wave tile (the count pass's tiling).
+__global__ __launch_bounds__(kBlock) void k_breduce_emit_narrow(BReduceOp op,
+                                                                const U2* tile_offs,
+                                                                uint64_t ntiles) {
+  __shared__ WaveSmem sm[kBlock / kWave];
+  if (*op.b.bad) return;
+  const uint64_t wt = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  if (wt >= ntiles) return;
+  WaveSmem& w = sm[threadIdx.x / kWave];
+  uint64_t r0, rp;
+  const uint32_t T = wave_tile_reduce(op.nr, op.vals, op.n, wt, w, &r0, &rp);
+  const uint64_t obase = tile_offs[wt].a;
+  const uint32_t lane = lane_id();
+  uint64_t col[kScanItems];
+  uint32_t row[kScanItems], si[kScanItems];
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    const uint32_t i = lane + (uint32_t)q * kWave;
+    const uint64_t lrow = i < T ? r0 + w.m[i] : 0;
+    row[q] = i < T ? op.b.map_row[lrow] : 0;
+    si[q] = i < T ? op.b.map_inst[lrow] : 0;
+  }
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    const uint32_t i = lane + (uint32_t)q * kWave;
+    col[q] = 0;
+    if (i < T) {
+      col[q] = op.b.ytab[op.b.inst[si[q]].yt_off + w.c[i]];
+      row[q] -= (uint32_t)op.b.inst_R[si[q]];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    const uint32_t i = lane + (uint32_t)q * kWave;
+    if (i >= T) continue;
+    op.out_row[obase + i] = row[q];
+    op.out_col[obase + i] = col[q];
+    op.out_val[obase + i] = w.v[i];
+    if (!op.b.bounds) continue;
+    // first output of every instance in (previous output's instance, s]
+    int64_t sp = -1;
+    if (i > 0) sp = op.b.map_inst[r0 + w.m[i - 1]];
+    else if (wt > 0) sp = op.b.map_inst[rp];
     for (int64_t t = sp + 1; t <= (int64_t)si[q]; ++t) op.b.bounds[t] = obase + i;
   }
 }
@@ -524,17 +569,20 @@ void run_breduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint3
                       const U2* tmp, hipStream_t s) {
   if (!n) return;
   BReduceOp op{keys, vals, n, out_row, out_col, out_val, K, b, NarrowRows{}};
-  hipLaunchKernelGGL(k_breduce_emit<false>, dim3((uint32_t)scan_tmp_count(n)), dim3(kBlock), 0,
-                     s, op, tmp);
+  hipLaunchKernelGGL(k_breduce_emit, dim3((uint32_t)scan_tmp_count(n)), dim3(kBlock), 0, s, op,
+                     tmp);
 }
 
 void run_breduce_emit_narrow(const NarrowRows& nr, const float* vals, uint64_t n,
                              uint32_t* out_row, uint64_t* out_col, float* out_val,
                              const BReduceArgs& b, const U2* tmp, hipStream_t s) {
   if (!n) return;
+  if ((reinterpret_cast<uintptr_t>(nr.cols) | reinterpret_cast<uintptr_t>(vals)) & 15)
+    throw std::logic_error("narrow products must be 16-byte aligned");
   BReduceOp op{nullptr, vals, n, out_row, out_col, out_val, 32, b, nr};
-  hipLaunchKernelGGL(k_breduce_emit<true>, dim3((uint32_t)scan_tmp_count(n)), dim3(kBlock), 0,
-                     s, op, tmp);
+  const uint64_t nt = narrow_tile_count(n);
+  hipLaunchKernelGGL(k_breduce_emit_narrow, dim3((uint32_t)ceil_div(nt, kBlock / kWave)),
+                     dim3(kBlock), 0, s, op, tmp, nt);
 }
 
 void launch_bdense_count(const BDenseArgs& a, int eb, uint64_t rows, hipStream_t s) {

@@ -370,7 +370,7 @@ sptc_status BatchEngine::build_chunks(hipStream_t s) {
       c.row_start.reset(c.lrows + 1);
       SPTC_HIP(hipMemcpyAsync(c.row_start.get(), c.rseg.row_start(), (c.lrows + 1) * 8,
                               hipMemcpyDeviceToDevice, s));
-      c.tile_row.reset(2 * (scan_tmp_count(c.n) + 1));
+      c.tile_row.reset(2 * (narrow_tile_count(c.n) + 1));
       launch_tile_rows(c.row_start.get(), c.lrows, c.tile_row.get(), s);
       SPTC_HIP(hipStreamSynchronize(s));  // rr and the scratch are released below
       c.rseg.scratch.release();
@@ -387,7 +387,8 @@ sptc_status BatchEngine::build_chunks(hipStream_t s) {
   }
   kA_.reset(chunk_cap);
   vA_.reset(chunk_cap);
-  if (tmp_.size() < reduce_tmp_count(chunk_cap)) tmp_.reset(reduce_tmp_count(chunk_cap));
+  const uint64_t ntmp = std::max(reduce_tmp_count(chunk_cap), narrow_tmp_count(chunk_cap));
+  if (tmp_.size() < ntmp) tmp_.reset(ntmp);
   info.esc_chunks = (int32_t)esc_.size();
   return SPTC_OK;
 }
@@ -598,7 +599,8 @@ sptc_status BatchEngine::run(hipStream_t s, bool P) {
     if (kt) kt->end(tok, SPTC_K_REDUCE, (c.rowsort ? 4.0 : 8.0) * (double)c.n, s);
     if (P) {
       U2 t;
-      readback(&t, tmp_.get() + scan_tmp_count(c.n), sizeof(U2), s);
+      readback(&t, tmp_.get() + (c.rowsort ? narrow_tile_count(c.n) : scan_tmp_count(c.n)),
+               sizeof(U2), s);
       if (DevicePool::get().live_bytes() + t.a * 16 > budget_) {
         msg = "batch output does not fit the memory budget";
         return SPTC_ERR_OOM_BUDGET;

@@ -74,6 +74,15 @@ __device__ __forceinline__ uint32_t popc_below(uint64_t m) {
 }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
+// Orders one wave's LDS accesses across its lanes: the fences keep the
+// compiler from moving memory operations across the point (the wave
+// barrier alone does not), LDS executes a wave's accesses in order.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Agent-scope (whole device, across XCD L2s) relaxed atomics for look-back.
 __device__ __forceinline__ uint64_t ld_agent(const uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -149,8 +158,8 @@ struct NarrowRows {
   const uint32_t* cols;
   const uint64_t* rstart;
   uint64_t nrows;
-  // per reduce tile t (kScanTile products from t * kScanTile): [2t] = row of
-  // its first product, [2t + 1] = row of the product before it
+  // per wave tile t (512 products from t * 512): [2t] = row of its first
+  // product, [2t + 1] = row of the product before it
   const uint32_t* tile_row;
 };
 

@@ -470,7 +470,8 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
     uint64_t max_rows = 0;
     for (size_t c = 1; c < bounds_r.size(); ++c)
       max_rows = std::max(max_rows, bounds_r[c] - bounds_r[c - 1]);
-    DevBuf<U2> tmp(reduce_tmp_count(std::max(chunk_cap, max_rows)));
+    DevBuf<U2> tmp(std::max(reduce_tmp_count(std::max(chunk_cap, max_rows)),
+                            narrow_tmp_count(chunk_cap)));
     int rb_max = 8;
     for (size_t c = 1; c < bounds_r.size(); ++c) {
       const int b = mode_bits(bounds_r[c] - bounds_r[c - 1]) + kyf;
@@ -488,7 +489,7 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
     // row-segmented product sort
     DevBuf<RowRange> rrange(1);
     RowSegPlan rseg;
-    DevBuf<uint32_t> trow(2 * (scan_tmp_count(chunk_cap) + 1));
+    DevBuf<uint32_t> trow(2 * (narrow_tile_count(chunk_cap) + 1));
     for (size_t c = 1; c < bounds_r.size(); ++c) {
       const uint64_t r0c = bounds_r[c - 1], r1c = bounds_r[c];
       const uint64_t P0 = bounds_p[c - 1], P1 = bounds_p[c];
@@ -568,7 +569,8 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       if (rowsort) run_reduce_count_narrow(nr, n, tmp.get(), s_);
       else run_reduce_count(sk, n, tmp.get(), s_);
       kt_.end(tok, SPTC_K_REDUCE, (rowsort ? 4.0 : 8.0) * (double)n, s_);
-      const uint64_t out_n = read_total(tmp.get() + scan_tmp_count(n)).a;
+      const uint64_t out_n =
+          read_total(tmp.get() + (rowsort ? narrow_tile_count(n) : scan_tmp_count(n))).a;
       info.nnz += out_n;
       info.nchunks++;
       if (o_.count_only) continue;

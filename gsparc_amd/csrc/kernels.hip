@@ -5,6 +5,7 @@
 #include "reduce_core.hpp"
 
 #include <cstdlib>
+#include <stdexcept>
 
 namespace sptc {
 
@@ -627,13 +628,11 @@ void run_reduce_count(const uint64_t* keys, uint64_t n, U2* tmp, hipStream_t s) 
 // Emit for a tile of kScanTile sorted products (same tiling as the count
 // pass, whose scanned tile sums give each tile's first output slot); the
 // tile's outputs are stored coalesced.
-template <bool NARROW>
 __global__ __launch_bounds__(kBlock) void k_reduce_emit(ReduceOp op, const U2* tile_offs) {
   __shared__ ReduceSmem sm;
   uint64_t prev;
   const uint32_t tile = blockIdx.x;
-  const uint32_t nout = NARROW ? reduce_tile_narrow(op.nr, op.vals, op.n, tile, sm, &prev)
-                               : reduce_tile(op.keys, op.vals, op.n, tile, sm, &prev);
+  const uint32_t nout = reduce_tile(op.keys, op.vals, op.n, tile, sm, &prev);
   const uint64_t obase = tile_offs[tile].a;
   // the thread's column-table gathers are all issued before its stores
   const uint64_t* __restrict__ ytab = op.ytab;
@@ -651,7 +650,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_emit(ReduceOp op, const U2* t
 #pragma unroll
   for (int q = 0; q < kRedRunsPerThread; ++q) {
     const uint32_t i = threadIdx.x + (uint32_t)q * kBlock;
-    if (i < nout) {  // streaming stores, as the loads
+    if (i < nout) {  // streaming stores
       __builtin_nontemporal_store((uint32_t)(op.r0 + shr64(key[q], op.ynf)), &orow[i]);
       __builtin_nontemporal_store(col[q], &ocol[i]);
       __builtin_nontemporal_store(sm.v[i], &oval[i]);
@@ -664,68 +663,80 @@ void run_reduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32
                      uint64_t fmask, const uint64_t* ytab, const U2* tmp, hipStream_t s) {
   if (!n) return;
   ReduceOp op{keys, vals, n, out_row, out_col, out_val, r0, ynf, fmask, ytab, NarrowRows{}};
-  hipLaunchKernelGGL(k_reduce_emit<false>, dim3((uint32_t)scan_tmp_count(n)), dim3(kBlock), 0,
-                     s, op, tmp);
+  hipLaunchKernelGGL(k_reduce_emit, dim3((uint32_t)scan_tmp_count(n)), dim3(kBlock), 0, s, op,
+                     tmp);
+}
+
+// Narrow-form emit, one wave per wave tile (the count pass's tiling): the
+// tile's outputs are stored coalesced, column-table gathers issued first.
+__global__ __launch_bounds__(kBlock) void k_reduce_emit_narrow(ReduceOp op, const U2* tile_offs,
+                                                               uint64_t ntiles) {
+  __shared__ WaveSmem sm[kBlock / kWave];
+  const uint64_t wt = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  if (wt >= ntiles) return;
+  WaveSmem& w = sm[threadIdx.x / kWave];
+  uint64_t r0, rp;
+  const uint32_t T = wave_tile_reduce(op.nr, op.vals, op.n, wt, w, &r0, &rp);
+  const uint64_t obase = tile_offs[wt].a;
+  const uint32_t lane = lane_id();
+  const uint64_t* __restrict__ ytab = op.ytab;
+  const uint32_t rb = (uint32_t)(op.r0 + r0);
+  uint64_t col[kScanItems];
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    const uint32_t i = lane + (uint32_t)q * kWave;
+    col[q] = i < T ? w.c[i] : 0;
+    if (ytab && i < T) col[q] = ytab[col[q]];
+  }
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    const uint32_t i = lane + (uint32_t)q * kWave;
+    if (i < T) {
+      __builtin_nontemporal_store(rb + w.m[i], &op.out_row[obase + i]);
+      __builtin_nontemporal_store(col[q], &op.out_col[obase + i]);
+      __builtin_nontemporal_store(w.v[i], &op.out_val[obase + i]);
+    }
+  }
+}
+
+static void check_narrow(const NarrowRows& nr, const float* vals) {
+  if ((reinterpret_cast<uintptr_t>(nr.cols) | reinterpret_cast<uintptr_t>(vals)) & 15)
+    throw std::logic_error("narrow products must be 16-byte aligned");
 }
 
 void run_reduce_emit_narrow(const NarrowRows& nr, const float* vals, uint64_t n,
                             uint32_t* out_row, uint64_t* out_col, float* out_val, uint64_t r0,
                             const uint64_t* ytab, const U2* tmp, hipStream_t s) {
   if (!n) return;
+  check_narrow(nr, vals);
   ReduceOp op{nullptr, vals, n, out_row, out_col, out_val, r0, 32, 0xFFFFFFFFull, ytab, nr};
-  hipLaunchKernelGGL(k_reduce_emit<true>, dim3((uint32_t)scan_tmp_count(n)), dim3(kBlock), 0,
-                     s, op, tmp);
+  const uint64_t nt = narrow_tile_count(n);
+  hipLaunchKernelGGL(k_reduce_emit_narrow, dim3((uint32_t)ceil_div(nt, kBlock / kWave)),
+                     dim3(kBlock), 0, s, op, tmp, nt);
 }
 
-// Narrow form of k_reduce_heads: a head where the column changes or a row
-// starts (row starts of the tile marked in LDS first; columns staged in LDS
-// so every predecessor is one LDS read).
+// Narrow-form count: per wave tile, the number of run heads in
+// tile_sums[wt].a (wave_tile_heads).
 __global__ __launch_bounds__(kBlock) void k_reduce_heads_narrow(NarrowRows nr, uint64_t n,
+                                                                uint64_t ntiles,
                                                                 U2* __restrict__ tile_sums) {
-  __shared__ U2 scratch[kBlock / kWave];
-  __shared__ uint8_t flag[kScanTile];
-  __shared__ uint32_t kc[kScanTile];
-  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
-  const uint64_t end = n - base < kScanTile ? n : base + kScanTile;
-  const uint64_t r0 = nr.tile_row[2 * (uint64_t)blockIdx.x];
-  const uint64_t r1 = r0 + 1 + threadIdx.x;
-  const uint64_t st1 = r1 < nr.nrows ? nr.rstart[r1] : ~0ull;
-  const bool at0 = nr.rstart[r0] == base;  // a row starts at the tile's first product
-  uint32_t kreg[kScanItems];
-#pragma unroll
-  for (int e = 0; e < kScanItems; ++e) {
-    const uint64_t i = base + (uint64_t)e * kBlock + threadIdx.x;
-    kreg[e] = i < n ? nr.cols[i] : 0;
-  }
-  const uint32_t pk0 = base > 0 ? nr.cols[base - 1] : 0;
-#pragma unroll
-  for (int e = 0; e < kScanItems; ++e) kc[(uint32_t)e * kBlock + threadIdx.x] = kreg[e];
-  for (uint32_t i = threadIdx.x; i < kScanTile; i += kBlock) flag[i] = 0;
-  // rows starting inside the tile (any shows in thread 0's candidate)
-  if (__syncthreads_or(st1 < end)) {
-    for (uint64_t r = r1; r < nr.nrows; r += kBlock) {
-      const uint64_t st = r == r1 ? st1 : nr.rstart[r];
-      if (st >= end) break;
-      flag[st - base] = 1;
-    }
-  }
-  if (threadIdx.x == 0 && at0) flag[0] = 1;
-  __syncthreads();
-  uint64_t c = 0;
-#pragma unroll
-  for (int e = 0; e < kScanItems; ++e) {
-    const uint32_t j = (uint32_t)e * kBlock + threadIdx.x;
-    const uint64_t i = base + j;
-    const uint32_t k = kreg[e];
-    const uint32_t pk = j ? kc[j - 1] : pk0;
-    c += i < n && (i == 0 || flag[j] || k != pk);
-  }
-  U2 tot;
-  block_excl_scan_u2(mk2(c, 0), scratch, &tot);
-  if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+  __shared__ __align__(16) uint32_t mark[kBlock / kWave][kWaveTile];
+  const uint64_t wt = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  if (wt >= ntiles) return;
+  const uint64_t b = wt * kWaveTile;
+  const uint32_t cnt = (uint32_t)(n - b < kWaveTile ? n - b : kWaveTile);
+  uint32_t col[kScanItems], mk[kScanItems];
+  float unused[kScanItems];
+  wave_tile_load<false>(nr.cols, nullptr, n, b + (uint64_t)kScanItems * lane_id(), col, unused);
+  bool rowsin;
+  const uint32_t hm = wave_tile_heads(nr, n, b, cnt, nr.tile_row[2 * wt], col,
+                                      mark[threadIdx.x / kWave], mk, &rowsin);
+  const uint32_t tot = __builtin_amdgcn_readlane(wave_incl_scan32((uint32_t)__popc(hm)),
+                                                 kWave - 1);
+  if (lane_id() == 0) tile_sums[wt] = mk2(tot, 0);
 }
 
-// One thread per row r with products [s, e), no search: reduce tiles whose
+// One thread per row r with products [s, e), no search: wave tiles whose
 // first product lies in [s, e) get [2t] = r, those whose preceding product
 // lies in [s, e) get [2t + 1] = r (tile_row holds 2 * (tiles + 1) entries).
 __global__ __launch_bounds__(kBlock) void k_tile_rows(const uint64_t* __restrict__ rstart,
@@ -736,8 +747,8 @@ __global__ __launch_bounds__(kBlock) void k_tile_rows(const uint64_t* __restrict
   for (; r < nrows; r += stride) {
     const uint64_t s = rstart[r], e = rstart[r + 1];
     if (e <= s) continue;
-    for (uint64_t t = ceil_div(s, kScanTile); t * kScanTile < e; ++t) tile_row[2 * t] = (uint32_t)r;
-    for (uint64_t t = ceil_div(s + 1, kScanTile); t * kScanTile < e + 1; ++t)
+    for (uint64_t t = ceil_div(s, kWaveTile); t * kWaveTile < e; ++t) tile_row[2 * t] = (uint32_t)r;
+    for (uint64_t t = ceil_div(s + 1, kWaveTile); t * kWaveTile < e + 1; ++t)
       tile_row[2 * t + 1] = (uint32_t)r;
   }
 }
@@ -750,10 +761,12 @@ void launch_tile_rows(const uint64_t* rstart, uint64_t nrows, uint32_t* tile_row
 }
 
 void run_reduce_count_narrow(const NarrowRows& nr, uint64_t n, U2* tmp, hipStream_t s) {
-  const uint64_t nt = scan_tmp_count(n);
-  if (n)
-    hipLaunchKernelGGL(k_reduce_heads_narrow, dim3((uint32_t)nt), dim3(kBlock), 0, s, nr, n,
-                       tmp);
+  const uint64_t nt = narrow_tile_count(n);
+  if (n) {
+    check_narrow(nr, nullptr);
+    hipLaunchKernelGGL(k_reduce_heads_narrow, dim3((uint32_t)ceil_div(nt, kBlock / kWave)),
+                       dim3(kBlock), 0, s, nr, n, nt, tmp);
+  }
   scan_tiles_parts(tmp, n ? nt : 0, tmp + nt, tmp + nt + 1, s);
 }
 

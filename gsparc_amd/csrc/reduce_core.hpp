@@ -116,143 +116,208 @@ __device__ __forceinline__ uint32_t reduce_tile(const uint64_t* __restrict__ key
   return nruns;
 }
 
-// reduce_tile for the narrow form: the tile's packed keys are rebuilt in LDS
-// from the row starts (a mark at each row's first product, then a prefix
-// maximum over the tile), the run crossing the tile end stops at its row's
-// end as well.  Same outputs, keys (row << 32 | column).
-__device__ __forceinline__ uint32_t reduce_tile_narrow(const NarrowRows& nr,
-                                                       const float* __restrict__ vals,
-                                                       uint64_t n, uint32_t tile,
-                                                       ReduceSmem& sm, uint64_t* prev) {
-  uint32_t* mark = reinterpret_cast<uint32_t*>(sm.v);  // aliases v until the keys are built
-  const uint32_t tid = threadIdx.x;
-  const uint64_t base = (uint64_t)tile * kScanTile;
-  const uint32_t cnt = (uint32_t)(n - base < kScanTile ? n - base : kScanTile);
-  // every load the tile needs first is issued at once: row table entries
-  // (same address for all threads), columns, values (held in registers
-  // while v holds the marks), the first candidate row start
-  const uint64_t r0 = nr.tile_row[2 * (uint64_t)tile];
-  const uint64_t rp = nr.tile_row[2 * (uint64_t)tile + 1];
-  const uint64_t pcol = base ? nr.cols[base - 1] : 0;
-  float vreg[kScanItems];
-  const uint64_t r1 = r0 + 1 + tid;
-  uint64_t st1 = r1 < nr.nrows ? nr.rstart[r1] : ~0ull;
+// ---- narrow form: wave tiles ----------------------------------------------
+// The narrow form is reduced in wave tiles of kWaveTile sorted products,
+// lane l holding products [8l, 8l + 8) of its tile in registers; every wave
+// works alone (no workgroup barriers).  A product starts a run (a head)
+// when it is the first product, a row starts at it or its column differs
+// from its predecessor's, as the 64-bit keys (row << 32 | column) would
+// give.  Each run is summed in sorted order with compensated FP32
+// arithmetic by the lane holding its head; a run that continues past the
+// lane reads the later lanes' values from LDS and, past the tile, global
+// memory up to another column or its row's end.
+static_assert(kScanItems == 8, "wave tiles hold 8 products per lane");
+// (may_alias: LDS words are written as vectors and read one by one)
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4), __may_alias__));
+typedef float f32x4 __attribute__((ext_vector_type(4), __may_alias__));
+
+struct alignas(16) WaveSmem {
+  float v[kWaveTile];     // the tile's values, then its outputs' sums
+  uint32_t c[kWaveTile];  // the outputs' columns
+  uint32_t m[kWaveTile];  // row-start marks, then the outputs' rows (- r0)
+};
+
+// The lane's 8 products from p: streaming vector loads when all lie below
+// n (cols and vals 16-byte aligned, p a multiple of 8), else one by one.
+template <bool VALS>
+__device__ __forceinline__ void wave_tile_load(const uint32_t* __restrict__ cols,
+                                               const float* __restrict__ vals, uint64_t n,
+                                               uint64_t p, uint32_t (&col)[kScanItems],
+                                               float (&val)[kScanItems]) {
+  if (p + kScanItems <= n) {
+    const u32x4* cp = reinterpret_cast<const u32x4*>(cols + p);
+    const u32x4 c0 = __builtin_nontemporal_load(cp), c1 = __builtin_nontemporal_load(cp + 1);
 #pragma unroll
-  for (int e = 0; e < kScanItems; ++e) {
-    const uint32_t i = (uint32_t)e * kBlock + tid;
-    // streaming loads (non-temporal): they should not evict the column
-    // table the emit gathers from
-    if (i < cnt) sm.k[i] = r0 << 32 | __builtin_nontemporal_load(&nr.cols[base + i]);
-    vreg[e] = i < cnt ? __builtin_nontemporal_load(&vals[base + i]) : 0.f;
+    for (int j = 0; j < 4; ++j) {
+      col[j] = c0[j];
+      col[4 + j] = c1[j];
+    }
+    if (VALS) {
+      const f32x4* vp = reinterpret_cast<const f32x4*>(vals + p);
+      const f32x4 v0 = __builtin_nontemporal_load(vp), v1 = __builtin_nontemporal_load(vp + 1);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        val[j] = v0[j];
+        val[4 + j] = v1[j];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+      col[j] = p + j < n ? cols[p + j] : 0;
+      if (VALS) val[j] = p + j < n ? vals[p + j] : 0.f;
+    }
   }
-  // most tiles lie inside one row (thread 0's candidate is the next row):
-  // their keys are complete; otherwise rows come from marks at the row
-  // starts and a prefix maximum
-  if (__syncthreads_or(st1 < base + cnt)) {
+}
+
+// Head mask of the lane's products of the wave tile at b (cnt products,
+// the first in row r0): bit j set when product 8 * lane + j starts a run.
+// When rows start after the tile's first product (*rowsin), mk[j] = row - r0
+// of the last row starting at the lane's product j (0: none), and `mark`
+// (the wave's kWaveTile words of LDS) holds them for the whole tile.
+__device__ __forceinline__ uint32_t wave_tile_heads(const NarrowRows& nr, uint64_t n, uint64_t b,
+                                                    uint32_t cnt, uint64_t r0,
+                                                    const uint32_t (&col)[kScanItems],
+                                                    uint32_t* __restrict__ mark,
+                                                    uint32_t (&mk)[kScanItems], bool* rowsin) {
+  const uint32_t lane = lane_id();
+  const uint64_t p = b + (uint64_t)kScanItems * lane;
+  const uint32_t pc0 = b ? nr.cols[b - 1] : 0;  // same address in every lane
+  const bool at0 = nr.rstart[r0] == b;
+  // lane l's candidate: row r0 + 1 + l (later rows start after product b)
+  const uint64_t r1 = r0 + 1 + lane;
+  const uint64_t st1 = r1 < nr.nrows ? nr.rstart[r1] : ~0ull;
+  const bool any = __ballot(st1 < b + cnt) != 0;
+  *rowsin = any;
 #pragma unroll
-    for (int e = 0; e < kScanItems; ++e) mark[(uint32_t)e * kBlock + tid] = 0;
-    __syncthreads();
-    for (uint64_t r = r1; r < nr.nrows; r += kBlock) {
+  for (int j = 0; j < kScanItems; ++j) mk[j] = 0;
+  if (any) {
+    u32x4* mp = reinterpret_cast<u32x4*>(mark + kScanItems * lane);
+    mp[0] = u32x4{0u, 0u, 0u, 0u};
+    mp[1] = u32x4{0u, 0u, 0u, 0u};
+    wave_sync();
+    for (uint64_t r = r1; r < nr.nrows; r += kWave) {
       const uint64_t st = r == r1 ? st1 : nr.rstart[r];
-      if (st >= base + cnt) break;
-      atomicMax(&mark[st - base], (uint32_t)(r - r0));
+      if (st >= b + cnt) break;
+      if (st > b) atomicMax(&mark[st - b], (uint32_t)(r - r0));  // empty rows: the last wins
     }
-    __syncthreads();
-    // Wave w walks positions [w * 512, w * 512 + 512) in 8 lane-striped
-    // steps (conflict-free LDS access), carrying the maximum; the waves
-    // before it contribute theirs.
-    constexpr uint32_t kSpan = (uint32_t)kScanTile / (kBlock / kWave);
-    const uint32_t lane = lane_id(), wv = tid / kWave;
-    uint32_t loc[kSpan / kWave];
-    uint32_t carry = 0;
+    wave_sync();
+    const u32x4 m0 = mp[0], m1 = mp[1];
 #pragma unroll
-    for (int e = 0; e < (int)(kSpan / kWave); ++e) {
-      uint32_t m = mark[wv * kSpan + e * kWave + lane];
-#pragma unroll
-      for (int d = 1; d < kWave; d <<= 1) {
-        const uint32_t o = __shfl_up(m, d, kWave);
-        if (lane >= (uint32_t)d) m = m > o ? m : o;
-      }
-      m = m > carry ? m : carry;
-      loc[e] = m;
-      carry = __shfl(m, kWave - 1, kWave);
-    }
-    __shared__ uint32_t s_wave[kBlock / kWave];
-    if (lane == 0) s_wave[wv] = carry;
-    __syncthreads();
-    uint32_t ex = 0;
-    for (uint32_t w = 0; w < wv; ++w) ex = s_wave[w] > ex ? s_wave[w] : ex;
-#pragma unroll
-    for (int e = 0; e < (int)(kSpan / kWave); ++e) {
-      const uint32_t i = wv * kSpan + e * kWave + lane;
-      if (i < cnt) {
-        const uint32_t rr = loc[e] > ex ? loc[e] : ex;
-        sm.k[i] = (r0 + rr) << 32 | (sm.k[i] & 0xFFFFFFFFull);
-      }
+    for (int j = 0; j < 4; ++j) {
+      mk[j] = m0[j];
+      mk[4 + j] = m1[j];
     }
   }
-  __syncthreads();  // marks consumed
+  uint32_t prev = __shfl_up(col[kScanItems - 1], 1, kWave);
+  if (lane == 0) prev = pc0;
+  uint32_t hm = 0;
 #pragma unroll
-  for (int e = 0; e < kScanItems; ++e) sm.v[(uint32_t)e * kBlock + tid] = vreg[e];
-  const uint64_t prev_key = base ? (rp << 32 | pcol) : 0;
-  *prev = prev_key;
-  __syncthreads();
-  for (uint32_t i = tid; i < kScanTile; i += kBlock) {
-    uint8_t h = 0;
-    if (i < cnt) h = (base + i == 0) || sm.k[i] != (i ? sm.k[i - 1] : prev_key);
-    sm.head[i] = h;
+  for (int j = 0; j < kScanItems; ++j) {
+    const uint64_t i = p + j;
+    const uint32_t pk = j ? col[j - 1] : prev;
+    const bool rs = mk[j] != 0 || (j == 0 && lane == 0 && at0);
+    hm |= (uint32_t)(i < n && (i == 0 || rs || col[j] != pk)) << j;
   }
-  __syncthreads();
-  const uint64_t f = reinterpret_cast<const uint64_t*>(sm.head)[tid];
-  U2 tot;
-  uint32_t rix = (uint32_t)block_excl_scan_u2(mk2((uint64_t)__popcll(f), 0), sm.scratch, &tot).a;
-  const uint32_t nruns = (uint32_t)tot.a;
-  for (uint64_t m = f; m; m &= m - 1)
-    sm.rs[rix++] = (uint16_t)(tid * 8 + (__builtin_ctzll(m) >> 3));
-  if (tid == 0) {
-    sm.rs[nruns] = (uint16_t)cnt;
-    sm.tail_end = base + cnt;
+  return hm;
+}
+
+// Reduces wave tile wt: leaves its outputs -- one per head in the tile, in
+// key order -- in (w.c, w.m, w.v)[0 .. T-1] (column, row - r0, sum) and
+// returns T; *r0 = row of the tile's first product, *rp = row of the
+// product before it.
+__device__ __forceinline__ uint32_t wave_tile_reduce(const NarrowRows& nr,
+                                                     const float* __restrict__ vals, uint64_t n,
+                                                     uint64_t wt, WaveSmem& w, uint64_t* r0p,
+                                                     uint64_t* rpp) {
+  const uint32_t lane = lane_id();
+  const uint64_t b = wt * kWaveTile;
+  const uint32_t cnt = (uint32_t)(n - b < kWaveTile ? n - b : kWaveTile);
+  const uint64_t r0 = nr.tile_row[2 * wt];
+  *r0p = r0;
+  *rpp = b ? nr.tile_row[2 * wt + 1] : 0;
+  uint32_t col[kScanItems], mk[kScanItems];
+  float val[kScanItems];
+  wave_tile_load<true>(nr.cols, vals, n, b + (uint64_t)kScanItems * lane, col, val);
+  bool rowsin;
+  const uint32_t hm = wave_tile_heads(nr, n, b, cnt, r0, col, w.m, mk, &rowsin);
+  // row of each product: prefix maximum of the marks (lane, then wave)
+  uint32_t ro[kScanItems];
+  if (rowsin) {
+    uint32_t pm = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+      pm = mk[j] > pm ? mk[j] : pm;
+      ro[j] = pm;
+    }
+    uint32_t inc = pm;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const uint32_t o = __shfl_up(inc, d, kWave);
+      if (lane >= (uint32_t)d) inc = o > inc ? o : inc;
+    }
+    uint32_t ex = __shfl_up(inc, 1, kWave);
+    if (lane == 0) ex = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) ro[j] = ro[j] > ex ? ro[j] : ex;
+  } else {
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) ro[j] = 0;
   }
-  __syncthreads();
-  // the last run may continue past the tile, up to another column or its row's end
-  if (nruns && base + cnt < n && tid < kWave) {
-    const uint64_t last = sm.k[sm.rs[nruns - 1]];
-    const uint32_t lcol = (uint32_t)last;
-    const uint64_t rend = nr.rstart[(last >> 32) + 1];
-    for (uint64_t g0 = base + cnt;; g0 += kWave) {
-      const uint64_t g = g0 + tid;
-      const uint32_t c = g < n ? nr.cols[g] : 0;  // issued beside the row end
-      const bool stop = g >= n || c != lcol || g >= rend;
-      const uint64_t m = __ballot(stop);
+  // output slots: exclusive scan of the lanes' head counts
+  const uint32_t h = (uint32_t)__popc(hm);
+  const uint32_t hinc = wave_incl_scan32(h);
+  const uint32_t o = hinc - h;
+  const uint32_t T = __builtin_amdgcn_readlane(hinc, kWave - 1);
+  // first head after the lane's products (kWaveTile: none), suffix minimum
+  uint32_t fs = hm ? kScanItems * lane + (uint32_t)__builtin_ctz(hm) : (uint32_t)kWaveTile;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint32_t x = __shfl_down(fs, d, kWave);
+    if (lane + (uint32_t)d < kWave) fs = x < fs ? x : fs;
+  }
+  uint32_t nxt = __shfl_down(fs, 1, kWave);
+  if (lane == kWave - 1) nxt = (uint32_t)kWaveTile;
+  // the tile's last run may continue past it: up to another column or its
+  // row's end (a full tile here, so its last product is lane 63's item 7)
+  uint64_t tail_end = b + cnt;
+  if (T && b + cnt < n) {
+    const uint32_t lc = __builtin_amdgcn_readlane(col[kScanItems - 1], kWave - 1);
+    const uint32_t lro = __builtin_amdgcn_readlane(ro[kScanItems - 1], kWave - 1);
+    const uint64_t rend = nr.rstart[r0 + lro + 1];
+    for (uint64_t g0 = b + cnt;; g0 += kWave) {
+      const uint64_t g = g0 + lane;
+      const uint32_t c = g < n ? nr.cols[g] : 0;
+      const uint64_t m = __ballot(g >= n || c != lc || g >= rend);
       if (m) {
-        if (tid == 0) sm.tail_end = g0 + __builtin_ctzll(m);
+        tail_end = g0 + (uint64_t)__builtin_ctzll(m);
         break;
       }
     }
   }
-  __syncthreads();
-  const uint64_t tail_end = sm.tail_end;
-  uint64_t ok[kRedRunsPerThread];
-  float ov[kRedRunsPerThread];
+  // values to LDS for the runs that continue into later lanes
+  {
+    f32x4* vp = reinterpret_cast<f32x4*>(w.v + kScanItems * lane);
+    vp[0] = f32x4{val[0], val[1], val[2], val[3]};
+    vp[1] = f32x4{val[4], val[5], val[6], val[7]};
+  }
+  wave_sync();
+  // sums: res[j] = the running sum after product j of the run it belongs to
+  const uint64_t p = b + (uint64_t)kScanItems * lane;
+  KahanF32 acc;
+  float res[kScanItems];
 #pragma unroll
-  for (int q = 0; q < kRedRunsPerThread; ++q) {
-    const uint32_t rn = tid + (uint32_t)q * kBlock;
-    ok[q] = 0;
-    ov[q] = 0.f;
-    if (rn >= nruns) continue;
-    const uint32_t j0 = sm.rs[rn], j1 = sm.rs[rn + 1];
-    KahanF32 acc;
-    uint32_t j = j0;
-    for (; j + 4 <= j1; j += 4) {
-      const float a = sm.v[j], b = sm.v[j + 1], c = sm.v[j + 2], d = sm.v[j + 3];
-      acc.add(a);
-      acc.add(b);
-      acc.add(c);
-      acc.add(d);
-    }
-    for (; j < j1; ++j) acc.add(sm.v[j]);
-    if (rn + 1 == nruns) {
-      for (uint64_t g = base + cnt; g < tail_end; g += 8) {
+  for (int j = 0; j < kScanItems; ++j) {
+    if ((hm >> j) & 1u) acc = KahanF32();
+    if (p + j < n) acc.add(val[j]);
+    res[j] = acc.value();
+  }
+  float last = 0.f;
+  if (hm) {  // the lane's last run: later lanes' products, then past the tile
+    const uint32_t stop = nxt < cnt ? nxt : cnt;
+    for (uint32_t q = kScanItems * lane + kScanItems; q < stop; ++q) acc.add(w.v[q]);
+    if (nxt >= cnt) {
+      for (uint64_t g = b + cnt; g < tail_end; g += 8) {
         float t[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) t[e] = g + e < tail_end ? vals[g + e] : 0.f;
@@ -261,20 +326,27 @@ __device__ __forceinline__ uint32_t reduce_tile_narrow(const NarrowRows& nr,
           if (g + e < tail_end) acc.add(t[e]);
       }
     }
-    ok[q] = sm.k[j0];
-    ov[q] = acc.value();
+    last = acc.value();
   }
-  __syncthreads();
+  wave_sync();  // every lane's LDS reads of values done
+  uint32_t k = o;
 #pragma unroll
-  for (int q = 0; q < kRedRunsPerThread; ++q) {
-    const uint32_t rn = tid + (uint32_t)q * kBlock;
-    if (rn < nruns) {
-      sm.k[rn] = ok[q];
-      sm.v[rn] = ov[q];
+  for (int j = 0; j < kScanItems; ++j) {
+    if ((hm >> j) & 1u) {
+      w.c[k] = col[j];
+      w.m[k] = ro[j];
+      ++k;
     }
   }
-  __syncthreads();
-  return nruns;
+  // a run ends before each later head of the lane; the lane's last run
+  // ends with `last`
+  k = o;
+#pragma unroll
+  for (int j = 0; j + 1 < kScanItems; ++j)
+    if (((hm >> (j + 1)) & 1u) && (hm & ((2u << j) - 1u))) w.v[k++] = res[j];
+  if (hm) w.v[o + h - 1] = last;
+  wave_sync();
+  return T;
 }
 
 }  // namespace sptc

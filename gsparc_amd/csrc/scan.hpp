@@ -112,6 +112,16 @@ inline uint64_t reduce_tmp_count(uint64_t n) {
   const uint64_t nt = scan_tmp_count(n);
   return nt + 1 + ceil_div(nt, kScanSumsPerBlock) + 1;
 }
+// The narrow form's reduce (count and emit) works in wave tiles of
+// kWaveTile products, kScanItems consecutive ones per lane: one tile sum
+// per wave tile, the grand total at tmp[narrow_tile_count(n)], and
+// narrow_tmp_count(n) entries of tmp in all.
+constexpr uint64_t kWaveTile = (uint64_t)kWave * kScanItems;
+inline uint64_t narrow_tile_count(uint64_t n) { return ceil_div(n ? n : 1, kWaveTile); }
+inline uint64_t narrow_tmp_count(uint64_t n) {
+  const uint64_t nt = narrow_tile_count(n);
+  return nt + 1 + ceil_div(nt, kScanSumsPerBlock) + 1;
+}
 // Exclusive scan of nt tile sums in place, total -> *total; parts holds
 // ceil(nt / kScanSumsPerBlock) + 1 entries (defined in kernels.hip).
 void scan_tiles_parts(U2* sums, uint64_t nt, U2* total, U2* parts, hipStream_t s);
