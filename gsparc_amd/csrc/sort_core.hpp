@@ -42,6 +42,24 @@ struct TileSmem {
   uint32_t tile;
 };
 
+// Lanes of the wave holding the same RB-bit digit d as this lane, among
+// the lanes in `valid`.  Per digit bit b: m = -(bit b of d) (0 or all ones)
+// and the ballot of that bit; (ballot ^ m) marks the lanes whose bit b
+// differs from this lane's, accumulated per 32-lane half.
+template <int RB>
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
+  uint32_t nlo = 0, nhi = 0;
+#pragma unroll
+  for (int b = 0; b < RB; ++b) {
+    uint32_t m;  // one v_bfe_i32 (the compiler splits a sign extract into two shifts)
+    asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(m) : "v"(d), "i"(b));
+    const uint64_t bb = __ballot(m != 0);
+    nlo |= (uint32_t)bb ^ m;
+    nhi |= (uint32_t)(bb >> 32) ^ m;
+  }
+  return valid & ~((uint64_t)nhi << 32 | nlo);
+}
+
 // Ranks the keys a thread holds (wave w owns keys [w*64*IPT, (w+1)*64*IPT)
 // of the tile, lane-striped: key i of a lane is wbase + i*64 + lane; valid
 // when below n) within its wave's run of equal digits.  sm.cnt must be zero;
@@ -59,13 +77,7 @@ __device__ __forceinline__ void tile_rank(TileSmem<RB, IPT, NT, ST>& sm, uint64_
     const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
     const bool valid = idx < n;
     const uint32_t d = (uint32_t)(k[i] >> shift) & (R - 1);
-    uint64_t peers = ballot(valid);
-#pragma unroll
-    for (int b = 0; b < RB; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bb = ballot(bit);
-      peers &= bit ? bb : ~bb;
-    }
+    const uint64_t peers = match_digit<RB>(d, ballot(valid));
     uint32_t before = 0;
     if (valid) before = sm.cnt[wave][d];
     __builtin_amdgcn_wave_barrier();
