@@ -215,13 +215,9 @@ __global__ __launch_bounds__(kBlock) void k_bexpand(BExpandArgs a) {
     const uint64_t pb = tl.p0;
     const uint32_t cnt = tl.cnt;
     __syncthreads();
-    if (threadIdx.x == 0) {
-      uint64_t lo = pc.A0, hi = pc.A1;  // last j in the piece with prefix <= pb
-      while (lo < hi) {
-        const uint64_t mid = (lo + hi) >> 1;
-        if (a.act_prefix[mid] <= pb) lo = mid + 1; else hi = mid;
-      }
-      s_ja = lo - 1;
+    if (threadIdx.x < kWave) {  // last j in the piece with prefix <= pb
+      const uint64_t j = wave_last_le(a.act_prefix, pc.A0, pc.A1, pb);
+      if (threadIdx.x == 0) s_ja = j;
     }
     __syncthreads();
     const uint64_t ja = s_ja;

@@ -111,6 +111,33 @@ __device__ __forceinline__ uint32_t wave_incl_scan32(uint32_t v) {
   return v;
 }
 
+// Last j in [lo, hi) with pre[j] <= p, for non-decreasing pre with
+// pre[lo] <= p; called by a whole wave, the result is uniform.  The first
+// probe reads the 64 entries from lo (an owner search that starts from the
+// previous owner usually ends there); then each round samples 64 evenly
+// spaced entries and keeps the interval between the last two that bracket
+// p, so a range of N entries takes about log64(N) dependent loads.
+__device__ __forceinline__ uint64_t wave_last_le(const uint64_t* __restrict__ pre, uint64_t lo,
+                                                 uint64_t hi, uint64_t p) {
+  const uint32_t lane = lane_id();
+  {
+    const uint64_t j = lo + lane;
+    const uint64_t m = __ballot(j < hi && pre[j] <= p);  // a prefix of the lanes
+    if (!(m >> (kWave - 1))) return lo + (uint64_t)__popcll(m) - 1;
+    lo += kWave - 1;
+  }
+  while (hi - lo > (uint64_t)kWave) {
+    const uint64_t step = (hi - lo + kWave - 1) / kWave;
+    const uint64_t j = lo + lane * step;
+    const uint64_t m = __ballot(j < hi && pre[j] <= p);  // lane 0 holds (j = lo)
+    lo += (uint64_t)(kWave - 1 - __builtin_clzll(m)) * step;
+    hi = hi < lo + step ? hi : lo + step;
+  }
+  const uint64_t j = lo + lane;
+  const uint64_t m = __ballot(j < hi && pre[j] <= p);
+  return lo + (uint64_t)__popcll(m) - 1;
+}
+
 // Block-wide exclusive scan (blockDim == kBlock), returns exclusive prefix
 // and the block total through *total.  `scratch` holds kBlock/kWave values.
 template <typename T>

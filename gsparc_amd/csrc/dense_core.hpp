@@ -36,13 +36,9 @@ __device__ __forceinline__ uint32_t dense_stage_owners(const uint64_t* __restric
                                                        uint64_t* pref, uint64_t* s_ja,
                                                        uint64_t* ja) {
   __syncthreads();
-  if (threadIdx.x == 0) {
-    uint64_t lo = j0, hi = j1;  // last j with act_prefix[j] <= pb
-    while (lo < hi) {
-      const uint64_t mid = (lo + hi) >> 1;
-      if (act_prefix[mid] <= pb) lo = mid + 1; else hi = mid;
-    }
-    *s_ja = lo - 1;
+  if (threadIdx.x < kWave) {  // last j with act_prefix[j] <= pb
+    const uint64_t j = wave_last_le(act_prefix, j0, j1, pb);
+    if (threadIdx.x == 0) *s_ja = j;
   }
   __syncthreads();
   *ja = *s_ja;

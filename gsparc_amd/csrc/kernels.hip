@@ -473,14 +473,9 @@ __global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs a) {
     const uint64_t pb = a.P0 + tile * kExpandTile;
     const uint64_t pe = pb + kExpandTile < a.P1 ? pb + kExpandTile : a.P1;
     __syncthreads();
-    if (threadIdx.x == 0) {
-      // last j in [A0, A1) with act_prefix[j] <= pb
-      uint64_t lo = a.A0, hi = a.A1;
-      while (lo < hi) {
-        uint64_t mid = (lo + hi) >> 1;
-        if (a.act_prefix[mid] <= pb) lo = mid + 1; else hi = mid;
-      }
-      s_ja = lo - 1;
+    if (threadIdx.x < kWave) {  // last j in [A0, A1) with act_prefix[j] <= pb
+      const uint64_t j = wave_last_le(a.act_prefix, a.A0, a.A1, pb);
+      if (threadIdx.x == 0) s_ja = j;
     }
     __syncthreads();
     const uint64_t ja = s_ja;

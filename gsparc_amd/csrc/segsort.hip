@@ -356,20 +356,18 @@ __global__ __launch_bounds__(kBlock) void k_expand_items(ExpandArgs a,
     float* oval = isbig ? a.big_val : a.out_vals;
     if (isbig)
       for (int i = threadIdx.x; i < R; i += kBlock) sh[i] = 0;
+    uint64_t jlo = a.A0;  // owners only move forward within an item
     for (uint32_t sub = 0; sub < count; sub += kExpandSub) {
       const uint64_t pb = a.P0 + start + sub;
       const uint64_t pe = pb + (count - sub < kExpandSub ? count - sub : kExpandSub);
       __syncthreads();
-      if (threadIdx.x == 0) {
-        uint64_t lo = a.A0, hi = a.A1;  // last j with act_prefix[j] <= pb
-        while (lo < hi) {
-          const uint64_t mid = (lo + hi) >> 1;
-          if (a.act_prefix[mid] <= pb) lo = mid + 1; else hi = mid;
-        }
-        s_ja = lo - 1;
+      if (threadIdx.x < kWave) {  // last j with act_prefix[j] <= pb
+        const uint64_t j = wave_last_le(a.act_prefix, jlo, a.A1, pb);
+        if (threadIdx.x == 0) s_ja = j;
       }
       __syncthreads();
       const uint64_t ja = s_ja;
+      jlo = ja;
       const uint64_t avail = a.A1 - ja;
       const uint32_t m = (uint32_t)(avail < kExpandSub + 1 ? avail : kExpandSub + 1);
       for (uint32_t k = threadIdx.x; k < m; k += kBlock) s_pref[k] = a.act_prefix[ja + k];
