@@ -205,7 +205,7 @@ static_assert(kBExpandTileItems == kBlock * kBExpandItems, "expand tile");
 __global__ __launch_bounds__(kBlock) void k_bexpand(BExpandArgs a) {
   constexpr uint32_t TILE = kBExpandTileItems;
   __shared__ uint64_t s_pref[TILE + 1];
-  __shared__ uint64_t s_ja;
+  __shared__ uint64_t s_ja, s_jl;
   if (*a.bad) return;
   uint64_t t0, t1, step;
   xcd_tiles(a.ntiles, a.remap, &t0, &t1, &step);
@@ -215,14 +215,18 @@ __global__ __launch_bounds__(kBlock) void k_bexpand(BExpandArgs a) {
     const uint64_t pb = tl.p0;
     const uint32_t cnt = tl.cnt;
     __syncthreads();
-    if (threadIdx.x < kWave) {  // last j in the piece with prefix <= pb
-      const uint64_t j = wave_last_le(a.act_prefix, pc.A0, pc.A1, pb);
+    // owners of the tile's first and last products (last j in the piece
+    // with prefix <= p; waves 0 and 1): only their prefixes are staged
+    if (threadIdx.x < 2 * kWave) {
+      const uint64_t j =
+          wave_last_le(a.act_prefix, pc.A0, pc.A1, threadIdx.x < kWave ? pb : pb + cnt - 1);
       if (threadIdx.x == 0) s_ja = j;
+      if (threadIdx.x == kWave) s_jl = j;
     }
     __syncthreads();
     const uint64_t ja = s_ja;
-    const uint64_t avail = pc.A1 - ja;
-    const uint32_t m = (uint32_t)(avail < TILE + 1 ? avail : TILE + 1);
+    const uint64_t need = s_jl - ja + 1;
+    const uint32_t m = (uint32_t)(need < TILE + 1 ? need : TILE + 1);
     for (uint32_t k = threadIdx.x; k < m; k += kBlock) s_pref[k] = a.act_prefix[ja + k];
     __syncthreads();
     uint32_t klo = 0, kc = ~0u;

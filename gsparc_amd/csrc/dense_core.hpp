@@ -27,23 +27,28 @@ struct DenseSmem {
   uint32_t bm[W];
   uint64_t pref[kDenseRound + 1];
   uint32_t red[kBlock / kWave];
+  uint64_t jl;  // owner of the round's last product
 };
 
-// Stages the round's owner prefixes: owners of products [pb, pb + cnt) are
+// Stages the round's owner prefixes: owners of products [pb, pl] are
 // entries [ja, ja + m) of [j0, j1).  Returns ja through *ja.
+template <typename SM>
 __device__ __forceinline__ uint32_t dense_stage_owners(const uint64_t* __restrict__ act_prefix,
                                                        uint64_t j0, uint64_t j1, uint64_t pb,
-                                                       uint64_t* pref, uint64_t* s_ja,
+                                                       uint64_t pl, SM& sm, uint64_t* s_ja,
                                                        uint64_t* ja) {
   __syncthreads();
-  if (threadIdx.x < kWave) {  // last j with act_prefix[j] <= pb
-    const uint64_t j = wave_last_le(act_prefix, j0, j1, pb);
+  // last j with act_prefix[j] <= pb (wave 0), <= pl (wave 1)
+  if (threadIdx.x < 2 * kWave) {
+    const uint64_t j = wave_last_le(act_prefix, j0, j1, threadIdx.x < kWave ? pb : pl);
     if (threadIdx.x == 0) *s_ja = j;
+    if (threadIdx.x == kWave) sm.jl = j;
   }
   __syncthreads();
   *ja = *s_ja;
-  const uint64_t avail = j1 - *ja;
-  const uint32_t m = (uint32_t)(avail < kDenseRound ? avail : kDenseRound);
+  uint64_t* pref = sm.pref;
+  const uint64_t need = sm.jl - *ja + 1;
+  const uint32_t m = (uint32_t)(need < kDenseRound ? need : kDenseRound);
   for (uint32_t k = threadIdx.x; k <= m; k += kBlock) pref[k] = act_prefix[*ja + k];
   __syncthreads();
   return m;
@@ -70,7 +75,8 @@ __device__ __forceinline__ uint32_t dense_row_count(const DenseArgs& a, uint64_t
   const uint64_t p1 = j0 < j1 ? a.act_prefix[j1] : 0;
   for (uint64_t pb = p0; pb < p1; pb += kDenseRound) {
     uint64_t ja;
-    const uint32_t m = dense_stage_owners(a.act_prefix, j0, j1, pb, sm.pref, s_ja, &ja);
+    const uint64_t pl = (pb + kDenseRound < p1 ? pb + kDenseRound : p1) - 1;
+    const uint32_t m = dense_stage_owners(a.act_prefix, j0, j1, pb, pl, sm, s_ja, &ja);
 #pragma unroll
     for (int g = 0; g < kDenseG; ++g) {
       const uint64_t p = pb + (uint64_t)g * kBlock + threadIdx.x;
@@ -115,7 +121,8 @@ __device__ __forceinline__ void dense_row_numeric(const DenseArgs& a, uint64_t j
   double vsum = 0.0;
   for (uint64_t pb = p0; pb < p1; pb += kDenseRound) {
     uint64_t ja;
-    const uint32_t m = dense_stage_owners(a.act_prefix, j0, j1, pb, sm.pref, s_ja, &ja);
+    const uint64_t pl = (pb + kDenseRound < p1 ? pb + kDenseRound : p1) - 1;
+    const uint32_t m = dense_stage_owners(a.act_prefix, j0, j1, pb, pl, sm, s_ja, &ja);
     uint32_t own[kDenseG], rank[kDenseG];
     float val[kDenseG];
 #pragma unroll
@@ -133,8 +140,7 @@ __device__ __forceinline__ void dense_row_numeric(const DenseArgs& a, uint64_t j
       }
     }
     // owners of this round in order: the first is own of product pb, the
-    // last that of min(pb + round, p1) - 1
-    const uint64_t pl = (pb + kDenseRound < p1 ? pb + kDenseRound : p1) - 1;
+    // last that of pl = min(pb + round, p1) - 1
     const uint32_t klast = dense_owner(sm.pref, m, pl);
     for (uint32_t k = 0; k <= klast; ++k) {
 #pragma unroll

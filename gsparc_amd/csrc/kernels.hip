@@ -460,7 +460,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs a) {
   constexpr int R = 1 << RB;
   __shared__ uint64_t s_pref[kExpandTile + 1];
   __shared__ uint32_t sh[HIST ? kMaxFusedPasses * R : 1];
-  __shared__ uint64_t s_ja;
+  __shared__ uint64_t s_ja, s_jl;
   __shared__ double s_sum[kBlock / kWave];
   if (HIST)
     for (int i = threadIdx.x; i < a.passes * R; i += kBlock) sh[i] = 0;
@@ -473,14 +473,17 @@ __global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs a) {
     const uint64_t pb = a.P0 + tile * kExpandTile;
     const uint64_t pe = pb + kExpandTile < a.P1 ? pb + kExpandTile : a.P1;
     __syncthreads();
-    if (threadIdx.x < kWave) {  // last j in [A0, A1) with act_prefix[j] <= pb
-      const uint64_t j = wave_last_le(a.act_prefix, a.A0, a.A1, pb);
+    // owners of the first and the last product (last j in [A0, A1) with
+    // act_prefix[j] <= p; waves 0 and 1): only their prefixes are staged
+    if (threadIdx.x < 2 * kWave) {
+      const uint64_t j = wave_last_le(a.act_prefix, a.A0, a.A1, threadIdx.x < kWave ? pb : pe - 1);
       if (threadIdx.x == 0) s_ja = j;
+      if (threadIdx.x == kWave) s_jl = j;
     }
     __syncthreads();
     const uint64_t ja = s_ja;
-    const uint64_t avail = a.A1 - ja;
-    const uint32_t m = (uint32_t)(avail < kExpandTile + 1 ? avail : kExpandTile + 1);
+    const uint64_t need = s_jl - ja + 1;
+    const uint32_t m = (uint32_t)(need < kExpandTile + 1 ? need : kExpandTile + 1);
     for (uint32_t k = threadIdx.x; k < m; k += kBlock) s_pref[k] = a.act_prefix[ja + k];
     __syncthreads();
     const uint64_t obase = pb - a.P0;

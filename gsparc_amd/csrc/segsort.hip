@@ -336,7 +336,7 @@ __global__ __launch_bounds__(kBlock) void k_expand_items(ExpandArgs a,
   constexpr int R = 1 << RB;
   __shared__ uint64_t s_pref[kExpandSub + 1];
   __shared__ uint32_t sh[R];
-  __shared__ uint64_t s_ja;
+  __shared__ uint64_t s_ja, s_jl;
   __shared__ double s_sum[kBlock / kWave];
   double vsum = 0.0;
   uint64_t t0, t1, step;
@@ -361,15 +361,18 @@ __global__ __launch_bounds__(kBlock) void k_expand_items(ExpandArgs a,
       const uint64_t pb = a.P0 + start + sub;
       const uint64_t pe = pb + (count - sub < kExpandSub ? count - sub : kExpandSub);
       __syncthreads();
-      if (threadIdx.x < kWave) {  // last j with act_prefix[j] <= pb
-        const uint64_t j = wave_last_le(a.act_prefix, jlo, a.A1, pb);
+      // owners of the first and the last product (waves 0 and 1): only
+      // their prefixes are staged
+      if (threadIdx.x < 2 * kWave) {
+        const uint64_t j = wave_last_le(a.act_prefix, jlo, a.A1, threadIdx.x < kWave ? pb : pe - 1);
         if (threadIdx.x == 0) s_ja = j;
+        if (threadIdx.x == kWave) s_jl = j;
       }
       __syncthreads();
       const uint64_t ja = s_ja;
       jlo = ja;
-      const uint64_t avail = a.A1 - ja;
-      const uint32_t m = (uint32_t)(avail < kExpandSub + 1 ? avail : kExpandSub + 1);
+      const uint64_t need = s_jl - ja + 1;
+      const uint32_t m = (uint32_t)(need < kExpandSub + 1 ? need : kExpandSub + 1);
       for (uint32_t k = threadIdx.x; k < m; k += kBlock) s_pref[k] = a.act_prefix[ja + k];
       __syncthreads();
       uint32_t klo = 0, kc = ~0u;
