@@ -679,6 +679,9 @@ __global__ __launch_bounds__(kBlock) void k_reduce_emit_narrow(ReduceOp op, cons
   const uint32_t lane = lane_id();
   const uint64_t* __restrict__ ytab = op.ytab;
   const uint32_t rb = (uint32_t)(op.r0 + r0);
+  uint32_t* __restrict__ orow = op.out_row + obase;  // uniform bases
+  uint64_t* __restrict__ ocol = op.out_col + obase;
+  float* __restrict__ oval = op.out_val + obase;
   uint64_t col[kScanItems];
 #pragma unroll
   for (int q = 0; q < kScanItems; ++q) {
@@ -690,9 +693,9 @@ __global__ __launch_bounds__(kBlock) void k_reduce_emit_narrow(ReduceOp op, cons
   for (int q = 0; q < kScanItems; ++q) {
     const uint32_t i = lane + (uint32_t)q * kWave;
     if (i < T) {
-      __builtin_nontemporal_store(rb + w.m[i], &op.out_row[obase + i]);
-      __builtin_nontemporal_store(col[q], &op.out_col[obase + i]);
-      __builtin_nontemporal_store(w.v[i], &op.out_val[obase + i]);
+      __builtin_nontemporal_store(rb + w.m[i], &orow[i]);
+      __builtin_nontemporal_store(col[q], &ocol[i]);
+      __builtin_nontemporal_store(w.v[i], &oval[i]);
     }
   }
 }
@@ -725,9 +728,9 @@ __global__ __launch_bounds__(kBlock) void k_reduce_heads_narrow(NarrowRows nr, u
   const uint32_t cnt = (uint32_t)(n - b < kWaveTile ? n - b : kWaveTile);
   uint32_t col[kScanItems], mk[kScanItems];
   float unused[kScanItems];
-  wave_tile_load<false>(nr.cols, nullptr, n, b + (uint64_t)kScanItems * lane_id(), col, unused);
+  wave_tile_load<false>(nr.cols + b, nullptr, cnt, kScanItems * lane_id(), col, unused);
   bool rowsin;
-  const uint32_t hm = wave_tile_heads(nr, n, b, cnt, nr.tile_row[2 * wt], col,
+  const uint32_t hm = wave_tile_heads(nr, b, cnt, nr.tile_row[2 * wt], col,
                                       mark[threadIdx.x / kWave], mk, &rowsin);
   const uint32_t tot = __builtin_amdgcn_readlane(wave_incl_scan32((uint32_t)__popc(hm)),
                                                  kWave - 1);

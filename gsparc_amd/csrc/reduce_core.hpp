@@ -137,15 +137,17 @@ struct alignas(16) WaveSmem {
   uint32_t m[kWaveTile];  // row-start marks, then the outputs' rows (- r0)
 };
 
-// The lane's 8 products from p: streaming vector loads when all lie below
-// n (cols and vals 16-byte aligned, p a multiple of 8), else one by one.
+// The lane's 8 products q0 .. q0 + 7 of the tile (cnt products at cols,
+// vals; uniform bases, so the addresses are 32-bit offsets): streaming
+// vector loads when all are in the tile (cols and vals 16-byte aligned, q0
+// a multiple of 8), else one by one.
 template <bool VALS>
 __device__ __forceinline__ void wave_tile_load(const uint32_t* __restrict__ cols,
-                                               const float* __restrict__ vals, uint64_t n,
-                                               uint64_t p, uint32_t (&col)[kScanItems],
+                                               const float* __restrict__ vals, uint32_t cnt,
+                                               uint32_t q0, uint32_t (&col)[kScanItems],
                                                float (&val)[kScanItems]) {
-  if (p + kScanItems <= n) {
-    const u32x4* cp = reinterpret_cast<const u32x4*>(cols + p);
+  if (q0 + kScanItems <= cnt) {
+    const u32x4* cp = reinterpret_cast<const u32x4*>(cols + q0);
     const u32x4 c0 = __builtin_nontemporal_load(cp), c1 = __builtin_nontemporal_load(cp + 1);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -153,7 +155,7 @@ __device__ __forceinline__ void wave_tile_load(const uint32_t* __restrict__ cols
       col[4 + j] = c1[j];
     }
     if (VALS) {
-      const f32x4* vp = reinterpret_cast<const f32x4*>(vals + p);
+      const f32x4* vp = reinterpret_cast<const f32x4*>(vals + q0);
       const f32x4 v0 = __builtin_nontemporal_load(vp), v1 = __builtin_nontemporal_load(vp + 1);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
@@ -164,8 +166,8 @@ __device__ __forceinline__ void wave_tile_load(const uint32_t* __restrict__ cols
   } else {
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
-      col[j] = p + j < n ? cols[p + j] : 0;
-      if (VALS) val[j] = p + j < n ? vals[p + j] : 0.f;
+      col[j] = q0 + j < cnt ? cols[q0 + j] : 0;
+      if (VALS) val[j] = q0 + j < cnt ? vals[q0 + j] : 0.f;
     }
   }
 }
@@ -175,13 +177,13 @@ __device__ __forceinline__ void wave_tile_load(const uint32_t* __restrict__ cols
 // When rows start after the tile's first product (*rowsin), mk[j] = row - r0
 // of the last row starting at the lane's product j (0: none), and `mark`
 // (the wave's kWaveTile words of LDS) holds them for the whole tile.
-__device__ __forceinline__ uint32_t wave_tile_heads(const NarrowRows& nr, uint64_t n, uint64_t b,
+__device__ __forceinline__ uint32_t wave_tile_heads(const NarrowRows& nr, uint64_t b,
                                                     uint32_t cnt, uint64_t r0,
                                                     const uint32_t (&col)[kScanItems],
                                                     uint32_t* __restrict__ mark,
                                                     uint32_t (&mk)[kScanItems], bool* rowsin) {
   const uint32_t lane = lane_id();
-  const uint64_t p = b + (uint64_t)kScanItems * lane;
+  const uint32_t q0 = kScanItems * lane;
   const uint32_t pc0 = b ? nr.cols[b - 1] : 0;  // same address in every lane
   const bool at0 = nr.rstart[r0] == b;
   // lane l's candidate: row r0 + 1 + l (later rows start after product b)
@@ -192,7 +194,7 @@ __device__ __forceinline__ uint32_t wave_tile_heads(const NarrowRows& nr, uint64
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) mk[j] = 0;
   if (any) {
-    u32x4* mp = reinterpret_cast<u32x4*>(mark + kScanItems * lane);
+    u32x4* mp = reinterpret_cast<u32x4*>(mark + q0);
     mp[0] = u32x4{0u, 0u, 0u, 0u};
     mp[1] = u32x4{0u, 0u, 0u, 0u};
     wave_sync();
@@ -211,15 +213,32 @@ __device__ __forceinline__ uint32_t wave_tile_heads(const NarrowRows& nr, uint64
   }
   uint32_t prev = __shfl_up(col[kScanItems - 1], 1, kWave);
   if (lane == 0) prev = pc0;
+  // the first product of all (b = 0, lane 0, j = 0) and a row starting at
+  // the tile's first product count as row starts
+  const bool first = lane == 0 && (b == 0 || at0);
   uint32_t hm = 0;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
-    const uint64_t i = p + j;
     const uint32_t pk = j ? col[j - 1] : prev;
-    const bool rs = mk[j] != 0 || (j == 0 && lane == 0 && at0);
-    hm |= (uint32_t)(i < n && (i == 0 || rs || col[j] != pk)) << j;
+    const bool rs = mk[j] != 0 || (j == 0 && first);
+    hm |= (uint32_t)(q0 + j < cnt && (rs || col[j] != pk)) << j;
   }
   return hm;
+}
+
+// Compensated sums of the lane's products: res[j] = the running sum after
+// product j of the run it belongs to (runs restart at the heads); acc is
+// left holding the lane's last run.  FULL: all 8 products are in the tile.
+template <bool FULL>
+__device__ __forceinline__ void wave_lane_sums(uint32_t hm, uint32_t q0, uint32_t cnt,
+                                               const float (&val)[kScanItems], KahanF32& acc,
+                                               float (&res)[kScanItems]) {
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    if ((hm >> j) & 1u) acc = KahanF32();
+    if (FULL || q0 + j < cnt) acc.add(val[j]);
+    res[j] = acc.value();
+  }
 }
 
 // Reduces wave tile wt: leaves its outputs -- one per head in the tile, in
@@ -236,11 +255,12 @@ __device__ __forceinline__ uint32_t wave_tile_reduce(const NarrowRows& nr,
   const uint64_t r0 = nr.tile_row[2 * wt];
   *r0p = r0;
   *rpp = b ? nr.tile_row[2 * wt + 1] : 0;
+  const uint32_t q0 = kScanItems * lane;
   uint32_t col[kScanItems], mk[kScanItems];
   float val[kScanItems];
-  wave_tile_load<true>(nr.cols, vals, n, b + (uint64_t)kScanItems * lane, col, val);
+  wave_tile_load<true>(nr.cols + b, vals + b, cnt, q0, col, val);
   bool rowsin;
-  const uint32_t hm = wave_tile_heads(nr, n, b, cnt, r0, col, w.m, mk, &rowsin);
+  const uint32_t hm = wave_tile_heads(nr, b, cnt, r0, col, w.m, mk, &rowsin);
   // row of each product: prefix maximum of the marks (lane, then wave)
   uint32_t ro[kScanItems];
   if (rowsin) {
@@ -270,7 +290,7 @@ __device__ __forceinline__ uint32_t wave_tile_reduce(const NarrowRows& nr,
   const uint32_t o = hinc - h;
   const uint32_t T = __builtin_amdgcn_readlane(hinc, kWave - 1);
   // first head after the lane's products (kWaveTile: none), suffix minimum
-  uint32_t fs = hm ? kScanItems * lane + (uint32_t)__builtin_ctz(hm) : (uint32_t)kWaveTile;
+  uint32_t fs = hm ? q0 + (uint32_t)__builtin_ctz(hm) : (uint32_t)kWaveTile;
 #pragma unroll
   for (int d = 1; d < kWave; d <<= 1) {
     const uint32_t x = __shfl_down(fs, d, kWave);
@@ -297,25 +317,19 @@ __device__ __forceinline__ uint32_t wave_tile_reduce(const NarrowRows& nr,
   }
   // values to LDS for the runs that continue into later lanes
   {
-    f32x4* vp = reinterpret_cast<f32x4*>(w.v + kScanItems * lane);
+    f32x4* vp = reinterpret_cast<f32x4*>(w.v + q0);
     vp[0] = f32x4{val[0], val[1], val[2], val[3]};
     vp[1] = f32x4{val[4], val[5], val[6], val[7]};
   }
   wave_sync();
-  // sums: res[j] = the running sum after product j of the run it belongs to
-  const uint64_t p = b + (uint64_t)kScanItems * lane;
   KahanF32 acc;
   float res[kScanItems];
-#pragma unroll
-  for (int j = 0; j < kScanItems; ++j) {
-    if ((hm >> j) & 1u) acc = KahanF32();
-    if (p + j < n) acc.add(val[j]);
-    res[j] = acc.value();
-  }
+  if (cnt == kWaveTile) wave_lane_sums<true>(hm, q0, cnt, val, acc, res);
+  else wave_lane_sums<false>(hm, q0, cnt, val, acc, res);
   float last = 0.f;
   if (hm) {  // the lane's last run: later lanes' products, then past the tile
     const uint32_t stop = nxt < cnt ? nxt : cnt;
-    for (uint32_t q = kScanItems * lane + kScanItems; q < stop; ++q) acc.add(w.v[q]);
+    for (uint32_t q = q0 + kScanItems; q < stop; ++q) acc.add(w.v[q]);
     if (nxt >= cnt) {
       for (uint64_t g = b + cnt; g < tail_end; g += 8) {
         float t[8];

@@ -131,7 +131,10 @@ __device__ __forceinline__ void tile_digit_starts(TileSmem<RB, IPT, NT, ST>& sm)
 
 // Given sm.gbase and sm.dstart (tile totals per digit on entry), scans the
 // totals into tile-local run starts, stages keys in digit order, writes them
-// in coalesced runs, then does the same for the values.
+// in coalesced runs, then does the same for the values.  The run starts are
+// folded into the per-wave prefixes (cnt[w][d] += dstart[d]) and the global
+// bases (gbase[d] -= dstart[d], modulo 2^32) first, so a key's staging slot
+// and destination take one LDS read each.
 template <int RB, int IPT, int NT, typename ST, typename KT, typename KO>
 __device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT, ST>& sm, uint64_t n,
                                                    int shift, uint64_t tile_begin,
@@ -148,11 +151,18 @@ __device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT, ST>& sm
   KT* kbuf = reinterpret_cast<KT*>(sm.buf);
   tile_digit_starts<RB, IPT, NT>(sm);
   __syncthreads();
+  for (int d = tid; d < R; d += NT) {
+    const uint32_t ds = sm.dstart[d];
+#pragma unroll
+    for (int w = 0; w < NT / kWave; ++w) sm.cnt[w][d] = (uint16_t)(sm.cnt[w][d] + ds);
+    sm.gbase[d] -= ds;
+  }
+  __syncthreads();
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {
     const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
     const uint32_t d = (uint32_t)(k[i] >> shift) & (R - 1);
-    pos[i] += (uint32_t)sm.dstart[d] + (uint32_t)sm.cnt[wave][d];
+    pos[i] += (uint32_t)sm.cnt[wave][d];
     if (idx < n) kbuf[pos[i]] = k[i];
   }
   __syncthreads();
@@ -164,7 +174,7 @@ __device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT, ST>& sm
     if (j < cnt) {
       const KT key = kbuf[j];
       const uint32_t d = (uint32_t)(key >> shift) & (R - 1);
-      dst[i] = sm.gbase[d] + (j - sm.dstart[d]);
+      dst[i] = sm.gbase[d] + j;
       kout[dst[i]] = (KO)key;
     }
   }
