@@ -371,7 +371,7 @@ sptc_status BatchEngine::build_chunks(hipStream_t s) {
       SPTC_HIP(hipMemcpyAsync(c.row_start.get(), c.rseg.row_start(), (c.lrows + 1) * 8,
                               hipMemcpyDeviceToDevice, s));
       c.tile_row.reset(2 * (narrow_tile_count(c.n) + 1));
-      launch_tile_rows(c.row_start.get(), c.lrows, c.tile_row.get(), s);
+      launch_tile_rows(c.row_start.get(), c.lrows, c.n, c.tile_row.get(), s);
       SPTC_HIP(hipStreamSynchronize(s));  // rr and the scratch are released below
       c.rseg.scratch.release();
     }

@@ -555,7 +555,7 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       bool alt = false;
       const NarrowRows nr{fcol, rowsort ? rseg.row_start() : nullptr, r1c - r0c, trow.get()};
       if (rowsort) {
-        launch_tile_rows(rseg.row_start(), r1c - r0c, trow.get(), s_);
+        launch_tile_rows(rseg.row_start(), r1c - r0c, n, trow.get(), s_);
         sort_row_segments(ws_, rseg, n, fcol, vA.get(), ws_.alt_keys.get(), ws_.alt_vals.get(),
                           rseg.row_start(), s_, &kt_, SPTC_K_SORT_PROD, /*counted=*/true);
       } else {

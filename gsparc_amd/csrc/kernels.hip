@@ -737,28 +737,35 @@ __global__ __launch_bounds__(kBlock) void k_reduce_heads_narrow(NarrowRows nr, u
   if (lane_id() == 0) tile_sums[wt] = mk2(tot, 0);
 }
 
-// One thread per row r with products [s, e), no search: wave tiles whose
-// first product lies in [s, e) get [2t] = r, those whose preceding product
-// lies in [s, e) get [2t + 1] = r (tile_row holds 2 * (tiles + 1) entries).
-__global__ __launch_bounds__(kBlock) void k_tile_rows(const uint64_t* __restrict__ rstart,
-                                                      uint64_t nrows,
-                                                      uint32_t* __restrict__ tile_row) {
-  uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-  for (; r < nrows; r += stride) {
-    const uint64_t s = rstart[r], e = rstart[r + 1];
-    if (e <= s) continue;
-    for (uint64_t t = ceil_div(s, kWaveTile); t * kWaveTile < e; ++t) tile_row[2 * t] = (uint32_t)r;
-    for (uint64_t t = ceil_div(s + 1, kWaveTile); t * kWaveTile < e + 1; ++t)
-      tile_row[2 * t + 1] = (uint32_t)r;
+// One thread per wave tile t: [2t] = row of its first product b = 512 t,
+// [2t + 1] = row of product b - 1 (t > 0); the row of product p is the last
+// row r with rstart[r] <= p (empty rows share their start with the next row).
+__device__ __forceinline__ uint32_t row_of(const uint64_t* __restrict__ rstart, uint64_t nrows,
+                                           uint64_t p) {
+  uint64_t lo = 0, hi = nrows;  // first r with rstart[r] > p, minus one
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (rstart[mid] <= p) lo = mid + 1; else hi = mid;
   }
+  return (uint32_t)(lo - 1);
 }
 
-void launch_tile_rows(const uint64_t* rstart, uint64_t nrows, uint32_t* tile_row,
+__global__ __launch_bounds__(kBlock) void k_tile_rows(const uint64_t* __restrict__ rstart,
+                                                      uint64_t nrows, uint64_t ntiles,
+                                                      uint32_t* __restrict__ tile_row) {
+  const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= ntiles) return;
+  const uint64_t b = t * kWaveTile;
+  tile_row[2 * t] = row_of(rstart, nrows, b);
+  if (t) tile_row[2 * t + 1] = row_of(rstart, nrows, b - 1);
+}
+
+void launch_tile_rows(const uint64_t* rstart, uint64_t nrows, uint64_t n, uint32_t* tile_row,
                       hipStream_t s) {
-  if (!nrows) return;
-  const uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(nrows, kBlock), 4096);
-  hipLaunchKernelGGL(k_tile_rows, dim3(grid), dim3(kBlock), 0, s, rstart, nrows, tile_row);
+  if (!nrows || !n) return;
+  const uint64_t nt = narrow_tile_count(n);
+  hipLaunchKernelGGL(k_tile_rows, dim3((uint32_t)ceil_div(nt, kBlock)), dim3(kBlock), 0, s,
+                     rstart, nrows, nt, tile_row);
 }
 
 void run_reduce_count_narrow(const NarrowRows& nr, uint64_t n, U2* tmp, hipStream_t s) {

@@ -124,7 +124,7 @@ void run_reduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32
                      uint64_t fmask, const uint64_t* ytab, const U2* tmp, hipStream_t s);
 // NarrowRows::tile_row for a chunk of rstart[nrows] products: 2 entries per
 // wave tile, 2 * (narrow_tile_count(n) + 1) in all.
-void launch_tile_rows(const uint64_t* rstart, uint64_t nrows, uint32_t* tile_row,
+void launch_tile_rows(const uint64_t* rstart, uint64_t nrows, uint64_t n, uint32_t* tile_row,
                       hipStream_t s);
 // Narrow-form count and emit (outputs' rows are r0 + chunk-local row): tmp
 // holds narrow_tmp_count(n) entries, the total at tmp[narrow_tile_count(n)];

@@ -21,13 +21,18 @@ __global__ __launch_bounds__(kSegNT) void k_seg_upsweep(const KT* __restrict__ k
   for (int i = threadIdx.x; i < R; i += kSegNT) sh[i] = 0;
   __syncthreads();
   const SegTile t = tiles[blockIdx.x];
-  const uint64_t end = t.start + t.count;
+  const KT* __restrict__ kb = kin + t.start;  // uniform base, 32-bit offsets
+  // all the thread's loads first (IPT in flight), then one LDS atomic per key
+  KT k[IPT];
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {
-    const uint64_t idx = t.start + (uint64_t)i * kSegNT + threadIdx.x;
-    const bool valid = idx < end;
-    const uint64_t k = valid ? (uint64_t)kin[idx] : 0;
-    hist_add_wave<RB>(sh, k >> shift, valid, 1);
+    const uint32_t j = (uint32_t)i * kSegNT + threadIdx.x;
+    k[i] = j < t.count ? kb[j] : 0;
+  }
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    const uint32_t j = (uint32_t)i * kSegNT + threadIdx.x;
+    if (j < t.count) atomicAdd(&sh[(uint32_t)(k[i] >> shift) & (R - 1)], 1u);
   }
   __syncthreads();
   for (int d = threadIdx.x; d < R; d += kSegNT) cnt[(uint64_t)blockIdx.x * R + d] = sh[d];
@@ -38,13 +43,14 @@ template <typename KI, typename KO, int RB, int IPT>
 __global__ __launch_bounds__(kSegNT) void k_seg_downsweep(
     const KI* __restrict__ kin, const float* __restrict__ vin, KO* __restrict__ kout,
     float* __restrict__ vout, int shift, const SegTile* __restrict__ tiles,
-    const uint32_t* __restrict__ offs, uint32_t remap) {
+    const uint32_t* __restrict__ offs, const uint32_t* __restrict__ dstarts, uint32_t remap) {
   constexpr int R = 1 << RB;
   __shared__ TileSmem<RB, IPT, kSegNT, KI> sm;
   const uint32_t tid = threadIdx.x;
   const SegTile t = tiles[remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x];
   for (int i = tid; i < (kSegNT / kWave) * R; i += kSegNT) (&sm.cnt[0][0])[i] = 0;
-  for (int d = tid; d < R; d += kSegNT) sm.gbase[d] = offs[t.obase + (uint64_t)d * t.ostride];
+  for (int d = tid; d < R; d += kSegNT)
+    sm.gbase[d] = offs[t.tile * R + d] + dstarts[(uint64_t)t.first * R + d];
   __syncthreads();
   const uint64_t end = t.start + t.count;
   const uint64_t wbase = t.start + (uint64_t)(tid / kWave) * kWave * IPT;
@@ -59,42 +65,39 @@ __global__ __launch_bounds__(kSegNT) void k_seg_downsweep(
   tile_stage_scatter<RB, IPT, kSegNT>(sm, end, shift, t.start, wbase, k, v, pos, kout, vout);
 }
 
-// Per segment (one workgroup, thread d = digit d): absolute destination of
-// each tile's run of each digit.
+// Per segment (one workgroup, thread d = digit d): off[t][d] = digit d's
+// keys in the segment's tiles before t (one coalesced read and write per
+// tile row), then cnt[first][d] = start + keys of smaller digits in the
+// segment (the first tile's counts are consumed by then).
 template <int RB>
 __global__ __launch_bounds__(1 << RB) void k_seg_scan(const SegDesc* __restrict__ segs,
-                                                     const uint32_t* __restrict__ cnt,
+                                                     uint32_t* __restrict__ cnt,
                                                      uint32_t* __restrict__ off) {
   constexpr int R = 1 << RB;
   __shared__ uint32_t scratch[R / kWave];
   const SegDesc g = segs[blockIdx.x];
   const uint32_t d = threadIdx.x;
-  const uint64_t t0 = g.first_tile, t1 = t0 + g.ntiles;
-  uint32_t tot = 0;
-  for (uint64_t t = t0; t < t1; ++t) tot += cnt[t * R + d];
-  uint32_t all;
-  uint32_t base = (uint32_t)g.start + block_scan_u32<R>(tot, scratch, &all);
-  // destinations are digit-major in the segment: o[d * nt + (t - t0)].  They
-  // go out through LDS in groups of up to 8 tiles, so the stores are
-  // contiguous (whole segment for nt <= 8, 32-byte pieces otherwise) rather
-  // than one 4-byte store every nt words.
-  __shared__ uint32_t buf[8 * R];
-  uint32_t* o = off + R * t0;
-  const uint64_t nt = g.ntiles;
-  for (uint64_t tg = 0; tg < nt; tg += 8) {
-    const uint32_t m = (uint32_t)(nt - tg < 8 ? nt - tg : 8);
-    for (uint32_t j = 0; j < m; ++j) {
-      buf[j * R + d] = base;
-      base += cnt[(t0 + tg + j) * R + d];
+  const uint64_t t0 = g.first_tile, nt = g.ntiles;
+  uint32_t run = 0;
+  uint64_t t = 0;
+  for (; t + 8 <= nt; t += 8) {  // eight loads in flight
+    uint32_t c[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) c[e] = cnt[(t0 + t + e) * R + d];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      off[(t0 + t + e) * R + d] = run;
+      run += c[e];
     }
-    __syncthreads();
-    for (uint32_t k = 0; k < m; ++k) {
-      const uint32_t q = d + k * R;
-      const uint32_t dd = q / m, j = q - dd * m;
-      o[(uint64_t)dd * nt + tg + j] = buf[j * R + dd];
-    }
-    __syncthreads();
   }
+  for (; t < nt; ++t) {
+    const uint32_t c = cnt[(t0 + t) * R + d];
+    off[(t0 + t) * R + d] = run;
+    run += c;
+  }
+  uint32_t all;
+  const uint32_t base = (uint32_t)g.start + block_scan_u32<R>(run, scratch, &all);
+  cnt[t0 * R + d] = base;
 }
 
 // ---- row segments: small-row groups (LDS) and big rows (multi-tile) --------
@@ -171,8 +174,8 @@ __global__ __launch_bounds__(kBlock) void k_rowseg_emit(
         SegTile d;
         d.start = first + t * tile;
         d.count = (uint32_t)(n - t * tile < tile ? n - t * tile : tile);
-        d.obase = (1ull << rb) * o + t;
-        d.ostride = (uint32_t)nt;
+        d.tile = o + t;
+        d.first = (uint32_t)o;
         big[o + t] = d;
       }
     } else {
@@ -441,8 +444,8 @@ void SegSortPlan::build(const std::vector<uint64_t>& seg_off, hipStream_t s) {
       SegTile d;
       d.start = seg_off[g] + t * kSegTile;
       d.count = (uint32_t)std::min<uint64_t>(kSegTile, len - t * kSegTile);
-      d.obase = (1ull << kSegRB) * t0 + t;
-      d.ostride = (uint32_t)nt;
+      d.tile = t0 + t;
+      d.first = (uint32_t)t0;
       tv.push_back(d);
     }
   }
@@ -500,7 +503,7 @@ bool segmented_sort(SortWorkspace& ws, const SegTile* tiles, uint64_t ntiles,
     tok = kt ? kt->begin(s) : -1;
     hipLaunchKernelGGL((k_seg_downsweep<uint64_t, uint64_t, kSegRB, kSegIPT>),
                        dim3((uint32_t)ntiles), dim3(kSegNT), 0, s, ka, va, kb, vb, shift, tiles,
-                       ws.rts_off.get(), xcd_remap());
+                       ws.rts_off.get(), ws.rts_cnt.get(), xcd_remap());
     if (kt) kt->end(tok, cls, 24.0 * (double)n, s);
     std::swap(ka, kb);
     std::swap(va, vb);
@@ -632,7 +635,8 @@ static void narrow_sort(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, u
     const bool last = p + 1 == passes;
     hipLaunchKernelGGL((k_seg_downsweep<uint32_t, uint32_t, RB, IPT>), dim3((uint32_t)ntiles),
                        dim3(kSegNT), 0, s, col[cur], val[cur], last ? fcol : col[1 - cur],
-                       last ? v : val[1 - cur], shift, tiles, ws.rts_off.get(), xcd_remap());
+                       last ? v : val[1 - cur], shift, tiles, ws.rts_off.get(),
+                       ws.rts_cnt.get(), xcd_remap());
     if (kt) kt->end(tok, cls, 16.0 * (double)nb, s);
     cur = 1 - cur;
   }

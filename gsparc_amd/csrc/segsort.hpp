@@ -21,19 +21,19 @@ constexpr uint64_t kSegTile = (uint64_t)kSegIPT * kSegNT;
 constexpr int kNarrowIPT = 16;
 
 // One tile of a segmented sort (never straddles a segment).  Its digit
-// counts are row t of a tile-major count matrix (coalesced writes); its
-// destination offsets sit digit-major within the segment, at
-// obase + d * ostride (spread over memory channels when many tiles start at
-// once).
+// counts are row `tile` of a tile-major count matrix; the scan leaves in
+// the same row of the offset matrix each digit's keys in the segment's
+// earlier tiles, and in row `first` of the count matrix the segment's digit
+// starts: a key's destination is their sum plus its rank (both rows read
+// coalesced).
 struct SegTile {
-  uint64_t start;    // first key (global index)
-  uint64_t obase;    // offset-matrix index of digit 0 (256 * first tile + index in segment)
-  uint32_t count;    // keys in the tile
-  uint32_t ostride;  // tiles in the segment
+  uint64_t start;  // first key (global index)
+  uint64_t tile;   // global tile index
+  uint32_t count;  // keys in the tile
+  uint32_t first;  // global index of the segment's first tile
 };
 // One segment: its tiles are first_tile .. first_tile + ntiles - 1; one
-// workgroup per segment turns their counts into absolute destinations
-// (start + digit start + earlier tiles of the digit).
+// workgroup per segment scans their counts (see SegTile).
 struct SegDesc {
   uint64_t start;
   uint32_t first_tile;
