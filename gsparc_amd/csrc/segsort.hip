@@ -52,17 +52,22 @@ __global__ __launch_bounds__(kSegNT) void k_seg_downsweep(
   for (int d = tid; d < R; d += kSegNT)
     sm.gbase[d] = offs[t.tile * R + d] + dstarts[(uint64_t)t.first * R + d];
   __syncthreads();
-  const uint64_t end = t.start + t.count;
-  const uint64_t wbase = t.start + (uint64_t)(tid / kWave) * kWave * IPT;
+  const bool full = t.count == kSegNT * IPT;
   KI k[IPT];
   float v[IPT];
   uint32_t pos[IPT];
-  tile_load_rank<RB, IPT, kSegNT>(sm, kin, vin, end, shift, wbase, k, v, pos);
+  if (full)
+    tile_load_rank<true, RB, IPT, kSegNT>(sm, kin + t.start, vin + t.start, t.count, shift, k, v,
+                                          pos);
+  else
+    tile_load_rank<false, RB, IPT, kSegNT>(sm, kin + t.start, vin + t.start, t.count, shift, k,
+                                           v, pos);
   __syncthreads();
   for (int d = tid; d < R; d += kSegNT)
     sm.dstart[d] = (uint16_t)wave_prefixes<RB, IPT, kSegNT>(sm, d);
   __syncthreads();
-  tile_stage_scatter<RB, IPT, kSegNT>(sm, end, shift, t.start, wbase, k, v, pos, kout, vout);
+  if (full) tile_stage_scatter<true, RB, IPT, kSegNT>(sm, t.count, shift, k, v, pos, kout, vout);
+  else tile_stage_scatter<false, RB, IPT, kSegNT>(sm, t.count, shift, k, v, pos, kout, vout);
 }
 
 // Per segment (one workgroup, thread d = digit d): off[t][d] = digit d's
@@ -281,7 +286,7 @@ __global__ __launch_bounds__(kSegNT) void k_local_sort(uint32_t* cols, float* va
     const int shift = p * kSegRB;
     for (int i = tid; i < (kSegNT / kWave) * R; i += kSegNT) (&sm.cnt[0][0])[i] = 0;
     __syncthreads();
-    tile_rank<kSegRB, kSegIPT, kSegNT>(sm, n, shift, wbase, k, pos);
+    tile_rank<false, kSegRB, kSegIPT, kSegNT>(sm, (uint32_t)n, shift, k, pos);
     __syncthreads();
     for (int d = tid; d < R; d += kSegNT)
       sm.dstart[d] = (uint16_t)wave_prefixes<kSegRB, kSegIPT, kSegNT>(sm, d);

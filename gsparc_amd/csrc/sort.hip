@@ -117,11 +117,12 @@ __global__ __launch_bounds__(NT) void k_onesweep(
   __syncthreads();
   const uint64_t tile = sm.tile;
   const uint64_t tile_begin = tile * TILE;
-  const uint64_t wbase = tile_begin + (uint64_t)(tid / kWave) * kWave * IPT;
+  const uint32_t cnt = (uint32_t)(n - tile_begin < (uint64_t)TILE ? n - tile_begin : TILE);
   uint64_t k[IPT];
   float v[IPT];
   uint32_t pos[IPT];
-  tile_load_rank<RB, IPT, NT>(sm, kin, vin, n, shift, wbase, k, v, pos);
+  tile_load_rank<false, RB, IPT, NT>(sm, kin + tile_begin, vin + tile_begin, cnt, shift, k, v,
+                                     pos);
   __syncthreads();
   for (int d = tid; d < R; d += NT) {
     const uint64_t total = wave_prefixes<RB, IPT, NT>(sm, d);
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(NT) void k_onesweep(
     sm.dstart[d] = (uint16_t)total;
   }
   __syncthreads();
-  tile_stage_scatter<RB, IPT, NT>(sm, n, shift, tile_begin, wbase, k, v, pos, kout, vout);
+  tile_stage_scatter<false, RB, IPT, NT>(sm, cnt, shift, k, v, pos, kout, vout);
 }
 
 template <int RB, int IPT, int NT>
@@ -193,15 +194,16 @@ __global__ __launch_bounds__(NT) void k_rts_downsweep(
   for (int d = tid; d < R; d += NT) sm.gbase[d] = offs[(uint64_t)d * ntiles + tile];
   __syncthreads();
   const uint64_t tile_begin = tile * TILE;
-  const uint64_t wbase = tile_begin + (uint64_t)(tid / kWave) * kWave * IPT;
+  const uint32_t cnt = (uint32_t)(n - tile_begin < (uint64_t)TILE ? n - tile_begin : TILE);
   uint64_t k[IPT];
   float v[IPT];
   uint32_t pos[IPT];
-  tile_load_rank<RB, IPT, NT>(sm, kin, vin, n, shift, wbase, k, v, pos);
+  tile_load_rank<false, RB, IPT, NT>(sm, kin + tile_begin, vin + tile_begin, cnt, shift, k, v,
+                                     pos);
   __syncthreads();
   for (int d = tid; d < R; d += NT) sm.dstart[d] = (uint16_t)wave_prefixes<RB, IPT, NT>(sm, d);
   __syncthreads();
-  tile_stage_scatter<RB, IPT, NT>(sm, n, shift, tile_begin, wbase, k, v, pos, kout, vout);
+  tile_stage_scatter<false, RB, IPT, NT>(sm, cnt, shift, k, v, pos, kout, vout);
 }
 
 // Exclusive scan of the uint32 count matrix into uint32 offsets.

@@ -60,24 +60,29 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
   return valid & ~((uint64_t)nhi << 32 | nlo);
 }
 
-// Ranks the keys a thread holds (wave w owns keys [w*64*IPT, (w+1)*64*IPT)
-// of the tile, lane-striped: key i of a lane is wbase + i*64 + lane; valid
-// when below n) within its wave's run of equal digits.  sm.cnt must be zero;
-// afterwards sm.cnt[w][d] = number of keys of digit d in wave w.  Keys are
-// 64-bit, or 32-bit column keys of the row-segmented product sort.
-template <int RB, int IPT, int NT, typename ST, typename KT>
-__device__ __forceinline__ void tile_rank(TileSmem<RB, IPT, NT, ST>& sm, uint64_t n, int shift,
-                                          uint64_t wbase, const KT (&k)[IPT],
-                                          uint32_t (&pos)[IPT]) {
+// Tile-local index of the thread's key i: wave w owns keys
+// [w*64*IPT, (w+1)*64*IPT) of the tile, lane-striped.
+template <int IPT>
+__device__ __forceinline__ uint32_t tile_local(int i) {
+  return (threadIdx.x / kWave) * (uint32_t)(kWave * IPT) + (uint32_t)i * kWave + lane_id();
+}
+
+// Ranks the keys a thread holds (tile_local layout; valid when below cnt,
+// FULL: cnt is the whole tile) within its wave's run of equal digits.
+// sm.cnt must be zero; afterwards sm.cnt[w][d] = number of keys of digit d
+// in wave w.  Keys are 64-bit, or 32-bit column keys of the row-segmented
+// product sort.
+template <bool FULL, int RB, int IPT, int NT, typename ST, typename KT>
+__device__ __forceinline__ void tile_rank(TileSmem<RB, IPT, NT, ST>& sm, uint32_t cnt, int shift,
+                                          const KT (&k)[IPT], uint32_t (&pos)[IPT]) {
   constexpr int R = 1 << RB;
   const uint32_t lane = lane_id();
   const uint32_t wave = threadIdx.x / kWave;
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {
-    const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
-    const bool valid = idx < n;
+    const bool valid = FULL || tile_local<IPT>(i) < cnt;
     const uint32_t d = (uint32_t)(k[i] >> shift) & (R - 1);
-    const uint64_t peers = match_digit<RB>(d, ballot(valid));
+    const uint64_t peers = match_digit<RB>(d, FULL ? ~0ull : ballot(valid));
     uint32_t before = 0;
     if (valid) before = sm.cnt[wave][d];
     __builtin_amdgcn_wave_barrier();
@@ -88,22 +93,22 @@ __device__ __forceinline__ void tile_rank(TileSmem<RB, IPT, NT, ST>& sm, uint64_
   }
 }
 
-// Loads the tile from global memory (same layout as tile_rank) and ranks it.
-template <int RB, int IPT, int NT, typename ST, typename KT>
+// Loads the tile (cnt keys at kb, values at vb: the tile's own bases, so
+// the addresses are 32-bit offsets) and ranks it.
+template <bool FULL, int RB, int IPT, int NT, typename ST, typename KT>
 __device__ __forceinline__ void tile_load_rank(TileSmem<RB, IPT, NT, ST>& sm,
-                                               const KT* __restrict__ kin,
-                                               const float* __restrict__ vin, uint64_t n,
-                                               int shift, uint64_t wbase, KT (&k)[IPT],
-                                               float (&v)[IPT], uint32_t (&pos)[IPT]) {
-  const uint32_t lane = lane_id();
+                                               const KT* __restrict__ kb,
+                                               const float* __restrict__ vb, uint32_t cnt,
+                                               int shift, KT (&k)[IPT], float (&v)[IPT],
+                                               uint32_t (&pos)[IPT]) {
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {
-    const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
-    const bool valid = idx < n;
-    k[i] = valid ? kin[idx] : 0;
-    v[i] = valid ? vin[idx] : 0.f;
+    const uint32_t li = tile_local<IPT>(i);
+    const bool valid = FULL || li < cnt;
+    k[i] = valid ? kb[li] : 0;
+    v[i] = valid ? vb[li] : 0.f;
   }
-  tile_rank<RB, IPT, NT>(sm, n, shift, wbase, k, pos);
+  tile_rank<FULL, RB, IPT, NT>(sm, cnt, shift, k, pos);
 }
 
 // sm.dstart holds each digit's tile total: turn it into the tile-local start
@@ -135,17 +140,14 @@ __device__ __forceinline__ void tile_digit_starts(TileSmem<RB, IPT, NT, ST>& sm)
 // folded into the per-wave prefixes (cnt[w][d] += dstart[d]) and the global
 // bases (gbase[d] -= dstart[d], modulo 2^32) first, so a key's staging slot
 // and destination take one LDS read each.
-template <int RB, int IPT, int NT, typename ST, typename KT, typename KO>
-__device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT, ST>& sm, uint64_t n,
-                                                   int shift, uint64_t tile_begin,
-                                                   uint64_t wbase, const KT (&k)[IPT],
+template <bool FULL, int RB, int IPT, int NT, typename ST, typename KT, typename KO>
+__device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT, ST>& sm, uint32_t cnt,
+                                                   int shift, const KT (&k)[IPT],
                                                    const float (&v)[IPT], uint32_t (&pos)[IPT],
                                                    KO* __restrict__ kout,
                                                    float* __restrict__ vout) {
   constexpr int R = 1 << RB;
-  constexpr int TILE = NT * IPT;
   const uint32_t tid = threadIdx.x;
-  const uint32_t lane = lane_id();
   const uint32_t wave = tid / kWave;
   static_assert(sizeof(KT) <= sizeof(ST), "staging element narrower than the key");
   KT* kbuf = reinterpret_cast<KT*>(sm.buf);
@@ -160,18 +162,16 @@ __device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT, ST>& sm
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {
-    const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
     const uint32_t d = (uint32_t)(k[i] >> shift) & (R - 1);
     pos[i] += (uint32_t)sm.cnt[wave][d];
-    if (idx < n) kbuf[pos[i]] = k[i];
+    if (FULL || tile_local<IPT>(i) < cnt) kbuf[pos[i]] = k[i];
   }
   __syncthreads();
-  const uint32_t cnt = (uint32_t)((n - tile_begin) < (uint64_t)TILE ? (n - tile_begin) : TILE);
   uint32_t dst[IPT];
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {
     const uint32_t j = tid + (uint32_t)i * NT;
-    if (j < cnt) {
+    if (FULL || j < cnt) {
       const KT key = kbuf[j];
       const uint32_t d = (uint32_t)(key >> shift) & (R - 1);
       dst[i] = sm.gbase[d] + j;
@@ -181,15 +181,13 @@ __device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT, ST>& sm
   __syncthreads();
   float* vbuf = reinterpret_cast<float*>(sm.buf);
 #pragma unroll
-  for (int i = 0; i < IPT; ++i) {
-    const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
-    if (idx < n) vbuf[pos[i]] = v[i];
-  }
+  for (int i = 0; i < IPT; ++i)
+    if (FULL || tile_local<IPT>(i) < cnt) vbuf[pos[i]] = v[i];
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {
     const uint32_t j = tid + (uint32_t)i * NT;
-    if (j < cnt) vout[dst[i]] = vbuf[j];
+    if (FULL || j < cnt) vout[dst[i]] = vbuf[j];
   }
 }
 
