@@ -341,9 +341,9 @@ __global__ __launch_bounds__(kBlock) void k_breduce_emit_narrow(BReduceOp op,
                                                                 uint64_t ntiles) {
   __shared__ WaveSmem sm[kBlock / kWave];
   if (*op.b.bad) return;
-  const uint64_t wt = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  const uint64_t wt = (uint64_t)blockIdx.x * (kBlock / kWave) + wave_id();
   if (wt >= ntiles) return;
-  WaveSmem& w = sm[threadIdx.x / kWave];
+  WaveSmem& w = sm[wave_id()];
   uint64_t r0, rp;
   const uint32_t T = wave_tile_reduce(op.nr, op.vals, op.n, wt, w, &r0, &rp);
   const uint64_t obase = tile_offs[wt].a;

@@ -64,6 +64,11 @@ __device__ __forceinline__ void xcd_tiles(uint64_t ntiles, bool remap, uint64_t*
   *step = (G + kXcds - 1 - x) / kXcds;  // workgroups on XCD x
 }
 
+// The wave's index in its workgroup, in a scalar register (the compiler
+// does not know threadIdx.x / 64 is the same across a wave).
+__device__ __forceinline__ uint32_t wave_id() {
+  return __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+}
 __device__ __forceinline__ uint32_t lane_id() {
   return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }

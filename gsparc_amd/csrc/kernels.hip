@@ -670,9 +670,9 @@ void run_reduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32
 __global__ __launch_bounds__(kBlock) void k_reduce_emit_narrow(ReduceOp op, const U2* tile_offs,
                                                                uint64_t ntiles) {
   __shared__ WaveSmem sm[kBlock / kWave];
-  const uint64_t wt = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  const uint64_t wt = (uint64_t)blockIdx.x * (kBlock / kWave) + wave_id();
   if (wt >= ntiles) return;
-  WaveSmem& w = sm[threadIdx.x / kWave];
+  WaveSmem& w = sm[wave_id()];
   uint64_t r0, rp;
   const uint32_t T = wave_tile_reduce(op.nr, op.vals, op.n, wt, w, &r0, &rp);
   const uint64_t obase = tile_offs[wt].a;
@@ -722,7 +722,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_heads_narrow(NarrowRows nr, u
                                                                 uint64_t ntiles,
                                                                 U2* __restrict__ tile_sums) {
   __shared__ __align__(16) uint32_t mark[kBlock / kWave][kWaveTile];
-  const uint64_t wt = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  const uint64_t wt = (uint64_t)blockIdx.x * (kBlock / kWave) + wave_id();
   if (wt >= ntiles) return;
   const uint64_t b = wt * kWaveTile;
   const uint32_t cnt = (uint32_t)(n - b < kWaveTile ? n - b : kWaveTile);
@@ -731,7 +731,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_heads_narrow(NarrowRows nr, u
   wave_tile_load<false>(nr.cols + b, nullptr, cnt, kScanItems * lane_id(), col, unused);
   bool rowsin;
   const uint32_t hm = wave_tile_heads(nr, b, cnt, nr.tile_row[2 * wt], col,
-                                      mark[threadIdx.x / kWave], mk, &rowsin);
+                                      mark[wave_id()], mk, &rowsin);
   const uint32_t tot = __builtin_amdgcn_readlane(wave_incl_scan32((uint32_t)__popc(hm)),
                                                  kWave - 1);
   if (lane_id() == 0) tile_sums[wt] = mk2(tot, 0);

@@ -64,7 +64,7 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
 // [w*64*IPT, (w+1)*64*IPT) of the tile, lane-striped.
 template <int IPT>
 __device__ __forceinline__ uint32_t tile_local(int i) {
-  return (threadIdx.x / kWave) * (uint32_t)(kWave * IPT) + (uint32_t)i * kWave + lane_id();
+  return wave_id() * (uint32_t)(kWave * IPT) + (uint32_t)i * kWave + lane_id();
 }
 
 // Ranks the keys a thread holds (tile_local layout; valid when below cnt,
@@ -77,7 +77,7 @@ __device__ __forceinline__ void tile_rank(TileSmem<RB, IPT, NT, ST>& sm, uint32_
                                           const KT (&k)[IPT], uint32_t (&pos)[IPT]) {
   constexpr int R = 1 << RB;
   const uint32_t lane = lane_id();
-  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t wave = wave_id();
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {
     const bool valid = FULL || tile_local<IPT>(i) < cnt;
@@ -148,7 +148,7 @@ __device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT, ST>& sm
                                                    float* __restrict__ vout) {
   constexpr int R = 1 << RB;
   const uint32_t tid = threadIdx.x;
-  const uint32_t wave = tid / kWave;
+  const uint32_t wave = wave_id();
   static_assert(sizeof(KT) <= sizeof(ST), "staging element narrower than the key");
   KT* kbuf = reinterpret_cast<KT*>(sm.buf);
   tile_digit_starts<RB, IPT, NT>(sm);
