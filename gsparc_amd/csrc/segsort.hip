@@ -414,7 +414,8 @@ __global__ __launch_bounds__(kBlock) void k_expand_items(ExpandArgs a,
           __builtin_nontemporal_store(v, &oval[p - a.P0]);
           vsum += (double)v;
         }
-        if (isbig) hist_add_wave<RB>(sh, col, valid, 1);
+        // first-pass digits are spread (low column bits): plain LDS atomics
+        if (isbig && valid) atomicAdd(&sh[col & (R - 1)], 1u);
       }
     }
     if (isbig) {
