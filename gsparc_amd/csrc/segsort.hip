@@ -39,35 +39,34 @@ __global__ __launch_bounds__(kSegNT) void k_seg_upsweep(const KT* __restrict__ k
 }
 
 // KI -> KO: 64 -> 64 (operand sorts), 32 -> 32 (column passes of big rows).
-template <typename KI, typename KO, int RB, int IPT>
-__global__ __launch_bounds__(kSegNT) void k_seg_downsweep(
+// NT threads x IPT keys per tile (NT * IPT = the plan's tile size).
+template <typename KI, typename KO, int RB, int IPT, int NT = kSegNT>
+__global__ __launch_bounds__(NT) void k_seg_downsweep(
     const KI* __restrict__ kin, const float* __restrict__ vin, KO* __restrict__ kout,
     float* __restrict__ vout, int shift, const SegTile* __restrict__ tiles,
     const uint32_t* __restrict__ offs, const uint32_t* __restrict__ dstarts, uint32_t remap) {
   constexpr int R = 1 << RB;
-  __shared__ TileSmem<RB, IPT, kSegNT, KI> sm;
+  __shared__ TileSmem<RB, IPT, NT, KI> sm;
   const uint32_t tid = threadIdx.x;
   const SegTile t = tiles[remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x];
-  for (int i = tid; i < (kSegNT / kWave) * R; i += kSegNT) (&sm.cnt[0][0])[i] = 0;
-  for (int d = tid; d < R; d += kSegNT)
+  for (int i = tid; i < (NT / kWave) * R; i += NT) (&sm.cnt[0][0])[i] = 0;
+  for (int d = tid; d < R; d += NT)
     sm.gbase[d] = offs[t.tile * R + d] + dstarts[(uint64_t)t.first * R + d];
   __syncthreads();
-  const bool full = t.count == kSegNT * IPT;
+  const bool full = t.count == NT * IPT;
   KI k[IPT];
   float v[IPT];
   uint32_t pos[IPT];
   if (full)
-    tile_load_rank<true, RB, IPT, kSegNT>(sm, kin + t.start, vin + t.start, t.count, shift, k, v,
-                                          pos);
+    tile_load_rank<true, RB, IPT, NT>(sm, kin + t.start, vin + t.start, t.count, shift, k, v, pos);
   else
-    tile_load_rank<false, RB, IPT, kSegNT>(sm, kin + t.start, vin + t.start, t.count, shift, k,
-                                           v, pos);
+    tile_load_rank<false, RB, IPT, NT>(sm, kin + t.start, vin + t.start, t.count, shift, k, v,
+                                       pos);
   __syncthreads();
-  for (int d = tid; d < R; d += kSegNT)
-    sm.dstart[d] = (uint16_t)wave_prefixes<RB, IPT, kSegNT>(sm, d);
+  for (int d = tid; d < R; d += NT) sm.dstart[d] = (uint16_t)wave_prefixes<RB, IPT, NT>(sm, d);
   __syncthreads();
-  if (full) tile_stage_scatter<true, RB, IPT, kSegNT>(sm, t.count, shift, k, v, pos, kout, vout);
-  else tile_stage_scatter<false, RB, IPT, kSegNT>(sm, t.count, shift, k, v, pos, kout, vout);
+  if (full) tile_stage_scatter<true, RB, IPT, NT>(sm, t.count, shift, k, v, pos, kout, vout);
+  else tile_stage_scatter<false, RB, IPT, NT>(sm, t.count, shift, k, v, pos, kout, vout);
 }
 
 // Per segment (one workgroup, thread d = digit d): off[t][d] = digit d's
@@ -535,6 +534,14 @@ static int narrow_rb(int col_bits) {
   const int p10 = std::max(1, radix_passes(col_bits, 10));
   return p10 < p8 ? 10 : 8;
 }
+// SPTC_SEG_WIDE=1: big-row scatter passes on 1024-thread workgroups
+static bool seg_wide() {
+  static const bool w = [] {
+    const char* e = getenv("SPTC_SEG_WIDE");
+    return e && atoi(e) == 1;
+  }();
+  return w;
+}
 static int narrow_ipt() {
   static const int ipt = [] {
     const char* e = getenv("SPTC_SEG_IPT");
@@ -639,10 +646,16 @@ static void narrow_sort(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, u
     if (kt) kt->end(tok, SPTC_K_SORT_COUNT, up + 16.0 * (double)m, s);
     tok = kt ? kt->begin(s) : -1;
     const bool last = p + 1 == passes;
-    hipLaunchKernelGGL((k_seg_downsweep<uint32_t, uint32_t, RB, IPT>), dim3((uint32_t)ntiles),
-                       dim3(kSegNT), 0, s, col[cur], val[cur], last ? fcol : col[1 - cur],
-                       last ? v : val[1 - cur], shift, tiles, ws.rts_off.get(),
-                       ws.rts_cnt.get(), xcd_remap());
+    if (IPT == 16 && seg_wide())  // same 8192-key tiles, 1024 threads x 8 keys
+      hipLaunchKernelGGL((k_seg_downsweep<uint32_t, uint32_t, RB, 8, 1024>),
+                         dim3((uint32_t)ntiles), dim3(1024), 0, s, col[cur], val[cur],
+                         last ? fcol : col[1 - cur], last ? v : val[1 - cur], shift, tiles,
+                         ws.rts_off.get(), ws.rts_cnt.get(), xcd_remap());
+    else
+      hipLaunchKernelGGL((k_seg_downsweep<uint32_t, uint32_t, RB, IPT>), dim3((uint32_t)ntiles),
+                         dim3(kSegNT), 0, s, col[cur], val[cur], last ? fcol : col[1 - cur],
+                         last ? v : val[1 - cur], shift, tiles, ws.rts_off.get(),
+                         ws.rts_cnt.get(), xcd_remap());
     if (kt) kt->end(tok, cls, 16.0 * (double)nb, s);
     cur = 1 - cur;
   }

@@ -346,3 +346,35 @@ def test_row_segments_big_and_small_rows(gpu):
                           weights=prod_per_c[X.idx[2].astype(np.int64)])
     assert per_row.max() > 3 * 6144 and np.median(per_row) < 3072  # both kinds present
     _compare_with_oracle(X, X, [2], [2], res)
+
+
+@pytest.mark.parametrize("ycols", [1, 3, 40])
+def test_long_runs_and_row_starts_across_wave_tiles(gpu, ycols):
+    """Reduce-tile edge cases, against the oracle: runs of hundreds to
+    thousands of products (Y has only `ycols` free columns) that cross lanes
+    and 512-product wave tiles, rows whose products end at arbitrary offsets
+    (sizes from 1 to 8000 products, big rows and small-row groups), rows with
+    no products (X nonzeros whose contracted index Y lacks), signed values."""
+    rng = np.random.default_rng(100 + ycols)
+    C = 3000
+    rows, xs = [], []
+    for r, n in enumerate([1, 2, 7, 64, 511, 512, 513, 900, 1500, 2600, 4000, 9000, 20, 3, 70]):
+        c = rng.choice(C, size=min(n, C), replace=False)
+        rows.append(np.full(len(c), r, np.uint32))
+        xs.append(c.astype(np.uint32))
+    xi, xc = np.concatenate(rows), np.concatenate(xs)
+    xv = (rng.random(len(xi)) - 0.5).astype(np.float32)
+    X = COOTensor((len(rows), C), [xi, xc], xv)
+    # Y: contracted index c in [0, 2700) only (X nonzeros at c >= 2700 give
+    # no products), free column j in [0, ycols)
+    yc, yj = [], []
+    for c in range(2700):
+        k = int(rng.integers(1, ycols + 1)) if ycols > 1 else 1
+        for j in rng.choice(ycols, size=k, replace=False):
+            yc.append(c)
+            yj.append(j)
+    yc, yj = np.asarray(yc, np.uint32), np.asarray(yj, np.uint32)
+    yv = (rng.random(len(yc)) - 0.5).astype(np.float32)
+    Y = COOTensor((C, ycols), [yc, yj], yv)
+    res = sptc.SpTC(X, Y, [1], [0], dense_accumulator=-1)
+    _compare_with_oracle(X, Y, [1], [0], res)
