@@ -250,6 +250,7 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
   int kyf = ynf;
   const uint64_t* ykeys = Y.keys.get();
   DevBuf<uint64_t> ykeys_rank, ytab;
+  DevBuf<uint32_t> ytab32;  // ytab as 32-bit words when Y free keys fit them
   if (ynf > 0 && ny > 1) {
     DevBuf<uint64_t> fk(ny);
     DevBuf<float> fv(ny);
@@ -270,6 +271,10 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
                    s_);
       ykeys = ykeys_rank.get();
       kyf = yrb;
+      if (ynf <= 32) {
+        ytab32.reset(U);
+        launch_ycol(ytab.get(), U, 0xFFFFFFFFull, ytab32.get(), s_);
+      }
     } else {
       ytab.release();
     }
@@ -588,7 +593,8 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       tok = kt_.begin(s_);
       if (rowsort)
         run_reduce_emit_narrow(nr, vA.get(), n, oc.row.get(), oc.col.get(), oc.val.get(), r0c,
-                               ytab.size() ? ytab.get() : nullptr, tmp.get(), s_);
+                               ytab.size() ? ytab.get() : nullptr,
+                               ytab32.size() ? ytab32.get() : nullptr, tmp.get(), s_);
       else
         run_reduce_emit(sk, sv, n, oc.row.get(), oc.col.get(), oc.val.get(), r0c, kyf, fmask,
                         ytab.size() ? ytab.get() : nullptr, tmp.get(), s_);

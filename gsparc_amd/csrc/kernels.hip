@@ -592,6 +592,7 @@ struct ReduceOp {
   uint64_t fmask;      // its mask
   const uint64_t* ytab;  // rank -> Y free key (NULL: the field is the key)
   NarrowRows nr;         // narrow form (keys unused): columns + row starts
+  const uint32_t* ytab32 = nullptr;  // ytab as 32-bit words (narrow emit; keys < 2^32)
 };
 
 // Per tile of kScanTile sorted keys: the number of run heads (key differs
@@ -678,6 +679,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_emit_narrow(ReduceOp op, cons
   const uint64_t obase = tile_offs[wt].a;
   const uint32_t lane = lane_id();
   const uint64_t* __restrict__ ytab = op.ytab;
+  const uint32_t* __restrict__ ytab32 = op.ytab32;
   const uint32_t rb = (uint32_t)(op.r0 + r0);
   uint32_t* __restrict__ orow = op.out_row + obase;  // uniform bases
   uint64_t* __restrict__ ocol = op.out_col + obase;
@@ -687,7 +689,8 @@ __global__ __launch_bounds__(kBlock) void k_reduce_emit_narrow(ReduceOp op, cons
   for (int q = 0; q < kScanItems; ++q) {
     const uint32_t i = lane + (uint32_t)q * kWave;
     col[q] = i < T ? w.c[i] : 0;
-    if (ytab && i < T) col[q] = ytab[col[q]];
+    if (ytab32 && i < T) col[q] = ytab32[col[q]];  // half the gathered table
+    else if (ytab && i < T) col[q] = ytab[col[q]];
   }
 #pragma unroll
   for (int q = 0; q < kScanItems; ++q) {
@@ -707,10 +710,12 @@ static void check_narrow(const NarrowRows& nr, const float* vals) {
 
 void run_reduce_emit_narrow(const NarrowRows& nr, const float* vals, uint64_t n,
                             uint32_t* out_row, uint64_t* out_col, float* out_val, uint64_t r0,
-                            const uint64_t* ytab, const U2* tmp, hipStream_t s) {
+                            const uint64_t* ytab, const uint32_t* ytab32, const U2* tmp,
+                            hipStream_t s) {
   if (!n) return;
   check_narrow(nr, vals);
-  ReduceOp op{nullptr, vals, n, out_row, out_col, out_val, r0, 32, 0xFFFFFFFFull, ytab, nr};
+  ReduceOp op{nullptr, vals, n, out_row, out_col, out_val, r0, 32, 0xFFFFFFFFull, ytab, nr,
+              ytab32};
   const uint64_t nt = narrow_tile_count(n);
   hipLaunchKernelGGL(k_reduce_emit_narrow, dim3((uint32_t)ceil_div(nt, kBlock / kWave)),
                      dim3(kBlock), 0, s, op, tmp, nt);

@@ -130,9 +130,11 @@ void launch_tile_rows(const uint64_t* rstart, uint64_t nrows, uint64_t n, uint32
 // holds narrow_tmp_count(n) entries, the total at tmp[narrow_tile_count(n)];
 // columns and values 16-byte aligned.
 void run_reduce_count_narrow(const NarrowRows& nr, uint64_t n, U2* tmp, hipStream_t s);
+// ytab32 (may be NULL): the same table as 32-bit words, used when set.
 void run_reduce_emit_narrow(const NarrowRows& nr, const float* vals, uint64_t n,
                             uint32_t* out_row, uint64_t* out_col, float* out_val, uint64_t r0,
-                            const uint64_t* ytab, const U2* tmp, hipStream_t s);
+                            const uint64_t* ytab, const uint32_t* ytab32, const U2* tmp,
+                            hipStream_t s);
 void launch_yfree_extract(const uint64_t* ykeys, uint64_t ny, uint64_t fmask, uint64_t* out,
                           hipStream_t s);
 void run_unique(const uint64_t* sorted, uint64_t n, uint64_t* out, U2* tmp, hipStream_t s);
