@@ -345,7 +345,8 @@ __global__ __launch_bounds__(kBlock) void k_breduce_emit_narrow(BReduceOp op,
   if (wt >= ntiles) return;
   WaveSmem& w = sm[wave_id()];
   uint64_t r0, rp;
-  const uint32_t T = wave_tile_reduce(op.nr, op.vals, op.n, wt, w, &r0, &rp);
+  bool rows;
+  const uint32_t T = wave_tile_reduce(op.nr, op.vals, op.n, wt, w, &r0, &rp, &rows);
   const uint64_t obase = tile_offs[wt].a;
   const uint32_t lane = lane_id();
   uint64_t col[kScanItems];
@@ -353,7 +354,7 @@ __global__ __launch_bounds__(kBlock) void k_breduce_emit_narrow(BReduceOp op,
 #pragma unroll
   for (int q = 0; q < kScanItems; ++q) {
     const uint32_t i = lane + (uint32_t)q * kWave;
-    const uint64_t lrow = i < T ? r0 + w.m[i] : 0;
+    const uint64_t lrow = i < T ? r0 + (rows ? w.m[i] : 0) : 0;
     row[q] = i < T ? op.b.map_row[lrow] : 0;
     si[q] = i < T ? op.b.map_inst[lrow] : 0;
   }
@@ -376,7 +377,7 @@ __global__ __launch_bounds__(kBlock) void k_breduce_emit_narrow(BReduceOp op,
     if (!op.b.bounds) continue;
     // first output of every instance in (previous output's instance, s]
     int64_t sp = -1;
-    if (i > 0) sp = op.b.map_inst[r0 + w.m[i - 1]];
+    if (i > 0) sp = op.b.map_inst[r0 + (rows ? w.m[i - 1] : 0)];
     else if (wt > 0) sp = op.b.map_inst[rp];
     for (int64_t t = sp + 1; t <= (int64_t)si[q]; ++t) op.b.bounds[t] = obase + i;
   }

@@ -675,7 +675,8 @@ __global__ __launch_bounds__(kBlock) void k_reduce_emit_narrow(ReduceOp op, cons
   if (wt >= ntiles) return;
   WaveSmem& w = sm[wave_id()];
   uint64_t r0, rp;
-  const uint32_t T = wave_tile_reduce(op.nr, op.vals, op.n, wt, w, &r0, &rp);
+  bool rows;
+  const uint32_t T = wave_tile_reduce(op.nr, op.vals, op.n, wt, w, &r0, &rp, &rows);
   const uint64_t obase = tile_offs[wt].a;
   const uint32_t lane = lane_id();
   const uint64_t* __restrict__ ytab = op.ytab;
@@ -696,7 +697,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_emit_narrow(ReduceOp op, cons
   for (int q = 0; q < kScanItems; ++q) {
     const uint32_t i = lane + (uint32_t)q * kWave;
     if (i < T) {
-      __builtin_nontemporal_store(rb + w.m[i], &orow[i]);
+      __builtin_nontemporal_store(rows ? rb + w.m[i] : rb, &orow[i]);
       __builtin_nontemporal_store(col[q], &ocol[i]);
       __builtin_nontemporal_store(w.v[i], &oval[i]);
     }

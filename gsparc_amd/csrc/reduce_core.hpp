@@ -244,11 +244,13 @@ __device__ __forceinline__ void wave_lane_sums(uint32_t hm, uint32_t q0, uint32_
 // Reduces wave tile wt: leaves its outputs -- one per head in the tile, in
 // key order -- in (w.c, w.m, w.v)[0 .. T-1] (column, row - r0, sum) and
 // returns T; *r0 = row of the tile's first product, *rp = row of the
-// product before it.
+// product before it.  When no row starts after the tile's first product
+// (*rows false, most tiles of long rows) every output is in row r0 and the
+// row offsets are not staged.
 __device__ __forceinline__ uint32_t wave_tile_reduce(const NarrowRows& nr,
                                                      const float* __restrict__ vals, uint64_t n,
                                                      uint64_t wt, WaveSmem& w, uint64_t* r0p,
-                                                     uint64_t* rpp) {
+                                                     uint64_t* rpp, bool* rows) {
   const uint32_t lane = lane_id();
   const uint64_t b = wt * kWaveTile;
   const uint32_t cnt = (uint32_t)(n - b < kWaveTile ? n - b : kWaveTile);
@@ -348,7 +350,7 @@ __device__ __forceinline__ uint32_t wave_tile_reduce(const NarrowRows& nr,
   for (int j = 0; j < kScanItems; ++j) {
     if ((hm >> j) & 1u) {
       w.c[k] = col[j];
-      w.m[k] = ro[j];
+      if (rowsin) w.m[k] = ro[j];
       ++k;
     }
   }
@@ -360,6 +362,7 @@ __device__ __forceinline__ uint32_t wave_tile_reduce(const NarrowRows& nr,
     if (((hm >> (j + 1)) & 1u) && (hm & ((2u << j) - 1u))) w.v[k++] = res[j];
   if (hm) w.v[o + h - 1] = last;
   wave_sync();
+  *rows = rowsin;
   return T;
 }
 

@@ -63,10 +63,13 @@ __global__ __launch_bounds__(NT) void k_seg_downsweep(
     tile_load_rank<false, RB, IPT, NT>(sm, kin + t.start, vin + t.start, t.count, shift, k, v,
                                        pos);
   __syncthreads();
-  for (int d = tid; d < R; d += NT) sm.dstart[d] = (uint16_t)wave_prefixes<RB, IPT, NT>(sm, d);
-  __syncthreads();
-  if (full) tile_stage_scatter<true, RB, IPT, NT>(sm, t.count, shift, k, v, pos, kout, vout);
-  else tile_stage_scatter<false, RB, IPT, NT>(sm, t.count, shift, k, v, pos, kout, vout);
+  tile_prefix_fold<RB, IPT, NT>(sm);
+  if (full)
+    tile_stage_scatter<true, RB, IPT, NT, KI, KI, KO, true>(sm, t.count, shift, k, v, pos, kout,
+                                                           vout);
+  else
+    tile_stage_scatter<false, RB, IPT, NT, KI, KI, KO, true>(sm, t.count, shift, k, v, pos, kout,
+                                                            vout);
 }
 
 // Per segment (one workgroup, thread d = digit d): off[t][d] = digit d's

@@ -134,13 +134,55 @@ __device__ __forceinline__ void tile_digit_starts(TileSmem<RB, IPT, NT, ST>& sm)
   }
 }
 
+// After ranking (sm.cnt = per-wave digit counts) in one sweep: each thread
+// takes PER consecutive digits, turns their per-wave counts into wave
+// prefixes in registers, the block scans the digit totals, and the thread
+// writes back cnt[w][d] = (tile-local start of d) + (wave prefix) and
+// gbase[d] -= start (modulo 2^32) -- what tile_digit_starts and the fold in
+// tile_stage_scatter do in three sweeps.
+template <int RB, int IPT, int NT, typename ST>
+__device__ __forceinline__ void tile_prefix_fold(TileSmem<RB, IPT, NT, ST>& sm) {
+  constexpr int R = 1 << RB;
+  constexpr int PER = (R + NT - 1) / NT;
+  constexpr int NW = NT / kWave;
+  const uint32_t tid = threadIdx.x;
+  uint32_t pre[PER][NW], tot[PER];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int d = tid * PER + j;
+    uint32_t acc = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      pre[j][w] = acc;
+      acc += d < R ? (uint32_t)sm.cnt[w][d] : 0u;
+    }
+    tot[j] = acc;
+    sum += acc;
+  }
+  uint32_t all;
+  uint32_t ex = block_scan_u32<NT>(sum, sm.scan, &all);  // barriers inside
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int d = tid * PER + j;
+    if (d < R) {
+#pragma unroll
+      for (int w = 0; w < NW; ++w) sm.cnt[w][d] = (uint16_t)(ex + pre[j][w]);
+      sm.gbase[d] -= ex;
+    }
+    ex += tot[j];
+  }
+}
+
 // Given sm.gbase and sm.dstart (tile totals per digit on entry), scans the
 // totals into tile-local run starts, stages keys in digit order, writes them
 // in coalesced runs, then does the same for the values.  The run starts are
 // folded into the per-wave prefixes (cnt[w][d] += dstart[d]) and the global
 // bases (gbase[d] -= dstart[d], modulo 2^32) first, so a key's staging slot
 // and destination take one LDS read each.
-template <bool FULL, int RB, int IPT, int NT, typename ST, typename KT, typename KO>
+// FOLDED: tile_prefix_fold has already run (sm.dstart unused).
+template <bool FULL, int RB, int IPT, int NT, typename ST, typename KT, typename KO,
+          bool FOLDED = false>
 __device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT, ST>& sm, uint32_t cnt,
                                                    int shift, const KT (&k)[IPT],
                                                    const float (&v)[IPT], uint32_t (&pos)[IPT],
@@ -151,13 +193,15 @@ __device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT, ST>& sm
   const uint32_t wave = wave_id();
   static_assert(sizeof(KT) <= sizeof(ST), "staging element narrower than the key");
   KT* kbuf = reinterpret_cast<KT*>(sm.buf);
-  tile_digit_starts<RB, IPT, NT>(sm);
-  __syncthreads();
-  for (int d = tid; d < R; d += NT) {
-    const uint32_t ds = sm.dstart[d];
+  if constexpr (!FOLDED) {
+    tile_digit_starts<RB, IPT, NT>(sm);
+    __syncthreads();
+    for (int d = tid; d < R; d += NT) {
+      const uint32_t ds = sm.dstart[d];
 #pragma unroll
-    for (int w = 0; w < NT / kWave; ++w) sm.cnt[w][d] = (uint16_t)(sm.cnt[w][d] + ds);
-    sm.gbase[d] -= ds;
+      for (int w = 0; w < NT / kWave; ++w) sm.cnt[w][d] = (uint16_t)(sm.cnt[w][d] + ds);
+      sm.gbase[d] -= ds;
+    }
   }
   __syncthreads();
 #pragma unroll
