@@ -290,17 +290,14 @@ __global__ __launch_bounds__(kSegNT) void k_local_sort(uint32_t* cols, float* va
     __syncthreads();
     tile_rank<false, kSegRB, kSegIPT, kSegNT>(sm, (uint32_t)n, shift, k, pos);
     __syncthreads();
-    for (int d = tid; d < R; d += kSegNT)
-      sm.dstart[d] = (uint16_t)wave_prefixes<kSegRB, kSegIPT, kSegNT>(sm, d);
-    __syncthreads();
-    tile_digit_starts<kSegRB, kSegIPT, kSegNT>(sm);
+    tile_prefix_fold<kSegRB, kSegIPT, kSegNT>(sm);  // cnt[w][d] = run start + wave prefix
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kSegIPT; ++i) {
       const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
       if (idx < n) {
         const uint32_t d = (uint32_t)(k[i] >> shift) & (R - 1);
-        const uint32_t q = pos[i] + (uint32_t)sm.dstart[d] + (uint32_t)sm.cnt[wave][d];
+        const uint32_t q = pos[i] + (uint32_t)sm.cnt[wave][d];
         sm.buf[q] = k[i];
         vbuf[q] = v[i];
       }
