@@ -80,6 +80,12 @@ __device__ __forceinline__ void tile_rank(TileSmem<RB, IPT, NT, ST>& sm, uint32_
   const uint32_t wave = wave_id();
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {
+    // a wave's item past the tile's end on all lanes (partly filled tiles,
+    // small-row groups) is skipped whole
+    if (!FULL && wave * (uint32_t)(kWave * IPT) + (uint32_t)i * kWave >= cnt) {
+      pos[i] = 0;
+      continue;
+    }
     const bool valid = FULL || tile_local<IPT>(i) < cnt;
     const uint32_t d = (uint32_t)(k[i] >> shift) & (R - 1);
     const uint64_t peers = match_digit<RB>(d, FULL ? ~0ull : ballot(valid));
