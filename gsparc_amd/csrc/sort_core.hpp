@@ -34,9 +34,10 @@ struct TileSmem {
   static constexpr int R = 1 << RB;
   static constexpr int TILE = NT * IPT;
   static constexpr int NW = NT / kWave;
-  uint16_t cnt[NW][R];     // per-wave running counts, then wave prefixes
-  uint32_t gbase[R];       // global start of this tile's run of each digit
-  uint16_t dstart[R];      // tile-local start of each digit's run
+  uint16_t cnt[NW][R];     // per-wave running counts, then each wave's staging start
+  uint32_t gbase[R];       // global start of this tile's run of each digit (then minus
+                           // the run's tile-local start)
+  uint16_t dstart[R];      // tile totals / run starts (paths without tile_prefix_fold)
   ST buf[TILE];            // staged keys, then staged values
   uint32_t scan[NW];
   uint32_t tile;
