@@ -37,8 +37,8 @@ def parse():
     p.add_argument("--qps", type=float, default=10.0)
     p.add_argument("--batch-instances", type=int, default=256)
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-sample-rows", type=int, default=0,
-                   help="X free rows in the CPU-baseline sample (0: auto)")
+    p.add_argument("--cpu-sample-products", type=float, default=0,
+                   help="intermediate products in the CPU-baseline sample (0: auto)")
     p.add_argument("--radix-bits", type=int, default=0, help="0: autotune, 8 or 11")
     p.add_argument("--copy-probe-bytes", type=int, default=0,
                    help="run one device-to-device copy of this size first (counter calibration)")
@@ -73,10 +73,22 @@ def measured_traffic(workload, kernel_class):
     return None
 
 
-def cpu_baseline(spec, X, n_rows):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(spec, X, target_products):
     """Reference CPU SpTC (Sparta ttt, EXPERIMENT_MODES=3) on a bounded
-    sample: the nonzeros of the first `n_rows` X free rows (in free-row key
-    order) contracted against all of X."""
+    sample: the nonzeros of every k-th X free row (rows in free-key order, k
+    chosen so the sample holds about target_products intermediate
+    products, so it spans the heavy head and the light tail of the row
+    distribution) contracted against all of X."""
     import oracle as orc
     from gsparc_amd.coo import COOTensor, write_tns_fast
     if not os.path.exists(orc.REF_TTT):
@@ -85,18 +97,18 @@ def cpu_baseline(spec, X, n_rows):
     fk = np.zeros(X.nnz, np.int64)
     for m in free:
         fk = fk * X.dims[m] + X.idx[m].astype(np.int64)
-    rows = np.unique(fk)
-    pick = rows[:n_rows]
-    sel = np.isin(fk, pick)
-    Xs = COOTensor(X.dims, [a[sel] for a in X.idx], X.val[sel])
-    # the sample's products (for the rate): sum over its nonzeros of the
-    # matching contract-key count in X
     ck = np.zeros(X.nnz, np.int64)
     for m in spec.cx:
         ck = ck * X.dims[m] + X.idx[m].astype(np.int64)
-    u, cnt = np.unique(ck, return_counts=True)
-    look = dict(zip(u.tolist(), cnt.tolist()))
-    prods = int(sum(look.get(k, 0) for k in ck[sel].tolist()))
+    _, cinv = np.unique(ck, return_inverse=True)
+    per_nnz = np.bincount(cinv).astype(np.int64)[cinv]  # products of each nonzero
+    rows, rinv = np.unique(fk, return_inverse=True)
+    total = int(per_nnz.sum())
+    stride = max(1, int(round(total / target_products)))
+    sel = (rinv % stride) == 0
+    Xs = COOTensor(X.dims, [a[sel] for a in X.idx], X.val[sel])
+    prods = int(per_nnz[sel].sum())
+    pick = rows[::stride]
     threads = os.cpu_count() or 1
     with tempfile.TemporaryDirectory() as d:
         xs_path = os.path.join(d, "xs.tns")
@@ -116,8 +128,9 @@ def cpu_baseline(spec, X, n_rows):
     if tot is None or not nnz_z:
         return None
     return {"total_s": tot, "threads": threads, "sample_rows": int(len(pick)),
+            "stride": stride, "rows": int(len(rows)),
             "sample_nnz": int(sel.sum()), "sample_products": prods,
-            "sample_out_nnz": int(nnz_z[-1])}
+            "sample_out_nnz": int(nnz_z[-1]), "cpu_model": cpu_model()}
 
 
 def bench_batch(args):
@@ -411,6 +424,38 @@ def bench_nell2(args):
     print(json.dumps(out))
 
 
+def self_check(X, spec, r):
+    """Checks the last timed step's result against what the input alone
+    fixes: ir = sum over X nonzeros of the count of nonzeros sharing their
+    contracted coordinates, the FP64 checksum of the products =
+    sum_c S(c)^2 (S(c) = sum of values at contracted coordinates c), and one
+    device pass over every emitted entry (FP64 sum of the values, strict
+    (row, column) order).  Exits non-zero on a mismatch."""
+    ck = np.zeros(X.nnz, np.int64)
+    for m in spec.cx:
+        ck = ck * np.int64(X.dims[m]) + X.idx[m].astype(np.int64)
+    u, inv = np.unique(ck, return_inverse=True)
+    cnt = np.bincount(inv).astype(np.int64)
+    S = np.bincount(inv, weights=X.val.astype(np.float64))
+    ir_cf = int((cnt * cnt).sum())
+    vs_cf = float((S * S).sum())
+    emitted, bad = r.verify()
+    u32 = 2.0 ** -24
+    out = {"ir_nnz": r.ir_nnz, "ir_closed_form": ir_cf, "value_sum": r.info.value_sum,
+           "value_sum_closed_form": vs_cf, "emitted_value_sum": emitted,
+           "order_violations": bad}
+    # positive values: FP32-rounded products sum within u32 of the exact
+    # checksum, compensated FP32 output sums within about 3 u32
+    ok = (r.ir_nnz == ir_cf and abs(r.info.value_sum - vs_cf) <= 2 * u32 * vs_cf
+          and abs(emitted - vs_cf) <= 6 * u32 * vs_cf and bad == 0)
+    out["ok"] = bool(ok)
+    log(f"self-check: {out}")
+    if not ok:
+        print(json.dumps({"error": "result check failed", "check": out}))
+        raise SystemExit(3)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -455,7 +500,8 @@ def main():
     out_nnz = ir = 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    last = None
+    for k_step in range(args.steps):
         ts = time.perf_counter()
         r = step(True)
         torch.cuda.synchronize()
@@ -465,10 +511,14 @@ def main():
             a0 = kstats.get(k, (0, 0.0, 0.0))
             kstats[k] = (a0[0] + n, a0[1] + kms, a0[2] + kb)
         out_nnz, ir = r.nnz, r.ir_nnz
+        if k_step == args.steps - 1:
+            last = r  # checked after the timed region
         del r
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms = elapsed / args.steps * 1e3
+    check = self_check(X, spec, last)
+    del last
     ph = np.median(np.array(phases), axis=0).tolist()
     res = {
         "metric": "SpTC output nnz/s",
@@ -482,6 +532,7 @@ def main():
                    "out_nnz": out_nnz},
         "phase_ms_median": {"linearize_sort": ph[1], "match": ph[2], "partition": ph[3],
                             "accumulate": ph[4], "total": ph[5]},
+        "check": check,
     }
     peak = sptc.device_info()["peak_hbm_gbs"]
     kern = {}
@@ -500,14 +551,15 @@ def main():
                            "alg_bytes_per_launch": d["alg_bytes_per_launch"]}
     res["pool"] = sptc.pool_stats()
     if not args.no_cpu_baseline and args.workload != "nell2":
-        rows = args.cpu_sample_rows or (1200 if args.workload == "nips" else 4000)
-        cb = cpu_baseline(spec, X, rows)
+        target = args.cpu_sample_products or (7e8 if args.workload == "nips" else 8e7)
+        cb = cpu_baseline(spec, X, target)
         if cb:
             res["cpu_baseline"] = {
                 "value": cb["sample_out_nnz"] / cb["total_s"], "unit": "nnz/s",
-                "cores": cb["threads"], "kind": "reference",
-                "sample": (f"Sparta ttt (EXPERIMENT_MODES=3, [Total time]) on the first "
-                           f"{cb['sample_rows']} X free rows ({cb['sample_nnz']} nnz, "
+                "cores": cb["threads"], "kind": "reference", "cpu_model": cb["cpu_model"],
+                "sample": (f"Sparta ttt (EXPERIMENT_MODES=3, [Total time]) on every "
+                           f"{cb['stride']}th of the {cb['rows']} X free rows "
+                           f"({cb['sample_rows']} rows, {cb['sample_nnz']} nnz, "
                            f"{cb['sample_products']} products, {cb['sample_out_nnz']} output "
                            f"nnz) against all of X: {cb['total_s']:.3f} s"),
                 "products_per_s": cb["sample_products"] / cb["total_s"],

@@ -15,7 +15,8 @@ from ._lib import LIB_DIR, PKG_DIR, REPO_DIR
 CSRC = os.path.join(PKG_DIR, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-HIP_SOURCES = ["sort.hip", "segsort.hip", "kernels.hip", "batch.hip", "device.cpp", "engine.cpp",
+HIP_SOURCES = ["sort.hip", "segsort.hip", "kernels.hip", "batch.hip", "result.hip",
+               "device.cpp", "engine.cpp",
                "batch_engine.cpp", "serve.cpp", "api.cpp"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

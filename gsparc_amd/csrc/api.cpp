@@ -5,8 +5,9 @@
 #include <vector>
 
 #include "batch_engine.hpp"
-#include "serve.hpp"
 #include "engine.hpp"
+#include "result.hpp"
+#include "serve.hpp"
 
 using namespace sptc;
 
@@ -158,6 +159,9 @@ sptc_status sptc_contract(const sptc_coo* X, const sptc_coo* Y, int32_t nc, cons
   } catch (const std::bad_alloc&) {
     delete res;
     return fail(SPTC_ERR_OOM_BUDGET, "host allocation failed");
+  } catch (const std::exception& ex) {
+    delete res;
+    return fail(SPTC_ERR_INTERNAL, ex.what());
   }
   *out = res;
   return SPTC_OK;
@@ -199,6 +203,9 @@ sptc_status sptc_plan(const sptc_coo* X, const sptc_coo* Y, int32_t nc, const in
   } catch (const std::bad_alloc&) {
     delete p;
     return fail(SPTC_ERR_OOM_BUDGET, "host allocation failed");
+  } catch (const std::exception& ex) {
+    delete p;
+    return fail(SPTC_ERR_INTERNAL, ex.what());
   }
   *out = p;
   return SPTC_OK;
@@ -228,6 +235,9 @@ sptc_status sptc_execute(sptc_plan_t* p, const sptc_opts* opts, void* stream,
   } catch (const std::bad_alloc&) {
     delete res;
     return fail(SPTC_ERR_OOM_BUDGET, "host allocation failed");
+  } catch (const std::exception& ex) {
+    delete res;
+    return fail(SPTC_ERR_INTERNAL, ex.what());
   }
   *out = res;
   return SPTC_OK;
@@ -267,6 +277,8 @@ sptc_status sptc_partition_plan(sptc_plan_t* p, uint64_t budget_bytes, int32_t m
     return from_hip(he);
   } catch (const std::bad_alloc&) {
     return fail(SPTC_ERR_OOM_BUDGET, "host allocation failed");
+  } catch (const std::exception& ex) {
+    return fail(SPTC_ERR_INTERNAL, ex.what());
   }
   const int32_t n = res.part_rows.empty() ? 0 : (int32_t)(res.part_rows.size() - 1);
   *nparts = n;
@@ -417,25 +429,111 @@ sptc_status sptc_result_copy_range(const sptc_result* r, uint64_t begin, uint64_
   return SPTC_OK;
 }
 
-sptc_status sptc_result_decode(const sptc_result* r, uint32_t* const* mode_idx, float* vals) {
-  if (!r || (!mode_idx && r->info.out_order)) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");
-  const uint64_t n = r->info.nnz;
-  std::vector<uint64_t> xf(n), yf(n);
-  sptc_status st = sptc_result_copy_to_host(r, nullptr, xf.data(), yf.data(), vals);
-  if (st != SPTC_OK) return st;
+// Output field layout of a result for the decode kernel; false when a free
+// mode is wider than 32 bits (it would not fit the uint32 outputs).
+static bool decode_args(const sptc_result* r, DecodeArgs* a) {
+  std::memset(a, 0, sizeof(*a));
   const Layout& xl = r->xl;
   const Layout& yl = r->yl;
+  a->nx = xl.nf;
+  a->ny = yl.nf;
   int o = 0;
   for (int i = 0; i < xl.nf; ++i, ++o) {
     const int m = xl.fmodes[i];
-    const int sh = xl.shift[m] - xl.ncbits;
-    const uint64_t mk = low_mask(xl.width[m]);
-    for (uint64_t k = 0; k < n; ++k) mode_idx[o][k] = (uint32_t)(shr64(xf[k], sh) & mk);
+    if (xl.width[m] > 32) return false;
+    a->shift[o] = xl.shift[m] - xl.ncbits;
+    a->mask[o] = low_mask(xl.width[m]);
   }
   for (int i = 0; i < yl.nf; ++i, ++o) {
     const int m = yl.fmodes[i];
-    const uint64_t mk = low_mask(yl.width[m]);
-    for (uint64_t k = 0; k < n; ++k) mode_idx[o][k] = (uint32_t)(shr64(yf[k], yl.shift[m]) & mk);
+    if (yl.width[m] > 32) return false;
+    a->shift[o] = yl.shift[m];
+    a->mask[o] = low_mask(yl.width[m]);
+  }
+  return true;
+}
+
+sptc_status sptc_result_decode_device(const sptc_result* r, uint32_t* const* mode_idx,
+                                      float* vals, void* stream) {
+  if (!r || (!mode_idx && r->info.out_order)) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");
+  DecodeArgs a;
+  if (!decode_args(r, &a))
+    return fail(SPTC_ERR_KEY_OVERFLOW, "an output mode is wider than 32 bits");
+  for (int m = 0; m < a.nx + a.ny; ++m) {
+    if (!mode_idx[m] && r->info.nnz) return fail(SPTC_ERR_BAD_ARGS, "NULL mode array");
+    a.out[m] = mode_idx[m];
+  }
+  a.vals = vals;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  try {
+    uint64_t off = 0;
+    for (const OutChunk& c : r->chunks) {
+      launch_decode(c.R(), c.C(), c.V(), c.nnz, r->RK(), a, off, s);
+      off += c.nnz;
+    }
+  } catch (const HipError& he) {
+    return from_hip(he);
+  }
+  return SPTC_OK;
+}
+
+sptc_status sptc_result_decode(const sptc_result* r, uint32_t* const* mode_idx, float* vals) {
+  if (!r || (!mode_idx && r->info.out_order)) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");
+  DecodeArgs a;
+  if (!decode_args(r, &a))
+    return fail(SPTC_ERR_KEY_OVERFLOW, "an output mode is wider than 32 bits");
+  const int order = a.nx + a.ny;
+  try {
+    // decode each chunk on the device in pieces, then copy the pieces out
+    const uint64_t piece = 1ull << 26;
+    uint64_t cap = 0;
+    for (const OutChunk& c : r->chunks) cap = std::max(cap, std::min(c.nnz, piece));
+    DevBuf<uint32_t> dev(cap * (uint64_t)std::max(order, 1));
+    DevBuf<float> dval(vals ? cap : 0);
+    for (int m = 0; m < order; ++m) a.out[m] = dev.get() + (uint64_t)m * cap;
+    a.vals = vals ? dval.get() : nullptr;
+    uint64_t off = 0;
+    for (const OutChunk& c : r->chunks) {
+      for (uint64_t b = 0; b < c.nnz; b += piece) {
+        const uint64_t n = std::min(piece, c.nnz - b);
+        launch_decode(c.R() + b, c.C() + b, c.V() + b, n, r->RK(), a, 0, nullptr);
+        for (int m = 0; m < order; ++m)
+          SPTC_HIP(hipMemcpy(mode_idx[m] + off, a.out[m], n * 4, hipMemcpyDeviceToHost));
+        if (vals) SPTC_HIP(hipMemcpy(vals + off, dval.get(), n * 4, hipMemcpyDeviceToHost));
+        off += n;
+      }
+    }
+  } catch (const HipError& he) {
+    return from_hip(he);
+  }
+  return SPTC_OK;
+}
+
+sptc_status sptc_result_verify(const sptc_result* r, double* value_sum, uint64_t* violations,
+                               void* stream) {
+  if (!r) return fail(SPTC_ERR_BAD_ARGS, "NULL result");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  try {
+    double sum = 0.0;
+    uint64_t bad = 0;
+    DevBuf<VerifyPart> part(kVerifyMaxBlocks);
+    std::vector<VerifyPart> host(kVerifyMaxBlocks);
+    for (const OutChunk& c : r->chunks) {
+      if (!c.nnz) continue;
+      launch_verify(c.R(), c.C(), c.V(), c.nnz, c.r0, c.r1, part.get(), s);
+      const uint32_t g = verify_grid(c.nnz);
+      SPTC_HIP(hipMemcpyAsync(host.data(), part.get(), g * sizeof(VerifyPart),
+                              hipMemcpyDeviceToHost, s));
+      SPTC_HIP(hipStreamSynchronize(s));
+      for (uint32_t i = 0; i < g; ++i) {
+        sum += host[i].sum;
+        bad += host[i].bad;
+      }
+    }
+    if (value_sum) *value_sum = sum;
+    if (violations) *violations = bad;
+  } catch (const HipError& he) {
+    return from_hip(he);
   }
   return SPTC_OK;
 }

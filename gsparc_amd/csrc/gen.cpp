@@ -12,6 +12,10 @@
 // rejected as duplicates.  Coordinates are drawn until exactly `nnz` distinct
 // ones exist ("duplicates merged"); entries are stored in first-draw order.
 //
+// Draws are computed in parallel blocks (each draw depends only on its
+// counter) and accepted in counter order by one thread, so the output does
+// not depend on the thread count.
+//
 // Zipf(s) over [0, d): inverse CDF over ranks 1..d with P(r) ~ r^-s, rank 1 ->
 // index 0.  Uniform ints use a 128-bit multiply-high (exact).  Values are
 // uniform on (0,1] in steps of 2^-24 (exact in FP32) or lognormal(0,1) via
@@ -22,6 +26,7 @@
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <thread>
 
 #include "../../include/sptc_gen.h"
 
@@ -96,34 +101,77 @@ extern "C" int sptc_gen_coo(uint64_t seed, int order, const uint64_t* dims,
   const uint64_t base = splitmix64(seed);
   const uint64_t vbase = splitmix64(seed ^ 0xA5A5A5A5DEADBEEFull);
   const uint64_t max_draws = 256 * nnz + (1u << 20);
-  uint64_t coord_idx[SPTC_GEN_MAX_ORDER];
+  unsigned nthr = std::thread::hardware_concurrency();
+  nthr = nthr < 1 ? 1 : (nthr > 16 ? 16 : nthr);
+  const uint64_t block = nnz < (1u << 16) ? 4096 : (1u << 20);
+  // draws [k0, k0 + block): per draw its coordinate and per-mode indices
+  std::vector<uint64_t> bcoord(block);
+  std::vector<uint32_t> bidx((size_t)block * order);
+  auto draw_range = [&](uint64_t k0, uint64_t lo, uint64_t hi) {
+    for (uint64_t j = lo; j < hi; ++j) {
+      const uint64_t k = k0 + j;
+      uint64_t coord = 0;
+      for (int m = 0; m < order; ++m) {
+        const uint64_t c = ms[m].draw(splitmix64(base + k * (uint64_t)order + m));
+        bidx[j * order + m] = (uint32_t)c;
+        coord = coord * dims[m] + c;
+      }
+      bcoord[j] = coord;
+    }
+  };
   uint64_t got = 0;
-  for (uint64_t k = 0; got < nnz; ++k) {
-    if (k >= max_draws) return -4;  // coordinate space too saturated
-    uint64_t coord = 0;
-    for (int m = 0; m < order; ++m) {
-      coord_idx[m] = ms[m].draw(splitmix64(base + k * (uint64_t)order + m));
-      coord = coord * dims[m] + coord_idx[m];
-    }
-    uint64_t h = splitmix64(coord) & mask;
-    bool dup = false;
-    while (table[h] != 0) {
-      if (table[h] == coord + 1) { dup = true; break; }
-      h = (h + 1) & mask;
-    }
-    if (dup) continue;
-    table[h] = coord + 1;
-    for (int m = 0; m < order; ++m) idx_out[m][got] = (uint32_t)coord_idx[m];
-    uint64_t x1 = splitmix64(vbase + 2 * got);
-    if (value_dist == SPTC_GEN_LOGNORMAL) {
-      uint64_t x2 = splitmix64(vbase + 2 * got + 1);
-      double u1 = unit_open0(x1), u2 = unit_closed0(x2);
-      double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
-      val_out[got] = (float)std::exp(z);
+  for (uint64_t k0 = 0; got < nnz; k0 += block) {
+    if (k0 >= max_draws) return -4;  // coordinate space too saturated
+    if (nthr == 1 || block < 8192) {
+      draw_range(k0, 0, block);
     } else {
-      val_out[got] = (float)((double)((x1 >> 40) + 1) * (1.0 / 16777216.0));
+      std::vector<std::thread> th;
+      const uint64_t per = (block + nthr - 1) / nthr;
+      for (unsigned t = 0; t < nthr; ++t) {
+        const uint64_t lo = t * per, hi = std::min<uint64_t>(block, lo + per);
+        if (lo < hi) th.emplace_back(draw_range, k0, lo, hi);
+      }
+      for (auto& x : th) x.join();
     }
-    ++got;
+    for (uint64_t j = 0; j < block && got < nnz; ++j) {
+      if (k0 + j >= max_draws) return -4;
+      const uint64_t coord = bcoord[j];
+      uint64_t h = splitmix64(coord) & mask;
+      bool dup = false;
+      while (table[h] != 0) {
+        if (table[h] == coord + 1) { dup = true; break; }
+        h = (h + 1) & mask;
+      }
+      if (dup) continue;
+      table[h] = coord + 1;
+      for (int m = 0; m < order; ++m) idx_out[m][got] = bidx[j * order + m];
+      ++got;
+    }
+  }
+  // values depend only on the accepted entry's position
+  auto values = [&](uint64_t lo, uint64_t hi) {
+    for (uint64_t g = lo; g < hi; ++g) {
+      uint64_t x1 = splitmix64(vbase + 2 * g);
+      if (value_dist == SPTC_GEN_LOGNORMAL) {
+        uint64_t x2 = splitmix64(vbase + 2 * g + 1);
+        double u1 = unit_open0(x1), u2 = unit_closed0(x2);
+        double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+        val_out[g] = (float)std::exp(z);
+      } else {
+        val_out[g] = (float)((double)((x1 >> 40) + 1) * (1.0 / 16777216.0));
+      }
+    }
+  };
+  if (nthr == 1 || nnz < (1u << 16)) {
+    values(0, nnz);
+  } else {
+    std::vector<std::thread> th;
+    const uint64_t per = (nnz + nthr - 1) / nthr;
+    for (unsigned t = 0; t < nthr; ++t) {
+      const uint64_t lo = t * per, hi = std::min<uint64_t>(nnz, lo + per);
+      if (lo < hi) th.emplace_back(values, lo, hi);
+    }
+    for (auto& x : th) x.join();
   }
   return 0;
 }

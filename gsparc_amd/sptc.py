@@ -106,6 +106,10 @@ def lib():
                                             POINTER(c_uint64)]
         L.sptc_result_copy_range.argtypes = [c_void_p, c_uint64, c_uint64, c_void_p, c_void_p,
                                              c_void_p]
+        L.sptc_result_decode_device.argtypes = [c_void_p, POINTER(c_void_p), c_void_p,
+                                                c_void_p]
+        L.sptc_result_verify.argtypes = [c_void_p, POINTER(c_double), POINTER(c_uint64),
+                                         c_void_p]
         L.sptc_result_free.argtypes = [c_void_p]
         L.sptc_result_free.restype = None
         L.sptc_layout.argtypes = [c_int32, POINTER(c_uint64), c_int32, POINTER(c_int32),
@@ -171,7 +175,8 @@ def lib():
                      "sptc_result_copy_to_host", "sptc_result_decode", "sptc_layout",
                      "sptc_result_row_keys", "sptc_result_row_range", "sptc_result_copy_range",
                      "sptc_linearize", "sptc_sort_pairs", "sptc_index_match",
-                     "sptc_exclusive_scan_u64", "sptc_device_info"):
+                     "sptc_exclusive_scan_u64", "sptc_device_info",
+                     "sptc_result_decode_device", "sptc_result_verify"):
             getattr(L, name).restype = c_int32
         _bound = True
     return L
@@ -303,6 +308,21 @@ class Result:
         check(lib().sptc_result_copy_range(self._h, begin, end, xf.ctypes.data, yf.ctypes.data,
                                            v.ctypes.data))
         return xf, yf, v
+
+    def verify(self, stream=None):
+        """(FP64 sum of the emitted values, order violations) from one pass
+        over the device-resident output."""
+        vs, bad = c_double(), c_uint64()
+        check(lib().sptc_result_verify(self._h, byref(vs), byref(bad), c_void_p(stream or 0)))
+        return vs.value, bad.value
+
+    def decode_device(self, mode_ptrs, vals_ptr=None, stream=None):
+        """Decodes into caller-owned device arrays (one uint32 array of nnz
+        per output mode, given as device addresses)."""
+        order = self.info.out_order
+        ptrs = (c_void_p * max(1, order))(*mode_ptrs)
+        check(lib().sptc_result_decode_device(self._h, ptrs, c_void_p(vals_ptr or 0),
+                                              c_void_p(stream or 0)))
 
     def decode(self):
         """Per-mode 0-based indices (X free modes ascending, then Y free

@@ -139,9 +139,28 @@ sptc_status sptc_result_copy_range(const sptc_result* r, uint64_t begin,
                                    uint64_t* yfree, float* vals);
 
 /* Decodes entries to 0-based per-mode uint32 indices (mode_idx[out_order]
- * host arrays of nnz each), X free modes ascending then Y free modes. */
+ * host arrays of nnz each), X free modes ascending then Y free modes; the
+ * decode runs on the device, pieces are copied out.  SPTC_ERR_KEY_OVERFLOW
+ * when a free mode is wider than 32 bits. */
 sptc_status sptc_result_decode(const sptc_result* r, uint32_t* const* mode_idx,
                                float* vals);
+
+/* Decodes entries to 0-based per-mode uint32 indices on the device:
+ * mode_idx is a host array of out_order DEVICE pointers (nnz entries each),
+ * vals a device array (may be NULL).  Asynchronous on `stream`.  The
+ * reference has no output decode (its COO output is discarded,
+ * contraction.cuh:353-357,596-597).  SPTC_ERR_KEY_OVERFLOW when a free mode
+ * is wider than 32 bits (as for sptc_result_decode). */
+sptc_status sptc_result_decode_device(const sptc_result* r, uint32_t* const* mode_idx,
+                                      float* vals, void* stream);
+
+/* One pass over the device-resident output: *value_sum = FP64 sum of the
+ * emitted values, *violations = entries that are not strictly ascending in
+ * (row, col) within their chunk or lie outside the chunk's row range
+ * (0 for a well-formed result).  Synchronizes the stream.  No reference
+ * counterpart: a check of the result's own invariants. */
+sptc_status sptc_result_verify(const sptc_result* r, double* value_sum, uint64_t* violations,
+                               void* stream);
 
 /* Kernel classes timed when opts.kernel_timing is set. */
 typedef enum {

@@ -10,8 +10,8 @@ import oracle as orc
 from gsparc_amd import sptc
 from gsparc_amd.coo import COOTensor
 from gsparc_amd.workloads import PLAIN_SMALL
-from helpers import (REL_TOL, assert_same_coo, decode_x, decode_y, load_fixtures, max_rel_err,
-                     random_coo)
+from helpers import (REL_TOL, assert_same_coo, compare_with_oracle, decode_x, decode_y,
+                     load_fixtures, max_rel_err, random_coo)
 
 pytestmark = pytest.mark.gpu
 FIXTURES = load_fixtures()
@@ -148,39 +148,7 @@ def test_contract_matches_golden(gpu, case):
     assert_same_coo(coords, v, case["zcoords"], case["zval"], rtol=REL_TOL)
 
 
-def _abs(t):
-    return COOTensor(t.dims, t.idx, np.abs(t.val))
-
-
-def _compare_with_oracle(X, Y, cx, cy, res, x_rows=None):
-    """Indices bit-exact; values within REL_TOL of the FP64 oracle relative
-    to the sum of |products| of each output (equal to the plain relative
-    error when values are positive; with signed values a cancelling sum
-    makes the plain relative error unbounded)."""
-    r = orc.pipeline(X, Y, cx, cy, x_rows=x_rows)
-    signed = bool((X.val < 0).any() or (Y.val < 0).any())
-    scale = orc.pipeline(_abs(X), _abs(Y), cx, cy, x_rows=x_rows)["v64"] if signed else None
-    if x_rows is None:
-        xf, yf, v = res.to_host()
-    else:
-        rk = res.row_keys()
-        parts = []
-        for key in np.sort(np.asarray(x_rows, np.uint64)):
-            j = int(np.searchsorted(rk, key))
-            if j < len(rk) and rk[j] == key:
-                b, e = res.row_range(j)
-                parts.append(res.copy_range(b, e))
-        xf = np.concatenate([p[0] for p in parts]) if parts else np.zeros(0, np.uint64)
-        yf = np.concatenate([p[1] for p in parts]) if parts else np.zeros(0, np.uint64)
-        v = np.concatenate([p[2] for p in parts]) if parts else np.zeros(0, np.float32)
-    np.testing.assert_array_equal(xf, r["xf"])
-    np.testing.assert_array_equal(yf, r["yf"])
-    if signed:
-        err = float((np.abs(v.astype(np.float64) - r["v64"]) / scale).max()) if len(v) else 0.0
-    else:
-        err = max_rel_err(v, r["v64"])
-    assert err <= REL_TOL, err
-    return r, err
+_compare_with_oracle = compare_with_oracle
 
 
 @pytest.mark.parametrize("seed", range(6))
