@@ -2,6 +2,10 @@
 
   lib/libsptc_gen.so  host-only workload generator (g++)
   lib/libsptc_hip.so  gfx950 kernels + C ABI (hipcc, --offload-arch=gfx950)
+  bin/gsparc          the reference-compatible command line (g++)
+  bin/main_block      the reference's main.cu SpTC block compiled unchanged
+                      against include/gsparc/ (only where the reference
+                      checkout is present)
 
 Every call recompiles everything (no incremental state), in parallel.
 """
@@ -16,7 +20,7 @@ CSRC = os.path.join(PKG_DIR, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 HIP_SOURCES = ["sort.hip", "segsort.hip", "kernels.hip", "batch.hip", "result.hip",
-               "device.cpp", "engine.cpp",
+               "device.cpp", "engine.cpp", "staticpart.cpp",
                "batch_engine.cpp", "serve.cpp", "api.cpp"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -85,8 +89,40 @@ def build_cli():
     bindir = os.path.join(PKG_DIR, "bin")
     os.makedirs(bindir, exist_ok=True)
     out = os.path.join(bindir, "gsparc")
-    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(REPO_DIR, "include"),
-          "-o", out, os.path.join(PKG_DIR, "cli", "gsparc_main.cpp"), "-L", LIB_DIR,
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-pthread", "-I",
+          os.path.join(REPO_DIR, "include"), "-o", out,
+          os.path.join(PKG_DIR, "cli", "gsparc_main.cpp"), "-L", LIB_DIR, "-lsptc_hip",
+          "-Wl,-rpath,$ORIGIN/../lib"])
+    return out
+
+
+REF_MAIN = "/root/reference/main.cu"
+
+
+def extract_main_block(path, dst):
+    """Writes the reference driver's SpTC iteration body -- from its
+    `if (tensor_x->nbits > 64)` to the line before `// delete SZ;` -- to dst
+    (the build tree) for tests/dropin/main_block.cpp to include."""
+    lines = open(path).read().splitlines(keepends=True)
+    b = next(i for i, l in enumerate(lines) if "if (tensor_x->nbits > 64)" in l)
+    e = next(i for i, l in enumerate(lines) if "// delete SZ;" in l)
+    with open(dst, "w") as f:
+        f.write("".join(lines[b:e]))
+    return b + 1, e
+
+
+def build_main_block(out=None, ref_main=REF_MAIN):
+    """bin/main_block: the reference's own main.cu SpTC block compiled
+    unchanged against include/gsparc/ and lib/libsptc_hip.so (needs the
+    reference checkout; the block itself stays in the build tree)."""
+    objdir = os.path.join(PKG_DIR, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    extract_main_block(ref_main, os.path.join(objdir, "main_block.inc"))
+    out = out or os.path.join(PKG_DIR, "bin", "main_block")
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    _run(["g++", "-O2", "-std=c++17", "-fopenmp", "-Wno-unknown-pragmas", "-I",
+          os.path.join(REPO_DIR, "include"), "-I", objdir, '-DMAIN_BLOCK="main_block.inc"',
+          "-o", out, os.path.join(REPO_DIR, "tests", "dropin", "main_block.cpp"), "-L", LIB_DIR,
           "-lsptc_hip", "-Wl,-rpath,$ORIGIN/../lib"])
     return out
 
@@ -96,11 +132,13 @@ def build_all():
     build_hip()
     build_cli()
     build_oracle()
+    if os.path.exists(REF_MAIN):
+        build_main_block()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
         {"all": build_all, "gen": build_gen, "hip": build_hip, "cli": build_cli,
-         "oracle": build_oracle}[w]()
+         "oracle": build_oracle, "main_block": build_main_block}[w]()
     print("built:", ", ".join(what))

@@ -1,198 +1,132 @@
 // gsparc -- command-line driver with the reference's flags and stderr lines
-// (cmdline_opts.cpp:9-171, main.cu:20-432), running the SpTC on the MI355X
-// engine through include/gsparc/sptc.hpp.
+// (cmdline_opts.cpp:9-171, main.cu:20-432), running main.cu's sequence on
+// the MI355X engine through the reference-shaped headers of
+// include/gsparc/: IOManager (.tns or quantum directory), TensorManager
+// (FindPartitionNum, ConvertTensor, SortSlitomXY, Partition,
+// convert_and_sort_64), then gsparc::SpTC, or for -m 1 the chained
+// multi_SpTC pair, iterated -n times.
 //
-//   gsparc -X x.tns -Y y.tns -c <nc> -x <modes...> -y <modes...> [-n iters]
-//          [-d dense_accumulator] [-g 1]
+//   gsparc -X x.tns -Y y.tns -c <nc> -x <modes...> -y <modes...>
+//          [-m 1 -a <nc2> -z <modes...>] [-q] [-n iters] [-d -1|0|1] [-g 1]
 //
-// .tns input: one nonzero per line, 1-based indices then the value; dims are
-// the observed maximum index + 1 (io_manager.tpp:74-259).  The quantum input
-// format (-q) and the chained second contraction (-m/-a/-z) are not part of
-// this engine and are rejected.
-#include <chrono>
+// The chained form contracts the first output's trailing nc2 modes (its Y
+// free modes) with modes -z of the second input (main.cu:323-371).
 #include <cstdio>
-#include <cstdlib>
-#include <cstring>
-#include <string>
+#include <memory>
+#include <vector>
 
 #include "gsparc/sptc.hpp"
 
-namespace {
-
-bool read_tns(const char* path, gsparc::SparseTensor<uint64_t, float>* t) {
-  FILE* f = std::fopen(path, "rb");
-  if (!f) return false;
-  std::fseek(f, 0, SEEK_END);
-  const long len = std::ftell(f);
-  std::fseek(f, 0, SEEK_SET);
-  std::string buf((size_t)(len > 0 ? len : 0), '\0');
-  const size_t got = std::fread(&buf[0], 1, buf.size(), f);
-  std::fclose(f);
-  buf.resize(got);
-  // order = fields on the first data line minus the value
-  size_t p = 0;
-  while (p < buf.size() && (buf[p] == '#' || buf[p] == '\n')) {
-    while (p < buf.size() && buf[p] != '\n') ++p;
-    ++p;
-  }
-  int fields = 0;
-  for (size_t q = p; q < buf.size() && buf[q] != '\n';) {
-    while (q < buf.size() && (buf[q] == ' ' || buf[q] == '\t' || buf[q] == '\r')) ++q;
-    if (q >= buf.size() || buf[q] == '\n') break;
-    ++fields;
-    while (q < buf.size() && !(buf[q] == ' ' || buf[q] == '\t' || buf[q] == '\n' || buf[q] == '\r')) ++q;
-  }
-  if (fields < 2 || fields - 1 > SPTC_MAX_ORDER) return false;
-  t->order = (unsigned short)(fields - 1);
-  t->indices.assign(t->order, {});
-  t->dims.assign(t->order, 0);
-  const char* s = buf.c_str();
-  const char* end = s + buf.size();
-  const char* c = s + p;
-  while (c < end) {
-    while (c < end && (*c == ' ' || *c == '\t' || *c == '\r' || *c == '\n')) ++c;
-    if (c >= end) break;
-    if (*c == '#') {
-      while (c < end && *c != '\n') ++c;
-      continue;
-    }
-    for (int m = 0; m < t->order; ++m) {
-      char* e;
-      const unsigned long long v = std::strtoull(c, &e, 10);
-      if (e == c || v == 0) return false;  // 1-based indices
-      t->indices[m].push_back(v - 1);
-      if (v > t->dims[m]) t->dims[m] = v;
-      c = e;
-    }
-    char* e;
-    const double v = std::strtod(c, &e);
-    if (e == c) return false;
-    t->values.push_back((float)v);
-    c = e;
-  }
-  t->nnz = t->values.size();
-  return true;
-}
-
-int usage() {
-  std::fprintf(stderr,
-               "usage: gsparc -X x.tns -Y y.tns -c <nc> -x <modes...> -y <modes...> "
-               "[-n iters] [-d -1|0|1] [-g 1]\n");
-  return 1;
-}
-
-}  // namespace
-
 int main(int argc, char** argv) {
-  std::string px, py;
-  int nc = -1, iters = 1, dense = 0, gpus = 1;
-  std::vector<int> cx, cy;
-  for (int i = 1; i < argc; ++i) {
-    const std::string a = argv[i];
-    auto next = [&]() -> const char* { return i + 1 < argc ? argv[++i] : nullptr; };
-    if (a == "-X" || a == "--X") {
-      const char* v = next();
-      if (!v) return usage();
-      px = v;
-      std::fprintf(stderr, "tensor X: %s\n", v);
-    } else if (a == "-Y" || a == "--Y") {
-      const char* v = next();
-      if (!v) return usage();
-      py = v;
-      std::fprintf(stderr, "tensor Y: %s\n", v);
-    } else if (a == "-c" || a == "--contract-mode") {
-      const char* v = next();
-      if (!v) return usage();
-      nc = std::atoi(v);
-    } else if (a == "-x" || a == "-y" || a == "--x" || a == "--y") {
-      if (nc < 0) return usage();
-      std::vector<int>& dst = (a.back() == 'x') ? cx : cy;
-      std::fprintf(stderr, "# contract modes of %c: %d, contract modes: ",
-                   a.back() == 'x' ? 'X' : 'Y', nc);
-      for (int k = 0; k < nc; ++k) {
-        const char* v = next();
-        if (!v) return usage();
-        dst.push_back(std::atoi(v));
-        std::fprintf(stderr, "%d ", dst.back());
-      }
-      std::fprintf(stderr, "\n");
-    } else if (a == "-n" || a == "--n") {
-      const char* v = next();
-      if (!v) return usage();
-      iters = std::atoi(v);
-    } else if (a == "-g" || a == "--g") {
-      const char* v = next();
-      if (!v) return usage();
-      gpus = std::atoi(v);
-      std::fprintf(stderr, "num_gpus: %d\n", gpus);
-    } else if (a == "-d" || a == "--d") {
-      const char* v = next();
-      if (!v) return usage();
-      dense = std::atoi(v);
-      std::fprintf(stderr, "dense_accumulator: %d\n", dense);
-    } else if (a == "-q" || a == "-m" || a == "-a" || a == "-z" || a == "-b") {
-      std::fprintf(stderr, "gsparc: %s (quantum input / chained contraction) is not supported\n",
-                   a.c_str());
-      return 1;
-    } else if (a == "-h" || a == "--help") {
-      return usage();
-    } else {
-      std::fprintf(stderr, "gsparc: unknown argument %s\n", a.c_str());
-      return usage();
-    }
-  }
-  if (px.empty() || py.empty() || nc < 0 || (int)cx.size() != nc || (int)cy.size() != nc ||
-      iters < 1)
-    return usage();
-  if (gpus != 1) {
+  using namespace gsparc;
+  CommandLineOptions options;
+  if (!options.Parse(argc, argv)) return 1;
+  if (options.get_gpu_count() != 1) {
     std::fprintf(stderr, "gsparc: this engine runs on one GPU (-g 1)\n");
     return 1;
   }
-  gsparc::SparseTensor<uint64_t, float> X, Y;
-  if (!read_tns(px.c_str(), &X) || !read_tns(py.c_str(), &Y)) {
-    std::fprintf(stderr, "gsparc: cannot read the input tensors\n");
-    return 1;
-  }
-  gsparc::Timers sum;
-  uint64_t result_nnz = 0;
+  using index_t = uint64_t;
+  using value_t = float;
+  using tensor_t = SparseTensor<index_t, value_t>;
+  using lindex_t = uint64_t;
+  using slitom_t = SLITOM<uint64_t, uint8_t, lindex_t, value_t>;
+  using tensor_manager_t = TensorManager<tensor_t, slitom_t>;
+
+  IOManager<tensor_t> io;
+  // ParseFromFile replaces the tensor it is given (io_manager.tpp:23-34)
+  auto parse = [&](const char* path, int number) {
+    std::unique_ptr<tensor_t> shell(new tensor_t(number));
+    tensor_t* t = shell.get();
+    io.ParseFromFile(path, &t, 1, options.get_quantum() != 0);
+    return std::unique_ptr<tensor_t>(t);
+  };
+  int rc = 0;
   try {
+    std::unique_ptr<tensor_t> X = parse(options.get_first_input_path(), 1);
+    std::unique_ptr<tensor_t> Y = parse(options.get_second_input_path(), 2);
+    std::unique_ptr<tensor_t> W;
+    if (options.is_multi_sptc()) W = parse(options.get_second_input_path(), 2);
+    tensor_t* tx = X.get();
+    tensor_t* ty = Y.get();
+    tensor_t* tz = W.get();
+
+    std::unique_ptr<CudaMemoryPool> pool_owner(new CudaMemoryPool(0));
+    CudaMemoryPool* pool = pool_owner.get();
+    CudaMemoryPool* pools[1] = {pool};
+    Timer slitom_timer, partition_timer;
+    Timer copy, idxmatch, dprtn, contraction;
+    Timer *copy_t[1] = {&copy}, *idx_t[1] = {&idxmatch}, *dprtn_t[1] = {&dprtn},
+          *contr_t[1] = {&contraction};
+    double reorder = 0.0;
+    uint64_t result_nnz = 0;
+    const int iters = options.get_num_iter();
     for (int it = 0; it < iters; ++it) {
-      // main.cu:293-376's sequence
-      gsparc::TensorManager tm(gpus);
-      tm.FindPartitionNum(X.nnz, Y.nnz, 0, 0);
-      gsparc::SLITOM SX(1), SY(2), SZ;
-      tm.ConvertTensor(&X, &SX, nc, cx.data());
-      tm.ConvertTensor(&Y, &SY, nc, cy.data());
-      tm.SortSlitomXY(&SX, &SY);
+      pool->reset();
+      tensor_manager_t tm(options.get_gpu_count());
+      const bool sort_gpu = tm.FindPartitionNum(tx->nnz, ty->nnz, pool->getPoolSize(), tx->nbits);
+      slitom_t SX(1, sort_gpu), SY(2, sort_gpu), SZ;
+      tm.ConvertTensor(tx, &SX, options.get_num_cmodes(), options.get_first_input_cmodes(),
+                       &slitom_timer);
+      tm.ConvertTensor(ty, &SY, options.get_num_cmodes(), options.get_second_input_cmodes(),
+                       &slitom_timer);
+      tm.SortSlitomXY(&SX, &SY, pools, &slitom_timer);
+      partition_timer.start();
       tm.Partition(&SX);
       tm.Partition(&SY);
-      gsparc::SpTC(&SX, &SY, &SZ, dense);
-      result_nnz += SZ.nnz;
-      const gsparc::Timers& t = SZ.timers;
-      sum.slitom += t.slitom;
-      sum.partition += t.partition;
-      sum.idxmatch += t.idxmatch;
-      sum.contraction += t.contraction;
-      sum.dynamic_partition += t.dynamic_partition;
-      sum.total += t.total;
+      partition_timer.stop();
+      if (SY.nprtn > 1) {
+        partition_timer.start();
+        tm.convert_and_sort_64(&SY, pools);
+        partition_timer.stop();
+      }
+      if (options.is_multi_sptc()) {
+        multi_SpTC<tensor_t, slitom_t>(&SX, &SY, &SZ, pools, 1, copy_t, idx_t, dprtn_t, contr_t);
+        const double first = slitom_timer.getTotalTime() + partition_timer.getTotalTime() +
+                             copy.getTotalTime() + idxmatch.getTotalTime() +
+                             contraction.getTotalTime() + dprtn.getTotalTime();
+        std::fprintf(stderr, "First SpTC time: %f\n", first);
+        // the intermediate stays on the device; its keys need no reordering
+        // (main.cu:352-358 rewrites them into the same order)
+        slitom_t Z;
+        tm.ConvertTensor(tz, &Z, options.get_add_num_cmodes(), options.get_third_cmodes(),
+                         &slitom_timer);
+        tm.SortSlitomXY(&SZ, &Z, pools, &slitom_timer);
+        multi_SpTC<tensor_t, slitom_t>(&SZ, &SY, &Z, pools, 1, copy_t, idx_t, dprtn_t, contr_t);
+        result_nnz += Z.nnz;
+      } else {
+        SpTC<tensor_t, slitom_t>(&SX, &SY, &SZ, pools, 1, copy_t, idx_t, dprtn_t, contr_t,
+                                 options.is_dense_accumulator());
+        result_nnz += SZ.nnz;
+      }
     }
-  } catch (const gsparc::Error& e) {
+    const double n = iters;
+    const double slitom_s = slitom_timer.getTotalTime(), part_s = partition_timer.getTotalTime();
+    const double copy_s = copy.getTotalTime(), idx_s = idxmatch.getTotalTime(),
+                 con_s = contraction.getTotalTime(), dp_s = dprtn.getTotalTime();
+    const double total = slitom_s + part_s + copy_s + idx_s + con_s + dp_s;
+    std::fprintf(stderr, "--------Break Down--------\n");
+    std::fprintf(stderr, "SLITOM time: %f\n", slitom_s / n);
+    std::fprintf(stderr, "Partition time: %f\n", part_s / n);
+    std::fprintf(stderr, "Idxmatch time: %f\n", idx_s / n);
+    std::fprintf(stderr, "Contraction time: %f\n", con_s / n);
+    std::fprintf(stderr, "Dynamic Partition time: %f\n", dp_s / n);
+    if (options.is_multi_sptc()) {
+      std::fprintf(stderr, "Bit reordering time: %f\n", reorder);
+      std::fprintf(stderr, "Bit reordering ratio: %f%%\n", total > 0 ? reorder / (total / n) * 100 : 0.0);
+    }
+    std::fprintf(stderr, "--------Elapsed Time--------\n");
+    std::fprintf(stderr, "Result NNZ: %llu\n", (unsigned long long)(result_nnz / iters));
+    std::fprintf(stderr, "SLITOM time: %f\n", slitom_s / n);
+    std::fprintf(stderr, "Partition time: %f\n", part_s / n);
+    std::fprintf(stderr, "SpTC(IdxMatch + Contraction + Data Copy) time: %f\n",
+                 (copy_s + idx_s + con_s + dp_s) / n);
+    std::fprintf(stderr, "Total time: %f\n", total / n);
+  } catch (const Error& e) {
     std::fprintf(stderr, "gsparc: %s (status %d)\n", e.what(), (int)e.status);
-    return 2;
+    rc = 2;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "gsparc: %s\n", e.what());
+    rc = 1;
   }
-  const double n = iters;
-  std::fprintf(stderr, "--------Break Down--------\n");
-  std::fprintf(stderr, "SLITOM time: %f\n", sum.slitom / n);
-  std::fprintf(stderr, "Partition time: %f\n", sum.partition / n);
-  std::fprintf(stderr, "Idxmatch time: %f\n", sum.idxmatch / n);
-  std::fprintf(stderr, "Contraction time: %f\n", sum.contraction / n);
-  std::fprintf(stderr, "Dynamic Partition time: %f\n", sum.dynamic_partition / n);
-  std::fprintf(stderr, "--------Elapsed Time--------\n");
-  std::fprintf(stderr, "Result NNZ: %llu\n", (unsigned long long)(result_nnz / iters));
-  std::fprintf(stderr, "SLITOM time: %f\n", sum.slitom / n);
-  std::fprintf(stderr, "Partition time: %f\n", sum.partition / n);
-  std::fprintf(stderr, "SpTC(IdxMatch + Contraction + Data Copy) time: %f\n",
-               (sum.idxmatch + sum.contraction + sum.dynamic_partition) / n);
-  std::fprintf(stderr, "Total time: %f\n", sum.total / n);
-  return 0;
+  return rc;
 }

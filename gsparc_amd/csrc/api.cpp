@@ -1,4 +1,5 @@
 // extern "C" boundary of libsptc_hip.so (declared in include/sptc.h).
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -8,6 +9,7 @@
 #include "engine.hpp"
 #include "result.hpp"
 #include "serve.hpp"
+#include "staticpart.hpp"
 
 using namespace sptc;
 
@@ -24,6 +26,12 @@ struct sptc_plan_s {
   sptc_opts opts;
   Operand x, y;
   double prepare_ms = 0;
+  // statically partitioned plan: host-side parts of both operands, staged
+  // and prepared pair by pair at each execution (x, y stay empty)
+  bool split = false;
+  Layout xl, yl;
+  std::vector<uint64_t> xdims, ydims;
+  OperandSplit xs, ys;
 };
 
 namespace {
@@ -128,21 +136,39 @@ sptc_status sptc_contract(const sptc_coo* X, const sptc_coo* Y, int32_t nc, cons
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   try {
     Engine e(s, o);
-    StagedCOO sx, sy;
-    Operand ox, oy;
-    e.record(0);
-    e.stage(*X, &sx);
-    e.stage(*Y, &sy);
-    e.record(1);
-    e.prepare(sx, xl, &ox);
-    e.prepare(sy, yl, &oy);
-    e.record(2);
-    st = e.contract(ox, oy, res);
+    int px = 1, py = 1;
+    if (!find_partition_num(X->nnz, X->order, Y->nnz, Y->order,
+                            std::max(X->idx_bytes, Y->idx_bytes), e.budget_avail(), &px, &py)) {
+      delete res;
+      return fail(SPTC_ERR_OOM_BUDGET, "operands need more than 64 parts each");
+    }
+    if (px > 1 || py > 1) {
+      OperandSplit xs, ys;
+      split_operand(*X, xl, px, false, &xs);
+      split_operand(*Y, yl, py, false, &ys);
+      st = contract_partitioned(e, *X, *Y, xl, yl, xs, ys, res);
+    } else {
+      Operand ox, oy;
+      e.record(0);
+      {
+        StagedCOO sx, sy;
+        e.stage(*X, &sx);
+        e.stage(*Y, &sy);
+        e.record(1);
+        if (o.check_indices) {
+          st = e.check_indices(sx, X->dims, "X");
+          if (st == SPTC_OK) st = e.check_indices(sy, Y->dims, "Y");
+        }
+        if (st == SPTC_OK) st = e.prepare_pair(sx, sy, xl, yl, &ox, &oy);
+        e.record(2);
+      }
+      if (st == SPTC_OK) st = e.contract(ox, oy, res);
+      if (st == SPTC_OK) std::memcpy(res->info.phase_ms, e.times.ms, sizeof(res->info.phase_ms));
+    }
     if (st != SPTC_OK) {
       delete res;
       return fail(st, e.msg);
     }
-    std::memcpy(res->info.phase_ms, e.times.ms, sizeof(res->info.phase_ms));
     // operand preparation ran before contract() collected: add its launches
     e.collect_kernel_stats(res->kstats);
     if (o.verbose) {
@@ -188,15 +214,46 @@ sptc_status sptc_plan(const sptc_coo* X, const sptc_coo* Y, int32_t nc, const in
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   try {
     Engine e(s, p->opts);
-    StagedCOO sx, sy;
-    e.record(0);
-    e.stage(*X, &sx);
-    e.stage(*Y, &sy);
-    e.prepare(sx, xl, &p->x);
-    e.prepare(sy, yl, &p->y);
-    e.record(1);
-    SPTC_HIP(hipStreamSynchronize(s));
-    p->prepare_ms = e.elapsed_ms(0, 1);
+    int px = 1, py = 1;
+    if (!find_partition_num(X->nnz, X->order, Y->nnz, Y->order,
+                            std::max(X->idx_bytes, Y->idx_bytes), e.budget_avail(), &px, &py)) {
+      delete p;
+      return fail(SPTC_ERR_OOM_BUDGET, "operands need more than 64 parts each");
+    }
+    if (px > 1 || py > 1) {
+      // the plan keeps host-side parts (owned copies); pairs are prepared
+      // on the device at each execution
+      const auto t0 = std::chrono::steady_clock::now();
+      p->split = true;
+      p->xl = xl;
+      p->yl = yl;
+      p->xdims.assign(X->dims, X->dims + X->order);
+      p->ydims.assign(Y->dims, Y->dims + Y->order);
+      split_operand(*X, xl, px, true, &p->xs);
+      split_operand(*Y, yl, py, true, &p->ys);
+      p->prepare_ms =
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    } else {
+      e.record(0);
+      {
+        StagedCOO sx, sy;
+        e.stage(*X, &sx);
+        e.stage(*Y, &sy);
+        sptc_status st2 = SPTC_OK;
+        if (p->opts.check_indices) {
+          st2 = e.check_indices(sx, X->dims, "X");
+          if (st2 == SPTC_OK) st2 = e.check_indices(sy, Y->dims, "Y");
+        }
+        if (st2 == SPTC_OK) st2 = e.prepare_pair(sx, sy, xl, yl, &p->x, &p->y);
+        if (st2 != SPTC_OK) {
+          delete p;
+          return fail(st2, e.msg);
+        }
+      }
+      e.record(1);
+      SPTC_HIP(hipStreamSynchronize(s));
+      p->prepare_ms = e.elapsed_ms(0, 1);
+    }
   } catch (const HipError& he) {
     delete p;
     return from_hip(he);
@@ -220,15 +277,22 @@ sptc_status sptc_execute(sptc_plan_t* p, const sptc_opts* opts, void* stream,
   std::memset(&res->info, 0, sizeof(res->info));
   try {
     Engine e(reinterpret_cast<hipStream_t>(stream), opts ? *opts : p->opts);
-    e.record(0);
-    e.record(1);
-    e.record(2);
-    const sptc_status st = e.contract(p->x, p->y, res);
+    sptc_status st;
+    if (p->split) {
+      sptc_coo X = p->xs.part[0].coo((int)p->xdims.size(), p->xdims.data());
+      sptc_coo Y = p->ys.part[0].coo((int)p->ydims.size(), p->ydims.data());
+      st = contract_partitioned(e, X, Y, p->xl, p->yl, p->xs, p->ys, res);
+    } else {
+      e.record(0);
+      e.record(1);
+      e.record(2);
+      st = e.contract(p->x, p->y, res);
+      if (st == SPTC_OK) std::memcpy(res->info.phase_ms, e.times.ms, sizeof(res->info.phase_ms));
+    }
     if (st != SPTC_OK) {
       delete res;
       return fail(st, e.msg);
     }
-    std::memcpy(res->info.phase_ms, e.times.ms, sizeof(res->info.phase_ms));
   } catch (const HipError& he) {
     delete res;
     return from_hip(he);
@@ -246,16 +310,53 @@ sptc_status sptc_execute(sptc_plan_t* p, const sptc_opts* opts, void* stream,
 sptc_status sptc_plan_get_info(const sptc_plan_t* p, sptc_plan_info* info) {
   if (!p || !info) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");
   std::memset(info, 0, sizeof(*info));
-  info->x_nnz = p->x.nnz;
-  info->y_nnz = p->y.nnz;
-  info->x_nfbits = p->x.lay.nfbits;
-  info->y_nfbits = p->y.lay.nfbits;
-  info->ncbits = p->x.lay.ncbits;
-  info->out_bits = p->x.lay.nfbits + p->y.lay.nfbits;
-  info->out_order = p->x.lay.nf + p->y.lay.nf;
+  const Layout& xl = p->split ? p->xl : p->x.lay;
+  const Layout& yl = p->split ? p->yl : p->y.lay;
+  info->x_nnz = p->split ? p->xs.nnz_lo.back() : p->x.nnz;
+  info->y_nnz = p->split ? p->ys.nnz_lo.back() : p->y.nnz;
+  info->x_nfbits = xl.nfbits;
+  info->y_nfbits = yl.nfbits;
+  info->ncbits = xl.ncbits;
+  info->out_bits = xl.nfbits + yl.nfbits;
+  info->out_order = xl.nf + yl.nf;
   info->device_bytes = p->x.keys.size() * 8 + p->x.vals.size() * 4 + p->y.keys.size() * 8 +
                        p->y.vals.size() * 4;
   info->prepare_ms = p->prepare_ms;
+  info->x_parts = p->split ? p->xs.parts : 1;
+  info->y_parts = p->split ? p->ys.parts : 1;
+  return SPTC_OK;
+}
+
+sptc_status sptc_plan_get_partitions(const sptc_plan_t* p, int32_t role, int32_t max_parts,
+                                     uint64_t* nnz_bounds, uint64_t* key_bounds,
+                                     int32_t* nparts) {
+  if (!p || !nparts || (role != 1 && role != 2))
+    return fail(SPTC_ERR_BAD_ARGS, "sptc_plan_get_partitions: bad arguments");
+  const OperandSplit* sp = p->split ? (role == 1 ? &p->xs : &p->ys) : nullptr;
+  const int32_t n = sp ? sp->parts : 1;
+  *nparts = n;
+  if (max_parts < n) return SPTC_OK;
+  const Layout& lay = role == 1 ? (p->split ? p->xl : p->x.lay) : (p->split ? p->yl : p->y.lay);
+  const uint64_t nnz = role == 1 ? (p->split ? p->xs.nnz_lo.back() : p->x.nnz)
+                                 : (p->split ? p->ys.nnz_lo.back() : p->y.nnz);
+  for (int32_t k = 0; k <= n; ++k) {
+    if (nnz_bounds) nnz_bounds[k] = sp ? sp->nnz_lo[k] : (k ? nnz : 0);
+    if (key_bounds)
+      key_bounds[k] = sp ? sp->key_lo[k] : (k ? (lay.nfbits >= 64 ? ~0ull : 1ull << lay.nfbits) : 0);
+  }
+  return SPTC_OK;
+}
+
+sptc_status sptc_find_partition_num(uint64_t nnz_x, int32_t order_x, uint64_t nnz_y,
+                                    int32_t order_y, int32_t idx_bytes, uint64_t budget_bytes,
+                                    int32_t* x_parts, int32_t* y_parts) {
+  if (!x_parts || !y_parts || order_x < 1 || order_y < 1 || (idx_bytes != 4 && idx_bytes != 8))
+    return fail(SPTC_ERR_BAD_ARGS, "sptc_find_partition_num: bad arguments");
+  int px = 1, py = 1;
+  if (!find_partition_num(nnz_x, order_x, nnz_y, order_y, idx_bytes, budget_bytes, &px, &py))
+    return fail(SPTC_ERR_OOM_BUDGET, "operands need more than 64 parts each");
+  *x_parts = px;
+  *y_parts = py;
   return SPTC_OK;
 }
 
@@ -263,6 +364,9 @@ sptc_status sptc_partition_plan(sptc_plan_t* p, uint64_t budget_bytes, int32_t m
                                 uint64_t* row_bounds, uint64_t* prod_bounds, int32_t* nparts,
                                 uint64_t* ir_nnz, void* stream) {
   if (!p || !nparts) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");
+  if (p->split)
+    return fail(SPTC_ERR_BAD_ARGS,
+                "the plan's operands are statically partitioned (sptc_plan_get_partitions)");
   sptc_opts o = p->opts;
   if (budget_bytes) o.mem_budget_bytes = budget_bytes;
   o.dense_accumulator = -1;
@@ -344,6 +448,10 @@ sptc_status sptc_result_copy_to_host(const sptc_result* r, uint64_t* keys, uint6
     }
   } catch (const HipError& he) {
     return from_hip(he);
+  } catch (const std::bad_alloc&) {
+    return fail(SPTC_ERR_OOM_BUDGET, "allocation failed");
+  } catch (const std::exception& ex) {
+    return fail(SPTC_ERR_INTERNAL, ex.what());
   }
   return SPTC_OK;
 }
@@ -355,6 +463,10 @@ sptc_status sptc_result_row_keys(const sptc_result* r, uint64_t* row_keys) {
       SPTC_HIP(hipMemcpy(row_keys, r->RK(), r->info.nrows * 8, hipMemcpyDeviceToHost));
   } catch (const HipError& he) {
     return from_hip(he);
+  } catch (const std::bad_alloc&) {
+    return fail(SPTC_ERR_OOM_BUDGET, "allocation failed");
+  } catch (const std::exception& ex) {
+    return fail(SPTC_ERR_INTERNAL, ex.what());
   }
   return SPTC_OK;
 }
@@ -392,6 +504,10 @@ sptc_status sptc_result_row_range(const sptc_result* r, uint64_t row, uint64_t* 
     *begin = *end = off;
   } catch (const HipError& he) {
     return from_hip(he);
+  } catch (const std::bad_alloc&) {
+    return fail(SPTC_ERR_OOM_BUDGET, "allocation failed");
+  } catch (const std::exception& ex) {
+    return fail(SPTC_ERR_INTERNAL, ex.what());
   }
   return SPTC_OK;
 }
@@ -425,6 +541,10 @@ sptc_status sptc_result_copy_range(const sptc_result* r, uint64_t begin, uint64_
     }
   } catch (const HipError& he) {
     return from_hip(he);
+  } catch (const std::bad_alloc&) {
+    return fail(SPTC_ERR_OOM_BUDGET, "allocation failed");
+  } catch (const std::exception& ex) {
+    return fail(SPTC_ERR_INTERNAL, ex.what());
   }
   return SPTC_OK;
 }
@@ -473,6 +593,10 @@ sptc_status sptc_result_decode_device(const sptc_result* r, uint32_t* const* mod
     }
   } catch (const HipError& he) {
     return from_hip(he);
+  } catch (const std::bad_alloc&) {
+    return fail(SPTC_ERR_OOM_BUDGET, "allocation failed");
+  } catch (const std::exception& ex) {
+    return fail(SPTC_ERR_INTERNAL, ex.what());
   }
   return SPTC_OK;
 }
@@ -505,7 +629,61 @@ sptc_status sptc_result_decode(const sptc_result* r, uint32_t* const* mode_idx, 
     }
   } catch (const HipError& he) {
     return from_hip(he);
+  } catch (const std::bad_alloc&) {
+    return fail(SPTC_ERR_OOM_BUDGET, "allocation failed");
+  } catch (const std::exception& ex) {
+    return fail(SPTC_ERR_INTERNAL, ex.what());
   }
+  return SPTC_OK;
+}
+
+sptc_status sptc_result_as_coo(sptc_result* r, void* stream, sptc_coo* out) {
+  if (!r || !out) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");
+  if (r->info.nnz && r->chunks.empty())
+    return fail(SPTC_ERR_BAD_ARGS, "a count-only result holds no entries");
+  DecodeArgs a;
+  if (!decode_args(r, &a))
+    return fail(SPTC_ERR_KEY_OVERFLOW, "an output mode is wider than 32 bits");
+  const int order = a.nx + a.ny;
+  if (order < 1) return fail(SPTC_ERR_BAD_ARGS, "the output has no modes");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  try {
+    if (r->coo_ptr.empty()) {
+      const uint64_t n = r->info.nnz;
+      r->coo_idx.resize(order);
+      r->coo_ptr.resize(order);
+      for (int m = 0; m < order; ++m) {
+        r->coo_idx[m].reset(n ? n : 1);
+        a.out[m] = r->coo_idx[m].get();
+        r->coo_ptr[m] = r->coo_idx[m].get();
+      }
+      r->coo_val.reset(n ? n : 1);
+      a.vals = r->coo_val.get();
+      uint64_t off = 0;
+      for (const OutChunk& c : r->chunks) {
+        launch_decode(c.R(), c.C(), c.V(), c.nnz, r->RK(), a, off, s);
+        off += c.nnz;
+      }
+      r->coo_dims.clear();
+      for (int i = 0; i < r->xl.nf; ++i) r->coo_dims.push_back(r->xl.dims[r->xl.fmodes[i]]);
+      for (int i = 0; i < r->yl.nf; ++i) r->coo_dims.push_back(r->yl.dims[r->yl.fmodes[i]]);
+      SPTC_HIP(hipStreamSynchronize(s));
+    }
+  } catch (const HipError& he) {
+    r->coo_ptr.clear();
+    return from_hip(he);
+  } catch (const std::exception& ex) {
+    r->coo_ptr.clear();
+    return fail(SPTC_ERR_INTERNAL, ex.what());
+  }
+  std::memset(out, 0, sizeof(*out));
+  out->order = order;
+  out->nnz = r->info.nnz;
+  out->dims = r->coo_dims.data();
+  out->idx = r->coo_ptr.data();
+  out->idx_bytes = 4;
+  out->val = r->coo_val.get();
+  out->on_device = 1;
   return SPTC_OK;
 }
 
@@ -534,6 +712,10 @@ sptc_status sptc_result_verify(const sptc_result* r, double* value_sum, uint64_t
     if (violations) *violations = bad;
   } catch (const HipError& he) {
     return from_hip(he);
+  } catch (const std::bad_alloc&) {
+    return fail(SPTC_ERR_OOM_BUDGET, "allocation failed");
+  } catch (const std::exception& ex) {
+    return fail(SPTC_ERR_INTERNAL, ex.what());
   }
   return SPTC_OK;
 }
@@ -587,6 +769,10 @@ sptc_status sptc_linearize(int32_t order, uint64_t nnz, const void* const* idx, 
     SPTC_HIP(hipGetLastError());
   } catch (const HipError& he) {
     return from_hip(he);
+  } catch (const std::bad_alloc&) {
+    return fail(SPTC_ERR_OOM_BUDGET, "allocation failed");
+  } catch (const std::exception& ex) {
+    return fail(SPTC_ERR_INTERNAL, ex.what());
   }
   return SPTC_OK;
 }
@@ -616,6 +802,10 @@ sptc_status sptc_sort_pairs(uint64_t* keys, float* vals, uint64_t n, int32_t nbi
     if (err) return fail(SPTC_ERR_INTERNAL, "look-back spin limit exceeded");
   } catch (const HipError& he) {
     return from_hip(he);
+  } catch (const std::bad_alloc&) {
+    return fail(SPTC_ERR_OOM_BUDGET, "allocation failed");
+  } catch (const std::exception& ex) {
+    return fail(SPTC_ERR_INTERNAL, ex.what());
   }
   return SPTC_OK;
 }
@@ -671,6 +861,10 @@ sptc_status sptc_index_match(const uint64_t* xkeys, uint64_t nx, const uint64_t*
     if (ir_nnz) *ir_nnz = t.a;
   } catch (const HipError& he) {
     return from_hip(he);
+  } catch (const std::bad_alloc&) {
+    return fail(SPTC_ERR_OOM_BUDGET, "allocation failed");
+  } catch (const std::exception& ex) {
+    return fail(SPTC_ERR_INTERNAL, ex.what());
   }
   return SPTC_OK;
 }
@@ -687,6 +881,10 @@ sptc_status sptc_exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t 
     SPTC_HIP(hipStreamSynchronize(s));
   } catch (const HipError& he) {
     return from_hip(he);
+  } catch (const std::bad_alloc&) {
+    return fail(SPTC_ERR_OOM_BUDGET, "allocation failed");
+  } catch (const std::exception& ex) {
+    return fail(SPTC_ERR_INTERNAL, ex.what());
   }
   return SPTC_OK;
 }
@@ -702,6 +900,10 @@ sptc_status sptc_device_info(int32_t* num_cus, uint64_t* free_bytes, uint64_t* t
     if (free_bytes) *free_bytes = device_free_bytes();
   } catch (const HipError& he) {
     return from_hip(he);
+  } catch (const std::bad_alloc&) {
+    return fail(SPTC_ERR_OOM_BUDGET, "allocation failed");
+  } catch (const std::exception& ex) {
+    return fail(SPTC_ERR_INTERNAL, ex.what());
   }
   return SPTC_OK;
 }
@@ -774,6 +976,10 @@ sptc_status sptc_batch_sync(sptc_batch* b, void* stream) {
     if (st != SPTC_OK) return fail(st, b->e.msg);
   } catch (const HipError& he) {
     return from_hip(he);
+  } catch (const std::bad_alloc&) {
+    return fail(SPTC_ERR_OOM_BUDGET, "allocation failed");
+  } catch (const std::exception& ex) {
+    return fail(SPTC_ERR_INTERNAL, ex.what());
   }
   return SPTC_OK;
 }
@@ -799,6 +1005,9 @@ sptc_status sptc_batch_result(sptc_batch* b, int32_t i, sptc_result** out) {
   } catch (const HipError& he) {
     delete r;
     return from_hip(he);
+  } catch (const std::exception& ex) {
+    delete r;
+    return fail(SPTC_ERR_INTERNAL, ex.what());
   }
   *out = r;
   return SPTC_OK;
@@ -832,6 +1041,10 @@ sptc_status sptc_server_create(const sptc_server_opts* opts, sptc_server** out) 
     *out = s;
   } catch (const HipError& he) {
     return from_hip(he);
+  } catch (const std::bad_alloc&) {
+    return fail(SPTC_ERR_OOM_BUDGET, "allocation failed");
+  } catch (const std::exception& ex) {
+    return fail(SPTC_ERR_INTERNAL, ex.what());
   }
   return SPTC_OK;
 }
@@ -846,6 +1059,10 @@ sptc_status sptc_server_add_tensor(sptc_server* s, const sptc_coo* t, int32_t* i
     if (st != SPTC_OK) return fail(st, m);
   } catch (const HipError& he) {
     return from_hip(he);
+  } catch (const std::bad_alloc&) {
+    return fail(SPTC_ERR_OOM_BUDGET, "allocation failed");
+  } catch (const std::exception& ex) {
+    return fail(SPTC_ERR_INTERNAL, ex.what());
   }
   return SPTC_OK;
 }
@@ -854,8 +1071,12 @@ sptc_status sptc_server_submit(sptc_server* s, int32_t x, int32_t y, int32_t nc,
                                const int32_t* cx, const int32_t* cy, uint64_t* request) {
   if (!s || !request || (nc && (!cx || !cy))) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");
   std::string m;
-  sptc_status st = s->srv->submit(x, y, nc, cx, cy, request, &m);
-  if (st != SPTC_OK) return fail(st, m);
+  try {
+    sptc_status st = s->srv->submit(x, y, nc, cx, cy, request, &m);
+    if (st != SPTC_OK) return fail(st, m);
+  } catch (const std::exception& ex) {
+    return fail(SPTC_ERR_INTERNAL, ex.what());
+  }
   return SPTC_OK;
 }
 
@@ -864,8 +1085,12 @@ sptc_status sptc_server_wait(sptc_server* s, uint64_t request, sptc_result** out
   if (!s) return fail(SPTC_ERR_BAD_ARGS, "NULL server");
   if (out) *out = nullptr;
   std::string m;
-  sptc_status st = s->srv->wait(request, out, info, &m);
-  if (st != SPTC_OK) return fail(st, m);
+  try {
+    sptc_status st = s->srv->wait(request, out, info, &m);
+    if (st != SPTC_OK) return fail(st, m);
+  } catch (const std::exception& ex) {
+    return fail(SPTC_ERR_INTERNAL, ex.what());
+  }
   return SPTC_OK;
 }
 

@@ -53,6 +53,10 @@ sptc_status BatchEngine::plan(int B, const sptc_coo* X, const sptc_coo* Y, const
     std::string m;
     sptc_status st = make_pair_layouts(X[i].order, X[i].dims, Y[i].order, Y[i].dims, nc[i],
                                        cx[i], cy[i], &xl_[i], &yl_[i], &m);
+    if (st == SPTC_OK && xl_[i].crank) {
+      st = SPTC_ERR_KEY_OVERFLOW;
+      m = "keys above 64 bits (contract ranks) run through sptc_contract / sptc_plan";
+    }
     if (st != SPTC_OK) {
       msg = "instance " + std::to_string(i) + ": " + m;
       return st;

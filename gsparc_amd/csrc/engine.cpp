@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <string>
+#include <vector>
 
 namespace sptc {
 
@@ -71,16 +73,41 @@ sptc_status make_pair_layouts(int xorder, const uint64_t* xdims, int yorder,
     cw[i] = std::max(mode_bits(xdims[cx[i]]), mode_bits(ydims[cy[i]]));
   }
   sptc_status st = make_layout(xorder, xdims, nc, cx, cw, 1, xl);
-  if (st != SPTC_OK) {
-    *msg = st == SPTC_ERR_KEY_OVERFLOW ? "X key needs more than 64 bits" : "bad X modes";
+  if (st == SPTC_ERR_BAD_ARGS) {
+    *msg = "bad X modes";
     return st;
   }
   st = make_layout(yorder, ydims, nc, cy, cw, 2, yl);
-  if (st != SPTC_OK) {
-    *msg = st == SPTC_ERR_KEY_OVERFLOW ? "Y key needs more than 64 bits" : "bad Y modes";
+  if (st == SPTC_ERR_BAD_ARGS) {
+    *msg = "bad Y modes";
     return st;
   }
+  if (xl->nbits > 64 || yl->nbits > 64) {
+    // keys above 64 bits: the contract field becomes a rank of the
+    // contracted coordinates (the free fields must fit 64 bits each)
+    if (xl->nfbits > 64 || yl->nfbits > 64) {
+      *msg = "a free-mode field needs more than 64 bits";
+      return SPTC_ERR_KEY_OVERFLOW;
+    }
+    for (int i = 0; i < nc; ++i)
+      if (cw[i] > 64) {
+        *msg = "a contracted mode needs more than 64 bits";
+        return SPTC_ERR_KEY_OVERFLOW;
+      }
+    xl->crank = yl->crank = 1;
+  }
   return SPTC_OK;
+}
+
+Layout rank_layout(const Layout& L, int rbits) {
+  Layout R = L;
+  R.crank = 1;
+  R.ncbits = rbits;
+  R.nbits = L.nfbits + rbits;
+  if (L.role == 1)
+    for (int i = 0; i < L.nf; ++i) R.shift[L.fmodes[i]] = L.shift[L.fmodes[i]] - L.ncbits + rbits;
+  for (int i = 0; i < L.nc; ++i) R.shift[L.cmodes[i]] = 0;  // not linearized directly
+  return R;
 }
 
 bool prefer_row_sort() {
@@ -137,6 +164,7 @@ Engine::Engine(hipStream_t s, const sptc_opts& o) : s_(s), o_(o) {
   // The budget caps what this call holds at once, on top of what was
   // already live when it started.
   const uint64_t live = DevicePool::get().live_bytes();
+  live0_ = live;
   if (o.mem_budget_bytes) {
     budget_ = live + o.mem_budget_bytes;
   } else {
@@ -196,7 +224,7 @@ void Engine::stage(const sptc_coo& t, StagedCOO* out) {
   out->val = reinterpret_cast<const float*>(out->own.back().get());
 }
 
-void Engine::prepare(const StagedCOO& t, const Layout& lay, Operand* out) {
+void Engine::prepare(const StagedCOO& t, const Layout& lay, Operand* out, const void* crank) {
   const uint64_t n = t.nnz;
   out->lay = lay;
   out->nnz = n;
@@ -205,11 +233,25 @@ void Engine::prepare(const StagedCOO& t, const Layout& lay, Operand* out) {
   if (!n) return;
   LinArgs a;
   std::memset(&a, 0, sizeof(a));
-  a.order = lay.order;
-  for (int m = 0; m < lay.order; ++m) {
-    a.shift[m] = lay.shift[m];
-    a.width[m] = lay.width[m];
-    a.idx[m] = t.idx[m];
+  if (lay.crank) {
+    // free modes, then the contract rank as one field
+    a.order = lay.nf + 1;
+    for (int j = 0; j < lay.nf; ++j) {
+      const int m = lay.fmodes[j];
+      a.shift[j] = lay.shift[m];
+      a.width[j] = lay.width[m];
+      a.idx[j] = t.idx[m];
+    }
+    a.shift[lay.nf] = lay.role == 1 ? 0 : lay.nfbits;
+    a.width[lay.nf] = lay.ncbits;
+    a.idx[lay.nf] = crank;
+  } else {
+    a.order = lay.order;
+    for (int m = 0; m < lay.order; ++m) {
+      a.shift[m] = lay.shift[m];
+      a.width[m] = lay.width[m];
+      a.idx[m] = t.idx[m];
+    }
   }
   SPTC_HIP(hipMemcpyAsync(out->vals.get(), t.val, n * sizeof(float),
                           hipMemcpyDeviceToDevice, s_));
@@ -228,6 +270,112 @@ void Engine::prepare(const StagedCOO& t, const Layout& lay, Operand* out) {
     std::swap(out->keys, ws_.alt_keys);
     std::swap(out->vals, ws_.alt_vals);
   }
+}
+
+sptc_status Engine::check_indices(const StagedCOO& t, const uint64_t* dims, const char* name) {
+  if (!t.nnz) return SPTC_OK;
+  DevBuf<uint64_t> mx(t.order);
+  fill_u64(mx.get(), 0, t.order, s_);
+  for (int m = 0; m < t.order; ++m) launch_mode_max(t.idx[m], t.idx_bytes, t.nnz, mx.get() + m, s_);
+  std::vector<uint64_t> h(t.order);
+  SPTC_HIP(hipMemcpyAsync(h.data(), mx.get(), t.order * 8, hipMemcpyDeviceToHost, s_));
+  SPTC_HIP(hipStreamSynchronize(s_));
+  for (int m = 0; m < t.order; ++m)
+    if (h[m] >= dims[m]) {
+      msg = std::string(name) + ": mode " + std::to_string(m) + " has index " +
+            std::to_string(h[m]) + ", not below its dim " + std::to_string(dims[m]);
+      return SPTC_ERR_BAD_ARGS;
+    }
+  return SPTC_OK;
+}
+
+uint64_t Engine::contract_ranks(const StagedCOO& sx, const StagedCOO& sy, const Layout& xl,
+                                const Layout& yl, DevBuf<uint64_t>* rank) {
+  const uint64_t nx = sx.nnz, ny = sy.nnz, n = nx + ny;
+  rank->reset(n ? n : 1);
+  if (!n) return 0;
+  fill_u64(rank->get(), 0, n, s_);
+  const int nc = xl.nc;
+  if (!nc) return 1;
+  DevBuf<uint64_t> key(n), sorted(n);
+  DevBuf<float> payload(n);  // moved with the keys, never read
+  DevBuf<uint64_t> uniq(n);
+  DevBuf<U2> tmp(scan_tmp_count(n) + 1);
+  uint64_t U = 1;
+  int rbits = 0;
+  bool prev = false;
+  for (int pos = 0; pos < nc;) {
+    // the next group of contracted modes that fits 64 bits above the rank
+    int W = 0, end = pos;
+    while (end < nc && rbits + W + xl.width[xl.cmodes[end]] <= 64) W += xl.width[xl.cmodes[end++]];
+    if (end == pos) {
+      msg = "contracted coordinates do not fit a 64-bit rank key";
+      return 0;
+    }
+    for (int side = 0; side < 2; ++side) {
+      const Layout& L = side ? yl : xl;
+      const StagedCOO& t = side ? sy : sx;
+      TupleArgs ta;
+      std::memset(&ta, 0, sizeof(ta));
+      int sh = W;
+      for (int j = pos; j < end; ++j) {
+        const int m = L.cmodes[j];
+        sh -= L.width[m];
+        ta.idx[j - pos] = t.idx[m];
+        ta.width[j - pos] = L.width[m];
+        ta.shift[j - pos] = sh;
+      }
+      ta.n = end - pos;
+      ta.bytes = t.idx_bytes;
+      ta.prev = prev ? rank->get() + (side ? nx : 0) : nullptr;
+      ta.prev_shift = W;
+      launch_tuple_key(ta, side ? ny : nx, key.get() + (side ? nx : 0), s_);
+    }
+    const int bits = rbits + W;
+    SPTC_HIP(hipMemcpyAsync(sorted.get(), key.get(), n * 8, hipMemcpyDeviceToDevice, s_));
+    const int rb = rb_ ? rb_ : choose_radix_bits(bits);
+    ws_.ensure(n, rb, s_);
+    ws_.prepare(radix_passes(bits, rb), s_);
+    if (sort_needs_histogram()) histogram(ws_, sorted.get(), n, radix_passes(bits, rb), s_);
+    const bool alt = sort_after_histogram(ws_, sorted.get(), payload.get(), n, bits, s_);
+    run_unique(alt ? ws_.alt_keys.get() : sorted.get(), n, uniq.get(), tmp.get(), s_);
+    U = read_total(tmp.get() + scan_tmp_count(n)).a;
+    launch_yrank(key.get(), n, 64, ~0ull, uniq.get(), U, 0, rank->get(), s_);
+    rbits = mode_bits(U);
+    prev = true;
+    pos = end;
+  }
+  return U;
+}
+
+sptc_status Engine::prepare_pair(const StagedCOO& sx, const StagedCOO& sy, const Layout& xl,
+                                 const Layout& yl, Operand* ox, Operand* oy) {
+  if (!xl.crank) {
+    prepare(sx, xl, ox);
+    prepare(sy, yl, oy);
+    return SPTC_OK;
+  }
+  DevBuf<uint64_t> rank;
+  const uint64_t U = contract_ranks(sx, sy, xl, yl, &rank);
+  if (!U && sx.nnz + sy.nnz) return SPTC_ERR_KEY_OVERFLOW;
+  const int rb = mode_bits(U);
+  if (xl.nfbits + rb > 64 || yl.nfbits + rb > 64) {
+    msg = "a free-mode field and the contract rank need more than 64 bits";
+    return SPTC_ERR_KEY_OVERFLOW;
+  }
+  const uint64_t nx = sx.nnz, n = sx.nnz + sy.nnz;
+  DevBuf<uint32_t> r32;
+  if (n && (sx.idx_bytes == 4 || sy.idx_bytes == 4)) {
+    r32.reset(n);
+    launch_ycol(rank.get(), n, 0xFFFFFFFFull, r32.get(), s_);
+  }
+  const void* rx = sx.idx_bytes == 4 ? (const void*)r32.get() : (const void*)rank.get();
+  const void* ry =
+      sy.idx_bytes == 4 ? (const void*)(r32.get() + nx) : (const void*)(rank.get() + nx);
+  prepare(sx, rank_layout(xl, rb), ox, rx);
+  prepare(sy, rank_layout(yl, rb), oy, ry);
+  SPTC_HIP(hipStreamSynchronize(s_));  // the rank buffers are freed on return
+  return SPTC_OK;
 }
 
 sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* res) {

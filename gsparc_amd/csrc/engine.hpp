@@ -24,6 +24,11 @@ struct Layout {
   int nf = 0, fmodes[SPTC_MAX_ORDER] = {0};
   int nfbits = 0, ncbits = 0, nbits = 0;
   int role = 1;
+  // 1: the packed contract field needs more than 64 bits with the free
+  // field, so the contract field holds the rank of the contracted
+  // coordinates among those of both operands (ncbits = bits(#distinct)),
+  // set when the operands are prepared (rank_layout)
+  int crank = 0;
 };
 
 int mode_bits(uint64_t d);
@@ -36,6 +41,10 @@ sptc_status make_layout(int order, const uint64_t* dims, int nc, const int32_t* 
 sptc_status make_pair_layouts(int xorder, const uint64_t* xdims, int yorder,
                               const uint64_t* ydims, int nc, const int32_t* cx,
                               const int32_t* cy, Layout* xl, Layout* yl, std::string* msg);
+
+// L with its contract field replaced by a rank of rbits bits (free fields
+// keep their place relative to it).
+Layout rank_layout(const Layout& L, int rbits);
 
 // A linearized, sorted operand resident in device memory.
 struct Operand {
@@ -74,6 +83,12 @@ struct sptc_result_s {
   // ESC chunk plan: first X row and first product of each chunk, then the
   // totals (empty for the dense accumulator)
   std::vector<uint64_t> part_rows, part_prods;
+  // the output as a device COO operand (sptc_result_as_coo): per-mode
+  // indices, values and dims, built on first request
+  std::vector<sptc::DevBuf<uint32_t>> coo_idx;
+  sptc::DevBuf<float> coo_val;
+  std::vector<const void*> coo_ptr;
+  std::vector<uint64_t> coo_dims;
   const uint64_t* RK() const { return vrow_key ? vrow_key : row_key.get(); }
 };
 
@@ -109,11 +124,27 @@ class Engine {
   ~Engine();
   // Copies host-resident COO arrays to the device (no-op for device ones).
   void stage(const sptc_coo& t, StagedCOO* out);
-  // Linearize and sort one operand.
-  void prepare(const StagedCOO& t, const Layout& lay, Operand* out);
+  // SPTC_ERR_BAD_ARGS (detail in msg) when an index of t is not below its
+  // mode's dim (the linearization would mask it into another coordinate).
+  sptc_status check_indices(const StagedCOO& t, const uint64_t* dims, const char* name);
+  // Linearize and sort one operand.  With lay.crank, `crank` holds each
+  // nonzero's contract rank (index width of t) in place of the contracted
+  // modes.
+  void prepare(const StagedCOO& t, const Layout& lay, Operand* out,
+               const void* crank = nullptr);
+  // Both operands of a contraction: for layouts with crank set, first the
+  // ranks of the contracted coordinates over X and Y together (one sort
+  // per group of modes that fits 64 bits), then each operand with its
+  // contract field replaced by the rank.  Returns SPTC_ERR_KEY_OVERFLOW
+  // (detail in msg) when a free field and the rank do not fit 64 bits.
+  sptc_status prepare_pair(const StagedCOO& sx, const StagedCOO& sy, const Layout& xl,
+                           const Layout& yl, Operand* ox, Operand* oy);
   // Match + partition + accumulate.  X and Y are prepared operands.
   sptc_status contract(const Operand& X, const Operand& Y, sptc_result_s* res);
   uint64_t budget() const { return budget_; }
+  const sptc_opts& options() const { return o_; }
+  // budget bytes above what was live when the engine was made
+  uint64_t budget_avail() const { return budget_ - live0_; }
   hipStream_t stream() const { return s_; }
   void record(int i);
   // ms between two recorded events (waits for the later one)
@@ -128,9 +159,14 @@ class Engine {
 
  private:
   U2 read_total(const U2* dev);
+  // Ranks of the contracted coordinates of X then Y in *rank (nx + ny);
+  // returns the number of distinct tuples (0 and msg set on failure).
+  uint64_t contract_ranks(const StagedCOO& sx, const StagedCOO& sy, const Layout& xl,
+                          const Layout& yl, DevBuf<uint64_t>* rank);
   hipStream_t s_;
   sptc_opts o_;
   uint64_t budget_ = 0;
+  uint64_t live0_ = 0;
   int rb_ = 8;
   SortWorkspace ws_;
   KTimer kt_;

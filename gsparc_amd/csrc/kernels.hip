@@ -366,6 +366,54 @@ __global__ __launch_bounds__(kBlock) void k_yrank(const uint64_t* __restrict__ y
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_tuple_key(TupleArgs a, uint64_t n,
+                                                      uint64_t* __restrict__ out) {
+  uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (; i < n; i += stride) {
+    uint64_t k = a.prev ? shl64(a.prev[i], a.prev_shift) : 0;
+    for (int m = 0; m < a.n; ++m) {
+      const uint64_t v = a.bytes == 4 ? (uint64_t) static_cast<const uint32_t*>(a.idx[m])[i]
+                                      : static_cast<const uint64_t*>(a.idx[m])[i];
+      k |= shl64(v & low_mask(a.width[m]), a.shift[m]);
+    }
+    out[i] = k;
+  }
+}
+
+// Largest index of one mode: a grid-stride max, a wave reduction, one
+// atomicMax per wave into *out (cleared by the caller).
+__global__ __launch_bounds__(kBlock) void k_mode_max(const void* __restrict__ idx, int bytes,
+                                                     uint64_t n,
+                                                     unsigned long long* __restrict__ out) {
+  uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  unsigned long long m = 0;
+  for (; i < n; i += stride) {
+    const unsigned long long v = bytes == 4 ? (unsigned long long)static_cast<const uint32_t*>(idx)[i]
+                                            : static_cast<const uint64_t*>(idx)[i];
+    m = v > m ? v : m;
+  }
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    const unsigned long long o = __shfl_down(m, d, kWave);
+    m = o > m ? o : m;
+  }
+  if (lane_id() == 0) atomicMax(out, m);
+}
+
+void launch_mode_max(const void* idx, int bytes, uint64_t n, uint64_t* out, hipStream_t s) {
+  if (!n) return;
+  uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(n, kBlock * 8), 2048);
+  hipLaunchKernelGGL(k_mode_max, dim3(grid), dim3(kBlock), 0, s, idx, bytes, n,
+                     reinterpret_cast<unsigned long long*>(out));
+}
+
+void launch_tuple_key(const TupleArgs& a, uint64_t n, uint64_t* out, hipStream_t s) {
+  if (!n) return;
+  uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(n, kBlock), 4096);
+  hipLaunchKernelGGL(k_tuple_key, dim3(grid), dim3(kBlock), 0, s, a, n, out);
+}
+
 void launch_yfree_extract(const uint64_t* ykeys, uint64_t ny, uint64_t fmask, uint64_t* out,
                           hipStream_t s) {
   if (!ny) return;

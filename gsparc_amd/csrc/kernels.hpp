@@ -16,6 +16,18 @@ struct LinArgs {
   int order;
 };
 
+// Packed key of a group of contracted coordinates, above the previous
+// group's rank: key = prev << prev_shift | OR_j (idx_j & (2^w_j-1)) << s_j.
+struct TupleArgs {
+  const void* idx[SPTC_MAX_ORDER];
+  int shift[SPTC_MAX_ORDER];
+  int width[SPTC_MAX_ORDER];
+  int n;
+  int bytes;               // 4 or 8 (index arrays)
+  const uint64_t* prev;    // NULL for the first group
+  int prev_shift;
+};
+
 struct YTable {
   int direct;
   const uint64_t* lo;      // direct: [2^ncbits]
@@ -138,6 +150,9 @@ void run_reduce_emit_narrow(const NarrowRows& nr, const float* vals, uint64_t n,
 void launch_yfree_extract(const uint64_t* ykeys, uint64_t ny, uint64_t fmask, uint64_t* out,
                           hipStream_t s);
 void run_unique(const uint64_t* sorted, uint64_t n, uint64_t* out, U2* tmp, hipStream_t s);
+void launch_tuple_key(const TupleArgs& a, uint64_t n, uint64_t* out, hipStream_t s);
+// *out = max(*out, largest of the n indices); *out cleared by the caller.
+void launch_mode_max(const void* idx, int bytes, uint64_t n, uint64_t* out, hipStream_t s);
 // ycol[i] = ykeys[i] & fmask (a column field of at most 32 bits).
 void launch_ycol(const uint64_t* ykeys, uint64_t ny, uint64_t fmask, uint32_t* ycol,
                  hipStream_t s);

@@ -80,12 +80,75 @@ __global__ void __launch_bounds__(kBlock) k_decode(const uint32_t* __restrict__ 
   }
 }
 
+__global__ void __launch_bounds__(kBlock) k_row_extent(const uint32_t* __restrict__ row,
+                                                       uint64_t n, uint64_t base,
+                                                       uint64_t* __restrict__ start,
+                                                       uint64_t* __restrict__ end) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const uint32_t r = row[i];
+    if (i == 0 || row[i - 1] != r) start[r] = base + i;
+    if (i + 1 == n || row[i + 1] != r) end[r] = base + i + 1;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_interleave_counts(ExtentSet e, uint64_t R,
+                                                              uint64_t* __restrict__ cnt) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  const uint64_t n = R * (uint64_t)e.Q;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const uint64_t r = i / (uint64_t)e.Q;
+    const int q = (int)(i - r * (uint64_t)e.Q);
+    cnt[i] = e.end[q][r] - e.start[q][r];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_merge_scatter(
+    const uint32_t* __restrict__ row, const uint64_t* __restrict__ col,
+    const float* __restrict__ val, uint64_t n, uint64_t base, const uint64_t* __restrict__ start,
+    const uint64_t* __restrict__ dst_base, int Q, int q, uint32_t row_off,
+    uint32_t* __restrict__ orow, uint64_t* __restrict__ ocol, float* __restrict__ oval) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const uint32_t r = row[i];
+    const uint64_t d = dst_base[(uint64_t)r * Q + q] + (base + i - start[r]);
+    orow[d] = r + row_off;
+    ocol[d] = col[i];
+    oval[d] = val[i];
+  }
+}
+
 uint32_t grid_for(uint64_t n, uint32_t cap) {
   const uint64_t g = ceil_div(n, (uint64_t)kBlock * kVerifyItems);
   return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
 }
 
 }  // namespace
+
+void launch_row_extent(const uint32_t* row, uint64_t n, uint64_t base, uint64_t* start,
+                       uint64_t* end, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_row_extent, dim3(grid_for(n, 8192)), dim3(kBlock), 0, s, row, n, base,
+                     start, end);
+  SPTC_HIP(hipGetLastError());
+}
+
+void launch_interleave_counts(const ExtentSet& e, uint64_t R, uint64_t* cnt, hipStream_t s) {
+  if (!R || e.Q < 1 || e.Q > kMaxMergeParts) return;
+  hipLaunchKernelGGL(k_interleave_counts, dim3(grid_for(R * (uint64_t)e.Q, 8192)), dim3(kBlock),
+                     0, s, e, R, cnt);
+  SPTC_HIP(hipGetLastError());
+}
+
+void launch_merge_scatter(const uint32_t* row, const uint64_t* col, const float* val,
+                          uint64_t n, uint64_t base, const uint64_t* start,
+                          const uint64_t* dst_base, int Q, int q, uint32_t row_off,
+                          uint32_t* orow, uint64_t* ocol, float* oval, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_merge_scatter, dim3(grid_for(n, 8192)), dim3(kBlock), 0, s, row, col,
+                     val, n, base, start, dst_base, Q, q, row_off, orow, ocol, oval);
+  SPTC_HIP(hipGetLastError());
+}
 
 uint32_t verify_grid(uint64_t n) { return grid_for(n, kVerifyMaxBlocks); }
 

@@ -141,6 +141,10 @@ sptc_status Server::submit(int32_t x, int32_t y, int32_t nc, const int32_t* cx,
   const sptc_coo& Y = tensors_[y]->desc;
   sptc_status st = make_pair_layouts(X.order, X.dims, Y.order, Y.dims, nc, cx, cy, &r->xl,
                                      &r->yl, msg);
+  if (st == SPTC_OK && r->xl.crank) {
+    *msg = "keys above 64 bits (contract ranks) run through sptc_contract / sptc_plan";
+    st = SPTC_ERR_KEY_OVERFLOW;
+  }
   if (st != SPTC_OK) return st;
   r->x = x;
   r->y = y;

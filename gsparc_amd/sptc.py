@@ -39,7 +39,7 @@ class _COO(Structure):
 class _Opts(Structure):
     _fields_ = [("dense_accumulator", c_int32), ("count_only", c_int32),
                 ("mem_budget_bytes", c_uint64), ("radix_bits", c_int32),
-                ("verbose", c_int32), ("kernel_timing", c_int32)]
+                ("verbose", c_int32), ("kernel_timing", c_int32), ("check_indices", c_int32)]
 
 
 KERNEL_CLASSES = ["linearize", "sort_input", "match", "expand", "sort_products", "reduce",
@@ -65,7 +65,8 @@ class _BatchInfo(Structure):
 class _PlanInfo(Structure):
     _fields_ = [("x_nnz", c_uint64), ("y_nnz", c_uint64), ("x_nfbits", c_int32),
                 ("y_nfbits", c_int32), ("ncbits", c_int32), ("out_bits", c_int32),
-                ("out_order", c_int32), ("device_bytes", c_uint64), ("prepare_ms", c_double)]
+                ("out_order", c_int32), ("device_bytes", c_uint64), ("prepare_ms", c_double),
+                ("x_parts", c_int32), ("y_parts", c_int32)]
 
 
 class _ServerOpts(Structure):
@@ -128,6 +129,11 @@ def lib():
         L.sptc_status_string.restype = c_char_p
         L.sptc_last_error.argtypes = []
         L.sptc_last_error.restype = c_char_p
+        L.sptc_plan_get_partitions.argtypes = [c_void_p, c_int32, c_int32, c_void_p,
+                                               c_void_p, POINTER(c_int32)]
+        L.sptc_find_partition_num.argtypes = [c_uint64, c_int32, c_uint64, c_int32, c_int32,
+                                              c_uint64, POINTER(c_int32), POINTER(c_int32)]
+        L.sptc_result_as_coo.argtypes = [c_void_p, c_void_p, POINTER(_COO)]
         L.sptc_trim_cache.argtypes = []
         L.sptc_trim_cache.restype = None
         L.sptc_result_kernel_stats.argtypes = [c_void_p, c_int32, POINTER(c_uint64),
@@ -202,6 +208,20 @@ def layout(dims, cmodes, role, cwidth=None):
                        byref(nf), byref(nc))
     check(st)
     return list(sh), list(wd), nf.value, nc.value
+
+
+class _ResultCOO:
+    """A result used as an operand: its device COO view (sptc_result_as_coo),
+    kept alive with the result."""
+
+    def __init__(self, r):
+        self.result = r
+        self.s = _COO()
+        check(lib().sptc_result_as_coo(r._h, None, byref(self.s)))
+
+
+def _operand(t, on_device):
+    return _ResultCOO(t) if isinstance(t, Result) else _HostCOO(t, on_device)
 
 
 class _HostCOO:
@@ -338,21 +358,24 @@ class Result:
 
 def SpTC(X, Y, cmodes_x, cmodes_y, dense_accumulator=0, count_only=False,
          mem_budget_bytes=0, radix_bits=0, verbose=False, stream=None, on_device=False,
-         kernel_timing=False):
+         kernel_timing=False, check_indices=False):
     """Z = X x_{cmodes_x, cmodes_y} Y on the GPU; returns a :class:`Result`.
 
-    X and Y are COOTensor (host numpy arrays) or, with on_device=True,
-    objects with .dims, .idx (torch device tensors) and .val."""
+    X and Y are COOTensor (host numpy arrays), with on_device=True objects
+    with .dims, .idx (torch device tensors) and .val, or the Result of an
+    earlier contraction, used from device memory (chained contractions:
+    its modes are the X free modes, then the Y free modes)."""
     L = lib()
-    hx = _HostCOO(X, on_device)
-    hy = X is Y and hx or _HostCOO(Y, on_device)
+    hx = _operand(X, on_device)
+    hy = X is Y and hx or _operand(Y, on_device)
     nc = len(cmodes_x)
     if len(cmodes_y) != nc:
         raise SpTCError(ERR_BAD_ARGS, "contracted mode lists differ in length")
     cx = (c_int32 * max(1, nc))(*cmodes_x)
     cy = (c_int32 * max(1, nc))(*cmodes_y)
     o = _Opts(int(dense_accumulator), int(bool(count_only)), int(mem_budget_bytes),
-              int(radix_bits), int(bool(verbose)), int(bool(kernel_timing)))
+              int(radix_bits), int(bool(verbose)), int(bool(kernel_timing)),
+              int(bool(check_indices)))
     out = c_void_p()
     st = L.sptc_contract(byref(hx.s), byref(hy.s), nc, cx, cy, byref(o),
                          c_void_p(stream or 0), byref(out))
@@ -368,17 +391,17 @@ class Plan:
 
     def __init__(self, X, Y, cmodes_x, cmodes_y, dense_accumulator=0, count_only=False,
                  mem_budget_bytes=0, radix_bits=0, kernel_timing=False, stream=None,
-                 on_device=False):
+                 on_device=False, check_indices=False):
         L = lib()
-        hx = _HostCOO(X, on_device)
-        hy = X is Y and hx or _HostCOO(Y, on_device)
+        hx = _operand(X, on_device)
+        hy = X is Y and hx or _operand(Y, on_device)
         nc = len(cmodes_x)
         if len(cmodes_y) != nc:
             raise SpTCError(ERR_BAD_ARGS, "contracted mode lists differ in length")
         cx = (c_int32 * max(1, nc))(*cmodes_x)
         cy = (c_int32 * max(1, nc))(*cmodes_y)
         o = _Opts(int(dense_accumulator), int(bool(count_only)), int(mem_budget_bytes),
-                  int(radix_bits), 0, int(bool(kernel_timing)))
+                  int(radix_bits), 0, int(bool(kernel_timing)), int(bool(check_indices)))
         h = c_void_p()
         check(L.sptc_plan(byref(hx.s), byref(hy.s), nc, cx, cy, byref(o),
                           c_void_p(stream or 0), byref(h)))
@@ -408,6 +431,18 @@ class Plan:
         out = c_void_p()
         check(lib().sptc_execute(self._h, byref(o), c_void_p(stream or 0), byref(out)))
         return Result(out.value)
+
+    def static_partitions(self, role):
+        """(nnz_bounds, key_bounds) of the plan's static parts of X (role 1)
+        or Y (role 2): nonzeros before each part and its first free key."""
+        L = lib()
+        n = c_int32()
+        check(L.sptc_plan_get_partitions(self._h, int(role), 0, None, None, byref(n)))
+        nb = np.zeros(n.value + 1, np.uint64)
+        kb = np.zeros(n.value + 1, np.uint64)
+        check(L.sptc_plan_get_partitions(self._h, int(role), n.value, nb.ctypes.data,
+                                         kb.ctypes.data, byref(n)))
+        return nb, kb
 
     def partitions(self, budget_bytes=0, stream=None):
         """(row_bounds, prod_bounds, ir_nnz): first X row and first product
@@ -547,6 +582,16 @@ class Server:
         st = _ServerStats()
         check(lib().sptc_server_get_stats(self._h, byref(st)))
         return {k: getattr(st, k) for k, _ in _ServerStats._fields_}
+
+
+def find_partition_num(nnz_x, order_x, nnz_y, order_y, idx_bytes, budget_bytes):
+    """(x_parts, y_parts) the engine splits operands of these sizes into
+    under a device budget (FindPartitionNum); (1, 1) when both fit whole."""
+    px, py = c_int32(), c_int32()
+    check(lib().sptc_find_partition_num(int(nnz_x), int(order_x), int(nnz_y), int(order_y),
+                                        int(idx_bytes), int(budget_bytes), byref(px),
+                                        byref(py)))
+    return px.value, py.value
 
 
 def pool_stats():

@@ -69,6 +69,12 @@ typedef struct {
   int32_t verbose;           /* 1: phase log on stderr                    */
   int32_t kernel_timing;     /* 1: HIP-event time every hot-kernel launch
                                 (see sptc_result_kernel_stats)          */
+  int32_t check_indices;     /* 1: one device pass per mode checks that
+                                every index is below its dim, else
+                                SPTC_ERR_BAD_ARGS; 0: an index >= its dim
+                                is masked to the mode's bit width, as the
+                                reference's pdep linearization does
+                                (tensor_manager.tpp:246-294)            */
 } sptc_opts;
 
 /* Result of a contraction (device-resident).  Entry i has output free
@@ -153,6 +159,17 @@ sptc_status sptc_result_decode(const sptc_result* r, uint32_t* const* mode_idx,
  * is wider than 32 bits (as for sptc_result_decode). */
 sptc_status sptc_result_decode_device(const sptc_result* r, uint32_t* const* mode_idx,
                                       float* vals, void* stream);
+
+/* The output as a device-resident COO operand, for chaining contractions
+ * without leaving the GPU (the reference's multi_SpTC keeps Z resident and
+ * feeds it to the next SpTC, sptc.cuh:362-698, main.cu:323-371): order =
+ * out_order, dims = the X free modes' dims then the Y free modes', 0-based
+ * uint32 indices and FP32 values in device buffers owned by the result
+ * (built by one decode on the first call, freed with the result).  *out can
+ * be passed as X or Y to sptc_contract / sptc_plan.  SPTC_ERR_KEY_OVERFLOW
+ * when a free mode is wider than 32 bits; SPTC_ERR_BAD_ARGS for a
+ * count-only result. */
+sptc_status sptc_result_as_coo(sptc_result* r, void* stream, sptc_coo* out);
 
 /* One pass over the device-resident output: *value_sum = FP64 sum of the
  * emitted values, *violations = entries that are not strictly ascending in
@@ -326,16 +343,25 @@ typedef struct {
   int32_t x_nfbits, y_nfbits, ncbits, out_bits;
   int32_t out_order;
   uint64_t device_bytes;   /* resident operand bytes held by the plan      */
-  double prepare_ms;       /* upload + linearize + sort of both operands   */
+  double prepare_ms;       /* upload + linearize + sort of both operands
+                              (a partitioned plan: the host-side split)   */
+  int32_t x_parts, y_parts;/* static operand parts (1: resident whole)    */
 } sptc_plan_info;
 
 /* TensorManager::ConvertTensor (x2) + SortSlitomXY + Partition
  * (main.cu:293-321, tensor_manager.tpp:212-298,487-522,633-749): lays out,
  * linearizes and sorts both operands into device memory owned by the plan
  * (the reference's SLITOM d_indices/d_values).  Host inputs are copied;
- * device inputs are only read during the call.  The static partitioning of
- * the reference has no counterpart: chunks are planned per execution from
- * the product counts (sptc_partition_plan). */
+ * device inputs are only read during the call.
+ * When the operands do not fit half of the memory budget
+ * (sptc_find_partition_num), the plan instead keeps both operands split
+ * into free-key ranges on the host (Partition, tensor_manager.tpp:633-749:
+ * X by free rows, Y by output columns) and every execution runs the
+ * reference's X x Y partition-pair loop (sptc.cuh:42-81): each Y part
+ * re-keyed to the Y layout and sorted on the device (convert_and_sort_64,
+ * tensor_manager.tpp:751-798), contracted with each X part, and the pair
+ * results merged into row order -- the same entries and values as an
+ * unpartitioned run.  sptc_contract partitions the same way. */
 sptc_status sptc_plan(const sptc_coo* X, const sptc_coo* Y, int32_t nc,
                       const int32_t* cx, const int32_t* cy,
                       const sptc_opts* opts, void* stream, sptc_plan_t** out);
@@ -363,6 +389,24 @@ sptc_status sptc_partition_plan(sptc_plan_t* p, uint64_t budget_bytes,
                                 int32_t max_parts, uint64_t* row_bounds,
                                 uint64_t* prod_bounds, int32_t* nparts,
                                 uint64_t* ir_nnz, void* stream);
+
+/* FindPartitionNum (tensor_manager.tpp:32-61) for this engine: the numbers
+ * of X and Y parts that sptc_plan / sptc_contract use for operands of these
+ * sizes under a device budget (1 and 1 when both stay resident whole).
+ * SPTC_ERR_OOM_BUDGET when more than 64 parts per operand would be needed. */
+sptc_status sptc_find_partition_num(uint64_t nnz_x, int32_t order_x, uint64_t nnz_y,
+                                    int32_t order_y, int32_t idx_bytes,
+                                    uint64_t budget_bytes, int32_t* x_parts,
+                                    int32_t* y_parts);
+
+/* Partition's output (the reference's SLITOM prtn_idx / prtn_coord,
+ * tensor_manager.tpp:633-749) for operand role 1 (X) or 2 (Y) of a plan:
+ * *nparts parts; when max_parts >= *nparts, nnz_bounds[k] = nonzeros in
+ * parts before k and key_bounds[k] = the first free key of part k
+ * (*nparts + 1 entries each, the last the totals / 2^nfbits). */
+sptc_status sptc_plan_get_partitions(const sptc_plan_t* p, int32_t role, int32_t max_parts,
+                                     uint64_t* nnz_bounds, uint64_t* key_bounds,
+                                     int32_t* nparts);
 
 void sptc_plan_destroy(sptc_plan_t* p);
 

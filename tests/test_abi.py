@@ -64,3 +64,24 @@ def test_layout_rejects_bad_modes():
     assert e.value.status == sptc.ERR_BAD_ARGS
     with pytest.raises(sptc.SpTCError):
         sptc.layout([10, 10], [0, 0], 1)
+
+
+def test_find_partition_num_host_only():
+    """FindPartitionNum's part counts: whole operands when they fit half the
+    budget, more parts as the budget shrinks (never fewer), an error when 64
+    parts per operand are not enough; a larger Y is split before X grows
+    past it."""
+    MB = 1 << 20
+    assert sptc.find_partition_num(10_000, 3, 10_000, 3, 4, 1 << 30) == (1, 1)
+    prev = (1, 1)
+    for budget in (2048, 1024, 512, 256, 128, 64):
+        px, py = sptc.find_partition_num(3_000_000, 3, 3_000_000, 3, 4, budget * MB)
+        assert px >= 1 and py >= 1
+        assert px * py >= prev[0] * prev[1]
+        prev = (px, py)
+    assert prev[0] * prev[1] > 1
+    with pytest.raises(sptc.SpTCError) as e:
+        sptc.find_partition_num(3_000_000, 3, 3_000_000, 3, 4, 1 * MB)
+    assert e.value.status == sptc.ERR_OOM_BUDGET
+    # Y alone too large: Y is split, X stays whole
+    assert sptc.find_partition_num(1000, 3, 20_000_000, 3, 4, 1800 * MB)[0] == 1

@@ -346,3 +346,20 @@ def test_long_runs_and_row_starts_across_wave_tiles(gpu, ycols):
     Y = COOTensor((C, ycols), [yc, yj], yv)
     res = sptc.SpTC(X, Y, [1], [0], dense_accumulator=-1)
     _compare_with_oracle(X, Y, [1], [0], res)
+
+
+def test_index_range_check(gpu):
+    """check_indices: an index at or above its mode's dim is an error; by
+    default it is masked to the mode's width, as the reference's pdep
+    linearization does."""
+    X = COOTensor((4, 5, 6), [np.array([0, 3], np.uint32), np.array([1, 2], np.uint32),
+                              np.array([5, 6], np.uint32)], np.array([1.0, 2.0], np.float32))
+    with pytest.raises(sptc.SpTCError) as e:
+        sptc.SpTC(X, X, [2], [2], check_indices=True)
+    assert e.value.status == sptc.ERR_BAD_ARGS and "mode 2" in str(e.value)
+    with pytest.raises(sptc.SpTCError) as e:
+        sptc.Plan(X, X, [2], [2], check_indices=True)
+    assert e.value.status == sptc.ERR_BAD_ARGS
+    sptc.SpTC(X, X, [2], [2])  # masked, no error
+    ok = COOTensor((4, 5, 7), X.idx, X.val)
+    assert sptc.SpTC(ok, ok, [2], [2], check_indices=True).nnz == 2

@@ -1,0 +1,33 @@
+"""Kernel variants selected by environment variables (read once per
+process): each runs the sort and reduce edge-case parity checks against the
+oracle in a process of its own, one after another.
+
+  SPTC_SEG_WIDE=1      1024-thread x 8-key big-row scatter tiles
+  SPTC_SEG_IPT=12/24   keys per thread of the big-row scatter tiles
+  SPTC_SEG_RB=8        8-bit digits for the big-row column passes
+  SPTC_SORT=onesweep   operand sorts with a decoupled look-back scan
+  SPTC_PROD_SORT=full  one sort over all product-key bits (64-bit keys)
+  SPTC_XCD_REMAP=0     blockIdx-order tiles instead of XCD-contiguous ones
+"""
+import os
+import subprocess
+import sys
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+VARIANTS = [{"SPTC_SEG_WIDE": "1"}, {"SPTC_SEG_IPT": "12"}, {"SPTC_SEG_IPT": "24"},
+            {"SPTC_SEG_RB": "8"}, {"SPTC_SORT": "onesweep"}, {"SPTC_PROD_SORT": "full"},
+            {"SPTC_XCD_REMAP": "0"}]
+
+
+@pytest.mark.parametrize("env", VARIANTS, ids=lambda e: ",".join(f"{k}={v}" for k, v in e.items()))
+def test_variant_against_oracle(gpu, env):
+    print(f"[variant] {env}", flush=True)
+    p = subprocess.run([sys.executable, "-u", os.path.join(HERE, "variant_check.py")],
+                       env=dict(os.environ, **env), stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=150)
+    assert p.returncode == 0, p.stdout[-3000:]
+    assert "variant ok" in p.stdout
