@@ -30,7 +30,8 @@ def parse():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="nips",
-                   choices=["nips", "plain_small", "nell2", "batch", "serving"])
+                   choices=["nips", "plain_small", "nell2", "batch", "batch_stated",
+                            "serving"])
     p.add_argument("--requests", type=int, default=200)
     p.add_argument("--nell2-products", type=float, default=1e11,
                    help="intermediate products of the nell2 row sample")
@@ -40,6 +41,8 @@ def parse():
     p.add_argument("--cpu-sample-products", type=float, default=0,
                    help="intermediate products in the CPU-baseline sample (0: auto)")
     p.add_argument("--radix-bits", type=int, default=0, help="0: autotune, 8 or 11")
+    p.add_argument("--accumulator", type=int, default=0,
+                   help="dense_accumulator: 0 planner's choice, -1 ESC, 1 dense, 2 tiled dense")
     p.add_argument("--copy-probe-bytes", type=int, default=0,
                    help="run one device-to-device copy of this size first (counter calibration)")
     return p.parse_args()
@@ -173,9 +176,10 @@ def bench_batch(args):
     info = b.info
     # kernel breakdown of one planning run (per-launch events; not timed)
     bt = sptc.Batch(items, on_device=True, kernel_timing=True)
+    bt_stats = bt.kernel_stats()
     kern = {k: {"launches": n, "total_ms": ms, "achieved_gbs": (by / (ms * 1e-3) / 1e9)
                 if ms else None}
-            for k, (n, ms, by) in bt.kernel_stats().items() if n}
+            for k, (n, ms, by) in bt_stats.items() if n}
     del bt
     if info["out_nnz"] != plain_out:
         raise SystemExit(f"batched output nnz {info['out_nnz']} != plain {plain_out}")
@@ -209,6 +213,129 @@ def bench_batch(args):
         "speedup_eager_vs_plain": plain_s / res["eager"],
         "batch_info": info,
         "plan_run_kernels": kern,
+    }
+    peak = sptc.device_info()["peak_hbm_gbs"]
+    ks = {k: v for k, v in bt_stats.items() if v[0] and v[1]}
+    if ks:
+        dom = max(ks, key=lambda k: ks[k][1])
+        n_l, ms_l, by_l = ks[dom]
+        ach = by_l / (ms_l * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": peak,
+                           "unit": "GB/s", "frac": ach / peak, "traffic": None,
+                           "avg_launch_ms": ms_l / n_l, "alg_bytes_per_launch": by_l / n_l}
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline_instances(specs, tens, budget_s=20.0)
+        if cb:
+            out["cpu_baseline"] = cb
+    print(json.dumps(out))
+
+
+def cpu_baseline_instances(specs, tens, budget_s=20.0):
+    """The reference's CPU SpTC (Sparta ttt) on whole instances of a batch,
+    in instance order, until about budget_s seconds of CPU time are spent;
+    output nnz/s over the instances run."""
+    import oracle as orc
+    from gsparc_amd.coo import write_tns_fast
+    if not os.path.exists(orc.REF_TTT):
+        return None
+    threads = os.cpu_count() or 1
+    tot_s, out_nnz, prods, done = 0.0, 0, 0, 0
+    with tempfile.TemporaryDirectory() as d:
+        for s_, t in zip(specs, tens):
+            path = os.path.join(d, "x.tns")
+            write_tns_fast(path, t)
+            cmd = [orc.REF_TTT, "-X", path, "-Y", path, "-m", str(len(s_.cx)), "-x",
+                   *[str(c) for c in s_.cx], "-y", *[str(c) for c in s_.cy], "-t", str(threads)]
+            p = subprocess.run(cmd, env=dict(os.environ, EXPERIMENT_MODES="3"),
+                               stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                               timeout=600)
+            tt = [float(l.split(":")[1].split()[0]) for l in p.stderr.splitlines()
+                  if l.startswith("[Total time]")]
+            nz = re.findall(r"NNZ=(\d+)", p.stdout)
+            if not tt or not nz:
+                return None
+            tot_s += tt[0]
+            out_nnz += int(nz[-1])
+            done += 1
+            if tot_s >= budget_s:
+                break
+    return {"value": out_nnz / tot_s if tot_s else None, "unit": "nnz/s", "cores": threads,
+            "kind": "reference", "cpu_model": cpu_model(),
+            "sample": f"Sparta ttt (EXPERIMENT_MODES=3, [Total time]) on the first {done} "
+                      f"instances, one after another: {out_nnz} output nnz in {tot_s:.3f} s"}
+
+
+def bench_batch_stated(args):
+    """The 256 batch instances as drawn (no nnz scaling): instances with at
+    most 2^28 products run as one batch (graph replay), the larger ones as
+    plain calls (count-only above 2^31 products, whose outputs would not fit
+    the device); against 256 plain calls.  Total output nnz/s."""
+    import torch
+    from gsparc_amd import sptc
+    from gsparc_amd.workloads import batch_instance, self_products
+
+    log("generating the instances as drawn")
+    specs = [batch_instance(i) for i in range(args.batch_instances)]
+    tens = [s.make() for s in specs]
+    irs = [self_products(t, s.cx) for s, t in zip(specs, tens)]
+    dev = [to_device(t) for t in tens]
+    small = [i for i, ir in enumerate(irs) if ir <= (1 << 28)]
+    large = [i for i, ir in enumerate(irs) if ir > (1 << 28)]
+    torch.cuda.synchronize()
+
+    def plain(i):
+        d, s_ = dev[i], specs[i]
+        return sptc.SpTC(d, d, list(s_.cx), list(s_.cy), on_device=True,
+                         count_only=irs[i] > (1 << 31))
+
+    def plain_all(ids):
+        tot = 0
+        for i in ids:
+            r = plain(i)
+            tot += r.nnz
+            del r
+        torch.cuda.synchronize()
+        return tot
+
+    plain_all(small)
+    log("plain calls (all instances)")
+    t0 = time.perf_counter()
+    out_small = plain_all(small)
+    t_small_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    out_large = plain_all(large)
+    t_large = time.perf_counter() - t0
+    log("batched (instances up to 2^28 products)")
+    b = sptc.Batch([(dev[i], dev[i], list(specs[i].cx), list(specs[i].cy)) for i in small],
+                   on_device=True)
+    if b.info["out_nnz"] != out_small:
+        raise SystemExit(f"batched output nnz {b.info['out_nnz']} != plain {out_small}")
+    for _ in range(max(1, args.warmup)):
+        b.run(use_graph=True)
+        b.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.run(use_graph=True)
+    b.sync()
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / args.steps
+    total_out = out_small + out_large
+    t_served = t_batch + t_large
+    out = {
+        "metric": "SpTC output nnz/s (batched, instances as drawn)",
+        "value": total_out / t_served, "unit": "nnz/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_served * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batch{len(specs)}_as_drawn", "instances": len(specs),
+                   "batched_instances": len(small), "plain_instances": len(large),
+                   "count_only_instances": sum(irs[i] > (1 << 31) for i in large),
+                   "ir_nnz": int(sum(irs)), "out_nnz": total_out},
+        "batched_ms": t_batch * 1e3, "batched_plain_ms": t_small_plain * 1e3,
+        "batched_speedup": t_small_plain / t_batch,
+        "large_plain_ms": t_large * 1e3,
+        "all_plain_ms": (t_small_plain + t_large) * 1e3,
+        "speedup_vs_plain": (t_small_plain + t_large) / t_served,
     }
     print(json.dumps(out))
 
@@ -349,12 +476,20 @@ def bench_serving(args):
         "config": {"workload": "serving", "requests": n, "qps": args.qps,
                    "large_fraction": float(kinds.mean()), "budget_bytes": budget},
         "server_poisson": {**_percentiles(lat_s), "wall_s": wall_s,
-                           "mean_batch": float(np.mean(sizes))},
-        "serial_poisson": {**_percentiles(lat_p), "wall_s": wall_p},
+                           "mean_batch": float(np.mean(sizes)),
+                           "small": _percentiles(np.asarray(lat_s)[~kinds]),
+                           "large": _percentiles(np.asarray(lat_s)[kinds])},
+        "serial_poisson": {**_percentiles(lat_p), "wall_s": wall_p,
+                           "small": _percentiles(np.asarray(lat_p)[~kinds]),
+                           "large": _percentiles(np.asarray(lat_p)[kinds])},
         "server_burst": {**_percentiles(burst_s[0]), "wall_s": burst_s[1],
-                         "requests_per_s": n / burst_s[1]},
+                         "requests_per_s": n / burst_s[1],
+                         "small": _percentiles(np.asarray(burst_s[0])[~kinds]),
+                         "large": _percentiles(np.asarray(burst_s[0])[kinds])},
         "serial_burst": {**_percentiles(burst_p[0]), "wall_s": burst_p[1],
-                         "requests_per_s": n / burst_p[1]},
+                         "requests_per_s": n / burst_p[1],
+                         "small": _percentiles(np.asarray(burst_p[0])[~kinds]),
+                         "large": _percentiles(np.asarray(burst_p[0])[kinds])},
         "burst_speedup": burst_p[1] / burst_s[1],
         "output_nnz_equal": out_s == out_p and burst_s[2] == out_p,
         "server_stats": st,
@@ -464,6 +599,8 @@ def main():
         sys.exit(2)
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "batch_stated":
+        return bench_batch_stated(args)
     if args.workload == "serving":
         return bench_serving(args)
     if args.workload == "nell2":
@@ -489,7 +626,8 @@ def main():
 
     def step(timing):
         return sptc.SpTC(dX, dX, list(spec.cx), list(spec.cy), on_device=True,
-                         radix_bits=args.radix_bits, kernel_timing=timing, **kw)
+                         radix_bits=args.radix_bits, kernel_timing=timing,
+                         dense_accumulator=args.accumulator, **kw)
 
     for _ in range(args.warmup):
         r = step(False)
@@ -497,7 +635,7 @@ def main():
     torch.cuda.synchronize()
     times, phases = [], []
     kstats = {}
-    out_nnz = ir = 0
+    out_nnz = ir = acc = 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     last = None
@@ -510,7 +648,7 @@ def main():
         for k, (n, kms, kb) in r.kernel_stats().items():
             a0 = kstats.get(k, (0, 0.0, 0.0))
             kstats[k] = (a0[0] + n, a0[1] + kms, a0[2] + kb)
-        out_nnz, ir = r.nnz, r.ir_nnz
+        out_nnz, ir, acc = r.nnz, r.ir_nnz, r.info.accumulator
         if k_step == args.steps - 1:
             last = r  # checked after the timed region
         del r
@@ -529,7 +667,7 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": spec.name, "contract_x": list(spec.cx),
                    "contract_y": list(spec.cy), "nnz": X.nnz, "ir_nnz": ir,
-                   "out_nnz": out_nnz},
+                   "out_nnz": out_nnz, "accumulator": ["esc", "dense", "tiled_dense"][acc]},
         "phase_ms_median": {"linearize_sort": ph[1], "match": ph[2], "partition": ph[3],
                             "accumulate": ph[4], "total": ph[5]},
         "check": check,

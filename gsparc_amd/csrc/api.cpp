@@ -78,6 +78,11 @@ bool device_ok() {
 
 }  // namespace
 
+namespace sptc {
+// Error reporting for entry points defined in other files.
+sptc_status api_fail(sptc_status s, const std::string& m) { return fail(s, m); }
+}  // namespace sptc
+
 extern "C" {
 
 const char* sptc_status_string(sptc_status s) {
@@ -305,6 +310,24 @@ sptc_status sptc_execute(sptc_plan_t* p, const sptc_opts* opts, void* stream,
   }
   *out = res;
   return SPTC_OK;
+}
+
+static sptc_status accumulate_forced(sptc_plan_t* p, const sptc_opts* opts, int32_t acc,
+                                     void* stream, sptc_result** out) {
+  if (!p || !out) return fail(SPTC_ERR_BAD_ARGS, "NULL argument");
+  sptc_opts o = opts ? *opts : p->opts;
+  o.dense_accumulator = acc;
+  return sptc_execute(p, &o, stream, out);
+}
+
+sptc_status sptc_accumulate_esc(sptc_plan_t* p, const sptc_opts* opts, void* stream,
+                                sptc_result** out) {
+  return accumulate_forced(p, opts, -1, stream, out);
+}
+
+sptc_status sptc_accumulate_dense(sptc_plan_t* p, const sptc_opts* opts, void* stream,
+                                  sptc_result** out) {
+  return accumulate_forced(p, opts, 1, stream, out);
 }
 
 sptc_status sptc_plan_get_info(const sptc_plan_t* p, sptc_plan_info* info) {

@@ -1,6 +1,8 @@
 // SpTC engine host orchestration (built with hipcc; launches live in .hip).
 #include "engine.hpp"
 
+#include "tdense.hpp"
+
 #include <algorithm>
 #include <cstring>
 #include <string>
@@ -510,13 +512,19 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       use_dense = o_.dense_accumulator == 1 ||
                   (!o_.count_only && ir * 32 >= R * E && ir >= kDenseMinRun * A);
   }
-  if (partition_only) use_dense = false;  // the plan asked for is ESC's chunking
-  if (o_.dense_accumulator == 1 && ir && !use_dense) {
-    msg = "dense accumulator does not apply (column space above 2^13 or duplicate Y "
-          "coordinates)";
+  // Tiled dense (column tiles in LDS, FP64 slots) for column spaces wider
+  // than an LDS row: forced with dense_accumulator 2, or by 1 when the LDS
+  // row does not apply.
+  bool use_tdense = false;
+  if (!use_dense && ir && !o_.count_only && kyf > 0 && kyf <= 32)
+    use_tdense = o_.dense_accumulator == 2 || o_.dense_accumulator == 1;
+  if (partition_only) use_dense = use_tdense = false;  // the plan asked for is ESC's chunking
+  if (o_.dense_accumulator >= 1 && ir && !use_dense && !use_tdense) {
+    msg = "dense accumulator does not apply (count-only run, or a column field above 32 "
+          "bits)";
     return SPTC_ERR_BAD_ARGS;
   }
-  info.accumulator = use_dense ? 1 : 0;
+  info.accumulator = use_dense ? 1 : (use_tdense ? 2 : 0);
   std::vector<uint64_t> bounds_r{0}, bounds_p{0}, bounds_a{0};
   if (use_dense) {
     record(4);
@@ -565,6 +573,66 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       kt_.end(tok, SPTC_K_DENSE, 12.0 * (double)ir + 16.0 * (double)out_n, s_);
       res->chunks.push_back(std::move(oc));
     }
+  } else if (use_tdense) {
+    record(4);
+    const uint64_t fmask = low_mask(kyf);
+    DevBuf<uint32_t> ycol32(ny);
+    launch_ycol(ykeys, ny, fmask, ycol32.get(), s_);
+    DevBuf<uint64_t> cur(A ? A : 1);
+    DevBuf<uint32_t> row_cnt(R);
+    DevBuf<uint64_t> out_off(R + 1);
+    DevBuf<U2> tmp(scan_tmp_count(R) + 1);
+    TDenseArgs ta;
+    std::memset(&ta, 0, sizeof(ta));
+    ta.act_prefix = act_prefix.get();
+    ta.act_ypos = act_ypos.get();
+    ta.act_xval = act_xval.get();
+    ta.row_act = row_act.get();
+    ta.ycol = ycol32.get();
+    ta.yvals = Y.vals.get();
+    ta.ytab = ytab.size() ? ytab.get() : nullptr;
+    ta.cur = cur.get();
+    ta.row_cnt = row_cnt.get();
+    ta.out_off = out_off.get();
+    ta.vsum = vsum.get();
+    // one workgroup per row, in launches of at most 2^30 rows
+    constexpr uint64_t kRowsPerLaunch = 1ull << 30;
+    int tok = kt_.begin(s_);
+    for (uint64_t b = 0; b < R; b += kRowsPerLaunch) {
+      ta.r0 = b;
+      ta.row_cnt = row_cnt.get() + b;
+      launch_tdense_count(ta, std::min(kRowsPerLaunch, R - b), s_);
+    }
+    kt_.end(tok, SPTC_K_DENSE, 4.0 * (double)ir, s_);
+    run_row_count_scan(row_cnt.get(), out_off.get(), R, tmp.get(), s_);
+    const uint64_t out_n = read_total(tmp.get() + scan_tmp_count(R)).a;
+    info.nnz = out_n;
+    info.nchunks = 1;
+    // the whole output stays resident: one chunk
+    if (DevicePool::get().live_bytes() + out_n * 16 > budget_) {
+      msg = "output does not fit the memory budget";
+      return SPTC_ERR_OOM_BUDGET;
+    }
+    OutChunk oc;
+    oc.nnz = out_n;
+    oc.r0 = 0;
+    oc.r1 = R;
+    oc.row.reset(out_n ? out_n : 1);
+    oc.col.reset(out_n ? out_n : 1);
+    oc.val.reset(out_n ? out_n : 1);
+    ta.out_row = oc.row.get();
+    ta.out_col = oc.col.get();
+    ta.out_val = oc.val.get();
+    ta.out0 = 0;
+    tok = kt_.begin(s_);
+    for (uint64_t b = 0; b < R; b += kRowsPerLaunch) {
+      ta.r0 = b;
+      ta.out_off = out_off.get() + b;
+      launch_tdense_numeric(ta, std::min(kRowsPerLaunch, R - b), s_);
+    }
+    // per product a Y column and value gathered; per output 16 bytes written
+    kt_.end(tok, SPTC_K_DENSE, 8.0 * (double)ir + 16.0 * (double)out_n, s_);
+    res->chunks.push_back(std::move(oc));
   } else if (ir) {
     DevBuf<uint64_t> scratch(4);
     launch_max_row(row_prod.get(), R, scratch.get(), s_);
@@ -575,7 +643,8 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
     const uint64_t free_b = budget_ > live ? budget_ - live : 0;
     const uint64_t per_prod = 12 + SortWorkspace::bytes_per_key(10) + 1;
     const uint64_t out_per = o_.count_only ? 0 : 16;
-    const uint64_t slack = 64ull << 20;
+    // room for the accumulation's small temporaries (scan tiles, row tables)
+    const uint64_t slack = std::min<uint64_t>(64ull << 20, budget_avail() / 8);
     uint64_t cap;
     if (free_b > slack + out_per * ir + per_prod * std::min<uint64_t>(ir, 1ull << 20))
       cap = (free_b - slack - out_per * ir) / per_prod;

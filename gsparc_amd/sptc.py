@@ -134,6 +134,9 @@ def lib():
         L.sptc_find_partition_num.argtypes = [c_uint64, c_int32, c_uint64, c_int32, c_int32,
                                               c_uint64, POINTER(c_int32), POINTER(c_int32)]
         L.sptc_result_as_coo.argtypes = [c_void_p, c_void_p, POINTER(_COO)]
+        L.sptc_accumulate_esc.argtypes = [c_void_p, POINTER(_Opts), c_void_p, POINTER(c_void_p)]
+        L.sptc_accumulate_dense.argtypes = [c_void_p, POINTER(_Opts), c_void_p,
+                                            POINTER(c_void_p)]
         L.sptc_trim_cache.argtypes = []
         L.sptc_trim_cache.restype = None
         L.sptc_result_kernel_stats.argtypes = [c_void_p, c_int32, POINTER(c_uint64),
@@ -430,6 +433,21 @@ class Plan:
             o.count_only = int(bool(count_only))
         out = c_void_p()
         check(lib().sptc_execute(self._h, byref(o), c_void_p(stream or 0), byref(out)))
+        return Result(out.value)
+
+    def accumulate_esc(self, stream=None):
+        """ESC_Contraction on the plan's operands (sptc_accumulate_esc)."""
+        out = c_void_p()
+        check(lib().sptc_accumulate_esc(self._h, byref(self._opts), c_void_p(stream or 0),
+                                        byref(out)))
+        return Result(out.value)
+
+    def accumulate_dense(self, stream=None):
+        """Dense_Contraction on the plan's operands (sptc_accumulate_dense):
+        the LDS row or, for wider column spaces, the tiled form."""
+        out = c_void_p()
+        check(lib().sptc_accumulate_dense(self._h, byref(self._opts), c_void_p(stream or 0),
+                                          byref(out)))
         return Result(out.value)
 
     def static_partitions(self, role):

@@ -57,8 +57,10 @@ typedef struct {
 } sptc_coo;
 
 /* Options.  dense_accumulator follows the reference's -d flag
- * (sptc.cuh:201-218): 1 forces the dense accumulator (error if it does not
- * apply), -1 forces ESC, 0 lets the planner choose. */
+ * (sptc.cuh:201-218): 1 forces the dense accumulator (the LDS row when the
+ * column space fits one, else the tiled form; error if neither applies),
+ * -1 forces ESC, 0 lets the planner choose, 2 forces the tiled dense
+ * accumulator. */
 typedef struct {
   int32_t dense_accumulator;
   int32_t count_only;        /* 1: return nnz and checksums, no entries   */
@@ -92,7 +94,7 @@ typedef struct {
   int32_t x_nfbits, y_nfbits, ncbits, out_bits;
   int32_t out_order;
   int32_t nchunks;       /* accumulation chunks (dynamic partitions)       */
-  int32_t accumulator;   /* 0 = ESC, 1 = dense                             */
+  int32_t accumulator;   /* 0 = ESC, 1 = dense LDS row, 2 = tiled dense    */
   double value_sum;      /* sum of all output values (FP64 on device)      */
   const uint32_t* row;   /* device [nnz]                                   */
   const uint64_t* col;   /* device [nnz]                                   */
@@ -375,6 +377,22 @@ sptc_status sptc_plan(const sptc_coo* X, const sptc_coo* Y, int32_t nc,
  * contraction.cuh:247-599) alone. */
 sptc_status sptc_execute(sptc_plan_t* p, const sptc_opts* opts, void* stream,
                          sptc_result** out);
+
+/* The accumulators on a plan's sorted operands, each forced regardless of
+ * opts->dense_accumulator (which is otherwise honoured as in sptc_execute):
+ *   sptc_accumulate_esc   -- ESC_Contraction (contraction.cuh:364-599):
+ *                            expand, row-segmented sort, reduce;
+ *   sptc_accumulate_dense -- Dense_Contraction (contraction.cuh:247-362):
+ *                            the LDS row when the column space fits it and Y
+ *                            has no duplicate coordinates, else the tiled
+ *                            dense accumulator (column tiles through LDS).
+ * The reference's accumulators receive the index match's mPos / mCnt /
+ * prefix arrays (sptc.cuh:118-318); here each call runs the match on the
+ * resident operands first (its time is phase_ms[2]). */
+sptc_status sptc_accumulate_esc(sptc_plan_t* p, const sptc_opts* opts, void* stream,
+                                sptc_result** out);
+sptc_status sptc_accumulate_dense(sptc_plan_t* p, const sptc_opts* opts, void* stream,
+                                  sptc_result** out);
 
 sptc_status sptc_plan_get_info(const sptc_plan_t* p, sptc_plan_info* info);
 

@@ -275,17 +275,24 @@ def test_dense_equals_esc_bitwise(gpu, seed):
     _compare_with_oracle(X, Y, [2], [0], d)
 
 
-def test_dense_rejected_when_it_cannot_apply(gpu):
+def test_dense_above_an_lds_row_is_tiled(gpu):
+    """A column space wider than one LDS row (300x200 Y free keys > 2^13):
+    -d 1 runs the tiled dense accumulator; a count-only run cannot."""
     rng = np.random.default_rng(9)
-    X = random_coo(rng, (300, 200, 40), 30_000)  # Y free space 300x200 > 2^13 slots
+    X = random_coo(rng, (300, 200, 40), 30_000)
+    res = sptc.SpTC(X, X, [2], [2], dense_accumulator=1)
+    assert res.info.accumulator == 2
+    _compare_with_oracle(X, X, [2], [2], res)
     with pytest.raises(sptc.SpTCError) as e:
-        sptc.SpTC(X, X, [2], [2], dense_accumulator=1)
+        sptc.SpTC(X, X, [2], [2], dense_accumulator=1, count_only=True)
     assert e.value.status == sptc.ERR_BAD_ARGS
 
 
 def test_duplicate_coordinates_sum_like_the_oracle(gpu):
     """The .tns reader keeps duplicate entries (io_manager.tpp:210-240); ESC
-    sums their products like any equal keys, dense refuses them."""
+    sums their products like any equal keys; the LDS dense row refuses them
+    (its ordered adds assume distinct columns per owner), so -d 1 runs the
+    tiled dense accumulator, which adds them like the reference's atomics."""
     rng = np.random.default_rng(10)
     X = random_coo(rng, (30, 20, 10), 1500)
     dup = COOTensor(X.dims, [np.concatenate([a, a[:200]]) for a in X.idx],
@@ -293,8 +300,9 @@ def test_duplicate_coordinates_sum_like_the_oracle(gpu):
     res = sptc.SpTC(dup, dup, [2], [2])
     assert res.info.accumulator == 0
     _compare_with_oracle(dup, dup, [2], [2], res)
-    with pytest.raises(sptc.SpTCError):
-        sptc.SpTC(dup, dup, [2], [2], dense_accumulator=1)
+    d = sptc.SpTC(dup, dup, [2], [2], dense_accumulator=1)
+    assert d.info.accumulator == 2
+    _compare_with_oracle(dup, dup, [2], [2], d)
 
 
 def test_row_segments_big_and_small_rows(gpu):

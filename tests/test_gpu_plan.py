@@ -115,3 +115,28 @@ def test_execute_options_per_call(gpu):
         _same(r, one)
     cnt = p.execute(count_only=True)
     assert cnt.nnz == one.nnz and cnt.ir_nnz == one.ir_nnz
+
+
+def test_accumulate_entry_points(gpu):
+    """sptc_accumulate_esc / sptc_accumulate_dense (ESC_Contraction /
+    Dense_Contraction on a plan's operands) are the forced accumulators of
+    sptc_execute: the LDS row for a narrow column space, the tiled form for
+    a wide one."""
+    rng = np.random.default_rng(16)
+    narrow = random_coo(rng, (40, 12, 30), 9_000)
+    wide = random_coo(rng, (300, 200, 40), 30_000)
+    for X, dense_kind in ((narrow, 1), (wide, 2)):
+        p = sptc.Plan(X, X, [2], [2])
+        e, d = p.accumulate_esc(), p.accumulate_dense()
+        assert e.info.accumulator == 0 and d.info.accumulator == dense_kind
+        _same(e, p.execute(dense_accumulator=-1))
+        if dense_kind == 1:
+            _same(d, p.execute(dense_accumulator=1))
+        else:  # FP64 LDS adds in no fixed order: equal up to the last FP32 bit
+            for a, b in zip(d.to_host(), p.execute(dense_accumulator=1).to_host()):
+                np.testing.assert_allclose(a.astype(np.float64), b.astype(np.float64),
+                                           rtol=2.0 ** -23, atol=0)
+        xe, ye, _ = e.to_host()
+        xd, yd, _ = d.to_host()
+        np.testing.assert_array_equal(xe, xd)
+        np.testing.assert_array_equal(ye, yd)

@@ -56,11 +56,11 @@ def whole(operands):
     return sptc.SpTC(X, Y, [2], [0])
 
 
-@pytest.mark.parametrize("budget_mb", [160, 100])
-def test_partitioned_equals_whole(gpu, operands, whole, budget_mb):
+@pytest.mark.parametrize("budget_mb,parts", [(200, (2, 2)), (240, (3, 1))])
+def test_partitioned_equals_whole(gpu, operands, whole, budget_mb, parts):
     X, Y = operands
     px, py = sptc.find_partition_num(X.nnz, X.order, Y.nnz, Y.order, 4, budget_mb * MB)
-    assert px > 1 and py > 1
+    assert (px, py) == parts
     res = sptc.SpTC(X, Y, [2], [0], mem_budget_bytes=budget_mb * MB)
     assert res.info.nchunks == px
     _same(res, whole)
@@ -73,7 +73,7 @@ def test_partitioned_equals_whole(gpu, operands, whole, budget_mb):
 
 def test_partitioned_plan(gpu, operands, whole):
     X, Y = operands
-    budget = 128 * MB
+    budget = 200 * MB
     p = sptc.Plan(X, Y, [2], [0], mem_budget_bytes=budget)
     info = p.info
     px, py = sptc.find_partition_num(X.nnz, X.order, Y.nnz, Y.order, 4, budget)
@@ -104,7 +104,7 @@ def test_partitioned_plan(gpu, operands, whole):
 
 def test_partitioned_rows_against_oracle(gpu, operands):
     X, Y = operands
-    res = sptc.SpTC(X, Y, [2], [0], mem_budget_bytes=100 * MB)
+    res = sptc.SpTC(X, Y, [2], [0], mem_budget_bytes=240 * MB)
     rk = res.row_keys()
     pick = rk[np.linspace(0, len(rk) - 1, 40).astype(np.int64)]
     compare_with_oracle(X, Y, [2], [0], res, x_rows=pick)
