@@ -19,7 +19,7 @@ from ._lib import LIB_DIR, PKG_DIR, REPO_DIR
 CSRC = os.path.join(PKG_DIR, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-HIP_SOURCES = ["sort.hip", "segsort.hip", "kernels.hip", "batch.hip", "result.hip",
+HIP_SOURCES = ["sort.hip", "segsort.hip", "kernels.hip", "batch.hip", "result.hip", "noalloc.hip",
                "tdense.hip",
                "device.cpp", "engine.cpp", "staticpart.cpp",
                "batch_engine.cpp", "serve.cpp", "api.cpp"]

@@ -123,6 +123,7 @@ def lib():
                                        c_int32, c_void_p, c_void_p, c_void_p,
                                        POINTER(c_uint64), c_void_p]
         L.sptc_exclusive_scan_u64.argtypes = [c_void_p, c_void_p, c_uint64, c_void_p]
+        L.sptc_index_match_noalloc.argtypes = L.sptc_index_match.argtypes
         L.sptc_device_info.argtypes = [POINTER(c_int32), POINTER(c_uint64), POINTER(c_uint64),
                                        POINTER(c_double)]
         L.sptc_status_string.argtypes = [c_int32]

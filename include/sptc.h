@@ -462,6 +462,22 @@ sptc_status sptc_index_match(const uint64_t* xkeys, uint64_t nx,
                              uint64_t* mcnt, uint64_t* prefix,
                              uint64_t* ir_nnz, void* stream);
 
+/* sptc_index_match with no allocation (no counterpart in the reference):
+ * for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Same
+ * arguments and results as sptc_index_match; its working memory (the Y
+ * contract table and the scan's tile sums) is a fixed 33 MiB device array
+ * that each GPU holds until the process ends.  Limits: a contract field of
+ * at most 21 bits (direct table) or fewer than 2^21 Y nonzeros (distinct-key
+ * table), and at most 65535 * 2048 nonzeros per operand; larger inputs
+ * return SPTC_ERR_BAD_ARGS before any device work.  Calls on one GPU share
+ * the array: calls that overlap on one GPU (two streams or threads) race. */
+sptc_status sptc_index_match_noalloc(const uint64_t* xkeys, uint64_t nx,
+                                     const uint64_t* ykeys, uint64_t ny,
+                                     int32_t ncbits, int32_t y_nfbits, uint64_t* mpos,
+                                     uint64_t* mcnt, uint64_t* prefix,
+                                     uint64_t* ir_nnz, void* stream);
+
 /* Exclusive scan of n uint64 (cub::DeviceScan::InclusiveSum shifted,
  * indexMatch.cuh:275-279); out has n+1 entries, out[n] = total. */
 sptc_status sptc_exclusive_scan_u64(const uint64_t* in, uint64_t* out,
