@@ -107,6 +107,10 @@ void* DevicePool::alloc(size_t bytes) {
   if (e != hipSuccess) {
     (void)hipGetLastError();
     release_empty_slabs_locked();
+    // the process-wide pool may cache the memory this pool needs (a worker
+    // pool of the server after large single contractions): release its free
+    // slabs too
+    if (this != &global()) global().trim();
     e = hipMalloc(reinterpret_cast<void**>(&base), want);
     slab = want;
     if (e != hipSuccess) {

@@ -58,6 +58,9 @@ Server::Server(const sptc_server_opts& o) : o_(o) {
   if (!o_.large_products) o_.large_products = 1ull << 28;
   if (!o_.large_budget_bytes) o_.large_budget_bytes = o_.budget_bytes / 3;
   SPTC_HIP(hipGetDevice(&device_));
+  // the workers allocate from pools of their own: hand back what the
+  // process-wide pool caches from earlier calls
+  DevicePool::global().trim();
   ledger_ = std::make_shared<Ledger>();
   SPTC_HIP(hipStreamCreateWithFlags(&plan_stream_, hipStreamNonBlocking));
   for (int w = 0; w < o_.workers; ++w) {
