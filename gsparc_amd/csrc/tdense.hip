@@ -46,20 +46,36 @@ __device__ __forceinline__ uint32_t block_min_u32(uint32_t v, uint32_t* red) {
   return m;
 }
 
+// 64-bit OR over the wave.
+__device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    const uint32_t lo = __shfl_xor((uint32_t)v, d, kWave);
+    const uint32_t hi = __shfl_xor((uint32_t)(v >> 32), d, kWave);
+    v |= ((uint64_t)hi << 32) | lo;
+  }
+  return v;
+}
+
 template <int TWB, bool NUM>
 __global__ __launch_bounds__(kBlock) void k_tdense(TDenseArgs a) {
   constexpr uint32_t TW = 1u << TWB;
   constexpr uint32_t NWORD = TW / 32;
   constexpr uint32_t S = TW / kBlock;  // slots per thread in the compaction
+  constexpr uint32_t NWAVE = kBlock / kWave;
   static_assert(!NUM || (S >= 1 && S <= 32 && 32 % S == 0), "numeric tile width");
   static_assert(NUM || NWORD % kBlock == 0, "symbolic tile width");
   __shared__ uint32_t bm[NWORD];
   __shared__ double slot[NUM ? TW : 1];
-  __shared__ uint32_t pref[kBlock];
-  __shared__ uint64_t base[kBlock];
-  __shared__ float xv[kBlock];
-  __shared__ uint32_t red[kBlock / kWave];
+  // the batch's owners with products in the tile, compacted: first product
+  // (exclusive prefix), first Y entry, X value; cpref[nk] = the total
+  __shared__ uint32_t cpref[kBlock + 1];
+  __shared__ uint64_t cbase[kBlock];
+  __shared__ float cxv[kBlock];
+  __shared__ uint64_t red64[NWAVE];
+  __shared__ uint32_t red[NWAVE];
   const uint32_t tid = threadIdx.x;
+  const uint32_t lane = lane_id();
+  const uint32_t wave = wave_id();
   const uint64_t r = a.r0 + blockIdx.x;
   const uint64_t j0 = a.row_act[r], j1 = a.row_act[r + 1];
   for (uint32_t i = tid; i < NWORD; i += kBlock) bm[i] = 0;
@@ -85,8 +101,9 @@ __global__ __launch_bounds__(kBlock) void k_tdense(TDenseArgs a) {
     for (uint64_t jb = j0; jb < j1; jb += kBlock) {
       const uint64_t j = jb + tid;
       uint32_t cnt = 0;
+      uint64_t s = 0;
       if (j < j1) {
-        const uint64_t s = a.cur[j];
+        s = a.cur[j];
         const uint64_t e = a.act_ypos[j] + (a.act_prefix[j + 1] - a.act_prefix[j]);
         uint64_t lo = s, hi = e;  // first entry at or past the tile's end
         while (lo < hi) {
@@ -94,32 +111,61 @@ __global__ __launch_bounds__(kBlock) void k_tdense(TDenseArgs a) {
           if (a.ycol[mid] < tend) lo = mid + 1; else hi = mid;
         }
         cnt = (uint32_t)(lo - s);
-        base[tid] = s;
-        if constexpr (NUM) xv[tid] = a.act_xval[j];
         a.cur[j] = lo;
         if (lo < e) {
           const uint32_t c = a.ycol[lo];
           nmin = c < nmin ? c : nmin;
         }
       }
-      uint32_t tot;
-      const uint32_t ex = block_excl_scan<uint32_t>(cnt, red, &tot);
-      pref[tid] = ex;
+      // one scan gives both the product prefix and the compacted position
+      uint64_t tot2;
+      const uint64_t ex =
+          block_excl_scan<uint64_t>(((uint64_t)cnt << 32) | (cnt ? 1u : 0u), red64, &tot2);
+      const uint32_t tot = (uint32_t)(tot2 >> 32), nk = (uint32_t)tot2;
+      if (cnt) {
+        const uint32_t k = (uint32_t)ex;
+        cpref[k] = (uint32_t)(ex >> 32);
+        cbase[k] = s;
+        if constexpr (NUM) cxv[k] = a.act_xval[j];
+      }
+      if (tid == 0) cpref[nk] = tot;
       __syncthreads();
-      for (uint32_t p = tid; p < tot; p += kBlock) {
-        uint32_t lo = 0, hi = kBlock;  // last owner k with pref[k] <= p
+      // wave w takes products [p0, p1), 64 consecutive ones per step; the
+      // owner of a lane's product is k0 plus the owners that start in the
+      // window at or below it (a mask built from the window's boundaries)
+      const uint32_t per = (uint32_t)ceil_div(ceil_div(tot, NWAVE), kWave) * kWave;
+      const uint32_t p0 = wave * per < tot ? wave * per : tot;
+      const uint32_t p1 = p0 + per < tot ? p0 + per : tot;
+      uint32_t k0 = 0;
+      if (p0 < p1) {  // last owner with cpref <= p0: sampled search over nk
+        uint32_t lo = 0, hi = nk;
         while (hi - lo > 1) {
           const uint32_t mid = (lo + hi) >> 1;
-          if (pref[mid] <= p) lo = mid; else hi = mid;
+          if (cpref[mid] <= p0) lo = mid; else hi = mid;
         }
-        const uint64_t y = base[lo] + (p - pref[lo]);
-        const uint32_t sl = (uint32_t)(a.ycol[y] - t0);
-        atomicOr(&bm[sl >> 5], 1u << (sl & 31));
-        if constexpr (NUM) {
-          const float v = xv[lo] * a.yvals[y];
-          vs += (double)v;
-          atomicAdd(&slot[sl], (double)v);
+        k0 = lo;
+      }
+      for (uint32_t pw = p0; pw < p1; pw += kWave) {
+        const uint32_t p = pw + lane;
+        const uint32_t kb = k0 + 1 + lane;
+        const uint32_t b = kb < nk ? cpref[kb] : ~0u;
+        // boundaries inside the window mark their offset (distinct: every
+        // compacted owner has products)
+        const uint64_t mark = (b >= pw && b < pw + kWave) ? (1ull << (b - pw)) : 0ull;
+        const uint64_t m = wave_or64(mark);
+        const uint64_t below = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1);
+        const uint32_t k = k0 + (uint32_t)__popcll(m & below);
+        if (p < p1) {
+          const uint64_t y = cbase[k] + (p - cpref[k]);
+          const uint32_t sl = (uint32_t)(a.ycol[y] - t0);
+          atomicOr(&bm[sl >> 5], 1u << (sl & 31));
+          if constexpr (NUM) {
+            const float v = cxv[k] * a.yvals[y];
+            vs += (double)v;
+            atomicAdd(&slot[sl], (double)v);
+          }
         }
+        k0 += (uint32_t)__popcll(m);  // owner of the next window's first product
       }
       __syncthreads();
     }
@@ -154,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_tdense(TDenseArgs a) {
   }
   if constexpr (NUM) {
     for (int d = kWave / 2; d > 0; d >>= 1) vs += __shfl_xor(vs, d, kWave);
-    if (lane_id() == 0 && vs != 0.0) atomicAdd(a.vsum, vs);
+    if (lane == 0 && vs != 0.0) atomicAdd(a.vsum, vs);
   } else {
     if (tid == 0) a.row_cnt[blockIdx.x] = (uint32_t)ocount;
   }
