@@ -526,6 +526,8 @@ def bench_nell2(args):
     del i, j, k, prod, rk, rinv
     dX, dS = to_device(X), to_device(Xs)
     torch.cuda.synchronize()
+    phases = full_size_phases(X, [2], [2])
+    log(f"full-size phases: {phases}")
     log(f"sample: every {stride}th row, {Xs.nnz} nnz, {ir_sample:.3e} products")
     times, res = [], None
     for _ in range(max(1, args.steps)):
@@ -553,10 +555,80 @@ def bench_nell2(args):
         "full_ir_nnz": ir_total,
         "full_time_s_extrapolated": ir_total / (res.ir_nnz / t),
         "value_sum": res.info.value_sum,
+        "full_size_phases": phases,
         "kernels": {k: {"launches": n, "avg_ms": ms / n, "achieved_gbs": by / (ms * 1e-3) / 1e9}
                     for k, (n, ms, by) in res.kernel_stats().items() if n and ms},
     }
     print(json.dumps(out))
+
+
+def full_size_phases(X, cx, cy, reps=5):
+    """Operand sort and index match over the whole operand through the
+    per-phase entry points (sptc_linearize, sptc_sort_pairs,
+    sptc_index_match), timed with HIP events on the launch stream: median
+    ms and algorithmic GB/s (sort: 24 B per key per radix pass, key + value
+    read and written; match: 8 B key read, 24 B of mpos/mcnt/prefix written,
+    per X nonzero, plus 8 B per Y key for the contract table)."""
+    import ctypes
+    import torch
+    from gsparc_amd import sptc
+    import oracle as orc
+    L = sptc.lib()
+    cw = orc.common_cwidth(X.dims, X.dims, cx, cy)
+    res = {}
+    for role, cm in ((1, cx), (2, cy)):
+        sh, wd, nf, ncb = sptc.layout(X.dims, cm, role, cw)
+        idx = [torch.from_numpy(a.view(np.int32)).cuda() for a in X.idx]
+        ptrs = (ctypes.c_void_p * X.order)(*[a.data_ptr() for a in idx])
+        keys = torch.empty(X.nnz, dtype=torch.int64, device="cuda")
+        vals = torch.from_numpy(X.val).cuda()
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        sptc.check(L.sptc_linearize(X.order, X.nnz, ptrs, 4, (ctypes.c_int32 * X.order)(*sh),
+                                    (ctypes.c_int32 * X.order)(*wd),
+                                    ctypes.c_void_p(keys.data_ptr()), stream))
+        base_k, base_v = keys.clone(), vals.clone()
+        nbits = nf + ncb
+        t = []
+        for _ in range(reps):
+            keys.copy_(base_k)
+            vals.copy_(base_v)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            sptc.check(L.sptc_sort_pairs(ctypes.c_void_p(keys.data_ptr()),
+                                         ctypes.c_void_p(vals.data_ptr()), X.nnz, nbits, 0,
+                                         stream))
+            e1.record()
+            torch.cuda.synchronize()
+            t.append(e0.elapsed_time(e1))
+        ms = float(np.median(t))
+        res[f"sort_{'x' if role == 1 else 'y'}"] = {
+            "ms": ms, "passes_assumed_8bit": -(-nbits // 8),
+            "gbs_per_8bit_pass_model": 24.0 * X.nnz * (-(-nbits // 8)) / (ms * 1e-3) / 1e9}
+        if role == 1:
+            xk, ncbits = keys, ncb
+        else:
+            yk, ynf = keys, nf
+    nx = X.nnz
+    mpos = torch.empty(nx, dtype=torch.int64, device="cuda")
+    mcnt = torch.empty(nx, dtype=torch.int64, device="cuda")
+    pre = torch.empty(nx + 1, dtype=torch.int64, device="cuda")
+    ir = ctypes.c_uint64()
+    t = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        sptc.check(L.sptc_index_match(ctypes.c_void_p(xk.data_ptr()), nx,
+                                      ctypes.c_void_p(yk.data_ptr()), nx, ncbits, ynf,
+                                      ctypes.c_void_p(mpos.data_ptr()),
+                                      ctypes.c_void_p(mcnt.data_ptr()),
+                                      ctypes.c_void_p(pre.data_ptr()), ctypes.byref(ir), stream))
+        e1.record()
+        torch.cuda.synchronize()
+        t.append(e0.elapsed_time(e1))
+    ms = float(np.median(t))
+    res["match"] = {"ms": ms, "ir": ir.value,
+                    "gbs": (32.0 * nx + 8.0 * nx) / (ms * 1e-3) / 1e9}
+    return res
 
 
 def self_check(X, spec, r):
