@@ -406,11 +406,10 @@ __global__ __launch_bounds__(kBlock) void k_bdense_count(BDenseArgs a) {
   if (threadIdx.x == 0) a.d.row_cnt[blockIdx.x] = t;
 }
 
-template <int EB>
+template <int EB, bool ORDERED>
 __global__ __launch_bounds__(kBlock) void k_bdense_numeric(BDenseArgs a) {
   constexpr int E = 1 << EB;
-  __shared__ float s_sum[E];
-  __shared__ float s_cmp[E];
+  __shared__ double s_acc[E];
   __shared__ DenseSmem<EB> sm;
   __shared__ uint64_t s_ja;
   if (*a.bad) return;
@@ -419,9 +418,9 @@ __global__ __launch_bounds__(kBlock) void k_bdense_numeric(BDenseArgs a) {
   bdense_row(a, blockIdx.x, r, in);
   const uint32_t g = find_seg(a.grp_rows, a.G, blockIdx.x);
   // instance-local row id
-  dense_row_numeric<EB>(a.d, a.d.row_act[r], a.d.row_act[r + 1], low_mask(in.kyf),
+  dense_row_numeric<EB, ORDERED>(a.d, a.d.row_act[r], a.d.row_act[r + 1], low_mask(in.kyf),
                         a.ytab + in.yt_off, (uint32_t)(blockIdx.x - a.grp_rows[g]),
-                        a.d.out_off[blockIdx.x], s_sum, s_cmp, sm, &s_ja);
+                        a.d.out_off[blockIdx.x], s_acc, sm, &s_ja);
 }
 
 __global__ __launch_bounds__(kBlock) void k_chunk_row_maps(const BPiece* pieces,
@@ -597,10 +596,19 @@ void launch_bdense_count(const BDenseArgs& a, int eb, uint64_t rows, hipStream_t
 
 void launch_bdense_numeric(const BDenseArgs& a, int eb, uint64_t rows, hipStream_t s) {
   if (!rows) return;
-  switch (eb) {
-    case 8: hipLaunchKernelGGL(k_bdense_numeric<8>, dim3((uint32_t)rows), dim3(kBlock), 0, s, a); break;
-    case 10: hipLaunchKernelGGL(k_bdense_numeric<10>, dim3((uint32_t)rows), dim3(kBlock), 0, s, a); break;
-    default: hipLaunchKernelGGL(k_bdense_numeric<13>, dim3((uint32_t)rows), dim3(kBlock), 0, s, a); break;
+  const dim3 g((uint32_t)rows), b(kBlock);
+  if (dense_ordered()) {
+    switch (eb) {
+      case 8: hipLaunchKernelGGL((k_bdense_numeric<8, true>), g, b, 0, s, a); break;
+      case 10: hipLaunchKernelGGL((k_bdense_numeric<10, true>), g, b, 0, s, a); break;
+      default: hipLaunchKernelGGL((k_bdense_numeric<13, true>), g, b, 0, s, a); break;
+    }
+  } else {
+    switch (eb) {
+      case 8: hipLaunchKernelGGL((k_bdense_numeric<8, false>), g, b, 0, s, a); break;
+      case 10: hipLaunchKernelGGL((k_bdense_numeric<10, false>), g, b, 0, s, a); break;
+      default: hipLaunchKernelGGL((k_bdense_numeric<13, false>), g, b, 0, s, a); break;
+    }
   }
 }
 

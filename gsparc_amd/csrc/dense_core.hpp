@@ -100,21 +100,25 @@ __device__ __forceinline__ uint32_t dense_row_count(const DenseArgs& a, uint64_t
   return t;
 }
 
-// Numeric pass: compensated sums per rank in LDS, then the touched slots in
-// rank order to out_* at obase (row id `row`, column ytab[rank] or rank).
-template <int EB>
+// Numeric pass, then the touched slots in rank order to out_* at obase (row
+// id `row`, column ytab[rank] or rank).  `acc` holds E doubles of LDS.
+//   ORDERED: per slot a Neumaier-compensated FP32 sum in owner order (the
+//            round's owners applied one after another, a barrier between
+//            owners) -- bit-identical to the ESC path;
+//   else:    per slot an FP64 sum by LDS atomic add (no barriers), rounded
+//            once to FP32 -- within FP32 rounding of the exact sum, the
+//            order of concurrent adds not fixed.
+template <int EB, bool ORDERED>
 __device__ __forceinline__ void dense_row_numeric(const DenseArgs& a, uint64_t j0, uint64_t j1,
                                                   uint64_t cmask, const uint64_t* ytab,
-                                                  uint32_t row, uint64_t obase, float* s_sum,
-                                                  float* s_cmp, DenseSmem<EB>& sm,
-                                                  uint64_t* s_ja) {
+                                                  uint32_t row, uint64_t obase, double* acc,
+                                                  DenseSmem<EB>& sm, uint64_t* s_ja) {
   constexpr int E = DenseSmem<EB>::E;
   constexpr int S = E / kBlock;  // slots per thread in the compaction (1..32)
   static_assert(E >= kBlock && E % kBlock == 0, "dense row width");
-  for (int i = threadIdx.x; i < E; i += kBlock) {
-    s_sum[i] = 0.f;
-    s_cmp[i] = 0.f;
-  }
+  float* s_sum = reinterpret_cast<float*>(acc);
+  float* s_cmp = s_sum + E;
+  for (int i = threadIdx.x; i < E; i += kBlock) acc[i] = 0.0;
   for (int i = threadIdx.x; i < DenseSmem<EB>::W; i += kBlock) sm.bm[i] = 0;
   const uint64_t p0 = j0 < j1 ? a.act_prefix[j0] : 0;
   const uint64_t p1 = j0 < j1 ? a.act_prefix[j1] : 0;
@@ -138,6 +142,17 @@ __device__ __forceinline__ void dense_row_numeric(const DenseArgs& a, uint64_t j
         rank[g] = (uint32_t)(a.ykeys[y] & cmask);
         val[g] = a.act_xval[ja + k] * a.yvals[y];
       }
+    }
+    if constexpr (!ORDERED) {
+#pragma unroll
+      for (int g = 0; g < kDenseG; ++g) {
+        if (own[g] != ~0u) {
+          vsum += (double)val[g];
+          atomicAdd(&acc[rank[g]], (double)val[g]);
+          atomicOr(&sm.bm[rank[g] >> 5], 1u << (rank[g] & 31));
+        }
+      }
+      continue;  // the next round's staging starts with a barrier
     }
     // owners of this round in order: the first is own of product pb, the
     // last that of pl = min(pb + round, p1) - 1
@@ -174,7 +189,7 @@ __device__ __forceinline__ void dense_row_numeric(const DenseArgs& a, uint64_t j
       const uint32_t rk = first + e;
       a.out_row[obase + o] = row;
       a.out_col[obase + o] = ytab ? ytab[rk] : rk;
-      a.out_val[obase + o] = s_sum[rk] + s_cmp[rk];
+      a.out_val[obase + o] = ORDERED ? s_sum[rk] + s_cmp[rk] : (float)acc[rk];
       ++o;
     }
   }

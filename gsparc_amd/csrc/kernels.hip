@@ -2,6 +2,8 @@
 // reduce.  Host launchers are declared in kernels.hpp.
 #include "dense_core.hpp"
 #include "kernels.hpp"
+
+#include <string>
 #include "reduce_core.hpp"
 
 #include <cstdlib>
@@ -854,16 +856,25 @@ __global__ __launch_bounds__(kBlock) void k_dense_count(DenseArgs a) {
   if (threadIdx.x == 0) a.row_cnt[blockIdx.x] = t;
 }
 
-template <int EB>
+template <int EB, bool ORDERED>
 __global__ __launch_bounds__(kBlock) void k_dense_numeric(DenseArgs a) {
   constexpr int E = 1 << EB;
-  __shared__ float s_sum[E];
-  __shared__ float s_cmp[E];
+  __shared__ double s_acc[E];
   __shared__ DenseSmem<EB> sm;
   __shared__ uint64_t s_ja;
   const uint64_t r = a.r0 + blockIdx.x;
-  dense_row_numeric<EB>(a, a.row_act[r], a.row_act[r + 1], a.fmask, a.ytab, (uint32_t)r,
-                        a.out_off[blockIdx.x], s_sum, s_cmp, sm, &s_ja);
+  dense_row_numeric<EB, ORDERED>(a, a.row_act[r], a.row_act[r + 1], a.fmask, a.ytab,
+                                 (uint32_t)r, a.out_off[blockIdx.x], s_acc, sm, &s_ja);
+}
+
+// Ordered (bit-identical to ESC) unless SPTC_DENSE_ORDER=atomic selects the
+// FP64 LDS-atomic accumulation.
+bool dense_ordered() {
+  static const bool ord = [] {
+    const char* e = getenv("SPTC_DENSE_ORDER");
+    return !(e && std::string(e) == "atomic");
+  }();
+  return ord;
 }
 
 void launch_dense_count(const DenseArgs& a, int eb, uint64_t rows, hipStream_t s) {
@@ -877,10 +888,19 @@ void launch_dense_count(const DenseArgs& a, int eb, uint64_t rows, hipStream_t s
 
 void launch_dense_numeric(const DenseArgs& a, int eb, uint64_t rows, hipStream_t s) {
   if (!rows) return;
-  switch (eb) {
-    case 8: hipLaunchKernelGGL(k_dense_numeric<8>, dim3((uint32_t)rows), dim3(kBlock), 0, s, a); break;
-    case 10: hipLaunchKernelGGL(k_dense_numeric<10>, dim3((uint32_t)rows), dim3(kBlock), 0, s, a); break;
-    default: hipLaunchKernelGGL(k_dense_numeric<13>, dim3((uint32_t)rows), dim3(kBlock), 0, s, a); break;
+  const dim3 g((uint32_t)rows), b(kBlock);
+  if (dense_ordered()) {
+    switch (eb) {
+      case 8: hipLaunchKernelGGL((k_dense_numeric<8, true>), g, b, 0, s, a); break;
+      case 10: hipLaunchKernelGGL((k_dense_numeric<10, true>), g, b, 0, s, a); break;
+      default: hipLaunchKernelGGL((k_dense_numeric<13, true>), g, b, 0, s, a); break;
+    }
+  } else {
+    switch (eb) {
+      case 8: hipLaunchKernelGGL((k_dense_numeric<8, false>), g, b, 0, s, a); break;
+      case 10: hipLaunchKernelGGL((k_dense_numeric<10, false>), g, b, 0, s, a); break;
+      default: hipLaunchKernelGGL((k_dense_numeric<13, false>), g, b, 0, s, a); break;
+    }
   }
 }
 

@@ -105,6 +105,9 @@ constexpr int kDenseMaxBits = 13;  // dense LDS rows up to 8192 slots
 // nonzero, against ~1e10 for ESC on the same instances).
 constexpr uint64_t kDenseMinRun = 16;
 
+// Dense numeric passes sum in owner order (bit-identical to ESC) unless the
+// FP64 LDS-atomic form is selected (SPTC_DENSE_ORDER=atomic).
+bool dense_ordered();
 void launch_dense_count(const DenseArgs& a, int eb, uint64_t rows, hipStream_t s);
 void launch_dense_numeric(const DenseArgs& a, int eb, uint64_t rows, hipStream_t s);
 void launch_any_dup(const uint64_t* keys, uint64_t n, uint32_t* flag, hipStream_t s);

@@ -8,6 +8,8 @@ oracle in a process of its own, one after another.
   SPTC_SORT=onesweep   operand sorts with a decoupled look-back scan
   SPTC_PROD_SORT=full  one sort over all product-key bits (64-bit keys)
   SPTC_XCD_REMAP=0     blockIdx-order tiles instead of XCD-contiguous ones
+  SPTC_DENSE_ORDER=atomic  dense rows summed by FP64 LDS atomics (checked
+                       against the oracle and the golden fixtures instead)
 """
 import os
 import subprocess
@@ -20,7 +22,7 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 VARIANTS = [{"SPTC_SEG_WIDE": "1"}, {"SPTC_SEG_IPT": "12"}, {"SPTC_SEG_IPT": "24"},
             {"SPTC_SEG_RB": "8"}, {"SPTC_SORT": "onesweep"}, {"SPTC_PROD_SORT": "full"},
-            {"SPTC_XCD_REMAP": "0"}]
+            {"SPTC_XCD_REMAP": "0"}, {"SPTC_DENSE_ORDER": "atomic"}]
 
 
 @pytest.mark.parametrize("env", VARIANTS, ids=lambda e: ",".join(f"{k}={v}" for k, v in e.items()))
