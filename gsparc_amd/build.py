@@ -20,7 +20,7 @@ CSRC = os.path.join(PKG_DIR, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 HIP_SOURCES = ["sort.hip", "segsort.hip", "kernels.hip", "batch.hip", "result.hip", "noalloc.hip",
-               "tdense.hip",
+               "tdense.hip", "bucket.hip",
                "device.cpp", "engine.cpp", "staticpart.cpp",
                "batch_engine.cpp", "serve.cpp", "api.cpp"]
 HIP_FLAGS = [

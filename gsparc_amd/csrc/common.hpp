@@ -166,15 +166,19 @@ __device__ __forceinline__ T block_excl_scan(T v, T* scratch, T* total) {
   return wave_off + inc - v;
 }
 
-// Neumaier-compensated FP32 accumulation (error ~2u independent of count).
+// One Neumaier-compensated FP32 step: sum s, compensation c, the sum's
+// value is s + c (error ~2u independent of the count).  Every accumulator
+// takes this one step, so sums of the same values in the same order agree
+// bit for bit whichever kernel adds them.
+__device__ __forceinline__ void kahan_step(float& s, float& c, float x) {
+  const float t = s + x;
+  if (fabsf(s) >= fabsf(x)) c += (s - t) + x;
+  else c += (x - t) + s;
+  s = t;
+}
 struct KahanF32 {
   float s = 0.f, c = 0.f;
-  __device__ __forceinline__ void add(float x) {
-    float t = s + x;
-    if (fabsf(s) >= fabsf(x)) c += (s - t) + x;
-    else c += (x - t) + s;
-    s = t;
-  }
+  __device__ __forceinline__ void add(float x) { kahan_step(s, c, x); }
   __device__ __forceinline__ float value() const { return s + c; }
 };
 
@@ -193,6 +197,10 @@ struct NarrowRows {
   // per wave tile t (512 products from t * 512): [2t] = row of its first
   // product, [2t + 1] = row of the product before it
   const uint32_t* tile_row;
+  // per row: nonzero when the row's outputs are produced elsewhere (big
+  // rows reduced by high-digit buckets): its products delimit runs but are
+  // neither counted nor emitted.  NULL: every row is reduced here.
+  const uint64_t* big = nullptr;
 };
 
 // ---- launchers (defined in the .hip files) --------------------------------

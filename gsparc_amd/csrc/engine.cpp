@@ -1,6 +1,8 @@
 // SpTC engine host orchestration (built with hipcc; launches live in .hip).
 #include "engine.hpp"
 
+#include "bucket.hpp"
+
 #include "tdense.hpp"
 
 #include <algorithm>
@@ -118,6 +120,14 @@ bool prefer_row_sort() {
     return !(e && std::string(e) == "full");
   }();
   return rows;
+}
+
+bool prefer_bucket_tail() {
+  static const bool msd = [] {
+    const char* e = getenv("SPTC_BIG_TAIL");
+    return e && std::string(e) == "buckets";
+  }();
+  return msd;
 }
 
 uint64_t count_products(const Operand& X, const Operand& Y, hipStream_t s) {
@@ -733,7 +743,7 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
         h->out0 = 0;
         SPTC_HIP(hipMemcpyAsync(rrange.get(), h, sizeof(RowRange), hipMemcpyHostToDevice, s_));
         build_row_segments(rrange.get(), 1, lrows, row_prod.get(), kyf, &rseg, ws_, tmp.get(),
-                           s_);
+                           s_, prefer_bucket_tail());
       }
       ExpandArgs ea;
       ea.act_prefix = act_prefix.get();
@@ -775,7 +785,28 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       // the narrow form of a row-segmented sort, 64-bit key + value otherwise
       kt_.end(tok, SPTC_K_EXPAND, rowsort ? 16.0 * (double)n : 24.0 * (double)n, s_);
       bool alt = false;
-      const NarrowRows nr{fcol, rowsort ? rseg.row_start() : nullptr, r1c - r0c, trow.get()};
+      NarrowRows nr{fcol, rowsort ? rseg.row_start() : nullptr, r1c - r0c, trow.get()};
+      // high-digit form: big rows are reduced by buckets, the narrow reduce
+      // takes the small rows
+      const bool buckets = rowsort && rseg.msd && rseg.nbig_rows > 0;
+      BucketArgs ba{};
+      if (buckets) {
+        nr.big = rseg.row_big();
+        ba.segs = rseg.bigseg.get();
+        ba.tiles = rseg.big.get();
+        ba.nsegs = rseg.nbig_rows;
+        ba.starts = ws_.rts_cnt.get();
+        ba.boff = ws_.rts_off.get();
+        ba.cols = fcol;
+        ba.vals = vA.get();
+        ba.bigrow = rseg.bigrow.get();
+        ba.low = kyf - rseg.rb;
+        ba.gshift = bucket_group_shift(n - rseg.nsmall, rseg.nbig_rows);
+        ba.tile_sums = tmp.get();
+        ba.r0 = r0c;
+        ba.ytab = ytab.size() ? ytab.get() : nullptr;
+        ba.ytab32 = ytab32.size() ? ytab32.get() : nullptr;
+      }
       if (rowsort) {
         launch_tile_rows(rseg.row_start(), r1c - r0c, n, trow.get(), s_);
         sort_row_segments(ws_, rseg, n, fcol, vA.get(), ws_.alt_keys.get(), ws_.alt_vals.get(),
@@ -788,8 +819,15 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       const uint64_t* sk = alt ? ws_.alt_keys.get() : kA.get();
       const float* sv = alt ? ws_.alt_vals.get() : vA.get();
       tok = kt_.begin(s_);
-      if (rowsort) run_reduce_count_narrow(nr, n, tmp.get(), s_);
-      else run_reduce_count(sk, n, tmp.get(), s_);
+      if (rowsort) {
+        run_reduce_count_narrow(nr, n, tmp.get(), s_, /*scan=*/!buckets);
+        if (buckets) {
+          run_bucket_count(ba, s_);
+          scan_narrow_tiles(n, tmp.get(), s_);
+        }
+      } else {
+        run_reduce_count(sk, n, tmp.get(), s_);
+      }
       kt_.end(tok, SPTC_K_REDUCE, (rowsort ? 4.0 : 8.0) * (double)n, s_);
       const uint64_t out_n =
           read_total(tmp.get() + (rowsort ? narrow_tile_count(n) : scan_tmp_count(n))).a;
@@ -808,11 +846,17 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       oc.col.reset(out_n);
       oc.val.reset(out_n);
       tok = kt_.begin(s_);
-      if (rowsort)
+      if (rowsort) {
         run_reduce_emit_narrow(nr, vA.get(), n, oc.row.get(), oc.col.get(), oc.val.get(), r0c,
                                ytab.size() ? ytab.get() : nullptr,
                                ytab32.size() ? ytab32.get() : nullptr, tmp.get(), s_);
-      else
+        if (buckets) {
+          ba.out_row = oc.row.get();
+          ba.out_col = oc.col.get();
+          ba.out_val = oc.val.get();
+          run_bucket_emit(ba, s_);
+        }
+      } else
         run_reduce_emit(sk, sv, n, oc.row.get(), oc.col.get(), oc.val.get(), r0c, kyf, fmask,
                         ytab.size() ? ytab.get() : nullptr, tmp.get(), s_);
       kt_.end(tok, SPTC_K_REDUCE, (rowsort ? 8.0 : 12.0) * (double)n + 16.0 * (double)out_n,

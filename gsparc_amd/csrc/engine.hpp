@@ -117,6 +117,10 @@ uint64_t count_products(const Operand& X, const Operand& Y, hipStream_t s);
 // X row, so only the column bits need sorting; see build_row_segments) unless
 // SPTC_PROD_SORT=full asks for one sort over all key bits.
 bool prefer_row_sort();
+// SPTC_BIG_TAIL=buckets: big rows of a row-segmented sort take the
+// high-digit form (one pass, then an LDS reduce per bucket) where the column
+// bits allow; by default they are sorted on all column bits.
+bool prefer_bucket_tail();
 
 class Engine {
  public:

@@ -723,6 +723,11 @@ __global__ __launch_bounds__(kBlock) void k_reduce_emit_narrow(ReduceOp op, cons
   __shared__ WaveSmem sm[kBlock / kWave];
   const uint64_t wt = (uint64_t)blockIdx.x * (kBlock / kWave) + wave_id();
   if (wt >= ntiles) return;
+  {  // a tile inside one big row has no outputs here
+    const uint64_t b = wt * kWaveTile;
+    const uint32_t cnt = (uint32_t)(op.n - b < kWaveTile ? op.n - b : kWaveTile);
+    if (wave_tile_all_big(op.nr, b, cnt, op.nr.tile_row[2 * wt])) return;
+  }
   WaveSmem& w = sm[wave_id()];
   uint64_t r0, rp;
   bool rows;
@@ -782,12 +787,21 @@ __global__ __launch_bounds__(kBlock) void k_reduce_heads_narrow(NarrowRows nr, u
   if (wt >= ntiles) return;
   const uint64_t b = wt * kWaveTile;
   const uint32_t cnt = (uint32_t)(n - b < kWaveTile ? n - b : kWaveTile);
+  const uint64_t r0 = nr.tile_row[2 * wt];
+  if (wave_tile_all_big(nr, b, cnt, r0)) {
+    if (lane_id() == 0) tile_sums[wt] = mk2(0, 0);
+    return;
+  }
   uint32_t col[kScanItems], mk[kScanItems];
   float unused[kScanItems];
   wave_tile_load<false>(nr.cols + b, nullptr, cnt, kScanItems * lane_id(), col, unused);
   bool rowsin;
-  const uint32_t hm = wave_tile_heads(nr, b, cnt, nr.tile_row[2 * wt], col,
-                                      mark[wave_id()], mk, &rowsin);
+  uint32_t hm = wave_tile_heads(nr, b, cnt, r0, col, mark[wave_id()], mk, &rowsin);
+  if (nr.big) {
+    uint32_t ro[kScanItems];
+    wave_tile_rows(mk, rowsin, ro);
+    hm = wave_emit_mask(nr, r0, rowsin, ro, hm);
+  }
   const uint32_t tot = __builtin_amdgcn_readlane(wave_incl_scan32((uint32_t)__popc(hm)),
                                                  kWave - 1);
   if (lane_id() == 0) tile_sums[wt] = mk2(tot, 0);
@@ -824,13 +838,19 @@ void launch_tile_rows(const uint64_t* rstart, uint64_t nrows, uint64_t n, uint32
                      rstart, nrows, nt, tile_row);
 }
 
-void run_reduce_count_narrow(const NarrowRows& nr, uint64_t n, U2* tmp, hipStream_t s) {
+void run_reduce_count_narrow(const NarrowRows& nr, uint64_t n, U2* tmp, hipStream_t s,
+                             bool scan) {
   const uint64_t nt = narrow_tile_count(n);
   if (n) {
     check_narrow(nr, nullptr);
     hipLaunchKernelGGL(k_reduce_heads_narrow, dim3((uint32_t)ceil_div(nt, kBlock / kWave)),
                        dim3(kBlock), 0, s, nr, n, nt, tmp);
   }
+  if (scan) scan_narrow_tiles(n, tmp, s);
+}
+
+void scan_narrow_tiles(uint64_t n, U2* tmp, hipStream_t s) {
+  const uint64_t nt = narrow_tile_count(n);
   scan_tiles_parts(tmp, n ? nt : 0, tmp + nt, tmp + nt + 1, s);
 }
 

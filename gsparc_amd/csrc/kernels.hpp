@@ -78,6 +78,9 @@ struct ExpandArgs {
   int passes;
   double* vsum;
   uint32_t remap;  // XCD-contiguous tile order (set by launch_expand)
+  // big-row tiles count the first pass's digits at this column shift
+  // (set by launch_expand_items from the plan)
+  int dshift = 0;
 };
 
 struct DenseArgs {
@@ -144,7 +147,11 @@ void launch_tile_rows(const uint64_t* rstart, uint64_t nrows, uint64_t n, uint32
 // Narrow-form count and emit (outputs' rows are r0 + chunk-local row): tmp
 // holds narrow_tmp_count(n) entries, the total at tmp[narrow_tile_count(n)];
 // columns and values 16-byte aligned.
-void run_reduce_count_narrow(const NarrowRows& nr, uint64_t n, U2* tmp, hipStream_t s);
+// scan = false leaves the per-tile counts in tmp unscanned (more counts are
+// added to them before scan_narrow_tiles).
+void run_reduce_count_narrow(const NarrowRows& nr, uint64_t n, U2* tmp, hipStream_t s,
+                             bool scan = true);
+void scan_narrow_tiles(uint64_t n, U2* tmp, hipStream_t s);
 // ytab32 (may be NULL): the same table as 32-bit words, used when set.
 void run_reduce_emit_narrow(const NarrowRows& nr, const float* vals, uint64_t n,
                             uint32_t* out_row, uint64_t* out_col, float* out_val, uint64_t r0,

@@ -226,6 +226,57 @@ __device__ __forceinline__ uint32_t wave_tile_heads(const NarrowRows& nr, uint64
   return hm;
 }
 
+// Row of each of the lane's products as an offset from the tile's first row
+// r0: a prefix maximum of the marks mk (lane, then wave); all 0 when no row
+// starts inside the tile.
+__device__ __forceinline__ void wave_tile_rows(const uint32_t (&mk)[kScanItems], bool rowsin,
+                                               uint32_t (&ro)[kScanItems]) {
+  if (!rowsin) {
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) ro[j] = 0;
+    return;
+  }
+  const uint32_t lane = lane_id();
+  uint32_t pm = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    pm = mk[j] > pm ? mk[j] : pm;
+    ro[j] = pm;
+  }
+  uint32_t inc = pm;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint32_t o = __shfl_up(inc, d, kWave);
+    if (lane >= (uint32_t)d) inc = o > inc ? o : inc;
+  }
+  uint32_t ex = __shfl_up(inc, 1, kWave);
+  if (lane == 0) ex = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) ro[j] = ro[j] > ex ? ro[j] : ex;
+}
+
+// The heads of hm that produce outputs here: all of them unless rows are
+// marked big (nr.big), whose heads only end the run before them.
+__device__ __forceinline__ uint32_t wave_emit_mask(const NarrowRows& nr, uint64_t r0,
+                                                   bool rowsin,
+                                                   const uint32_t (&ro)[kScanItems],
+                                                   uint32_t hm) {
+  if (!nr.big || !hm) return hm;
+  if (!rowsin) return nr.big[r0] ? 0u : hm;
+  uint32_t em = hm;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j)
+    if (((hm >> j) & 1u) && nr.big[r0 + ro[j]]) em &= ~(1u << j);
+  return em;
+}
+
+// True when all cnt products of the wave tile at b lie in row r0 and that
+// row is big (uniform across the wave).
+__device__ __forceinline__ bool wave_tile_all_big(const NarrowRows& nr, uint64_t b, uint32_t cnt,
+                                                  uint64_t r0) {
+  return nr.big && nr.big[r0] && (r0 + 1 >= nr.nrows || nr.rstart[r0 + 1] >= b + cnt);
+}
+
 // Compensated sums of the lane's products: res[j] = the running sum after
 // product j of the run it belongs to (runs restart at the heads); acc is
 // left holding the lane's last run.  FULL: all 8 products are in the tile.
@@ -265,29 +316,10 @@ __device__ __forceinline__ uint32_t wave_tile_reduce(const NarrowRows& nr,
   const uint32_t hm = wave_tile_heads(nr, b, cnt, r0, col, w.m, mk, &rowsin);
   // row of each product: prefix maximum of the marks (lane, then wave)
   uint32_t ro[kScanItems];
-  if (rowsin) {
-    uint32_t pm = 0;
-#pragma unroll
-    for (int j = 0; j < kScanItems; ++j) {
-      pm = mk[j] > pm ? mk[j] : pm;
-      ro[j] = pm;
-    }
-    uint32_t inc = pm;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-      const uint32_t o = __shfl_up(inc, d, kWave);
-      if (lane >= (uint32_t)d) inc = o > inc ? o : inc;
-    }
-    uint32_t ex = __shfl_up(inc, 1, kWave);
-    if (lane == 0) ex = 0;
-#pragma unroll
-    for (int j = 0; j < kScanItems; ++j) ro[j] = ro[j] > ex ? ro[j] : ex;
-  } else {
-#pragma unroll
-    for (int j = 0; j < kScanItems; ++j) ro[j] = 0;
-  }
-  // output slots: exclusive scan of the lanes' head counts
-  const uint32_t h = (uint32_t)__popc(hm);
+  wave_tile_rows(mk, rowsin, ro);
+  // output slots: exclusive scan of the lanes' emitting head counts
+  const uint32_t em = wave_emit_mask(nr, r0, rowsin, ro, hm);
+  const uint32_t h = (uint32_t)__popc(em);
   const uint32_t hinc = wave_incl_scan32(h);
   const uint32_t o = hinc - h;
   const uint32_t T = __builtin_amdgcn_readlane(hinc, kWave - 1);
@@ -348,19 +380,22 @@ __device__ __forceinline__ uint32_t wave_tile_reduce(const NarrowRows& nr,
   uint32_t k = o;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
-    if ((hm >> j) & 1u) {
+    if ((em >> j) & 1u) {
       w.c[k] = col[j];
       if (rowsin) w.m[k] = ro[j];
       ++k;
     }
   }
   // a run ends before each later head of the lane; the lane's last run
-  // ends with `last`
+  // ends with `last`; only emitting heads' runs take an output slot
   k = o;
+  bool emits = false;  // the current run (started in this lane) emits
 #pragma unroll
-  for (int j = 0; j + 1 < kScanItems; ++j)
-    if (((hm >> (j + 1)) & 1u) && (hm & ((2u << j) - 1u))) w.v[k++] = res[j];
-  if (hm) w.v[o + h - 1] = last;
+  for (int j = 0; j + 1 < kScanItems; ++j) {
+    if ((hm >> j) & 1u) emits = (em >> j) & 1u;
+    if (((hm >> (j + 1)) & 1u) && emits) w.v[k++] = res[j];
+  }
+  if (hm && ((em >> (31 - __builtin_clz(hm))) & 1u)) w.v[o + h - 1] = last;
   wave_sync();
   *rows = rowsin;
   return T;

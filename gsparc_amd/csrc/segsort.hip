@@ -163,7 +163,7 @@ __global__ __launch_bounds__(kBlock) void k_rowseg_emit(
     const RowRange* ranges, uint32_t nr, uint64_t lrows, const uint64_t* row_prod, int rb,
     uint64_t tile,
     const uint64_t* big_t, const uint64_t* bt, const uint64_t* bs, const uint64_t* head,
-    const uint64_t* hs, LocalSeg* local, SegTile* big, SegDesc* bigseg) {
+    const uint64_t* hs, LocalSeg* local, SegTile* big, SegDesc* bigseg, uint32_t* bigrow) {
   uint64_t l = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (; l < lrows; l += stride) {
@@ -177,6 +177,7 @@ __global__ __launch_bounds__(kBlock) void k_rowseg_emit(
       g.first_tile = (uint32_t)o;
       g.ntiles = (uint32_t)nt;
       bigseg[bs[l]] = g;
+      if (bigrow) bigrow[bs[l]] = (uint32_t)l;
       for (uint64_t t = 0; t < nt; ++t) {
         SegTile d;
         d.start = first + t * tile;
@@ -413,8 +414,26 @@ __global__ __launch_bounds__(kBlock) void k_expand_items(ExpandArgs a,
           __builtin_nontemporal_store(v, &oval[p - a.P0]);
           vsum += (double)v;
         }
-        // first-pass digits are spread (low column bits): plain LDS atomics
-        if (isbig && valid) atomicAdd(&sh[col & (R - 1)], 1u);
+        if (isbig) {
+          if (!a.dshift) {
+            // low column bits are spread: plain LDS atomics
+            if (valid) atomicAdd(&sh[col & (R - 1)], 1u);
+          } else {
+            // high digits come in runs (each owner's Y segment is sorted by
+            // column): the first lane of each run adds the run's length
+            const uint32_t lane = lane_id();
+            const uint32_t d = (col >> a.dshift) & (R - 1);
+            const uint32_t pd = __shfl_up(d, 1, kWave);
+            const bool lead = valid && (lane == 0 || pd != d);
+            const uint64_t lm = __ballot(lead);
+            const uint32_t nv = (uint32_t)__popcll(__ballot(valid));  // a prefix of the lanes
+            if (lead) {
+              const uint64_t above = lane + 1 < (uint32_t)kWave ? lm >> (lane + 1) : 0ull;
+              const uint32_t end = above ? lane + 1 + (uint32_t)__builtin_ctzll(above) : nv;
+              atomicAdd(&sh[d], end - lane);
+            }
+          }
+        }
       }
     }
     if (isbig) {
@@ -553,15 +572,19 @@ static int narrow_ipt() {
 static int narrow_passes(int col_bits) {
   return std::max(1, radix_passes(col_bits, narrow_rb(col_bits)));
 }
+static int plan_passes(const RowSegPlan& plan) {
+  return plan.msd ? 1 : narrow_passes(plan.col_bits);
+}
 
 void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows,
                         const uint64_t* row_prod, int col_bits, RowSegPlan* plan,
-                        SortWorkspace& ws, U2* tmp, hipStream_t s) {
+                        SortWorkspace& ws, U2* tmp, hipStream_t s, bool msd) {
   if (col_bits > 32) throw std::logic_error("row-segmented sort of more than 32 column bits");
   plan->nlocal = plan->nbig_tiles = plan->nbig_rows = plan->nsmall = 0;
   plan->lrows = lrows;
   plan->col_bits = col_bits;
-  plan->rb = narrow_rb(col_bits);
+  plan->msd = msd && col_bits > kBucketRB && col_bits <= kBucketRB + kBucketMaxLow;
+  plan->rb = plan->msd ? kBucketRB : narrow_rb(col_bits);
   plan->ipt = narrow_ipt();
   const uint64_t tile = (uint64_t)plan->ipt * kSegNT;
   if (!lrows) return;
@@ -600,12 +623,14 @@ void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows
   if (plan->local.size() < plan->nlocal) plan->local.reset(plan->nlocal);
   if (plan->big.size() < plan->nbig_tiles) plan->big.reset(plan->nbig_tiles);
   if (plan->bigseg.size() < plan->nbig_rows) plan->bigseg.reset(plan->nbig_rows);
+  if (plan->msd && plan->bigrow.size() < plan->nbig_rows) plan->bigrow.reset(plan->nbig_rows);
   if (plan->nlocal)
     hipLaunchKernelGGL(k_zero_local, dim3((uint32_t)ceil_div(plan->nlocal, 256)), dim3(256), 0,
                        s, plan->local.get(), plan->nlocal);
   hipLaunchKernelGGL(k_rowseg_emit, dim3(grid), dim3(kBlock), 0, s, ranges, nranges, lrows,
                      row_prod, plan->rb, tile, big_t, bt, bs, head, hs, plan->local.get(),
-                     plan->big.get(), plan->bigseg.get());
+                     plan->big.get(), plan->bigseg.get(),
+                     plan->msd ? plan->bigrow.get() : nullptr);
   if (plan->nbig_tiles) segmented_sort_reserve(ws, plan->nbig_tiles, s, plan->rb);
 }
 
@@ -617,7 +642,7 @@ void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows
 NarrowTarget narrow_target(const RowSegPlan& plan, uint64_t n, float* v, uint64_t* k1,
                            float* v1) {
   uint32_t* c = reinterpret_cast<uint32_t*>(k1);
-  if (narrow_passes(plan.col_bits) & 1) return NarrowTarget{c, v1};
+  if (plan_passes(plan) & 1) return NarrowTarget{c, v1};
   return NarrowTarget{c + n, v};
 }
 
@@ -631,11 +656,11 @@ static void narrow_sort(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, u
   const uint64_t nb = n - plan.nsmall;  // big-row products
   uint32_t* col[2] = {reinterpret_cast<uint32_t*>(k1), reinterpret_cast<uint32_t*>(k1) + n};
   float* val[2] = {v1, v};
-  const int passes = narrow_passes(plan.col_bits);
+  const int passes = plan_passes(plan);
   int cur = (passes & 1) ? 0 : 1;
   const SegTile* tiles = plan.big.get();
   for (int p = 0; p < passes; ++p) {
-    const int shift = p * RB;
+    const int shift = plan.msd ? plan.col_bits - RB : p * RB;
     int tok = kt ? kt->begin(s) : -1;
     if (p > 0 || !counted)  // else the expansion counted the first pass
       hipLaunchKernelGGL((k_seg_upsweep<uint32_t, RB, IPT>), dim3((uint32_t)ntiles),
@@ -668,6 +693,7 @@ void launch_expand_items(const ExpandArgs& a, const RowSegPlan& plan, SortWorksp
   const uint32_t grid = (uint32_t)std::min<uint64_t>(items, 8192);
   ExpandArgs b = a;
   b.remap = xcd_remap_enabled();
+  b.dshift = plan.msd ? plan.col_bits - plan.rb : 0;
   if (plan.rb == 10)
     hipLaunchKernelGGL(k_expand_items<10>, dim3(grid), dim3(kBlock), 0, s, b, plan.big.get(),
                        (uint32_t)plan.nbig_tiles, plan.local.get(), (uint32_t)plan.nlocal,

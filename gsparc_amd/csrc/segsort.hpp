@@ -19,6 +19,10 @@ constexpr uint64_t kSegTile = (uint64_t)kSegIPT * kSegNT;
 // 16 or 24; SPTC_SEG_IPT overrides the default).  16 keeps two 512-thread
 // workgroups per CU (112 VGPRs); 24 drops to one.
 constexpr int kNarrowIPT = 16;
+// High-digit form of the big-row sort: one pass on the top kBucketRB column
+// bits, leaving buckets of at most 2^kBucketMaxLow columns.
+constexpr int kBucketRB = 10;
+constexpr int kBucketMaxLow = 10;
 
 // One tile of a segmented sort (never straddles a segment).  Its digit
 // counts are row `tile` of a tile-major count matrix; the scan leaves in
@@ -116,9 +120,15 @@ struct RowSegPlan {
   int col_bits = 0;
   int rb = kSegRB;  // digit bits of the big-row passes (8 or 10)
   int ipt = kNarrowIPT;  // keys per thread of their tiles
+  // high-digit form: the big rows take one pass on their top rb column bits
+  // (10 < col_bits <= 20), which leaves each row's products in 2^rb buckets
+  // of 2^(col_bits - rb) columns, still in expansion order; the buckets are
+  // reduced in LDS (bucket.hpp) instead of being sorted on the low bits
+  bool msd = false;
   DevBuf<LocalSeg> local;
   DevBuf<SegTile> big;
   DevBuf<SegDesc> bigseg;
+  DevBuf<uint32_t> bigrow;   // high-digit form: chunk-local row of each big segment
   DevBuf<uint64_t> scratch;  // 9 x (rows + 1) per-row values and scans
   // per chunk-local row: nonzero when the row is a big segment
   const uint64_t* row_big() const { return scratch.get() + (lrows + 1); }
@@ -127,9 +137,10 @@ struct RowSegPlan {
 };
 // Builds the plan (one host synchronization for the counts); tmp holds
 // scan_tmp_count(lrows) + 1 entries.  Reserves ws for the big-row passes.
+// msd: use the high-digit form where the column bits allow it.
 void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows,
                         const uint64_t* row_prod, int col_bits, RowSegPlan* plan,
-                        SortWorkspace& ws, U2* tmp, hipStream_t s);
+                        SortWorkspace& ws, U2* tmp, hipStream_t s, bool msd = false);
 // Where the expansion writes big-row products of an n-product chunk sorted
 // into (col, v) with spare arrays (k1, v1) of n 64-bit keys and n values: a
 // 32-bit column array and a value array, chosen so that the big-row passes

@@ -356,6 +356,50 @@ def test_long_runs_and_row_starts_across_wave_tiles(gpu, ycols):
     _compare_with_oracle(X, Y, [1], [0], res)
 
 
+def bucket_case(name):
+    """Big rows (tens of thousands of products each) over wide column spaces:
+    the shapes of the high-digit bucket reduce (SPTC_BIG_TAIL=buckets).
+    zipf15: Y columns drawn Zipf, so
+    a bucket holds one column from many owners, often several inside one
+    64-product window (signed values); wide20: 20 column bits, buckets of
+    1024 columns; low11: 11 column bits, buckets of 2 columns."""
+    from gsparc_amd.coo import UNIFORM, ZIPF, VAL_UNIFORM01, generate
+    if name == "zipf15":
+        X = generate(0xB0C1, (24, 3000), (UNIFORM, UNIFORM), (1.0, 1.0), 24 * 500, VAL_UNIFORM01)
+        Y = generate(0xB0C2, (3000, 300_000), (UNIFORM, ZIPF), (1.0, 1.1), 3000 * 40,
+                     VAL_UNIFORM01)
+        rng = np.random.default_rng(17)
+        Y = COOTensor(Y.dims, Y.idx, Y.val * np.where(rng.random(Y.nnz) < 0.5, -1, 1).astype(
+            np.float32))
+    elif name == "wide20":
+        X = generate(0xB0C3, (16, 3000), (UNIFORM, UNIFORM), (1.0, 1.0), 16 * 300, VAL_UNIFORM01)
+        Y = generate(0xB0C4, (3000, 1 << 20), (UNIFORM, UNIFORM), (1.0, 1.0), 3000 * 250,
+                     VAL_UNIFORM01)
+    else:
+        X = generate(0xB0C5, (24, 3000), (UNIFORM, UNIFORM), (1.0, 1.0), 24 * 500, VAL_UNIFORM01)
+        Y = generate(0xB0C6, (3000, 2000), (UNIFORM, UNIFORM), (1.0, 1.0), 3000 * 40,
+                     VAL_UNIFORM01)
+    return X, Y
+
+
+BUCKET_CASES = ["zipf15", "wide20", "low11"]
+
+
+@pytest.mark.parametrize("name", BUCKET_CASES)
+def test_big_rows_wide_columns_against_oracle(gpu, name):
+    """Big rows over 11 to 20 column bits (two column passes, or with
+    SPTC_BIG_TAIL=buckets one pass and buckets of 2 to 1024 columns),
+    against the oracle, count-only included."""
+    X, Y = bucket_case(name)
+    res = sptc.SpTC(X, Y, [1], [0], dense_accumulator=-1)
+    assert res.info.accumulator == 0
+    bits = int(len(np.unique(Y.idx[1])) - 1).bit_length()  # column ranks
+    assert bits == {"zipf15": 15, "wide20": 20, "low11": 11}[name], bits
+    _compare_with_oracle(X, Y, [1], [0], res)
+    c = sptc.SpTC(X, Y, [1], [0], dense_accumulator=-1, count_only=True)
+    assert c.nnz == res.nnz
+
+
 def test_index_range_check(gpu):
     """check_indices: an index at or above its mode's dim is an error; by
     default it is masked to the mode's width, as the reference's pdep

@@ -32,12 +32,33 @@ def dense_checks():
     tb.test_batch_matches_golden_fixtures(None)
 
 
+def digests():
+    """One line per big-row case: a digest of the output's indices and value
+    bits, for comparing two variants bit for bit."""
+    import hashlib
+    from gsparc_amd.coo import UNIFORM, ZIPF, VAL_UNIFORM01, generate
+    cases = [t.bucket_case(n) + ([1], [0]) for n in t.BUCKET_CASES]
+    X = generate(0x5EED00AA, (300, 200, 48), (ZIPF, ZIPF, UNIFORM), (1.3, 1.3, 1.0), 40_000,
+                 VAL_UNIFORM01)
+    cases.append((X, X, [2], [2]))
+    for i, (X, Y, cx, cy) in enumerate(cases):
+        res = sptc.SpTC(X, Y, cx, cy, dense_accumulator=-1)
+        h = hashlib.sha256()
+        for a in res.to_host():
+            h.update(np.ascontiguousarray(a).tobytes())
+        print(f"digest {i} {res.nnz} {h.hexdigest()}", flush=True)
+
+
 def main():
     sptc.lib()
-    if os.environ.get("SPTC_DENSE_ORDER") == "atomic":
+    if "--digest" in sys.argv:
+        digests()
+    elif os.environ.get("SPTC_DENSE_ORDER") == "atomic":
         dense_checks()
     else:
         t.test_row_segments_big_and_small_rows(None)
+        for name in t.BUCKET_CASES:
+            t.test_big_rows_wide_columns_against_oracle(None, name)
         for ycols in (1, 3, 40):
             t.test_long_runs_and_row_starts_across_wave_tiles(None, ycols)
         for n, bits, rb in ((100_000, 43, 8), (1_000_003, 48, 8), (700_000, 36, 9),
