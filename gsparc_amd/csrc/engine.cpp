@@ -6,6 +6,8 @@
 #include "tdense.hpp"
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -722,6 +724,12 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
     DevBuf<RowRange> rrange(1);
     RowSegPlan rseg;
     DevBuf<uint32_t> trow(2 * (narrow_tile_count(chunk_cap) + 1));
+    // verbose: a progress line on stderr at least every 10 s of chunks
+    const auto t_loop = std::chrono::steady_clock::now();
+    auto t_log = t_loop;
+    if (o_.verbose && bounds_r.size() > 1)
+      fprintf(stderr, "[sptc] accumulate: %zu chunks, %llu products\n", bounds_r.size() - 1,
+              (unsigned long long)ir);
     for (size_t c = 1; c < bounds_r.size(); ++c) {
       const uint64_t r0c = bounds_r[c - 1], r1c = bounds_r[c];
       const uint64_t P0 = bounds_p[c - 1], P1 = bounds_p[c];
@@ -833,6 +841,17 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
           read_total(tmp.get() + (rowsort ? narrow_tile_count(n) : scan_tmp_count(n))).a;
       info.nnz += out_n;
       info.nchunks++;
+      if (o_.verbose) {
+        const auto now = std::chrono::steady_clock::now();
+        if (now - t_log >= std::chrono::seconds(10) || c + 1 == bounds_r.size()) {
+          t_log = now;
+          fprintf(stderr, "[sptc] chunk %zu/%zu: %llu of %llu products, %llu outputs, %.1f s\n", c,
+                  bounds_r.size() - 1, (unsigned long long)P1, (unsigned long long)ir,
+                  (unsigned long long)info.nnz,
+                  std::chrono::duration<double>(now - t_loop).count());
+          fflush(stderr);
+        }
+      }
       if (o_.count_only) continue;
       if (DevicePool::get().live_bytes() + out_n * 16 > budget_) {
         msg = "output does not fit the memory budget";

@@ -194,6 +194,22 @@ def test_chunked_equals_unchunked(gpu):
         np.testing.assert_array_equal(x, y)
 
 
+def test_verbose_progress_lines(gpu, capfd):
+    """verbose: the accumulation announces its chunks and reports the last
+    one (products and outputs so far) on stderr."""
+    rng = np.random.default_rng(7)
+    X = random_coo(rng, (60, 50, 40), 60_000)
+    full = sptc.SpTC(X, X, [2], [2], dense_accumulator=-1)
+    capfd.readouterr()
+    res = sptc.SpTC(X, X, [2], [2], dense_accumulator=-1, verbose=True,
+                    mem_budget_bytes=full.nnz * 16 + full.ir_nnz * 8 + (128 << 20))
+    err = capfd.readouterr().err
+    n = res.info.nchunks
+    assert n > 1
+    assert f"[sptc] accumulate: {n} chunks, {res.ir_nnz} products" in err
+    assert f"[sptc] chunk {n}/{n}: {res.ir_nnz} of {res.ir_nnz} products, {res.nnz} outputs" in err
+
+
 def test_plain_small_config(gpu):
     """The 2000x2000x500, 2e5-nnz self-contraction over mode 2: global
     invariants on the whole output plus oracle parity on sampled rows."""
