@@ -7,8 +7,10 @@
 //   k_scan_reduce : per tile of kBlock*kScanItems items, the tile sum
 //   k_scan_tiles  : exclusive scan of tile sums (single workgroup), total
 //   k_scan_emit   : recompute values, block scan + tile offset, emit()
-// Items are handled in a blocked arrangement: thread t of a tile owns items
-// [t*kScanItems, t*kScanItems + kScanItems).
+// Values are computed and emitted tile-striped (item e*kBlock + t of a tile
+// to thread t, so each of the kScanItems loads and stores of a workgroup is
+// one contiguous run); the emit's scan goes through LDS, where thread t
+// takes items [t*kScanItems, t*kScanItems + kScanItems) in order.
 #pragma once
 #include "common.hpp"
 
@@ -60,39 +62,51 @@ __device__ __forceinline__ U2 block_excl_scan_u2(U2 v, U2* scratch, U2* total) {
 template <class Op>
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(Op op, uint64_t n, U2* tile_sums) {
   __shared__ U2 scratch[kBlock / kWave];
-  const uint64_t tile = blockIdx.x;
-  const uint64_t first = tile * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + threadIdx.x;
   U2 s = mk2(0, 0);
 #pragma unroll
   for (int e = 0; e < kScanItems; ++e) {
-    uint64_t i = first + e;
+    const uint64_t i = base + (uint64_t)e * kBlock;
     if (i < n) s = s + op.value(i);
   }
   U2 tot;
   block_excl_scan_u2(s, scratch, &tot);
-  if (threadIdx.x == 0) tile_sums[tile] = tot;
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
 }
 
 template <class Op>
 __global__ __launch_bounds__(kBlock) void k_scan_emit(Op op, uint64_t n, const U2* tile_offs) {
   __shared__ U2 scratch[kBlock / kWave];
-  const uint64_t tile = blockIdx.x;
-  const uint64_t first = tile * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+  __shared__ U2 sv[kScanTile];  // values, then exclusive prefixes, by tile index
+  const uint32_t tid = threadIdx.x;
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
   U2 vals[kScanItems];
+#pragma unroll
+  for (int e = 0; e < kScanItems; ++e) {
+    const uint64_t i = base + (uint64_t)e * kBlock + tid;
+    vals[e] = i < n ? op.value(i) : mk2(0, 0);
+    sv[e * kBlock + tid] = vals[e];
+  }
+  __syncthreads();
+  U2 mine[kScanItems];
   U2 s = mk2(0, 0);
 #pragma unroll
   for (int e = 0; e < kScanItems; ++e) {
-    uint64_t i = first + e;
-    vals[e] = i < n ? op.value(i) : mk2(0, 0);
-    s = s + vals[e];
+    mine[e] = sv[tid * kScanItems + e];
+    s = s + mine[e];
   }
-  U2 tot;
-  U2 excl = block_excl_scan_u2(s, scratch, &tot) + tile_offs[tile];
+  U2 tot;  // (barriers inside: every thread's reads of sv are done after it)
+  U2 excl = block_excl_scan_u2(s, scratch, &tot) + tile_offs[blockIdx.x];
 #pragma unroll
   for (int e = 0; e < kScanItems; ++e) {
-    uint64_t i = first + e;
-    if (i < n) op.emit(i, excl, vals[e]);
-    excl = excl + vals[e];
+    sv[tid * kScanItems + e] = excl;
+    excl = excl + mine[e];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < kScanItems; ++e) {
+    const uint64_t i = base + (uint64_t)e * kBlock + tid;
+    if (i < n) op.emit(i, sv[e * kBlock + tid], vals[e]);
   }
 }
 
