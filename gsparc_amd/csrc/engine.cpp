@@ -775,12 +775,17 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       ea.ycol = ycol.size() ? ycol.get() : nullptr;
       // narrow form: sorted columns land in kA's first 4n bytes
       uint32_t* fcol = reinterpret_cast<uint32_t*>(kA.get());
+      // count-only runs sort the columns alone: no value is written or moved
+      const bool cols_only = rowsort && o_.count_only;
+      float* const pv = cols_only ? nullptr : vA.get();
+      float* const pv1 = cols_only ? nullptr : ws_.alt_vals.get();
       if (rowsort) {
         const NarrowTarget nt =
             narrow_target(rseg, n, vA.get(), ws_.alt_keys.get(), ws_.alt_vals.get());
         ea.big_col = nt.col;
-        ea.big_val = nt.val;
+        ea.big_val = cols_only ? nullptr : nt.val;
         ea.out_col32 = fcol;
+        ea.out_vals = pv;
       }
       ea.hist = ws_.hist.get();
       const bool hist = sort_needs_histogram();
@@ -791,7 +796,8 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       else launch_expand(ea, rb, s_);
       // matched Y (column, value) read; product written: column + value in
       // the narrow form of a row-segmented sort, 64-bit key + value otherwise
-      kt_.end(tok, SPTC_K_EXPAND, rowsort ? 16.0 * (double)n : 24.0 * (double)n, s_);
+      kt_.end(tok, SPTC_K_EXPAND,
+              (rowsort ? (cols_only ? 12.0 : 16.0) : 24.0) * (double)n, s_);
       bool alt = false;
       NarrowRows nr{fcol, rowsort ? rseg.row_start() : nullptr, r1c - r0c, trow.get()};
       // high-digit form: big rows are reduced by buckets, the narrow reduce
@@ -817,8 +823,8 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
       }
       if (rowsort) {
         launch_tile_rows(rseg.row_start(), r1c - r0c, n, trow.get(), s_);
-        sort_row_segments(ws_, rseg, n, fcol, vA.get(), ws_.alt_keys.get(), ws_.alt_vals.get(),
-                          rseg.row_start(), s_, &kt_, SPTC_K_SORT_PROD, /*counted=*/true);
+        sort_row_segments(ws_, rseg, n, fcol, pv, ws_.alt_keys.get(), pv1, rseg.row_start(), s_,
+                          &kt_, SPTC_K_SORT_PROD, /*counted=*/true);
       } else {
         ws_.ensure(n, rb, s_);
         if (hist && passes > kMaxFusedPasses) histogram(ws_, kA.get(), n, passes, s_);

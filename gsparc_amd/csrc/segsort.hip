@@ -57,11 +57,11 @@ __global__ __launch_bounds__(NT) void k_seg_downsweep(
   KI k[IPT];
   float v[IPT];
   uint32_t pos[IPT];
+  const float* vt = vin ? vin + t.start : nullptr;  // NULL values: keys only
   if (full)
-    tile_load_rank<true, RB, IPT, NT>(sm, kin + t.start, vin + t.start, t.count, shift, k, v, pos);
+    tile_load_rank<true, RB, IPT, NT>(sm, kin + t.start, vt, t.count, shift, k, v, pos);
   else
-    tile_load_rank<false, RB, IPT, NT>(sm, kin + t.start, vin + t.start, t.count, shift, k, v,
-                                       pos);
+    tile_load_rank<false, RB, IPT, NT>(sm, kin + t.start, vt, t.count, shift, k, v, pos);
   __syncthreads();
   tile_prefix_fold<RB, IPT, NT>(sm);
   if (full)
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(kSegNT) void k_local_sort(uint32_t* cols, float* va
     const bool valid = idx < n;
     const uint64_t row = pos[i] > before ? pos[i] : before;
     k[i] = valid ? (shl64(row, col_bits) | cols[sg.start + idx]) : 0;
-    v[i] = valid ? vals[sg.start + idx] : 0.f;
+    v[i] = valid && vals ? vals[sg.start + idx] : 0.f;  // vals NULL: columns only
   }
   if (tid == 0) s_k0 = k[0];
   __syncthreads();  // marks read, sm.buf free for the sort
@@ -319,7 +319,7 @@ __global__ __launch_bounds__(kSegNT) void k_local_sort(uint32_t* cols, float* va
     const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
     if (idx < n) {
       cols[sg.start + idx] = (uint32_t)(k[i] & cmask);
-      vals[sg.start + idx] = v[i];
+      if (vals) vals[sg.start + idx] = v[i];
     }
   }
 }
@@ -411,7 +411,7 @@ __global__ __launch_bounds__(kBlock) void k_expand_items(ExpandArgs a,
           const float v = xv * a.yvals[y];
           col = a.ycol[y];
           __builtin_nontemporal_store(col, &ocol[p - a.P0]);
-          __builtin_nontemporal_store(v, &oval[p - a.P0]);
+          if (oval) __builtin_nontemporal_store(v, &oval[p - a.P0]);  // NULL: count-only
           vsum += (double)v;
         }
         if (isbig) {
@@ -681,7 +681,7 @@ static void narrow_sort(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, u
                          dim3(kSegNT), 0, s, col[cur], val[cur], last ? fcol : col[1 - cur],
                          last ? v : val[1 - cur], shift, tiles, ws.rts_off.get(),
                          ws.rts_cnt.get(), xcd_remap());
-    if (kt) kt->end(tok, cls, 16.0 * (double)nb, s);
+    if (kt) kt->end(tok, cls, (v ? 16.0 : 8.0) * (double)nb, s);  // column (+ value) moved
     cur = 1 - cur;
   }
 }
@@ -723,7 +723,7 @@ void sort_row_segments(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, ui
     int tok = kt ? kt->begin(s) : -1;
     hipLaunchKernelGGL(k_local_sort, dim3((uint32_t)plan.nlocal), dim3(kSegNT), 0, s, col, v,
                        plan.local.get(), row_start, plan.lrows, plan.col_bits);
-    if (kt) kt->end(tok, SPTC_K_SORT_LOCAL, 16.0 * (double)plan.nsmall, s);
+    if (kt) kt->end(tok, SPTC_K_SORT_LOCAL, (v ? 16.0 : 8.0) * (double)plan.nsmall, s);
   }
 }
 

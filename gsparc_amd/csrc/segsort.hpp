@@ -155,6 +155,7 @@ NarrowTarget narrow_target(const RowSegPlan& plan, uint64_t n, float* v, uint64_
 // row_start as RowSegPlan::row_start (kept by the caller when the plan's
 // scratch is released).  Launches only (replayable).
 // counted: the first pass's digit counts are already in ws (launch_expand_items).
+// v and v1 NULL: the columns are sorted alone (count-only runs).
 void sort_row_segments(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, uint32_t* col,
                        float* v, uint64_t* k1, float* v1, const uint64_t* row_start,
                        hipStream_t s, KTimer* kt = nullptr, int cls = SPTC_K_SORT_PROD,
@@ -163,7 +164,8 @@ void sort_row_segments(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, ui
 // then small-row groups): a.big_col/big_val from narrow_target, a.out_col32/
 // out_vals the sorted arrays, a.ycol required.  Also writes the big tiles'
 // first-pass digit counts into ws, so sort_row_segments(..., counted=true)
-// skips that pass's upsweep.
+// skips that pass's upsweep.  NULL value targets (a.big_val, a.out_vals):
+// columns only, the value sum still taken.
 void launch_expand_items(const ExpandArgs& a, const RowSegPlan& plan, SortWorkspace& ws,
                          hipStream_t s);
 

@@ -113,7 +113,7 @@ __device__ __forceinline__ void tile_load_rank(TileSmem<RB, IPT, NT, ST>& sm,
     const uint32_t li = tile_local<IPT>(i);
     const bool valid = FULL || li < cnt;
     k[i] = valid ? kb[li] : 0;
-    v[i] = valid ? vb[li] : 0.f;
+    v[i] = valid && vb ? vb[li] : 0.f;  // vb NULL: keys only
   }
   tile_rank<FULL, RB, IPT, NT>(sm, cnt, shift, k, pos);
 }
@@ -181,6 +181,7 @@ __device__ __forceinline__ void tile_prefix_fold(TileSmem<RB, IPT, NT, ST>& sm) 
   }
 }
 
+// vout NULL: keys only (no value staging or stores).
 // Given sm.gbase and sm.dstart (tile totals per digit on entry), scans the
 // totals into tile-local run starts, stages keys in digit order, writes them
 // in coalesced runs, then does the same for the values.  The run starts are
@@ -229,6 +230,7 @@ __device__ __forceinline__ void tile_stage_scatter(TileSmem<RB, IPT, NT, ST>& sm
       kout[dst[i]] = (KO)key;
     }
   }
+  if (!vout) return;  // keys only
   __syncthreads();
   float* vbuf = reinterpret_cast<float*>(sm.buf);
 #pragma unroll
