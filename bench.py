@@ -215,6 +215,8 @@ def bench_batch(args):
         "speedup_vs_plain": plain_s / res["graph"],
         "speedup_eager_vs_plain": plain_s / res["eager"],
         "batch_info": info,
+        # launches per run: the captured graph's nodes (one per kernel or copy)
+        "graph_nodes": b.info["graph_nodes"],
         "plan_run_kernels": kern,
     }
     peak = sptc.device_info()["peak_hbm_gbs"]

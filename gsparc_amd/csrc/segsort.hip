@@ -207,21 +207,23 @@ __global__ void k_zero_local(LocalSeg* local, uint64_t n) {
 }
 
 // One workgroup per small group of (32-bit column, value) products of the
-// consecutive rows row0..: each product's row within the group comes from
-// the row starts (the last row starting at or before it: marks, then a
-// prefix maximum over the wave-striped layout), the composite key
-// (row in group << col_bits | column) is sorted on the bits that vary inside
-// the group entirely in LDS (stable LSD passes), and columns and values are
-// stored back in place.
+// consecutive rows row0..: each product's row rank within the group comes
+// from the row starts (marks at the starts, then a prefix count over the
+// wave-striped layout), the composite key (row rank << col_bits | column)
+// is sorted on the bits that vary inside the group entirely in LDS (stable
+// LSD passes), and columns and values are stored back in place.
+// KT/RB: 32-bit keys with 10-bit digits when (row in group, column) fits 32
+// bits (fewer passes, half the key staging), else 64-bit keys, 8-bit digits.
+template <typename KT, int RB>
 __global__ __launch_bounds__(kSegNT) void k_local_sort(uint32_t* cols, float* vals,
                                                        const LocalSeg* __restrict__ segs,
                                                        const uint64_t* __restrict__ rstart,
                                                        uint64_t lrows, int col_bits) {
-  constexpr int R = 1 << kSegRB;
-  __shared__ TileSmem<kSegRB, kSegIPT, kSegNT> sm;
+  constexpr int R = 1 << RB;
+  __shared__ TileSmem<RB, kSegIPT, kSegNT, KT> sm;
   __shared__ float vbuf[kSegTile];
   __shared__ uint64_t s_or[kSegNT / kWave];
-  __shared__ uint32_t s_wmax[kSegNT / kWave];
+  __shared__ uint32_t s_wmax[kSegNT / kWave];  // row starts per wave
   __shared__ uint64_t s_k0;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = lane_id();
@@ -229,54 +231,50 @@ __global__ __launch_bounds__(kSegNT) void k_local_sort(uint32_t* cols, float* va
   const LocalSeg sg = segs[blockIdx.x];
   const uint64_t n = sg.count;
   const uint64_t wbase = (uint64_t)wave * kWave * kSegIPT;  // tile-local
-  // mark[p]: (r - row0) of the last row r > row0 starting at start + p, else 0
+  // mark[p] = 1 where a row starts inside the group (p > 0); a product's
+  // row key is the number of such starts at or before it -- the rank of
+  // its row among the group's non-empty rows (< 6144, 13 bits), which
+  // orders like the row itself
   uint32_t* mark = reinterpret_cast<uint32_t*>(sm.buf);
   for (uint32_t i = tid; i < n; i += kSegNT) mark[i] = 0;
   __syncthreads();
   for (uint64_t r = (uint64_t)sg.row0 + 1 + tid; r < lrows; r += kSegNT) {
     const uint64_t st = rstart[r];
     if (st >= sg.start + n) break;
-    atomicMax(&mark[st - sg.start], (uint32_t)(r - sg.row0));
+    if (st > sg.start) mark[st - sg.start] = 1;  // empty rows share the next row's start
   }
   __syncthreads();
-  uint64_t k[kSegIPT];
+  KT k[kSegIPT];
   float v[kSegIPT];
   uint32_t pos[kSegIPT];
   uint32_t carry = 0;
 #pragma unroll
   for (int i = 0; i < kSegIPT; ++i) {
     const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
-    uint32_t m = idx < n ? mark[idx] : 0;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-      const uint32_t o = __shfl_up(m, d, kWave);
-      if (lane >= (uint32_t)d) m = m > o ? m : o;
-    }
-    pos[i] = m > carry ? m : carry;  // row in group so far (this wave's span)
-    const uint32_t last = __shfl(pos[i], kWave - 1, kWave);
-    carry = last;
+    pos[i] = carry + wave_incl_scan32(idx < n ? mark[idx] : 0u);  // starts so far (this wave)
+    carry = __shfl(pos[i], kWave - 1, kWave);
   }
   if (lane == 0) s_wmax[wave] = carry;
   __syncthreads();
   uint32_t before = 0;
-  for (uint32_t w = 0; w < wave; ++w) before = s_wmax[w] > before ? s_wmax[w] : before;
+  for (uint32_t w = 0; w < wave; ++w) before += s_wmax[w];
   const uint64_t cmask = low_mask(col_bits);
 #pragma unroll
   for (int i = 0; i < kSegIPT; ++i) {
     const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
     const bool valid = idx < n;
-    const uint64_t row = pos[i] > before ? pos[i] : before;
-    k[i] = valid ? (shl64(row, col_bits) | cols[sg.start + idx]) : 0;
+    const uint64_t row = pos[i] + before;
+    k[i] = valid ? (KT)(shl64(row, col_bits) | cols[sg.start + idx]) : (KT)0;
     v[i] = valid && vals ? vals[sg.start + idx] : 0.f;  // vals NULL: columns only
   }
-  if (tid == 0) s_k0 = k[0];
+  if (tid == 0) s_k0 = (uint64_t)k[0];
   __syncthreads();  // marks read, sm.buf free for the sort
   const uint64_t k0 = s_k0;
   uint64_t vary = 0;
 #pragma unroll
   for (int i = 0; i < kSegIPT; ++i) {
     const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
-    if (idx < n) vary |= k[i] ^ k0;
+    if (idx < n) vary |= (uint64_t)k[i] ^ k0;
   }
   for (int off = kWave / 2; off > 0; off >>= 1) vary |= __shfl_xor(vary, off, kWave);
   if (lane == 0) s_or[wave] = vary;
@@ -284,14 +282,14 @@ __global__ __launch_bounds__(kSegNT) void k_local_sort(uint32_t* cols, float* va
   vary = 0;
 #pragma unroll
   for (int w = 0; w < kSegNT / kWave; ++w) vary |= s_or[w];
-  const int passes = vary ? (64 - __builtin_clzll(vary) + kSegRB - 1) / kSegRB : 0;
+  const int passes = vary ? (64 - __builtin_clzll(vary) + RB - 1) / RB : 0;
   for (int p = 0; p < passes; ++p) {
-    const int shift = p * kSegRB;
+    const int shift = p * RB;
     for (int i = tid; i < (kSegNT / kWave) * R; i += kSegNT) (&sm.cnt[0][0])[i] = 0;
     __syncthreads();
-    tile_rank<false, kSegRB, kSegIPT, kSegNT>(sm, (uint32_t)n, shift, k, pos);
+    tile_rank<false, RB, kSegIPT, kSegNT>(sm, (uint32_t)n, shift, k, pos);
     __syncthreads();
-    tile_prefix_fold<kSegRB, kSegIPT, kSegNT>(sm);  // cnt[w][d] = run start + wave prefix
+    tile_prefix_fold<RB, kSegIPT, kSegNT>(sm);  // cnt[w][d] = run start + wave prefix
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kSegIPT; ++i) {
@@ -721,8 +719,14 @@ void sort_row_segments(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, ui
   }
   if (plan.nlocal) {
     int tok = kt ? kt->begin(s) : -1;
-    hipLaunchKernelGGL(k_local_sort, dim3((uint32_t)plan.nlocal), dim3(kSegNT), 0, s, col, v,
-                       plan.local.get(), row_start, plan.lrows, plan.col_bits);
+    // a group's row offset takes at most 13 bits (6144 products)
+    if (plan.col_bits + 13 <= 32)
+      hipLaunchKernelGGL((k_local_sort<uint32_t, 10>), dim3((uint32_t)plan.nlocal), dim3(kSegNT),
+                         0, s, col, v, plan.local.get(), row_start, plan.lrows, plan.col_bits);
+    else
+      hipLaunchKernelGGL((k_local_sort<uint64_t, kSegRB>), dim3((uint32_t)plan.nlocal),
+                         dim3(kSegNT), 0, s, col, v, plan.local.get(), row_start, plan.lrows,
+                         plan.col_bits);
     if (kt) kt->end(tok, SPTC_K_SORT_LOCAL, (v ? 16.0 : 8.0) * (double)plan.nsmall, s);
   }
 }
