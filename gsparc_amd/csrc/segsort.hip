@@ -363,14 +363,19 @@ __global__ __launch_bounds__(kBlock) void k_expand_items(ExpandArgs a,
     if (isbig)
       for (int i = threadIdx.x; i < R; i += kBlock) sh[i] = 0;
     uint64_t jlo = a.A0;  // owners only move forward within an item
-    for (uint32_t sub = 0; sub < count; sub += kExpandSub) {
-      const uint64_t pb = a.P0 + start + sub;
-      const uint64_t pe = pb + (count - sub < kExpandSub ? count - sub : kExpandSub);
+    // windows start at multiples of 64 products, so each wave's stores are
+    // whole 256-byte runs; the first window's products before the item are
+    // masked off
+    const uint64_t iend = start + count;
+    for (uint64_t sub = count ? start & ~63ull : iend; sub < iend; sub += kExpandSub) {
+      const uint64_t pb = a.P0 + sub;                          // window start
+      const uint64_t p0 = a.P0 + (sub > start ? sub : start);  // its first product
+      const uint64_t pe = a.P0 + (iend - sub < kExpandSub ? iend : sub + kExpandSub);
       __syncthreads();
       // owners of the first and the last product (waves 0 and 1): only
       // their prefixes are staged
       if (threadIdx.x < 2 * kWave) {
-        const uint64_t j = wave_last_le(a.act_prefix, jlo, a.A1, threadIdx.x < kWave ? pb : pe - 1);
+        const uint64_t j = wave_last_le(a.act_prefix, jlo, a.A1, threadIdx.x < kWave ? p0 : pe - 1);
         if (threadIdx.x == 0) s_ja = j;
         if (threadIdx.x == kWave) s_jl = j;
       }
@@ -387,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void k_expand_items(ExpandArgs a,
 #pragma unroll
       for (int e = 0; e < (int)(kExpandSub / kBlock); ++e) {
         const uint64_t p = pb + (uint64_t)e * kBlock + threadIdx.x;
-        const bool valid = p < pe;
+        const bool valid = p >= p0 && p < pe;
         uint32_t col = 0;
         if (valid) {
           uint32_t k = klo;  // same owner as the previous product: one LDS read
@@ -422,12 +427,14 @@ __global__ __launch_bounds__(kBlock) void k_expand_items(ExpandArgs a,
             const uint32_t lane = lane_id();
             const uint32_t d = (col >> a.dshift) & (R - 1);
             const uint32_t pd = __shfl_up(d, 1, kWave);
-            const bool lead = valid && (lane == 0 || pd != d);
+            const bool pv = __shfl_up((uint32_t)valid, 1, kWave) != 0u;
+            const bool lead = valid && (lane == 0 || !pv || pd != d);
             const uint64_t lm = __ballot(lead);
-            const uint32_t nv = (uint32_t)__popcll(__ballot(valid));  // a prefix of the lanes
+            const uint64_t vm = __ballot(valid);  // one run of lanes
             if (lead) {
+              const uint32_t hi = (uint32_t)kWave - (uint32_t)__builtin_clzll(vm);
               const uint64_t above = lane + 1 < (uint32_t)kWave ? lm >> (lane + 1) : 0ull;
-              const uint32_t end = above ? lane + 1 + (uint32_t)__builtin_ctzll(above) : nv;
+              const uint32_t end = above ? lane + 1 + (uint32_t)__builtin_ctzll(above) : hi;
               atomicAdd(&sh[d], end - lane);
             }
           }
