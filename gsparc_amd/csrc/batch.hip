@@ -73,7 +73,8 @@ __global__ __launch_bounds__(kBlock) void k_brank(const uint64_t* __restrict__ y
                                                   const int* __restrict__ kyf,
                                                   const uint64_t* __restrict__ ytab,
                                                   const uint64_t* __restrict__ yt_off, uint64_t n,
-                                                  uint64_t* __restrict__ out) {
+                                                  uint64_t* __restrict__ out,
+                                                  uint32_t* __restrict__ ycol) {
   uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (; i < n; i += stride) {
@@ -87,6 +88,7 @@ __global__ __launch_bounds__(kBlock) void k_brank(const uint64_t* __restrict__ y
       if (ytab[mid] < f) lo = mid + 1; else hi = mid;
     }
     out[i] = shl64(shr64(k, ynf[s]), kyf[s]) | (lo - base);
+    ycol[i] = (uint32_t)(lo - base);  // the column field (ranks stay below 2^32)
   }
 }
 
@@ -230,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void k_bexpand(BExpandArgs a) {
     for (uint32_t k = threadIdx.x; k < m; k += kBlock) s_pref[k] = a.act_prefix[ja + k];
     __syncthreads();
     uint32_t klo = 0, kc = ~0u;
-    uint64_t rbits = 0, cmask = 0;
+    uint64_t rbits = 0;
     bool big = false;
 #pragma unroll
     for (int e = 0; e < kBExpandItems; ++e) {
@@ -248,22 +250,21 @@ __global__ __launch_bounds__(kBlock) void k_bexpand(BExpandArgs a) {
         }
         klo = k;
         const uint64_t j = ja + k;
-        if (k != kc) {  // new owner: its row's key bits and column mask
+        if (k != kc) {  // new owner: its row's key bits
           kc = k;
           const uint32_t row = a.act_row[j];
           const uint64_t lr = pc.lrow0 + (row - pc.ra);
           rbits = shl64(lr, a.K);
-          cmask = low_mask(a.inst[find_seg(a.inst_R, a.B, row)].kyf);
           big = a.row_big && a.row_big[lr] != 0;
         }
         const uint64_t y = a.act_ypos[j] + (p - s_pref[k]);
-        const uint64_t col = a.ykeys[y] & cmask;
+        const uint32_t col = a.ycol[y];
         const float v = a.act_xval[j] * a.yvals[y];
         if (big) {
-          a.big_col[tl.out0 + q] = (uint32_t)col;
+          a.big_col[tl.out0 + q] = col;
           a.big_val[tl.out0 + q] = v;
         } else if (a.row_big) {
-          a.out_col32[tl.out0 + q] = (uint32_t)col;
+          a.out_col32[tl.out0 + q] = col;
           a.out_vals[tl.out0 + q] = v;
         } else {
           a.out_keys[tl.out0 + q] = rbits | col;
@@ -538,10 +539,10 @@ void run_bunique(const uint64_t* sorted, const uint64_t* seg_off, uint32_t nseg,
 
 void launch_brank(const uint64_t* ykeys, const uint64_t* yoff, uint32_t B, const int* ynf,
                   const int* kyf, const uint64_t* ytab, const uint64_t* yt_off, uint64_t n,
-                  uint64_t* out, hipStream_t s) {
+                  uint64_t* out, uint32_t* ycol, hipStream_t s) {
   if (!n) return;
   hipLaunchKernelGGL(k_brank, dim3(grid_for(n)), dim3(kBlock), 0, s, ykeys, yoff, B, ynf, kyf,
-                     ytab, yt_off, n, out);
+                     ytab, yt_off, n, out, ycol);
 }
 
 void run_bycontract(const uint64_t* ykeys, const uint64_t* yoff, uint32_t B, const int* kyf,

@@ -50,9 +50,10 @@ void launch_bfree_extract(const uint64_t* ykeys, const uint64_t* yoff, uint32_t 
 void run_bunique(const uint64_t* sorted, const uint64_t* seg_off, uint32_t nseg, uint64_t n,
                  uint64_t* table, uint64_t* seg_first, U2* tmp, hipStream_t s);
 // Y key (c << ynf | f) -> (c << kyf | rank of f in its instance's table).
+// ycol: the rank (the key's column field) of every Y key as 32 bits.
 void launch_brank(const uint64_t* ykeys, const uint64_t* yoff, uint32_t B, const int* ynf,
                   const int* kyf, const uint64_t* ytab, const uint64_t* yt_off, uint64_t n,
-                  uint64_t* out, hipStream_t s);
+                  uint64_t* out, uint32_t* ycol, hipStream_t s);
 // Distinct contract values per Y instance: ukey/ustart (global Y index),
 // u_off[s] = first table entry of instance s; ustart[total] = n.
 void run_bycontract(const uint64_t* ykeys, const uint64_t* yoff, uint32_t B, const int* kyf,
@@ -99,15 +100,12 @@ struct BExpandArgs {
   const uint64_t* act_ypos;
   const float* act_xval;
   const uint32_t* act_row;
-  const uint64_t* ykeys;  // Y region (rank-compressed keys)
+  const uint32_t* ycol;   // Y region: every key's column field (rank) as 32 bits
   const float* yvals;
   const BPiece* pieces;
   const BETile* tiles;
   uint32_t ntiles;
   int K;
-  const uint64_t* inst_R;  // [B+1] global row starts
-  const BInst* inst;
-  uint32_t B;
   uint64_t* out_keys;
   float* out_vals;
   // narrow form (row_big non-null): products go out as 32-bit columns, to

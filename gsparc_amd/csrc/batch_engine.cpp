@@ -159,6 +159,7 @@ sptc_status BatchEngine::plan(int B, const sptc_coo* X, const sptc_coo* Y, const
   v1_.reset(n1);
   fk_.reset(NY_ ? NY_ : 1);
   fv_.reset(NY_ ? NY_ : 1);
+  ycol_.reset(NY_ ? NY_ : 1);
   ytab_.reset(NY_ ? NY_ : 1);
   yt_off_.reset(B + 1);
   ukey_.reset(NY_ + 1);
@@ -450,7 +451,7 @@ sptc_status BatchEngine::run(hipStream_t s, bool P) {
     SPTC_HIP(hipStreamSynchronize(s));
   }
   launch_brank(yk, yoff_.get(), B, ynf_.get(), kyf_.get(), ytab_.get(), yt_off_.get(), NY_, yk,
-               s);
+               ycol_.get(), s);
 
   // ---- (c) contract tables + index match --------------------------------------
   fill_u64(u_off_.get(), 0, B + 1, s);
@@ -558,15 +559,12 @@ sptc_status BatchEngine::run(hipStream_t s, bool P) {
     ea.act_ypos = act_ypos_.get();
     ea.act_xval = act_xval_.get();
     ea.act_row = act_row_.get();
-    ea.ykeys = yk;
+    ea.ycol = ycol_.get();
     ea.yvals = yv;
     ea.pieces = c.pieces.get();
     ea.tiles = c.tiles.get();
     ea.ntiles = (uint32_t)c.ht.size();
     ea.K = c.K;
-    ea.inst_R = iR;
-    ea.inst = inst_.get();
-    ea.B = B;
     ea.out_keys = kA_.get();
     ea.out_vals = vA_.get();
     if (c.rowsort) {

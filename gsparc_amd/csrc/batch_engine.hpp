@@ -106,6 +106,7 @@ class BatchEngine {
   bool lin_alt_ = false;
   bool presorted_ = false;
   DevBuf<uint64_t> k0_, k1_, fk_;
+  DevBuf<uint32_t> ycol_;  // column field (rank) of every Y key, 32 bits
   DevBuf<float> v0_, v1_, fv_;
   DevBuf<uint64_t> ytab_, yt_off_, ukey_, ustart_, u_off_;
   DevBuf<BInst> inst_;
