@@ -364,7 +364,8 @@ __global__ __launch_bounds__(kBlock) void k_breduce_emit_narrow(BReduceOp op,
     const uint32_t i = lane + (uint32_t)q * kWave;
     col[q] = 0;
     if (i < T) {
-      col[q] = op.b.ytab[op.b.inst[si[q]].yt_off + w.c[i]];
+      const uint64_t t = op.b.inst[si[q]].yt_off + w.c[i];
+      col[q] = op.b.ytab32 ? (uint64_t)op.b.ytab32[t] : op.b.ytab[t];  // 32-bit: half the gather
       row[q] -= (uint32_t)op.b.inst_R[si[q]];
     }
   }

@@ -127,6 +127,7 @@ struct BReduceArgs {
   const uint64_t* inst_R;
   const BInst* inst;
   const uint64_t* ytab;
+  const uint32_t* ytab32;  // ytab as 32-bit words when every Y free key fits (or NULL)
   uint32_t B;
   uint64_t* bounds;
   const uint32_t* bad;

@@ -160,6 +160,7 @@ sptc_status BatchEngine::plan(int B, const sptc_coo* X, const sptc_coo* Y, const
   fk_.reset(NY_ ? NY_ : 1);
   fv_.reset(NY_ ? NY_ : 1);
   ycol_.reset(NY_ ? NY_ : 1);
+  if (ynf_max_ <= 32) ytab32_.reset(NY_ ? NY_ : 1);
   ytab_.reset(NY_ ? NY_ : 1);
   yt_off_.reset(B + 1);
   ukey_.reset(NY_ + 1);
@@ -432,6 +433,7 @@ sptc_status BatchEngine::run(hipStream_t s, bool P) {
   run_bunique(falt ? k1_.get() : fk_.get(), yoff_.get(), B, NY_, ytab_.get(), yt_off_.get(),
               tmp_.get(), s);
   launch_fix_empty(yt_off_.get(), yoff_.get(), B, s);
+  if (ynf_max_ <= 32) launch_ycol(ytab_.get(), NY_, 0xFFFFFFFFull, ytab32_.get(), s);
   if (P) {
     ytoff_h_.assign(B + 1, 0);
     readback(ytoff_h_.data(), yt_off_.get(), (B + 1) * 8, s);
@@ -633,6 +635,7 @@ sptc_status BatchEngine::run(hipStream_t s, bool P) {
     rb.inst_R = iR;
     rb.inst = inst_.get();
     rb.ytab = ytab_.get();
+    rb.ytab32 = ytab32_.size() ? ytab32_.get() : nullptr;
     rb.B = B;
     rb.bad = bad_.get();
     tok = kt ? kt->begin(s) : -1;

@@ -107,6 +107,7 @@ class BatchEngine {
   bool presorted_ = false;
   DevBuf<uint64_t> k0_, k1_, fk_;
   DevBuf<uint32_t> ycol_;  // column field (rank) of every Y key, 32 bits
+  DevBuf<uint32_t> ytab32_;  // ytab_ as 32-bit words when every Y free key fits
   DevBuf<float> v0_, v1_, fv_;
   DevBuf<uint64_t> ytab_, yt_off_, ukey_, ustart_, u_off_;
   DevBuf<BInst> inst_;

@@ -338,6 +338,11 @@ def test_row_segments_big_and_small_rows(gpu):
                           weights=prod_per_c[X.idx[2].astype(np.int64)])
     assert per_row.max() > 3 * 6144 and np.median(per_row) < 3072  # both kinds present
     _compare_with_oracle(X, X, [2], [2], res)
+    # count-only runs sort the columns alone: same count, same value sum
+    c = sptc.SpTC(X, X, [2], [2], dense_accumulator=-1, count_only=True)
+    assert c.nnz == res.nnz and c.ir_nnz == res.ir_nnz
+    # (FP64 sums added by workgroups in any order)
+    assert np.isclose(c.info.value_sum, res.info.value_sum, rtol=1e-12, atol=0)
 
 
 @pytest.mark.parametrize("ycols", [1, 3, 40])
@@ -370,6 +375,9 @@ def test_long_runs_and_row_starts_across_wave_tiles(gpu, ycols):
     Y = COOTensor((C, ycols), [yc, yj], yv)
     res = sptc.SpTC(X, Y, [1], [0], dense_accumulator=-1)
     _compare_with_oracle(X, Y, [1], [0], res)
+    c = sptc.SpTC(X, Y, [1], [0], dense_accumulator=-1, count_only=True)
+    assert c.nnz == res.nnz
+    assert np.isclose(c.info.value_sum, res.info.value_sum, rtol=1e-12, atol=1e-12)
 
 
 def bucket_case(name):
