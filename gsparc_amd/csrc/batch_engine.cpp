@@ -197,9 +197,8 @@ sptc_status BatchEngine::plan(int B, const sptc_coo* X, const sptc_coo* Y, const
 
 // Accumulator per instance, with the plain engine's rule (Engine::contract):
 // dense when the column ranks fit an LDS row (<= 2^13), Y has no duplicate
-// coordinates, the rows are dense enough (ir * 32 >= rows * slots) and the
-// matched Y segments long enough (ir >= kDenseMinRun * active), or dense is
-// forced.
+// coordinates and dense_pays (rows dense enough, Y segments long enough,
+// few owners per row), or dense is forced.
 sptc_status BatchEngine::plan_accumulators(hipStream_t s) {
   const uint32_t B = B_;
   std::vector<uint32_t> dup(B, 0);
@@ -213,8 +212,7 @@ sptc_status BatchEngine::plan_accumulators(hipStream_t s) {
     if (o_.dense_accumulator >= 0 && ir && kyf <= kDenseMaxBits && !dup[i]) {
       const uint64_t E = 1ull << (kyf < 8 ? 8 : kyf);
       const uint64_t A = iA_[i + 1] - iA_[i];
-      acc_[i] = (o_.dense_accumulator == 1 || (ir * 32 >= R * E && ir >= kDenseMinRun * A))
-                    ? 1 : 0;
+      acc_[i] = (o_.dense_accumulator == 1 || dense_pays(ir, R, A, E)) ? 1 : 0;
     }
     if (o_.dense_accumulator == 1 && ir && !acc_[i]) {
       msg = "instance " + std::to_string(i) +

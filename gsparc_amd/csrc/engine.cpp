@@ -522,7 +522,7 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
     // count-only runs stay on ESC, whose expansion also yields the value sum
     if (!dups)
       use_dense = o_.dense_accumulator == 1 ||
-                  (!o_.count_only && ir * 32 >= R * E && ir >= kDenseMinRun * A);
+                  (!o_.count_only && dense_pays(ir, R, A, E));
   }
   // Tiled dense (column tiles in LDS, FP64 slots) for column spaces wider
   // than an LDS row: forced with dense_accumulator 2, or by 1 when the LDS

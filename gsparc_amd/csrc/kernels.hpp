@@ -107,6 +107,19 @@ constexpr int kDenseMaxBits = 13;  // dense LDS rows up to 8192 slots
 // threads idle and ESC is faster (measured: ~3e9 products/s at 3-4 per
 // nonzero, against ~1e10 for ESC on the same instances).
 constexpr uint64_t kDenseMinRun = 16;
+// The ordered dense pass applies a round's owners (active X nonzeros) one
+// after another, a barrier between them, and a row is one workgroup: rows
+// with many owners serialise on those barriers.  On the batched workload's
+// 57 dense-eligible instances dense won (0.74-0.99x of ESC's time) up to 5.1
+// owners per row and lost from 5.5 on (1.03x, up to 2.9x at 4000 per row).
+constexpr uint64_t kDenseMaxOwnersPerRow = 5;
+// The automatic choice (single and batched planners alike): dense when its
+// rows are dense enough for the slots (ir * 32 >= rows * slots), the Y
+// segments long enough and the owners per row few enough.
+inline bool dense_pays(uint64_t ir, uint64_t rows, uint64_t active, uint64_t slots) {
+  return ir * 32 >= rows * slots && ir >= kDenseMinRun * active &&
+         active <= kDenseMaxOwnersPerRow * rows;
+}
 
 // Dense numeric passes sum in owner order (bit-identical to ESC) unless the
 // FP64 LDS-atomic form is selected (SPTC_DENSE_ORDER=atomic).

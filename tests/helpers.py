@@ -191,3 +191,17 @@ def pick_rows(X, Y, cx, cy, per_row_cap=2_000_000, total_cap=12_000_000, strided
         pick.append(i)
         total += rprod[i]
     return rows[np.array(sorted(pick), np.int64)]
+
+
+def dense_friendly(rng, rows=500, owners=2, cdim=20, ycols=200, per_c=150):
+    """Few active X nonzeros per row (`owners`), long Y segments (`per_c`
+    entries per contract index) over a narrow column space: the shape the
+    automatic rule sends to the dense accumulator."""
+    xr = np.repeat(np.arange(rows, dtype=np.uint32), owners)
+    xc = rng.integers(0, cdim, size=len(xr)).astype(np.uint32)
+    X = COOTensor((rows, cdim), [xr, xc], (rng.random(len(xr)) + 0.1).astype(np.float32))
+    yc = np.repeat(np.arange(cdim, dtype=np.uint32), per_c)
+    yj = np.concatenate([rng.choice(ycols, size=per_c, replace=False)
+                         for _ in range(cdim)]).astype(np.uint32)
+    Y = COOTensor((cdim, ycols), [yc, yj], (rng.random(len(yc)) + 0.1).astype(np.float32))
+    return X, Y

@@ -8,7 +8,7 @@ import pytest
 from gsparc_amd import sptc
 from gsparc_amd.coo import COOTensor
 from gsparc_amd.workloads import batch_specs
-from helpers import REL_TOL, assert_same_coo, load_fixtures, random_coo
+from helpers import REL_TOL, assert_same_coo, dense_friendly, load_fixtures, random_coo
 
 pytestmark = pytest.mark.gpu
 FIXTURES = load_fixtures()
@@ -48,6 +48,9 @@ def _mixed_items(seed):
         if k == 9:
             Y = X if xo == yo and cx == cy else Y
         items.append((X, Y, cx, cy))
+    # few owners per row over long Y segments: the automatic rule's dense shape
+    X, Y = dense_friendly(rng)
+    items.append((X, Y, [1], [0]))
     return items
 
 

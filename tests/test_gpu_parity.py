@@ -11,7 +11,7 @@ from gsparc_amd import sptc
 from gsparc_amd.coo import COOTensor
 from gsparc_amd.workloads import PLAIN_SMALL
 from helpers import (REL_TOL, assert_same_coo, compare_with_oracle, decode_x, decode_y,
-                     load_fixtures, max_rel_err, random_coo)
+                     dense_friendly, load_fixtures, max_rel_err, random_coo)
 
 pytestmark = pytest.mark.gpu
 FIXTURES = load_fixtures()
@@ -188,10 +188,26 @@ def test_chunked_equals_unchunked(gpu):
         np.testing.assert_array_equal(x, y)
     cnt = sptc.SpTC(X, X, [2], [2], count_only=True)
     assert cnt.nnz == full.nnz and cnt.ir_nnz == full.ir_nnz
-    auto = sptc.SpTC(X, X, [2], [2])  # narrow column space: the planner picks dense
-    assert auto.info.accumulator == 1
+    # narrow column space but 20 owners per row: the planner keeps ESC
+    auto = sptc.SpTC(X, X, [2], [2])
+    assert auto.info.accumulator == 0
     for x, y in zip(a, auto.to_host()):
         np.testing.assert_array_equal(x, y)
+
+
+def test_automatic_accumulator_rule(gpu):
+    """The planner picks dense for few owners per row over long Y segments
+    and keeps ESC above 5 owners per row; both give the same bits."""
+    rng = np.random.default_rng(21)
+    X, Y = dense_friendly(rng)
+    auto = sptc.SpTC(X, Y, [1], [0])
+    assert auto.info.accumulator == 1
+    esc = sptc.SpTC(X, Y, [1], [0], dense_accumulator=-1)
+    for x, y in zip(auto.to_host(), esc.to_host()):
+        np.testing.assert_array_equal(x, y)
+    _compare_with_oracle(X, Y, [1], [0], auto)
+    X8, _ = dense_friendly(rng, owners=8)
+    assert sptc.SpTC(X8, Y, [1], [0]).info.accumulator == 0
 
 
 def test_verbose_progress_lines(gpu, capfd):
