@@ -209,7 +209,8 @@ __global__ void k_zero_local(LocalSeg* local, uint64_t n) {
 // One workgroup per small group of (32-bit column, value) products of the
 // consecutive rows row0..: each product's row rank within the group comes
 // from the row starts (marks at the starts, then a prefix count over the
-// wave-striped layout), the composite key (row rank << col_bits | column)
+// wave-striped layout), the composite key (row rank << b | column, b the
+// bits the group's columns use)
 // is sorted on the bits that vary inside the group entirely in LDS (stable
 // LSD passes), and columns and values are stored back in place.
 // KT/RB: 32-bit keys with 10-bit digits when (row in group, column) fits 32
@@ -218,12 +219,13 @@ template <typename KT, int RB>
 __global__ __launch_bounds__(kSegNT) void k_local_sort(uint32_t* cols, float* vals,
                                                        const LocalSeg* __restrict__ segs,
                                                        const uint64_t* __restrict__ rstart,
-                                                       uint64_t lrows, int col_bits) {
+                                                       uint64_t lrows) {
   constexpr int R = 1 << RB;
   __shared__ TileSmem<RB, kSegIPT, kSegNT, KT> sm;
   __shared__ float vbuf[kSegTile];
   __shared__ uint64_t s_or[kSegNT / kWave];
   __shared__ uint32_t s_wmax[kSegNT / kWave];  // row starts per wave
+  __shared__ uint32_t s_cor[kSegNT / kWave];   // OR of the wave's columns
   __shared__ uint64_t s_k0;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = lane_id();
@@ -254,17 +256,36 @@ __global__ __launch_bounds__(kSegNT) void k_local_sort(uint32_t* cols, float* va
     pos[i] = carry + wave_incl_scan32(idx < n ? mark[idx] : 0u);  // starts so far (this wave)
     carry = __shfl(pos[i], kWave - 1, kWave);
   }
-  if (lane == 0) s_wmax[wave] = carry;
+  // the group's columns, and the bits they use: the row rank goes right
+  // above those (a chunk's column field is as wide as its widest instance)
+  uint32_t cv[kSegIPT];
+  uint32_t cor = 0;
+#pragma unroll
+  for (int i = 0; i < kSegIPT; ++i) {
+    const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+    cv[i] = idx < n ? cols[sg.start + idx] : 0u;
+    cor |= cv[i];
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1) cor |= __shfl_xor(cor, off, kWave);
+  if (lane == 0) {
+    s_wmax[wave] = carry;
+    s_cor[wave] = cor;
+  }
   __syncthreads();
   uint32_t before = 0;
-  for (uint32_t w = 0; w < wave; ++w) before += s_wmax[w];
-  const uint64_t cmask = low_mask(col_bits);
+  cor = 0;
+  for (uint32_t w = 0; w < kSegNT / kWave; ++w) {
+    if (w < wave) before += s_wmax[w];
+    cor |= s_cor[w];
+  }
+  const int cb = cor ? 32 - __builtin_clz(cor) : 0;  // at most the chunk's column bits
+  const uint64_t cmask = low_mask(cb);
 #pragma unroll
   for (int i = 0; i < kSegIPT; ++i) {
     const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
     const bool valid = idx < n;
     const uint64_t row = pos[i] + before;
-    k[i] = valid ? (KT)(shl64(row, col_bits) | cols[sg.start + idx]) : (KT)0;
+    k[i] = valid ? (KT)(shl64(row, cb) | cv[i]) : (KT)0;
     v[i] = valid && vals ? vals[sg.start + idx] : 0.f;  // vals NULL: columns only
   }
   if (tid == 0) s_k0 = (uint64_t)k[0];
@@ -729,11 +750,10 @@ void sort_row_segments(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, ui
     // a group's row offset takes at most 13 bits (6144 products)
     if (plan.col_bits + 13 <= 32)
       hipLaunchKernelGGL((k_local_sort<uint32_t, 10>), dim3((uint32_t)plan.nlocal), dim3(kSegNT),
-                         0, s, col, v, plan.local.get(), row_start, plan.lrows, plan.col_bits);
+                         0, s, col, v, plan.local.get(), row_start, plan.lrows);
     else
       hipLaunchKernelGGL((k_local_sort<uint64_t, kSegRB>), dim3((uint32_t)plan.nlocal),
-                         dim3(kSegNT), 0, s, col, v, plan.local.get(), row_start, plan.lrows,
-                         plan.col_bits);
+                         dim3(kSegNT), 0, s, col, v, plan.local.get(), row_start, plan.lrows);
     if (kt) kt->end(tok, SPTC_K_SORT_LOCAL, (v ? 16.0 : 8.0) * (double)plan.nsmall, s);
   }
 }
