@@ -751,10 +751,13 @@ __global__ __launch_bounds__(kBlock) void k_reduce_emit_narrow(ReduceOp op, cons
 #pragma unroll
   for (int q = 0; q < kScanItems; ++q) {
     const uint32_t i = lane + (uint32_t)q * kWave;
+    // plain (cached) stores: the partial lines at the ends of neighbouring
+    // tiles' output runs merge in L2 (streaming stores wrote 1.14x the
+    // output bytes, these 1.03x)
     if (i < T) {
-      __builtin_nontemporal_store(rows ? rb + w.m[i] : rb, &orow[i]);
-      __builtin_nontemporal_store(col[q], &ocol[i]);
-      __builtin_nontemporal_store(w.v[i], &oval[i]);
+      orow[i] = rows ? rb + w.m[i] : rb;
+      ocol[i] = col[q];
+      oval[i] = w.v[i];
     }
   }
 }
