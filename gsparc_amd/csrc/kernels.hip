@@ -716,48 +716,83 @@ void run_reduce_emit(const uint64_t* keys, const float* vals, uint64_t n, uint32
                      tmp);
 }
 
-// Narrow-form emit, one wave per wave tile (the count pass's tiling): the
-// tile's outputs are stored coalesced, column-table gathers issued first.
+// Narrow-form emit, one wave per wave tile (the count pass's tiling).  The
+// workgroup's four wave tiles are consecutive, so their outputs are one
+// contiguous run: after a barrier the workgroup stores that run together,
+// column-table gathers issued first, with streaming stores whose lanes are
+// counted from the 32-output boundary at or below the run's start, so every
+// store instruction covers whole 128-byte lines except at the run's ends
+// (streaming stores do not merge partial lines in L2).
 __global__ __launch_bounds__(kBlock) void k_reduce_emit_narrow(ReduceOp op, const U2* tile_offs,
                                                                uint64_t ntiles) {
-  __shared__ WaveSmem sm[kBlock / kWave];
-  const uint64_t wt = (uint64_t)blockIdx.x * (kBlock / kWave) + wave_id();
-  if (wt >= ntiles) return;
-  {  // a tile inside one big row has no outputs here
+  constexpr uint32_t NW = kBlock / kWave;
+  __shared__ WaveSmem sm[NW];
+  __shared__ uint32_t s_T[NW], s_rb[NW], s_rows[NW];
+  const uint32_t wave = wave_id();
+  const uint64_t wt0 = (uint64_t)blockIdx.x * NW;
+  const uint64_t wt = wt0 + wave;
+  uint32_t T = 0, rb = 0;
+  bool rows = false;
+  if (wt < ntiles) {
     const uint64_t b = wt * kWaveTile;
     const uint32_t cnt = (uint32_t)(op.n - b < kWaveTile ? op.n - b : kWaveTile);
-    if (wave_tile_all_big(op.nr, b, cnt, op.nr.tile_row[2 * wt])) return;
+    // a tile inside one big row has no outputs here
+    if (!wave_tile_all_big(op.nr, b, cnt, op.nr.tile_row[2 * wt])) {
+      uint64_t r0, rp;
+      T = wave_tile_reduce(op.nr, op.vals, op.n, wt, sm[wave], &r0, &rp, &rows);
+      rb = (uint32_t)(op.r0 + r0);
+    }
   }
-  WaveSmem& w = sm[wave_id()];
-  uint64_t r0, rp;
-  bool rows;
-  const uint32_t T = wave_tile_reduce(op.nr, op.vals, op.n, wt, w, &r0, &rp, &rows);
-  const uint64_t obase = tile_offs[wt].a;
-  const uint32_t lane = lane_id();
+  if (lane_id() == 0) {
+    s_T[wave] = T;
+    s_rb[wave] = rb;
+    s_rows[wave] = rows;
+  }
+  __syncthreads();
+  uint32_t pre[NW + 1];
+  pre[0] = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < NW; ++k) pre[k + 1] = pre[k] + s_T[k];
+  const uint32_t total = pre[NW];
+  const uint64_t obase = tile_offs[wt0].a;  // the first wave tile's outputs start the run
+  const uint64_t abase = obase & ~31ull;     // a 128-byte line of rows and values
+  const uint32_t lead = (uint32_t)(obase - abase);
   const uint64_t* __restrict__ ytab = op.ytab;
   const uint32_t* __restrict__ ytab32 = op.ytab32;
-  const uint32_t rb = (uint32_t)(op.r0 + r0);
-  uint32_t* __restrict__ orow = op.out_row + obase;  // uniform bases
-  uint64_t* __restrict__ ocol = op.out_col + obase;
-  float* __restrict__ oval = op.out_val + obase;
-  uint64_t col[kScanItems];
+  uint32_t* __restrict__ orow = op.out_row + abase;  // uniform bases
+  uint64_t* __restrict__ ocol = op.out_col + abase;
+  float* __restrict__ oval = op.out_val + abase;
+  // slots [lead, lead + total) of at most 31 + 4 * 512
+  constexpr int kPer = (int)((NW * kWaveTile + 31 + kBlock - 1) / kBlock);
+  uint64_t col[kPer];
+  uint32_t row[kPer];
+  float val[kPer];
 #pragma unroll
-  for (int q = 0; q < kScanItems; ++q) {
-    const uint32_t i = lane + (uint32_t)q * kWave;
-    col[q] = i < T ? w.c[i] : 0;
-    if (ytab32 && i < T) col[q] = ytab32[col[q]];  // half the gathered table
-    else if (ytab && i < T) col[q] = ytab[col[q]];
+  for (int q = 0; q < kPer; ++q) {
+    const uint32_t jj = threadIdx.x + (uint32_t)q * kBlock;
+    const uint32_t j = jj - lead;  // wraps below lead: fails j < total
+    col[q] = 0;
+    row[q] = 0;
+    val[q] = 0.f;
+    if (jj >= lead && j < total) {
+      uint32_t k = 0;
+#pragma unroll
+      for (uint32_t e = 1; e < NW; ++e) k += j >= pre[e] ? 1u : 0u;
+      const uint32_t i = j - pre[k];
+      const WaveSmem& w = sm[k];
+      row[q] = s_rows[k] ? s_rb[k] + w.m[i] : s_rb[k];
+      val[q] = w.v[i];
+      const uint32_t c = w.c[i];
+      col[q] = ytab32 ? (uint64_t)ytab32[c] : ytab ? ytab[c] : (uint64_t)c;  // half the gathered table
+    }
   }
 #pragma unroll
-  for (int q = 0; q < kScanItems; ++q) {
-    const uint32_t i = lane + (uint32_t)q * kWave;
-    // plain (cached) stores: the partial lines at the ends of neighbouring
-    // tiles' output runs merge in L2 (streaming stores wrote 1.14x the
-    // output bytes, these 1.03x)
-    if (i < T) {
-      orow[i] = rows ? rb + w.m[i] : rb;
-      ocol[i] = col[q];
-      oval[i] = w.v[i];
+  for (int q = 0; q < kPer; ++q) {
+    const uint32_t jj = threadIdx.x + (uint32_t)q * kBlock;
+    if (jj >= lead && jj - lead < total) {
+      __builtin_nontemporal_store(row[q], &orow[jj]);
+      __builtin_nontemporal_store(col[q], &ocol[jj]);
+      __builtin_nontemporal_store(val[q], &oval[jj]);
     }
   }
 }
