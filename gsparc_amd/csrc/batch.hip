@@ -352,21 +352,40 @@ __global__ __launch_bounds__(kBlock) void k_breduce_emit_narrow(BReduceOp op,
   const uint32_t lane = lane_id();
   uint64_t col[kScanItems];
   uint32_t row[kScanItems], si[kScanItems];
+  if (!rows) {
+    // one row for the whole tile: its global row, instance and column table
+    // once for the wave instead of once per output
+    const uint32_t s0 = op.b.map_inst[r0];
+    const uint64_t yt = op.b.inst[s0].yt_off;
+    const uint32_t rr = op.b.map_row[r0] - (uint32_t)op.b.inst_R[s0];
 #pragma unroll
-  for (int q = 0; q < kScanItems; ++q) {
-    const uint32_t i = lane + (uint32_t)q * kWave;
-    const uint64_t lrow = i < T ? r0 + (rows ? w.m[i] : 0) : 0;
-    row[q] = i < T ? op.b.map_row[lrow] : 0;
-    si[q] = i < T ? op.b.map_inst[lrow] : 0;
-  }
+    for (int q = 0; q < kScanItems; ++q) {
+      const uint32_t i = lane + (uint32_t)q * kWave;
+      row[q] = rr;
+      si[q] = s0;
+      col[q] = 0;
+      if (i < T) {
+        const uint64_t t = yt + w.c[i];
+        col[q] = op.b.ytab32 ? (uint64_t)op.b.ytab32[t] : op.b.ytab[t];  // 32-bit: half the gather
+      }
+    }
+  } else {
 #pragma unroll
-  for (int q = 0; q < kScanItems; ++q) {
-    const uint32_t i = lane + (uint32_t)q * kWave;
-    col[q] = 0;
-    if (i < T) {
-      const uint64_t t = op.b.inst[si[q]].yt_off + w.c[i];
-      col[q] = op.b.ytab32 ? (uint64_t)op.b.ytab32[t] : op.b.ytab[t];  // 32-bit: half the gather
-      row[q] -= (uint32_t)op.b.inst_R[si[q]];
+    for (int q = 0; q < kScanItems; ++q) {
+      const uint32_t i = lane + (uint32_t)q * kWave;
+      const uint64_t lrow = i < T ? r0 + w.m[i] : 0;
+      row[q] = i < T ? op.b.map_row[lrow] : 0;
+      si[q] = i < T ? op.b.map_inst[lrow] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) {
+      const uint32_t i = lane + (uint32_t)q * kWave;
+      col[q] = 0;
+      if (i < T) {
+        const uint64_t t = op.b.inst[si[q]].yt_off + w.c[i];
+        col[q] = op.b.ytab32 ? (uint64_t)op.b.ytab32[t] : op.b.ytab[t];
+        row[q] -= (uint32_t)op.b.inst_R[si[q]];
+      }
     }
   }
 #pragma unroll
@@ -377,6 +396,7 @@ __global__ __launch_bounds__(kBlock) void k_breduce_emit_narrow(BReduceOp op,
     op.out_col[obase + i] = col[q];
     op.out_val[obase + i] = w.v[i];
     if (!op.b.bounds) continue;
+    if (!rows && i > 0) continue;  // the previous output has the same row and instance
     // first output of every instance in (previous output's instance, s]
     int64_t sp = -1;
     if (i > 0) sp = op.b.map_inst[r0 + (rows ? w.m[i - 1] : 0)];
