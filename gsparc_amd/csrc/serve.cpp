@@ -266,14 +266,15 @@ void Server::worker(int w) {
       }
       const uint64_t cap = o_.budget_bytes > resident_bytes_ ? o_.budget_bytes - resident_bytes_ : 0;
       const uint64_t avail = cap > in_use ? cap - in_use : 0;
-      // Arrival order: a large request at the head runs alone when it fits;
-      // small requests are taken in order while they fit (and may pass a
-      // large request that does not fit yet).
+      // Arrival order: a large request at the head runs alone when it fits
+      // and no other large request is running; small requests are taken in
+      // order while they fit (and may pass a large request that waits).
       for (auto it = ready_.begin(); it != ready_.end();) {
         const auto& r = *it;
         if (r->large) {
-          if (pick.empty() && r->footprint <= avail) {
+          if (pick.empty() && !large_running_ && r->footprint <= avail) {
             big = r;
+            large_running_ = true;
             ready_.erase(it);
             break;
           }
@@ -319,6 +320,7 @@ void Server::worker(int w) {
     std::lock_guard<std::mutex> g(mu_);
     for (auto& r : pick) finish(r);
     if (big) finish(big);
+    if (big) large_running_ = false;
     if (big) ++n_large_;
     else {
       ++n_batches_;

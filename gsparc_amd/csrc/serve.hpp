@@ -99,6 +99,9 @@ class Server {
   uint64_t resident_bytes_ = 0;
   uint64_t next_id_ = 1;
   bool stop_ = false;
+  // a large request is running (they run one at a time: each is HBM-bound,
+  // so two at once only share the bandwidth and delay the earlier one)
+  bool large_running_ = false;
   hipStream_t plan_stream_ = nullptr;
   std::vector<hipStream_t> streams_;
   std::vector<std::shared_ptr<DevicePool>> pools_;

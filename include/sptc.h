@@ -275,8 +275,8 @@ void sptc_batch_free(sptc_batch* b);
  * that are ready together run as one batch (sptc_batch_* machinery); a
  * request above the large-product threshold runs alone, count-only (nnz and
  * value checksum: the reference's plain path likewise returns only the nnz,
- * main.cu:375).  Several workers, each with its own stream, overlap their
- * launches.  No reference counterpart. */
+ * main.cu:375), one large request at a time.  Several workers, each with
+ * its own stream, overlap their launches.  No reference counterpart. */
 typedef struct sptc_server_s sptc_server;
 
 typedef struct {
