@@ -262,17 +262,20 @@ struct MatchOp {
   MatchOut o;
   int api;  // 1: write mpos/mcnt/prefix (per-phase entry point)
 
-  __device__ U2 value(uint64_t i) const {
+  // the X key's Y segment start, from value() to emit() in registers
+  struct Aux {
+    uint64_t pos;
+  };
+  __device__ U2 value(uint64_t i, Aux& x) const {
     const uint64_t k = xkeys[i];
     uint64_t pos, cnt;
     ytable_lookup(t, k & cmask, pos, cnt);
+    x.pos = pos;
     const uint64_t head = (i == 0) || (shr64(xkeys[i - 1], ncbits) != shr64(k, ncbits));
     return mk2(cnt, ((uint64_t)(cnt != 0) << 32) | head);
   }
-  __device__ void emit(uint64_t i, U2 excl, U2 v) const {
-    const uint64_t k = xkeys[i];
-    uint64_t pos, cnt;
-    ytable_lookup(t, k & cmask, pos, cnt);
+  __device__ void emit(uint64_t i, U2 excl, U2 v, const Aux& x) const {
+    const uint64_t pos = x.pos, cnt = v.a;
     if (api) {
       o.mpos[i] = cnt ? pos : 0;
       o.mcnt[i] = cnt;
@@ -283,7 +286,7 @@ struct MatchOp {
     const uint64_t head = v.b & 1;
     const uint64_t row = (excl.b & 0xFFFFFFFFull) + head - 1;
     if (head) {
-      o.row_key[row] = shr64(k, ncbits);
+      o.row_key[row] = shr64(xkeys[i], ncbits);
       o.row_prod[row] = excl.a;
       o.row_act[row] = excl.b >> 32;
     }

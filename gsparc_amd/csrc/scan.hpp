@@ -3,7 +3,10 @@
 // An Op supplies, for item i of [0, n):
 //   __device__ V value(uint64_t i) const;            // the item's contribution
 //   __device__ void emit(uint64_t i, V excl, V val);   // consumer (optional)
-// where V is U2 (two uint64 lanes, added lane-wise).  Three kernels:
+// where V is U2 (two uint64 lanes, added lane-wise).  An Op that declares a
+// type Aux instead supplies value(i, Aux&) and emit(i, excl, val, const
+// Aux&): what value() looked up reaches emit() in registers, not loaded
+// again.  Three kernels:
 //   k_scan_reduce : per tile of kBlock*kScanItems items, the tile sum
 //   k_scan_tiles  : exclusive scan of tile sums (single workgroup), total
 //   k_scan_emit   : recompute values, block scan + tile offset, emit()
@@ -13,6 +16,8 @@
 // takes items [t*kScanItems, t*kScanItems + kScanItems) in order.
 #pragma once
 #include "common.hpp"
+
+#include <type_traits>
 
 namespace sptc {
 
@@ -59,6 +64,24 @@ __device__ __forceinline__ U2 block_excl_scan_u2(U2 v, U2* scratch, U2* total) {
   return off + inc - v;
 }
 
+// Op::Aux when the Op declares one (see above).
+template <class Op, class = void>
+struct ScanAux {
+  struct type {};
+  static constexpr bool on = false;
+};
+template <class Op>
+struct ScanAux<Op, std::void_t<typename Op::Aux>> {
+  using type = typename Op::Aux;
+  static constexpr bool on = true;
+};
+template <class Op>
+__device__ __forceinline__ U2 scan_value(const Op& op, uint64_t i,
+                                         typename ScanAux<Op>::type& aux) {
+  if constexpr (ScanAux<Op>::on) return op.value(i, aux);
+  else return op.value(i);
+}
+
 template <class Op>
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(Op op, uint64_t n, U2* tile_sums) {
   __shared__ U2 scratch[kBlock / kWave];
@@ -67,7 +90,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(Op op, uint64_t n, U2* t
 #pragma unroll
   for (int e = 0; e < kScanItems; ++e) {
     const uint64_t i = base + (uint64_t)e * kBlock;
-    if (i < n) s = s + op.value(i);
+    typename ScanAux<Op>::type aux;
+    if (i < n) s = s + scan_value(op, i, aux);
   }
   U2 tot;
   block_excl_scan_u2(s, scratch, &tot);
@@ -81,10 +105,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_emit(Op op, uint64_t n, const U
   const uint32_t tid = threadIdx.x;
   const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
   U2 vals[kScanItems];
+  typename ScanAux<Op>::type aux[kScanItems];
 #pragma unroll
   for (int e = 0; e < kScanItems; ++e) {
     const uint64_t i = base + (uint64_t)e * kBlock + tid;
-    vals[e] = i < n ? op.value(i) : mk2(0, 0);
+    vals[e] = i < n ? scan_value(op, i, aux[e]) : mk2(0, 0);
     sv[e * kBlock + tid] = vals[e];
   }
   __syncthreads();
@@ -106,7 +131,10 @@ __global__ __launch_bounds__(kBlock) void k_scan_emit(Op op, uint64_t n, const U
 #pragma unroll
   for (int e = 0; e < kScanItems; ++e) {
     const uint64_t i = base + (uint64_t)e * kBlock + tid;
-    if (i < n) op.emit(i, sv[e * kBlock + tid], vals[e]);
+    if (i < n) {
+      if constexpr (ScanAux<Op>::on) op.emit(i, sv[e * kBlock + tid], vals[e], aux[e]);
+      else op.emit(i, sv[e * kBlock + tid], vals[e]);
+    }
   }
 }
 
