@@ -146,23 +146,28 @@ struct BMatchOp {
       cnt = 0;
     }
   }
-  __device__ U2 value(uint64_t i) const {
+  // the item's instance and Y segment start, from value() to emit() in
+  // registers (both are searches)
+  struct Aux {
+    uint64_t pos;
+    uint32_t s;
+  };
+  __device__ U2 value(uint64_t i, Aux& x) const {
     const uint32_t s = find_seg(xoff, B, i);
     const BInst in = inst[s];
     const uint64_t k = xkeys[i];
     uint64_t pos, cnt;
     lookup(s, k & in.cmask, pos, cnt);
+    x.pos = pos;
+    x.s = s;
     const uint64_t head =
         (i == xoff[s]) || (shr64(xkeys[i - 1], in.ncbits) != shr64(k, in.ncbits));
     return mk2(cnt, ((uint64_t)(cnt != 0) << 32) | head);
   }
-  __device__ void emit(uint64_t i, U2 excl, U2 v) const {
+  __device__ void emit(uint64_t i, U2 excl, U2 v, const Aux& x) const {
     if (*o.bad) return;  // the row table is part of the planned results
-    const uint32_t s = find_seg(xoff, B, i);
-    const BInst in = inst[s];
-    const uint64_t k = xkeys[i];
-    uint64_t pos, cnt;
-    lookup(s, k & in.cmask, pos, cnt);
+    const uint32_t s = x.s;
+    const uint64_t pos = x.pos, cnt = v.a;
     const uint64_t head = v.b & 1;
     const uint64_t row = (excl.b & 0xFFFFFFFFull) + head - 1;
     if (i == xoff[s]) {
@@ -171,7 +176,7 @@ struct BMatchOp {
       o.inst_R[s] = excl.b & 0xFFFFFFFFull;
     }
     if (head) {
-      o.row_key[row] = shr64(k, in.ncbits);
+      o.row_key[row] = shr64(xkeys[i], inst[s].ncbits);
       o.row_prod[row] = excl.a;
       o.row_act[row] = excl.b >> 32;
     }
