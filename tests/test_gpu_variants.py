@@ -6,6 +6,8 @@ oracle in a process of its own, one after another.
   SPTC_SEG_IPT=12/24   keys per thread of the big-row scatter tiles
   SPTC_SEG_RB=8        8-bit digits for the big-row column passes
   SPTC_SORT=onesweep   operand sorts with a decoupled look-back scan
+  SPTC_SORT_TILE=256x16/512x16  operand-sort tile (threads x keys, 8-bit digits)
+  SPTC_SORT_RB9=0      operand sorts keep 8-bit digits where 9 would save a pass
   SPTC_PROD_SORT=full  one sort over all product-key bits (64-bit keys)
   SPTC_XCD_REMAP=0     blockIdx-order tiles instead of XCD-contiguous ones
   SPTC_DENSE_ORDER=atomic  dense rows summed by FP64 LDS atomics (checked
@@ -24,7 +26,8 @@ pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 VARIANTS = [{"SPTC_SEG_WIDE": "1"}, {"SPTC_SEG_IPT": "12"}, {"SPTC_SEG_IPT": "24"},
-            {"SPTC_SEG_RB": "8"}, {"SPTC_SORT": "onesweep"}, {"SPTC_PROD_SORT": "full"},
+            {"SPTC_SEG_RB": "8"}, {"SPTC_SORT": "onesweep"}, {"SPTC_SORT_TILE": "256x16"},
+            {"SPTC_SORT_TILE": "512x16"}, {"SPTC_SORT_RB9": "0"}, {"SPTC_PROD_SORT": "full"},
             {"SPTC_XCD_REMAP": "0"}, {"SPTC_DENSE_ORDER": "atomic"},
             {"SPTC_BIG_TAIL": "buckets"}]
 
