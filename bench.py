@@ -503,7 +503,7 @@ def bench_serving(args):
 
 
 def bench_nell2(args):
-    """12092x9184x28818, 7.69e7 nnz, contracted on mode 2 with
+    """NELL-2-shaped workload: 12092x9184x28818, 7.69e7 nnz, contracted on mode 2 with
     itself under a 64 GB working-set cap.  The full contraction has 3.03e13
     intermediate products (about 9 minutes count-only), so by default the
     run contracts an evenly strided sample of X's free rows (about
@@ -708,8 +708,9 @@ def self_check(X, spec, r):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
-              f"(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported: the library "
+              f"runs on one device, and sharding across GPUs is a design, not code",
+              file=sys.stderr)
         sys.exit(2)
     if args.workload == "batch":
         return bench_batch(args)

@@ -41,7 +41,7 @@ def _sub(X, mask):
     return COOTensor(X.dims, [a[mask] for a in X.idx], X.val[mask])
 
 
-# ---- NIPS-shaped, order 4, 3.1e6 nnz, contract {2,3} ------------
+# ---- NIPS-shaped: order 4, 3.1e6 nnz, contract {2,3} ------------------------
 
 def test_nips_full_size(gpu):
     X = NIPS.make()
@@ -57,7 +57,7 @@ def test_nips_full_size(gpu):
     _log(f"nips: {len(rows)} sampled rows match the oracle, max rel err {err:.2e}")
 
 
-# ---- NELL-2-shaped, 7.69e7 nnz, contract mode 2, 64 GB cap ------
+# ---- NELL-2-shaped: 7.69e7 nnz, contract mode 2, 64 GB cap -----------------
 
 @pytest.fixture(scope="module")
 def nell2():
@@ -113,7 +113,7 @@ def test_nell2_rows_full_output(gpu, nell2):
     _log(f"nell2 rows: {Xs.nnz} nnz, out {res.nnz}, max rel err {err:.2e}")
 
 
-# ---- 256 instances drawn as stated ------------------------------
+# ---- the 256-instance batch, instances as drawn -----------------------------
 
 BATCH_FULL_IR = 1 << 28  # instances up to this many products run batched, in full
 
@@ -168,7 +168,7 @@ def test_batch_as_stated_large_instances(gpu, batch_as_stated):
     _log(f"batch large: {len(big)} instances, {total:.3e} products, closed forms hold")
 
 
-# ---- the serving mix ------------------------------------------------
+# ---- the serving mix ------------------------------------------------------------
 
 def test_serving_mix_against_oracle(gpu, batch_as_stated):
     """A request mix of the serving distribution -- 90 % instances of the
