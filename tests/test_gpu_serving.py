@@ -2,6 +2,8 @@
 the same served alone, served in a mix with other requests (batched and
 overlapping a large request on another stream), and computed by the plain
 single-call path."""
+import time
+
 import numpy as np
 import pytest
 
@@ -82,6 +84,29 @@ def test_results_same_alone_and_in_the_mix(gpu):
     assert st["peak_in_use_bytes"] <= st["budget_bytes"]
     assert max(sizes) > 1, sizes  # some requests were batched together
     srv.close()
+
+
+def test_large_requests_run_one_at_a_time(gpu):
+    """Large requests are HBM-bound: three submitted together, with budget
+    and workers for all three, still run one after another (each one's
+    launch starts after the previous one's result is ready)."""
+    _, large = _tensors()
+    srv = sptc.Server(budget_bytes=24 << 30, workers=3, large_products=50_000_000,
+                      large_budget_bytes=6 << 30)
+    big = srv.add_tensor(large)
+    srv.wait(srv.submit(big, big, [2], [2]), want_result=False)  # operands prepared
+    subs = []
+    for _ in range(3):
+        t0 = time.perf_counter() * 1e3
+        subs.append((srv.submit(big, big, [2], [2]), t0))
+    spans = []
+    for rid, t0 in subs:
+        _, info = srv.wait(rid, want_result=False)
+        assert info["status"] == 0 and info["count_only"] == 1
+        spans.append((t0 + info["queue_ms"], t0 + info["latency_ms"]))
+    spans.sort()
+    for (_, end0), (start1, _) in zip(spans, spans[1:]):
+        assert start1 >= end0 - 2.0, spans  # ms; submit times are taken just before the call
 
 
 def test_server_errors(gpu):
