@@ -237,7 +237,8 @@ __global__ __launch_bounds__(kBlock) void k_bexpand(BExpandArgs a) {
     for (uint32_t k = threadIdx.x; k < m; k += kBlock) s_pref[k] = a.act_prefix[ja + k];
     __syncthreads();
     uint32_t klo = 0, kc = ~0u;
-    uint64_t rbits = 0;
+    uint64_t rbits = 0, ybase = 0;
+    float xv = 0.f;
     bool big = false;
 #pragma unroll
     for (int e = 0; e < kBExpandItems; ++e) {
@@ -255,25 +256,28 @@ __global__ __launch_bounds__(kBlock) void k_bexpand(BExpandArgs a) {
         }
         klo = k;
         const uint64_t j = ja + k;
-        if (k != kc) {  // new owner: its row's key bits
+        if (k != kc) {  // new owner: its row's key bits, Y segment and X value
           kc = k;
           const uint32_t row = a.act_row[j];
           const uint64_t lr = pc.lrow0 + (row - pc.ra);
           rbits = shl64(lr, a.K);
           big = a.row_big && a.row_big[lr] != 0;
+          ybase = a.act_ypos[j] - s_pref[k];
+          xv = a.act_xval[j];
         }
-        const uint64_t y = a.act_ypos[j] + (p - s_pref[k]);
+        const uint64_t y = ybase + p;
         const uint32_t col = a.ycol[y];
-        const float v = a.act_xval[j] * a.yvals[y];
+        const float v = xv * a.yvals[y];
+        // streaming stores: the gathered Y arrays stay cached
         if (big) {
-          a.big_col[tl.out0 + q] = col;
-          a.big_val[tl.out0 + q] = v;
+          __builtin_nontemporal_store(col, &a.big_col[tl.out0 + q]);
+          __builtin_nontemporal_store(v, &a.big_val[tl.out0 + q]);
         } else if (a.row_big) {
-          a.out_col32[tl.out0 + q] = col;
-          a.out_vals[tl.out0 + q] = v;
+          __builtin_nontemporal_store(col, &a.out_col32[tl.out0 + q]);
+          __builtin_nontemporal_store(v, &a.out_vals[tl.out0 + q]);
         } else {
-          a.out_keys[tl.out0 + q] = rbits | col;
-          a.out_vals[tl.out0 + q] = v;
+          __builtin_nontemporal_store(rbits | col, &a.out_keys[tl.out0 + q]);
+          __builtin_nontemporal_store(v, &a.out_vals[tl.out0 + q]);
         }
       }
     }
