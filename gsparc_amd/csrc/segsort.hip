@@ -227,6 +227,7 @@ __global__ __launch_bounds__(kSegNT) void k_local_sort(uint32_t* cols, float* va
   __shared__ uint32_t s_wmax[kSegNT / kWave];  // row starts per wave
   __shared__ uint32_t s_cor[kSegNT / kWave];   // OR of the wave's columns
   __shared__ uint64_t s_k0;
+  __shared__ KT s_last[kSegNT / kWave];  // each wave's last key
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = lane_id();
   const uint32_t wave = tid / kWave;
@@ -286,10 +287,30 @@ __global__ __launch_bounds__(kSegNT) void k_local_sort(uint32_t* cols, float* va
     const bool valid = idx < n;
     const uint64_t row = pos[i] + before;
     k[i] = valid ? (KT)(shl64(row, cb) | cv[i]) : (KT)0;
-    v[i] = valid && vals ? vals[sg.start + idx] : 0.f;  // vals NULL: columns only
   }
+  if (lane == kWave - 1) s_last[wave] = k[kSegIPT - 1];
   if (tid == 0) s_k0 = (uint64_t)k[0];
   __syncthreads();  // marks read, sm.buf free for the sort
+  // a group already in (row, column) order -- a row whose products are one
+  // owner's Y segment is -- stays as it is: no sort, no value loads, no
+  // stores.  The key before item i of lane 0 is lane 63's of item i - 1
+  // (or the previous wave's last key, that wave being full).
+  uint32_t desc = 0;
+#pragma unroll
+  for (int i = 0; i < kSegIPT; ++i) {
+    const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+    const KT up = __shfl_up(k[i], 1, kWave);
+    const KT p63 = i > 0 ? (KT)__shfl(k[i > 0 ? i - 1 : 0], kWave - 1, kWave)
+                         : (wave > 0 ? s_last[wave > 0 ? wave - 1 : 0] : (KT)0);
+    const KT prev = lane ? up : p63;
+    if (idx < n && idx > 0 && k[i] < prev) desc = 1;
+  }
+  if (!__syncthreads_or(desc)) return;
+#pragma unroll
+  for (int i = 0; i < kSegIPT; ++i) {
+    const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+    v[i] = idx < n && vals ? vals[sg.start + idx] : 0.f;  // vals NULL: columns only
+  }
   const uint64_t k0 = s_k0;
   uint64_t vary = 0;
 #pragma unroll
