@@ -364,7 +364,7 @@ sptc_status BatchEngine::build_chunks(hipStream_t s) {
       SPTC_HIP(hipMemcpyAsync(rr.get(), c.hr.data(), c.hr.size() * sizeof(RowRange),
                               hipMemcpyHostToDevice, s));
       build_row_segments(rr.get(), (uint32_t)c.hr.size(), c.lrows, row_prod_.get(), c.K,
-                         &c.rseg, ws_, tmp_.get(), s);
+                         &c.rseg, ws_, tmp_.get(), s, /*msd=*/false, row_act_.get());
       c.rowsort = true;
       // keep the per-row big-segment flags the expansion reads and the row
       // starts of the narrow form, release the rest

@@ -751,7 +751,7 @@ sptc_status Engine::contract(const Operand& X, const Operand& Y, sptc_result_s* 
         h->out0 = 0;
         SPTC_HIP(hipMemcpyAsync(rrange.get(), h, sizeof(RowRange), hipMemcpyHostToDevice, s_));
         build_row_segments(rrange.get(), 1, lrows, row_prod.get(), kyf, &rseg, ws_, tmp.get(),
-                           s_, prefer_bucket_tail());
+                           s_, prefer_bucket_tail(), row_act.get());
       }
       ExpandArgs ea;
       ea.act_prefix = act_prefix.get();

@@ -112,7 +112,7 @@ constexpr uint64_t kHalfTile = kSegTile / 2;
 
 __device__ __forceinline__ uint64_t rowseg_locate(const RowRange* ranges, uint32_t nranges,
                                                   uint64_t l, const uint64_t* row_prod,
-                                                  uint64_t* first) {
+                                                  uint64_t* first, uint64_t* grow = nullptr) {
   uint32_t lo = 0, hi = nranges;  // last range with lrow0 <= l
   while (lo + 1 < hi) {
     const uint32_t mid = (lo + hi) >> 1;
@@ -121,40 +121,54 @@ __device__ __forceinline__ uint64_t rowseg_locate(const RowRange* ranges, uint32
   const RowRange rr = ranges[lo];
   const uint64_t g = rr.ra + (l - rr.lrow0);
   *first = rr.out0 + (row_prod[g] - row_prod[rr.ra]);
+  if (grow) *grow = g;
   return row_prod[g + 1] - row_prod[g];
 }
 
-// Per row: small-row products (0 for big rows) and big-row tile counts.
+// Per row: products in small-row groups (0 for big rows), big-row tile
+// counts, and for a row above half a tile with one owner -- its products are
+// that owner's Y segment, already in column order -- the number of
+// kSegTile-product pieces it is cut into (group items the group sort finds
+// in order; row_act NULL: no such rows).
 __global__ __launch_bounds__(kBlock) void k_rowseg_sizes(const RowRange* ranges, uint32_t nr,
                                                          uint64_t lrows, const uint64_t* row_prod,
-                                                         uint64_t tile, uint64_t* small_n,
-                                                         uint64_t* big_t, uint64_t* big_f,
-                                                         uint64_t* rstart) {
+                                                         const uint64_t* row_act, uint64_t tile,
+                                                         uint64_t* small_n, uint64_t* big_t,
+                                                         uint64_t* big_f, uint64_t* rstart,
+                                                         uint64_t* pre_p) {
   uint64_t l = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (; l < lrows; l += stride) {
-    uint64_t first;
-    const uint64_t n = rowseg_locate(ranges, nr, l, row_prod, &first);
-    const bool big = n > kHalfTile;
+    uint64_t first, g;
+    const uint64_t n = rowseg_locate(ranges, nr, l, row_prod, &first, &g);
+    const bool one = row_act && row_act[g + 1] - row_act[g] == 1;
+    const bool pre = n > kHalfTile && one;
+    const bool big = n > kHalfTile && !one;
     small_n[l] = big ? 0 : n;
     big_t[l] = big ? ceil_div(n, tile) : 0;
     big_f[l] = big ? 1 : 0;
+    pre_p[l] = pre ? ceil_div(n, kSegTile) : 0;
     rstart[l] = first;
     if (l + 1 == lrows) rstart[lrows] = first + n;
   }
 }
 
-// Per row: 1 when a small row starts a group (after a big row, or where the
-// running small-row total enters a new half-tile window).
+// Per row: the group items it starts -- 1 when a small row starts a group
+// (after a big or pieced row, or where the running small-row total enters a
+// new half-tile window), a pieced row's piece count.
 __global__ __launch_bounds__(kBlock) void k_rowseg_heads(uint64_t lrows, const uint64_t* small_n,
                                                          const uint64_t* big_t,
+                                                         const uint64_t* pre_p,
                                                          const uint64_t* ps, uint64_t* head) {
   uint64_t l = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (; l < lrows; l += stride) {
     uint64_t h = 0;
-    if (!big_t[l])
-      h = l == 0 || big_t[l - 1] != 0 || ps[l] / kHalfTile != ps[l - 1] / kHalfTile;
+    if (pre_p[l])
+      h = pre_p[l];
+    else if (!big_t[l])
+      h = l == 0 || big_t[l - 1] != 0 || pre_p[l - 1] != 0 ||
+          ps[l] / kHalfTile != ps[l - 1] / kHalfTile;
     head[l] = h;
   }
 }
@@ -163,7 +177,8 @@ __global__ __launch_bounds__(kBlock) void k_rowseg_emit(
     const RowRange* ranges, uint32_t nr, uint64_t lrows, const uint64_t* row_prod, int rb,
     uint64_t tile,
     const uint64_t* big_t, const uint64_t* bt, const uint64_t* bs, const uint64_t* head,
-    const uint64_t* hs, LocalSeg* local, SegTile* big, SegDesc* bigseg, uint32_t* bigrow) {
+    const uint64_t* hs, const uint64_t* pre_p, LocalSeg* local, SegTile* big, SegDesc* bigseg,
+    uint32_t* bigrow) {
   uint64_t l = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (; l < lrows; l += stride) {
@@ -185,6 +200,14 @@ __global__ __launch_bounds__(kBlock) void k_rowseg_emit(
         d.tile = o + t;
         d.first = (uint32_t)o;
         big[o + t] = d;
+      }
+    } else if (pre_p[l]) {
+      for (uint64_t t = 0; t < pre_p[l]; ++t) {  // pieces of a row already in order
+        LocalSeg q;
+        q.start = first + t * kSegTile;
+        q.count = (uint32_t)(n - t * kSegTile < kSegTile ? n - t * kSegTile : kSegTile);
+        q.row0 = (uint32_t)l;
+        local[hs[l] + t] = q;
       }
     } else {
       // the group holding row l: its head is the last head at or before l
@@ -625,7 +648,8 @@ static int plan_passes(const RowSegPlan& plan) {
 
 void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows,
                         const uint64_t* row_prod, int col_bits, RowSegPlan* plan,
-                        SortWorkspace& ws, U2* tmp, hipStream_t s, bool msd) {
+                        SortWorkspace& ws, U2* tmp, hipStream_t s, bool msd,
+                        const uint64_t* row_act) {
   if (col_bits > 32) throw std::logic_error("row-segmented sort of more than 32 column bits");
   plan->nlocal = plan->nbig_tiles = plan->nbig_rows = plan->nsmall = 0;
   plan->lrows = lrows;
@@ -636,7 +660,7 @@ void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows
   const uint64_t tile = (uint64_t)plan->ipt * kSegNT;
   if (!lrows) return;
   const uint64_t m = lrows + 1;
-  if (plan->scratch.size() < 9 * m) plan->scratch.reset(9 * m);
+  if (plan->scratch.size() < 10 * m) plan->scratch.reset(10 * m);
   uint64_t* small_n = plan->scratch.get();
   uint64_t* big_t = small_n + m;
   uint64_t* ps = big_t + m;
@@ -646,14 +670,15 @@ void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows
   uint64_t* big_f = hs + m;
   uint64_t* bs = big_f + m;
   uint64_t* rstart = bs + m;
+  uint64_t* pre_p = rstart + m;
   const uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(lrows, kBlock), 4096);
   hipLaunchKernelGGL(k_rowseg_sizes, dim3(grid), dim3(kBlock), 0, s, ranges, nranges, lrows,
-                     row_prod, tile, small_n, big_t, big_f, rstart);
+                     row_prod, row_act, tile, small_n, big_t, big_f, rstart, pre_p);
   run_exclusive_scan(small_n, ps, lrows, tmp, s);
   run_exclusive_scan(big_t, bt, lrows, tmp, s);
   run_exclusive_scan(big_f, bs, lrows, tmp, s);
-  hipLaunchKernelGGL(k_rowseg_heads, dim3(grid), dim3(kBlock), 0, s, lrows, small_n, big_t, ps,
-                     head);
+  hipLaunchKernelGGL(k_rowseg_heads, dim3(grid), dim3(kBlock), 0, s, lrows, small_n, big_t,
+                     pre_p, ps, head);
   run_exclusive_scan(head, hs, lrows, tmp, s);
   uint64_t tot[4];
   SPTC_HIP(hipMemcpyAsync(&tot[0], hs + lrows, 8, hipMemcpyDeviceToHost, s));
@@ -675,8 +700,8 @@ void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows
     hipLaunchKernelGGL(k_zero_local, dim3((uint32_t)ceil_div(plan->nlocal, 256)), dim3(256), 0,
                        s, plan->local.get(), plan->nlocal);
   hipLaunchKernelGGL(k_rowseg_emit, dim3(grid), dim3(kBlock), 0, s, ranges, nranges, lrows,
-                     row_prod, plan->rb, tile, big_t, bt, bs, head, hs, plan->local.get(),
-                     plan->big.get(), plan->bigseg.get(),
+                     row_prod, plan->rb, tile, big_t, bt, bs, head, hs, pre_p,
+                     plan->local.get(), plan->big.get(), plan->bigseg.get(),
                      plan->msd ? plan->bigrow.get() : nullptr);
   if (plan->nbig_tiles) segmented_sort_reserve(ws, plan->nbig_tiles, s, plan->rb);
 }

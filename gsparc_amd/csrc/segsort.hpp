@@ -129,7 +129,7 @@ struct RowSegPlan {
   DevBuf<SegTile> big;
   DevBuf<SegDesc> bigseg;
   DevBuf<uint32_t> bigrow;   // high-digit form: chunk-local row of each big segment
-  DevBuf<uint64_t> scratch;  // 9 x (rows + 1) per-row values and scans
+  DevBuf<uint64_t> scratch;  // 10 x (rows + 1) per-row values and scans
   // per chunk-local row: nonzero when the row is a big segment
   const uint64_t* row_big() const { return scratch.get() + (lrows + 1); }
   // per chunk-local row: its first product; [lrows] = the product count
@@ -138,9 +138,15 @@ struct RowSegPlan {
 // Builds the plan (one host synchronization for the counts); tmp holds
 // scan_tmp_count(lrows) + 1 entries.  Reserves ws for the big-row passes.
 // msd: use the high-digit form where the column bits allow it.
+// row_act (global row -> its first active X nonzero, [rows + 1]; may be
+// NULL): a row above half a tile with a single owner is already in column
+// order (its products are one Y segment), so it skips the big-row passes
+// and becomes group items of at most kSegTile products each, which the
+// group sort finds in order.
 void build_row_segments(const RowRange* ranges, uint32_t nranges, uint64_t lrows,
                         const uint64_t* row_prod, int col_bits, RowSegPlan* plan,
-                        SortWorkspace& ws, U2* tmp, hipStream_t s, bool msd = false);
+                        SortWorkspace& ws, U2* tmp, hipStream_t s, bool msd = false,
+                        const uint64_t* row_act = nullptr);
 // Where the expansion writes big-row products of an n-product chunk sorted
 // into (col, v) with spare arrays (k1, v1) of n 64-bit keys and n values: a
 // 32-bit column array and a value array, chosen so that the big-row passes
