@@ -239,10 +239,12 @@ __global__ void k_zero_local(LocalSeg* local, uint64_t n) {
 // KT/RB: 32-bit keys with 10-bit digits when (row in group, column) fits 32
 // bits (fewer passes, half the key staging), else 64-bit keys, 8-bit digits.
 template <typename KT, int RB>
+// moved (may be NULL): gets the products of the groups that are sorted.
 __global__ __launch_bounds__(kSegNT) void k_local_sort(uint32_t* cols, float* vals,
                                                        const LocalSeg* __restrict__ segs,
                                                        const uint64_t* __restrict__ rstart,
-                                                       uint64_t lrows) {
+                                                       uint64_t lrows,
+                                                       unsigned long long* moved) {
   constexpr int R = 1 << RB;
   __shared__ TileSmem<RB, kSegIPT, kSegNT, KT> sm;
   __shared__ float vbuf[kSegTile];
@@ -329,6 +331,7 @@ __global__ __launch_bounds__(kSegNT) void k_local_sort(uint32_t* cols, float* va
     if (idx < n && idx > 0 && k[i] < prev) desc = 1;
   }
   if (!__syncthreads_or(desc)) return;
+  if (moved && tid == 0) atomicAdd(moved, (unsigned long long)n);
 #pragma unroll
   for (int i = 0; i < kSegIPT; ++i) {
     const uint64_t idx = wbase + (uint64_t)i * kWave + lane;
@@ -793,14 +796,19 @@ void sort_row_segments(SortWorkspace& ws, const RowSegPlan& plan, uint64_t n, ui
   }
   if (plan.nlocal) {
     int tok = kt ? kt->begin(s) : -1;
+    // every group's columns are read (4 B a product); a group not already in
+    // order also reads its values and writes both back (12 B, or 4 B for
+    // columns alone), counted on the device
+    unsigned long long* moved = kt ? kt->counter(tok, v ? 12.0 : 4.0, s) : nullptr;
     // a group's row offset takes at most 13 bits (6144 products)
     if (plan.col_bits + 13 <= 32)
       hipLaunchKernelGGL((k_local_sort<uint32_t, 10>), dim3((uint32_t)plan.nlocal), dim3(kSegNT),
-                         0, s, col, v, plan.local.get(), row_start, plan.lrows);
+                         0, s, col, v, plan.local.get(), row_start, plan.lrows, moved);
     else
       hipLaunchKernelGGL((k_local_sort<uint64_t, kSegRB>), dim3((uint32_t)plan.nlocal),
-                         dim3(kSegNT), 0, s, col, v, plan.local.get(), row_start, plan.lrows);
-    if (kt) kt->end(tok, SPTC_K_SORT_LOCAL, (v ? 16.0 : 8.0) * (double)plan.nsmall, s);
+                         dim3(kSegNT), 0, s, col, v, plan.local.get(), row_start, plan.lrows,
+                         moved);
+    if (kt) kt->end(tok, SPTC_K_SORT_LOCAL, 4.0 * (double)plan.nsmall, s);
   }
 }
 
