@@ -107,15 +107,20 @@ struct RowRange {
 // (col, val).  Rows holding more than half a 6144-product tile are segments
 // of their own, sorted on the column bits by the multi-tile passes above
 // (digits of 8 bits, or 10 where that saves a pass; tiles of ipt * kSegNT
-// columns) from their narrow target (narrow_target) into (col, val).  Runs
-// of smaller rows are cut into groups of at most 6144 products (a group
-// starts where the running small-row total crosses a multiple of half a
-// tile); each group is sorted on its (row in group, column) bits entirely in
-// one workgroup's LDS, in place in (col, val).  Every product is read and
-// written once for the groups and once per digit pass (at least one) for
-// the big rows, 8 bytes each way.
+// columns) from their narrow target (narrow_target) into (col, val) --
+// except rows with a single owner (with row_act given), which are already
+// in column order and are cut into group items of at most 6144 products
+// instead.  Runs of smaller rows are cut into groups of at most 6144
+// products (a group starts where the running small-row total crosses a
+// multiple of half a tile); each group item is sorted on its (row in group,
+// column) bits entirely in one workgroup's LDS, in place in (col, val),
+// unless it is already in that order.  Every big-row product is read and
+// written once per digit pass (at least one), 8 bytes each way; a group
+// item reads its columns, and when it needs sorting its values, and writes
+// both back.
 struct RowSegPlan {
-  uint64_t nlocal = 0, nbig_tiles = 0, nbig_rows = 0, nsmall = 0;  // nsmall: products in groups
+  // nsmall: products in group items (small-row groups and one-owner rows)
+  uint64_t nlocal = 0, nbig_tiles = 0, nbig_rows = 0, nsmall = 0;
   uint64_t lrows = 0;
   int col_bits = 0;
   int rb = kSegRB;  // digit bits of the big-row passes (8 or 10)
